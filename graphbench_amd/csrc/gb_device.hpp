@@ -1,0 +1,71 @@
+// gb_device.hpp -- wave64 primitives shared by the CDNA4 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GB_WAVE 64
+
+__device__ __forceinline__ int gb_lane() { return (int)__lane_id(); }
+
+// Exclusive prefix sum over the 64 lanes of a wave; *total gets the sum.
+__device__ __forceinline__ uint32_t gb_wave_excl_sum(uint32_t x, uint32_t *total) {
+  uint32_t incl = x;
+#pragma unroll
+  for (int d = 1; d < GB_WAVE; d <<= 1) {
+    uint32_t y = __shfl_up(incl, d, GB_WAVE);
+    if (gb_lane() >= d) incl += y;
+  }
+  *total = __shfl(incl, GB_WAVE - 1, GB_WAVE);
+  return incl - x;
+}
+
+__device__ __forceinline__ uint64_t gb_shfl_u64(uint64_t v, int src) {
+  uint32_t lo = __shfl((uint32_t)v, src, GB_WAVE);
+  uint32_t hi = __shfl((uint32_t)(v >> 32), src, GB_WAVE);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Reserve `cnt` consecutive slots of a device queue for this lane.  One
+// atomicAdd per wave.  Must be reached by every active lane of the wave.
+__device__ __forceinline__ uint64_t gb_wave_reserve(unsigned long long *counter, uint32_t cnt) {
+  uint32_t total;
+  uint32_t excl = gb_wave_excl_sum(cnt, &total);
+  uint64_t base = 0;
+  if (total) {
+    // the lowest active lane does the atomic
+    uint64_t active = __ballot(1);
+    int leader = __ffsll((unsigned long long)active) - 1;
+    if (gb_lane() == leader) base = atomicAdd(counter, (unsigned long long)total);
+    base = gb_shfl_u64(base, leader);
+  }
+  return base + excl;
+}
+
+// Deterministic wave sum (fixed butterfly order).
+__device__ __forceinline__ double gb_wave_sum_f64(double v) {
+#pragma unroll
+  for (int d = GB_WAVE / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, GB_WAVE);
+  return v;
+}
+
+// Compensated (double-float) accumulation: value ~= s + c.
+struct F2 {
+  float s, c;
+};
+__device__ __forceinline__ void f2_add(F2 &a, float v) {
+  float t = a.s + v;
+  float bp = t - a.s;
+  float err = (a.s - (t - bp)) + (v - bp);
+  a.s = t;
+  a.c += err;
+}
+__device__ __forceinline__ F2 f2_merge(F2 a, F2 b) {
+  float t = a.s + b.s;
+  float bp = t - a.s;
+  float err = (a.s - (t - bp)) + (b.s - bp);
+  F2 r;
+  r.s = t;
+  r.c = (a.c + b.c) + err;
+  return r;
+}
+__device__ __forceinline__ float f2_value(F2 a) { return a.s + a.c; }

@@ -1,0 +1,615 @@
+// gb_pagerank.hip -- PageRank pull gather for gfx950.
+//
+// Replaces the local gather and update/termination of run_pagerank_csr_mpi
+// (pagerank/pagerank_mpi.c:833-846 and 995-1020) on one GPU.
+//
+// Configured form (gb_pagerank): x_k[v] = base_k + d * sum_{u->v} x_{k-1}[u]/outdeg[u],
+// base_k = (1-d)/N + d * dangling_{k-1}/N.  One iteration = three launches:
+//   pr_gather  merge-path tiles over the in-CSR (rows + edges split evenly),
+//              edge values gathered into LDS with coalesced col reads, each
+//              thread consumes IPT items with a compensated FP32 sum, a block
+//              segmented scan joins rows split between threads, the epilogue
+//              writes x_k, contrib_k = x_k/outdeg and FP64 residual/dangling
+//              block partials.  A row that began in an earlier tile is left to
+//   pr_fixup   which adds the earlier tiles' tail partials in tile order.
+//   pr_finalize one block: FP64 sums of the partials in a fixed order, the
+//              next base, and the stop flag (L1 residual < tol or max_iter).
+// Every reduction has a fixed order, so results are run-to-run identical.
+// Kernels read the stop flag first, so chunks of iterations are captured in
+// one hipGraph and replayed until the flag is set.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "gb_device.hpp"
+#include "gb_internal.hpp"
+
+namespace {
+
+constexpr int PR_BLOCK = 256;
+
+struct PrState {
+  double residual;
+  double dangling;
+  double tol;
+  float base;
+  float damping;
+  int32_t iter;
+  int32_t done;
+  int32_t max_iter;
+  int32_t V;
+};
+
+__global__ void inv_outdeg_kernel(int32_t V, const int32_t *__restrict__ rp,
+                                  float *__restrict__ inv) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  int32_t d = rp[v + 1] - rp[v];
+  inv[v] = d > 0 ? 1.0f / (float)d : 0.0f;
+}
+
+// Out-degrees from the in-CSR when the forward CSR is absent.
+__global__ void count_outdeg_kernel(int64_t E, const int32_t *__restrict__ icol,
+                                    int32_t *__restrict__ deg) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) atomicAdd(&deg[icol[e]], 1);
+}
+__global__ void inv_from_deg_kernel(int32_t V, const int32_t *__restrict__ deg,
+                                    float *__restrict__ inv) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < V) inv[v] = deg[v] > 0 ? 1.0f / (float)deg[v] : 0.0f;
+}
+
+__device__ __forceinline__ double block_sum_f64(double v, double *s_red) {
+  v = gb_wave_sum_f64(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if (gb_lane() == 0) s_red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = blockDim.x >> 6;
+  for (int k = 0; k < nw; k++) t += s_red[k];
+  return t;
+}
+
+__global__ __launch_bounds__(PR_BLOCK) void pr_init_kernel(
+    int32_t V, const float *__restrict__ x0, const float *__restrict__ inv_od,
+    float *__restrict__ x, float *__restrict__ contrib, double *__restrict__ part_dang) {
+  __shared__ double s_red[PR_BLOCK / 64];
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double dang = 0.0;
+  if (v < V) {
+    float xv = x0 ? x0[v] : (float)(1.0 / (double)V);
+    float io = inv_od[v];
+    x[v] = xv;
+    contrib[v] = xv * io;
+    if (io == 0.0f) dang = (double)xv;
+  }
+  double t = block_sum_f64(dang, s_red);
+  if (threadIdx.x == 0) part_dang[blockIdx.x] = t;
+}
+
+__global__ void pr_init_finalize_kernel(int32_t nparts, const double *__restrict__ part_dang,
+                                        PrState *__restrict__ st) {
+  __shared__ double s_red[1024 / 64];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) a += part_dang[i];
+  double dang = block_sum_f64(a, s_red);
+  if (threadIdx.x == 0) {
+    const double N = (double)st->V;
+    const double d = (double)st->damping;
+    st->dangling = dang;
+    st->base = (float)((1.0 - d) / N + d * dang / N);
+    st->iter = 0;
+    st->residual = INFINITY;
+    st->done = st->max_iter <= 0 ? 1 : 0;
+  }
+}
+
+struct RowOut {
+  float base, damping;
+  const float *__restrict__ x_old;
+  const float *__restrict__ inv_od;
+  float *__restrict__ x_new;
+  float *__restrict__ contrib_new;
+  __device__ __forceinline__ void operator()(int32_t v, F2 tot, double &res, double &dang) const {
+    const float s = f2_value(tot);
+    const float xn = base + damping * s;
+    const float xo = x_old[v];
+    res += fabs((double)xn - (double)xo);
+    const float io = inv_od[v];
+    x_new[v] = xn;
+    contrib_new[v] = xn * io;
+    if (io == 0.0f) dang += (double)xn;
+  }
+};
+
+template <int IPT>
+__global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
+    const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
+    const int32_t *__restrict__ tile_row, const int32_t *__restrict__ tile_edge,
+    const float *__restrict__ contrib, RowOut out, F2 *__restrict__ tail,
+    F2 *__restrict__ head, double *__restrict__ part_res, double *__restrict__ part_dang,
+    const PrState *__restrict__ st) {
+  constexpr int TILE = PR_BLOCK * IPT;
+  __shared__ int32_t s_end[TILE];
+  __shared__ float s_val[TILE];
+  __shared__ F2 s_scan[PR_BLOCK];
+  __shared__ int32_t s_flag[PR_BLOCK];
+  __shared__ double s_red[PR_BLOCK / 64];
+  if (st->done) return;
+  out.base = st->base;
+  const int t = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int32_t r0 = tile_row[t], e0 = tile_edge[t];
+  const int32_t r1 = tile_row[t + 1], e1 = tile_edge[t + 1];
+  const int nrows = r1 - r0, nedges = e1 - e0;
+  for (int k = tid; k < nrows; k += PR_BLOCK) s_end[k] = irp[r0 + k + 1] - e0;
+  for (int k = tid; k < nedges; k += PR_BLOCK) s_val[k] = contrib[icol[e0 + k]];
+  __syncthreads();
+
+  // this thread's start on the merge path, searched in LDS
+  const int total = nrows + nedges;
+  int dg = tid * IPT;
+  if (dg > total) dg = total;
+  int lo = dg - nedges > 0 ? dg - nedges : 0;
+  int hi = dg < nrows ? dg : nrows;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (s_end[mid] <= dg - mid - 1)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  int i = lo, j = dg - lo;
+  F2 acc = {0.0f, 0.0f}, first = {0.0f, 0.0f};
+  int first_row = -1;
+  double res = 0.0, dang = 0.0;
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    if (i < nrows && s_end[i] <= j) {
+      if (first_row < 0) {
+        first_row = i;
+        first = acc;
+      } else {
+        out(r0 + i, acc, res, dang);
+      }
+      acc.s = 0.0f;
+      acc.c = 0.0f;
+      ++i;
+    } else if (j < nedges) {
+      f2_add(acc, s_val[j]);
+      ++j;
+    }
+  }
+  // segmented inclusive scan of the open-row partials over the threads
+  s_scan[tid] = acc;
+  s_flag[tid] = first_row >= 0;
+  __syncthreads();
+  for (int off = 1; off < PR_BLOCK; off <<= 1) {
+    F2 vb = s_scan[tid];
+    int fb = s_flag[tid];
+    F2 va;
+    int fa = 0;
+    const bool has = tid >= off;
+    if (has) {
+      va = s_scan[tid - off];
+      fa = s_flag[tid - off];
+    }
+    __syncthreads();
+    if (has) {
+      s_scan[tid] = fb ? vb : f2_merge(va, vb);
+      s_flag[tid] = fa | fb;
+    }
+    __syncthreads();
+  }
+  if (first_row >= 0) {
+    F2 cin = tid > 0 ? s_scan[tid - 1] : F2{0.0f, 0.0f};
+    F2 tot = f2_merge(cin, first);
+    if (first_row == 0 && irp[r0] < e0) {
+      head[t] = tot;  // row r0 began in an earlier tile: pr_fixup finishes it
+    } else {
+      out(r0 + first_row, tot, res, dang);
+    }
+  }
+  if (tid == PR_BLOCK - 1) tail[t] = s_scan[PR_BLOCK - 1];
+  double rs = block_sum_f64(res, s_red);
+  double ds = block_sum_f64(dang, s_red);
+  if (tid == 0) {
+    part_res[t] = rs;
+    part_dang[t] = ds;
+  }
+}
+
+__global__ __launch_bounds__(PR_BLOCK) void pr_fixup_kernel(
+    int64_t ntiles, const int32_t *__restrict__ irp, const int32_t *__restrict__ tile_row,
+    const int32_t *__restrict__ tile_edge, const F2 *__restrict__ tail,
+    const F2 *__restrict__ head, RowOut out, double *__restrict__ fix_res,
+    double *__restrict__ fix_dang, const PrState *__restrict__ st) {
+  __shared__ double s_red[PR_BLOCK / 64];
+  if (st->done) return;
+  out.base = st->base;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double res = 0.0, dang = 0.0;
+  if (t < ntiles) {
+    const int32_t r0 = tile_row[t], e0 = tile_edge[t], r1 = tile_row[t + 1];
+    if (r1 > r0) {
+      const int32_t rs = irp[r0];
+      if (rs < e0) {
+        F2 acc = head[t];
+        for (int64_t k = t - 1; k >= 0; --k) {
+          acc = f2_merge(tail[k], acc);
+          if (tile_edge[k] <= rs) break;
+        }
+        out(r0, acc, res, dang);
+      }
+    }
+  }
+  double rsum = block_sum_f64(res, s_red);
+  double dsum = block_sum_f64(dang, s_red);
+  if (threadIdx.x == 0) {
+    fix_res[blockIdx.x] = rsum;
+    fix_dang[blockIdx.x] = dsum;
+  }
+}
+
+__global__ void pr_finalize_kernel(int64_t n1, const double *__restrict__ p_res,
+                                   const double *__restrict__ p_dang, int64_t n2,
+                                   const double *__restrict__ f_res,
+                                   const double *__restrict__ f_dang, PrState *__restrict__ st) {
+  __shared__ double s_red[1024 / 64];
+  if (st->done) return;
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < n1; i += blockDim.x) {
+    a += p_res[i];
+    b += p_dang[i];
+  }
+  for (int64_t i = threadIdx.x; i < n2; i += blockDim.x) {
+    a += f_res[i];
+    b += f_dang[i];
+  }
+  double res = block_sum_f64(a, s_red);
+  double dang = block_sum_f64(b, s_red);
+  if (threadIdx.x == 0) {
+    const double N = (double)st->V;
+    const double d = (double)st->damping;
+    st->iter += 1;
+    st->residual = res;
+    st->dangling = dang;
+    st->base = (float)((1.0 - d) / N + d * dang / N);
+    st->done = (res < st->tol || st->iter >= st->max_iter) ? 1 : 0;
+  }
+}
+
+// ---- reference-semantics form (FP64, bit-identical to the C loop) ---------
+
+__global__ void pr_ref_init_kernel(int32_t V, const double *__restrict__ recip,
+                                   double *__restrict__ x, double *__restrict__ diff,
+                                   double *__restrict__ wb) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  x[v] = 1.0;                          // init_weight, pagerank_mpi.c:63-68
+  diff[v] = __dmul_rn(1.0, recip[v]);  // 752-754
+  wb[v] = 0.3;                         // RDM_JMP, 808-809
+}
+
+__global__ void pr_ref_gather_kernel(int32_t V, const int32_t *__restrict__ irp,
+                                     const int32_t *__restrict__ icol,
+                                     const double *__restrict__ diff, double *__restrict__ wb,
+                                     double pj) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  double acc = wb[v];
+  for (int32_t e = irp[v]; e < irp[v + 1]; e++)  // 833-846, in-CSR order
+    acc = __dadd_rn(acc, __dmul_rn(pj, diff[icol[e]]));
+  wb[v] = acc;
+}
+
+__global__ void pr_ref_update_kernel(int32_t V, const double *__restrict__ recip,
+                                     double *__restrict__ x, double *__restrict__ diff,
+                                     const double *__restrict__ wb, int *__restrict__ not_done) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool big = false;
+  if (v < V) {
+    double dv = __dmul_rn(__dsub_rn(wb[v], x[v]), recip[v]);  // 995-997
+    diff[v] = dv;
+    big = fabs(dv) > 0.000001;  // THRESH; NaN compares false (converged)
+    x[v] = wb[v];
+  }
+  if (__ballot(big) != 0ull && gb_lane() == 0) atomicOr(not_done, 1);
+}
+
+struct PrWork {
+  DevBuf xb, ca, cb, tail, head, pres, pdang, fres, fdang, state;
+  void release() {
+    for (DevBuf *b : {&xb, &ca, &cb, &tail, &head, &pres, &pdang, &fres, &fdang, &state})
+      b->release();
+  }
+};
+PrWork g_prw;
+
+int default_ipt(gb_graph g) {
+  (void)g;
+  const char *e = getenv("GB_PR_IPT");
+  if (e) return atoi(e);
+  return 8;
+}
+
+template <int IPT>
+void launch_gather(int64_t ntiles, hipStream_t st, const int32_t *irp, const int32_t *icol,
+                   const int32_t *trow, const int32_t *tedge, const float *contrib, RowOut out,
+                   F2 *tail, F2 *head, double *pres, double *pdang, const PrState *state) {
+  hipLaunchKernelGGL(pr_gather_kernel<IPT>, dim3((unsigned)ntiles), dim3(PR_BLOCK), 0, st, irp,
+                     icol, trow, tedge, contrib, out, tail, head, pres, pdang, state);
+}
+
+}  // namespace
+
+void gb_pr_release() { g_prw.release(); }
+
+static gb_status ensure_inv_outdeg(gb_graph g, hipStream_t st) {
+  if (g->inv_outdeg_valid) return GB_OK;
+  GB_TRY(g->inv_outdeg.ensure(sizeof(float) * (g->V + 1)));
+  int blocks = (int)((g->V + 255) / 256);
+  if (g->V == 0) return GB_OK;
+  if (g->rp) {
+    hipLaunchKernelGGL(inv_outdeg_kernel, dim3(blocks), dim3(256), 0, st, g->V, g->rp,
+                       g->inv_outdeg.as<float>());
+  } else {
+    DevBuf deg;
+    GB_TRY(deg.ensure(sizeof(int32_t) * g->V));
+    GB_HIP(hipMemsetAsync(deg.p, 0, sizeof(int32_t) * g->V, st));
+    int eb = (int)((g->E + 255) / 256);
+    if (eb > 0)
+      hipLaunchKernelGGL(count_outdeg_kernel, dim3(eb), dim3(256), 0, st, g->E, g->rcol,
+                         deg.as<int32_t>());
+    hipLaunchKernelGGL(inv_from_deg_kernel, dim3(blocks), dim3(256), 0, st, g->V,
+                       deg.as<int32_t>(), g->inv_outdeg.as<float>());
+    GB_HIP(hipStreamSynchronize(st));
+  }
+  GB_HIP(hipGetLastError());
+  g->inv_outdeg_valid = true;
+  return GB_OK;
+}
+
+extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
+                                 gb_pr_result *res, void *stream) {
+  if (!g || !p || !d_x) return gb_fail(GB_ERR_INVALID, "gb_pagerank: null argument");
+  if (!g->rrp || (!g->rcol && g->E > 0))
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: graph has no reverse CSR");
+  if (g->V <= 0) return gb_fail(GB_ERR_INVALID, "gb_pagerank: empty graph");
+  if (!(p->damping >= 0.0 && p->damping <= 1.0) || p->max_iter < 0)
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: bad damping/max_iter");
+  hipStream_t st = gb_stream(stream);
+  hipStream_t own = nullptr;
+  if (!st && p->use_graph && !p->time_kernels) {
+    // the null stream cannot be captured: use a blocking stream (it orders
+    // itself against the null stream) for this call
+    GB_HIP(hipStreamCreate(&own));
+    st = own;
+  }
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() {
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    }
+  } guard{own};
+  const int32_t V = g->V;
+  int ipt = p->items_per_thread > 0 ? p->items_per_thread : default_ipt(g);
+  if (ipt != 4 && ipt != 8 && ipt != 12 && ipt != 16) ipt = 8;
+  const int64_t tile = (int64_t)PR_BLOCK * ipt;
+  GB_TRY(ensure_inv_outdeg(g, st));
+  if (g->pr_tile_items != tile) {
+    const int64_t ntiles = ((int64_t)V + g->E + tile - 1) / tile;
+    GB_TRY(g->pr_tile_row.ensure(sizeof(int32_t) * (ntiles + 1)));
+    GB_TRY(g->pr_tile_edge.ensure(sizeof(int32_t) * (ntiles + 1)));
+    GB_TRY(gb_merge_path_split(V, g->rrp, tile, ntiles, g->pr_tile_row.as<int32_t>(),
+                               g->pr_tile_edge.as<int32_t>(), st));
+    g->pr_tile_items = (int32_t)tile;
+    g->pr_ntiles = ntiles;
+  }
+  const int64_t ntiles = g->pr_ntiles;
+  const int64_t nfix = (ntiles + PR_BLOCK - 1) / PR_BLOCK;
+  const int64_t vblocks = ((int64_t)V + PR_BLOCK - 1) / PR_BLOCK;
+  PrWork &w = g_prw;
+  GB_TRY(w.xb.ensure(sizeof(float) * V));
+  GB_TRY(w.ca.ensure(sizeof(float) * V));
+  GB_TRY(w.cb.ensure(sizeof(float) * V));
+  GB_TRY(w.tail.ensure(sizeof(F2) * ntiles));
+  GB_TRY(w.head.ensure(sizeof(F2) * ntiles));
+  GB_TRY(w.pres.ensure(sizeof(double) * std::max(ntiles, vblocks)));
+  GB_TRY(w.pdang.ensure(sizeof(double) * std::max(ntiles, vblocks)));
+  GB_TRY(w.fres.ensure(sizeof(double) * nfix));
+  GB_TRY(w.fdang.ensure(sizeof(double) * nfix));
+  GB_TRY(w.state.ensure(sizeof(PrState)));
+  PrState hs{};
+  hs.tol = p->tol;
+  hs.damping = (float)p->damping;
+  hs.max_iter = p->max_iter;
+  hs.V = V;
+  GB_HIP(hipMemcpyAsync(w.state.p, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
+  PrState *state = w.state.as<PrState>();
+  float *xa = d_x, *xb = w.xb.as<float>();
+  float *ca = w.ca.as<float>(), *cb = w.cb.as<float>();
+  const float *inv = g->inv_outdeg.as<float>();
+  hipLaunchKernelGGL(pr_init_kernel, dim3((unsigned)vblocks), dim3(PR_BLOCK), 0, st, V, p->d_x0,
+                     inv, xa, ca, w.pdang.as<double>());
+  hipLaunchKernelGGL(pr_init_finalize_kernel, dim3(1), dim3(1024), 0, st, (int32_t)vblocks,
+                     w.pdang.as<double>(), state);
+  GB_HIP(hipGetLastError());
+
+  const int32_t *trow = g->pr_tile_row.as<int32_t>();
+  const int32_t *tedge = g->pr_tile_edge.as<int32_t>();
+  auto one_iter = [&](const float *x_old, const float *c_old, float *x_new, float *c_new) {
+    RowOut out{0.0f, (float)p->damping, x_old, inv, x_new, c_new};
+    switch (ipt) {
+      case 4:
+        launch_gather<4>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
+                         w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
+                         w.pdang.as<double>(), state);
+        break;
+      case 12:
+        launch_gather<12>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
+                          w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
+                          w.pdang.as<double>(), state);
+        break;
+      case 16:
+        launch_gather<16>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
+                          w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
+                          w.pdang.as<double>(), state);
+        break;
+      default:
+        launch_gather<8>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
+                         w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
+                         w.pdang.as<double>(), state);
+    }
+    hipLaunchKernelGGL(pr_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, ntiles,
+                       g->rrp, trow, tedge, w.tail.as<F2>(), w.head.as<F2>(), out,
+                       w.fres.as<double>(), w.fdang.as<double>(), state);
+    hipLaunchKernelGGL(pr_finalize_kernel, dim3(1), dim3(1024), 0, st, ntiles,
+                       w.pres.as<double>(), w.pdang.as<double>(), nfix, w.fres.as<double>(),
+                       w.fdang.as<double>(), state);
+  };
+
+  PrState host_state{};
+  const int CH = 10;  // iterations per chunk (even: buffer parity restarts)
+  double gather_ms = 0.0;
+  int gather_launches = 0;
+  if (p->use_graph && !p->time_kernels) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    GB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < CH; k++) {
+      if (k % 2 == 0)
+        one_iter(xa, ca, xb, cb);
+      else
+        one_iter(xb, cb, xa, ca);
+    }
+    GB_HIP(hipStreamEndCapture(st, &graph));
+    GB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    for (int done_it = 0; done_it < p->max_iter; done_it += CH) {
+      GB_HIP(hipGraphLaunch(exec, st));
+      GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
+      GB_HIP(hipStreamSynchronize(st));
+      if (host_state.done) break;
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+  } else {
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (p->time_kernels) {
+      GB_HIP(hipEventCreate(&ev0));
+      GB_HIP(hipEventCreate(&ev1));
+    }
+    for (int k = 0; k < p->max_iter; k++) {
+      const float *xo = (k % 2 == 0) ? xa : xb;
+      const float *co = (k % 2 == 0) ? ca : cb;
+      float *xn = (k % 2 == 0) ? xb : xa;
+      float *cn = (k % 2 == 0) ? cb : ca;
+      if (p->time_kernels) {
+        // time the gather alone: same launch as one_iter's first kernel
+        RowOut out{0.0f, (float)p->damping, xo, inv, xn, cn};
+        GB_HIP(hipEventRecord(ev0, st));
+        switch (ipt) {
+          case 4:
+            launch_gather<4>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
+                             w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
+            break;
+          case 12:
+            launch_gather<12>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
+                              w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
+            break;
+          case 16:
+            launch_gather<16>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
+                              w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
+            break;
+          default:
+            launch_gather<8>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
+                             w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
+        }
+        GB_HIP(hipEventRecord(ev1, st));
+        hipLaunchKernelGGL(pr_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, ntiles,
+                           g->rrp, trow, tedge, w.tail.as<F2>(), w.head.as<F2>(), out,
+                           w.fres.as<double>(), w.fdang.as<double>(), state);
+        hipLaunchKernelGGL(pr_finalize_kernel, dim3(1), dim3(1024), 0, st, ntiles,
+                           w.pres.as<double>(), w.pdang.as<double>(), nfix, w.fres.as<double>(),
+                           w.fdang.as<double>(), state);
+        GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
+        GB_HIP(hipStreamSynchronize(st));
+        float ms = 0.0f;
+        GB_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        gather_ms += ms;
+        gather_launches++;
+        if (host_state.done) break;
+      } else {
+        one_iter(xo, co, xn, cn);
+        if ((k + 1) % CH == 0 || k + 1 == p->max_iter) {
+          GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
+          GB_HIP(hipStreamSynchronize(st));
+          if (host_state.done) break;
+        }
+      }
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+  GB_HIP(hipGetLastError());
+  GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  if (host_state.iter % 2 == 1)
+    GB_HIP(hipMemcpyAsync(d_x, xb, sizeof(float) * V, hipMemcpyDeviceToDevice, st));
+  GB_HIP(hipStreamSynchronize(st));
+  const bool conv = host_state.residual < p->tol;
+  if (res) {
+    res->iterations = host_state.iter;
+    res->converged = conv ? 1 : 0;
+    res->residual = host_state.residual;
+    res->dangling = host_state.dangling;
+    res->gather_ms = gather_ms;
+    res->gather_launches = gather_launches;
+    res->tiles = (int32_t)ntiles;
+  }
+  return GB_OK;
+}
+
+extern "C" gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t max_iter,
+                                     double *d_x, int32_t *iterations, void *stream) {
+  if (!g || !d_recip || !d_x) return gb_fail(GB_ERR_INVALID, "gb_pagerank_ref: null argument");
+  if (!g->rrp || (!g->rcol && g->E > 0))
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank_ref: no reverse CSR");
+  hipStream_t st = gb_stream(stream);
+  const int32_t V = g->V;
+  if (V <= 0) return gb_fail(GB_ERR_INVALID, "gb_pagerank_ref: empty graph");
+  DevBuf diff, wb, flag;
+  GB_TRY(diff.ensure(sizeof(double) * V));
+  GB_TRY(wb.ensure(sizeof(double) * V));
+  GB_TRY(flag.ensure(sizeof(int)));
+  const int blocks = (V + 255) / 256;
+  hipLaunchKernelGGL(pr_ref_init_kernel, dim3(blocks), dim3(256), 0, st, V, d_recip, d_x,
+                     diff.as<double>(), wb.as<double>());
+  const double pj = 1.0 - 0.3;  // purpose_jump, pagerank_mpi.c:689
+  int32_t it = 0;
+  for (;;) {
+    if (max_iter > 0 && it >= max_iter) {
+      it = -1;
+      break;
+    }
+    it++;
+    GB_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(pr_ref_gather_kernel, dim3(blocks), dim3(256), 0, st, V, g->rrp, g->rcol,
+                       diff.as<double>(), wb.as<double>(), pj);
+    hipLaunchKernelGGL(pr_ref_update_kernel, dim3(blocks), dim3(256), 0, st, V, d_recip, d_x,
+                       diff.as<double>(), wb.as<double>(), flag.as<int>());
+    int h = 0;
+    GB_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    if (!h) break;
+  }
+  GB_HIP(hipGetLastError());
+  if (iterations) *iterations = it;
+  return it < 0 ? GB_ERR_NOT_CONVERGED : GB_OK;
+}

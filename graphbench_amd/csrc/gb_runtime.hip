@@ -1,0 +1,428 @@
+// gb_runtime.hip -- status/errors, device memory, graph handles and the
+// graph-structure kernels (merge-path split, edge-balanced parts, light/heavy
+// separation, max weight).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "gb_device.hpp"
+#include "gb_internal.hpp"
+
+namespace {
+thread_local char g_err[1024] = "";
+std::mutex g_mem_mu;
+uint64_t g_mem_cap = 0;
+uint64_t g_mem_used = 0;
+int g_device = -1;
+int g_cus = 256;
+std::unordered_map<void *, size_t> g_sizes;  // live allocations, for the cap
+}  // namespace
+
+gb_status gb_fail(gb_status s, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return s;
+}
+
+hipStream_t gb_stream(void *s) { return (hipStream_t)s; }
+int gb_num_cus() { return g_cus; }
+
+gb_status gb_dmalloc(void **p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    if (g_mem_cap && g_mem_used + bytes > g_mem_cap)
+      return gb_fail(GB_ERR_NOMEM, "device allocation of %zu bytes exceeds the cap (%llu used of %llu)",
+                     bytes, (unsigned long long)g_mem_used, (unsigned long long)g_mem_cap);
+  }
+  void *raw = nullptr;
+  hipError_t e = hipMalloc(&raw, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return gb_fail(GB_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    g_mem_used += bytes;
+    g_sizes[raw] = bytes;
+  }
+  *p = raw;
+  return GB_OK;
+}
+
+void gb_dfree(void *p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_mem_mu);
+    auto it = g_sizes.find(p);
+    if (it != g_sizes.end()) {
+      g_mem_used -= it->second;
+      g_sizes.erase(it);
+    }
+  }
+  (void)hipFree(p);
+}
+
+extern "C" {
+
+const char *gb_version(void) { return "graphbench-mi355x 0.1 (gfx950)"; }
+
+const char *gb_status_string(gb_status s) {
+  switch (s) {
+    case GB_OK: return "ok";
+    case GB_ERR_INVALID: return "invalid argument";
+    case GB_ERR_NOMEM: return "out of device memory";
+    case GB_ERR_HIP: return "HIP runtime error";
+    case GB_ERR_OVERFLOW: return "uint32 distance overflow";
+    case GB_ERR_NOT_CONVERGED: return "iteration cap reached";
+    case GB_ERR_NO_DEVICE: return "no HIP device";
+    case GB_ERR_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown";
+}
+
+const char *gb_last_error(void) { return g_err; }
+
+void gb_release_workspaces(void) {
+  gb_pr_release();
+  gb_sssp_release();
+  gb_bc_release();
+}
+
+void gb_set_mem_cap(uint64_t bytes) {
+  std::lock_guard<std::mutex> lk(g_mem_mu);
+  g_mem_cap = bytes;
+}
+
+gb_status gb_init(int32_t device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return gb_fail(GB_ERR_NO_DEVICE, "no HIP device available");
+  }
+  if (device < 0 || device >= n) return gb_fail(GB_ERR_INVALID, "device %d out of range", device);
+  GB_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  GB_HIP(hipGetDeviceProperties(&prop, device));
+  g_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  g_device = device;
+  return GB_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// graph handles
+
+extern "C" gb_status gb_graph_wrap(gb_graph *out, int32_t V, int64_t E,
+                                   const int32_t *d_row_ptr, const int32_t *d_col,
+                                   const uint32_t *d_w, const int32_t *d_rrow_ptr,
+                                   const int32_t *d_rcol, const uint32_t *d_rw) {
+  if (!out || V < 0 || E < 0 || E >= (1ll << 31) || (!d_row_ptr && V > 0) || (!d_col && E > 0))
+    return gb_fail(GB_ERR_INVALID, "gb_graph_wrap: bad graph (V=%d E=%lld)", V, (long long)E);
+  if (g_device < 0) GB_TRY(gb_init(0));
+  gb_graph g = new gb_graph_s();
+  g->V = V;
+  g->E = E;
+  g->rp = d_row_ptr;
+  g->col = d_col;
+  g->w = d_w;
+  g->rrp = d_rrow_ptr;
+  g->rcol = d_rcol;
+  g->rw = d_rw;
+  *out = g;
+  return GB_OK;
+}
+
+extern "C" gb_status gb_graph_upload(gb_graph *out, int32_t V, int64_t E,
+                                     const int32_t *row_ptr, const int32_t *col,
+                                     const uint32_t *w, const int32_t *rrow_ptr,
+                                     const int32_t *rcol, const uint32_t *rw) {
+  if (!out || V < 0 || E < 0 || E >= (1ll << 31) || !row_ptr || (!col && E > 0))
+    return gb_fail(GB_ERR_INVALID, "gb_graph_upload: bad graph");
+  if (row_ptr[0] != 0 || row_ptr[V] != E)
+    return gb_fail(GB_ERR_INVALID, "gb_graph_upload: row_ptr[0]=%d row_ptr[V]=%d, E=%lld",
+                   row_ptr[0], row_ptr[V], (long long)E);
+  if (g_device < 0) GB_TRY(gb_init(0));
+  gb_graph g = new gb_graph_s();
+  g->V = V;
+  g->E = E;
+  g->owns = true;
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> gb_status {
+    if (!src) return GB_OK;
+    gb_status s = b.ensure(bytes);
+    if (s != GB_OK) return s;
+    if (bytes) GB_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return GB_OK;
+  };
+  gb_status s = GB_OK;
+  if ((s = up(g->own_rp, row_ptr, sizeof(int32_t) * (V + 1))) != GB_OK ||
+      (s = up(g->own_col, col, sizeof(int32_t) * E)) != GB_OK ||
+      (s = up(g->own_w, w, sizeof(uint32_t) * E)) != GB_OK ||
+      (s = up(g->own_rrp, rrow_ptr, sizeof(int32_t) * (V + 1))) != GB_OK ||
+      (s = up(g->own_rcol, rcol, sizeof(int32_t) * E)) != GB_OK ||
+      (s = up(g->own_rw, rw, sizeof(uint32_t) * E)) != GB_OK) {
+    delete g;
+    return s;
+  }
+  g->rp = g->own_rp.as<int32_t>();
+  g->col = g->own_col.as<int32_t>();
+  g->w = w ? g->own_w.as<uint32_t>() : nullptr;
+  g->rrp = rrow_ptr ? g->own_rrp.as<int32_t>() : nullptr;
+  g->rcol = rcol ? g->own_rcol.as<int32_t>() : nullptr;
+  g->rw = rw ? g->own_rw.as<uint32_t>() : nullptr;
+  *out = g;
+  return GB_OK;
+}
+
+extern "C" gb_status gb_graph_set_f64_weights(gb_graph g, const double *d_w64,
+                                              const double *d_rw64) {
+  if (!g || (!d_w64 && g->E > 0)) return gb_fail(GB_ERR_INVALID, "gb_graph_set_f64_weights");
+  g->w64 = d_w64;
+  g->rw64 = d_rw64;
+  g->split64_valid = false;
+  return GB_OK;
+}
+
+extern "C" gb_status gb_graph_destroy(gb_graph g) {
+  if (!g) return GB_OK;
+  delete g;
+  return GB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// merge-path split: one thread per tile boundary.
+// Row end i (= row_ptr[i+1]) is consumed before edge j iff row_ptr[i+1] <= j.
+
+__global__ void merge_path_split_kernel(int32_t V, const int32_t *__restrict__ rp,
+                                        int64_t items, int64_t ntiles,
+                                        int32_t *__restrict__ tile_row,
+                                        int32_t *__restrict__ tile_edge) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const int64_t E = rp[V];
+  int64_t d = t * items;
+  const int64_t total = (int64_t)V + E;
+  if (d > total) d = total;
+  int64_t lo = d - E > 0 ? d - E : 0;
+  int64_t hi = d < V ? d : V;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)rp[mid + 1] <= d - mid - 1)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  tile_row[t] = (int32_t)lo;
+  tile_edge[t] = (int32_t)(d - lo);
+}
+
+extern "C" gb_status gb_merge_path_split(int32_t V, const int32_t *d_row_ptr, int64_t items,
+                                         int64_t ntiles, int32_t *d_tile_row,
+                                         int32_t *d_tile_edge, void *stream) {
+  if (V < 0 || items <= 0 || ntiles < 0 || !d_row_ptr || !d_tile_row || !d_tile_edge)
+    return gb_fail(GB_ERR_INVALID, "gb_merge_path_split: bad arguments");
+  int64_t n = ntiles + 1;
+  int blocks = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(merge_path_split_kernel, dim3(blocks), dim3(256), 0, gb_stream(stream), V,
+                     d_row_ptr, items, ntiles, d_tile_row, d_tile_edge);
+  GB_HIP(hipGetLastError());
+  return GB_OK;
+}
+
+// comp_distrib_csr rule (partition/csr_partition.c:5-30): part j grows from
+// the vertex after the previous cut; it is cut at the first vertex i where the
+// part's degree sum exceeds q = floor(E/p).  With prefix sums this is one
+// binary search per part; the cut chain is sequential, so one thread walks it.
+__global__ void edge_balanced_parts_kernel(int32_t V, const int32_t *__restrict__ rp, int32_t p,
+                                           int64_t *__restrict__ distrib) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int64_t E = rp[V];
+  const int64_t q = E / p;
+  for (int32_t k = 0; k < p; k++) distrib[k] = 0;
+  int64_t start = 0;  // first vertex of the current part
+  for (int32_t j = 0; j < p; j++) {
+    // smallest i >= start with rp[i+1] - rp[start] > q
+    int64_t base = rp[start];
+    int64_t lo = start, hi = V;  // answer in [start, V) or V = none
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)rp[mid + 1] - base > q)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    if (lo >= V) break;
+    distrib[j] = lo;
+    start = lo + 1;
+    if (start >= V) break;
+  }
+  distrib[p - 1] = (int64_t)V - 1;
+}
+
+extern "C" gb_status gb_edge_balanced_parts(int32_t V, const int32_t *d_row_ptr, int32_t p,
+                                            int64_t *d_distrib, void *stream) {
+  if (V <= 0 || p <= 0 || !d_row_ptr || !d_distrib)
+    return gb_fail(GB_ERR_INVALID, "gb_edge_balanced_parts: bad arguments");
+  hipLaunchKernelGGL(edge_balanced_parts_kernel, dim3(1), dim3(64), 0, gb_stream(stream), V,
+                     d_row_ptr, p, d_distrib);
+  GB_HIP(hipGetLastError());
+  return GB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// max weight
+
+__global__ void max_u32_kernel(const uint32_t *__restrict__ a, int64_t n, unsigned int *out) {
+  uint32_t m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = a[i] > m ? a[i] : m;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    uint32_t o = __shfl_xor(m, d, 64);
+    m = o > m ? o : m;
+  }
+  if (gb_lane() == 0) atomicMax(out, m);
+}
+
+gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st) {
+  if (g->max_w_valid) {
+    *out = g->max_w;
+    return GB_OK;
+  }
+  if (g->E == 0) {
+    *out = 0;
+    return GB_OK;
+  }
+  if (!g->w) return gb_fail(GB_ERR_INVALID, "graph has no uint32 weights");
+  DevBuf tmp;
+  GB_TRY(tmp.ensure(sizeof(unsigned int)));
+  GB_HIP(hipMemsetAsync(tmp.p, 0, sizeof(unsigned int), st));
+  int blocks = (int)std::min<int64_t>((g->E + 255) / 256, 4096);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(max_u32_kernel, dim3(blocks), dim3(256), 0, st, g->w, g->E,
+                     tmp.as<unsigned int>());
+  GB_HIP(hipGetLastError());
+  unsigned int h = 0;
+  GB_HIP(hipMemcpyAsync(&h, tmp.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  g->max_w = h;
+  g->max_w_valid = true;
+  *out = h;
+  return GB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// light/heavy separation (separate_csr, graph/csr.c:75-130) as a stable
+// in-row partition: one wave per row, ballot ranks keep the original order.
+
+template <class W, class Pred>
+__global__ void separate_rows_kernel(int32_t V, const int32_t *__restrict__ rp,
+                                     const int32_t *__restrict__ col, const W *__restrict__ w,
+                                     Pred light, int32_t *__restrict__ col_out,
+                                     W *__restrict__ w_out, int32_t *__restrict__ le) {
+  const int lane = gb_lane();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < V; v += nwaves) {
+    const int32_t s = rp[v], e = rp[v + 1];
+    // pass 1: count light
+    int32_t nl = 0;
+    for (int32_t b = s; b < e; b += 64) {
+      int32_t i = b + lane;
+      bool f = i < e && light(w[i]);
+      nl += __popcll(__ballot(f));
+    }
+    int32_t lpos = s, hpos = s + nl;
+    for (int32_t b = s; b < e; b += 64) {
+      int32_t i = b + lane;
+      bool in = i < e;
+      bool f = in && light(w[i]);
+      uint64_t bl = __ballot(f);
+      uint64_t bh = __ballot(in && !f);
+      uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      if (in) {
+        int32_t dst = f ? lpos + __popcll(bl & below) : hpos + __popcll(bh & below);
+        col_out[dst] = col[i];
+        w_out[dst] = w[i];
+      }
+      lpos += __popcll(bl);
+      hpos += __popcll(bh);
+    }
+    if (lane == 0) le[v] = s + nl;
+  }
+}
+
+struct LightU32 {
+  uint32_t t;
+  __device__ bool operator()(uint32_t x) const { return x <= t; }
+};
+struct LightF64 {
+  double t;
+  __device__ bool operator()(double x) const { return x <= t; }
+};
+
+static int rows_grid(int64_t V) {
+  int64_t waves = V;
+  int64_t blocks = (waves + 3) / 4;
+  int64_t cap = (int64_t)gb_num_cus() * 32;
+  if (blocks > cap) blocks = cap;
+  return blocks < 1 ? 1 : (int)blocks;
+}
+
+gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st) {
+  if (g->split_valid && g->split_light_max == light_max) return GB_OK;
+  if (!g->w && g->E > 0) return gb_fail(GB_ERR_INVALID, "graph has no uint32 weights");
+  GB_TRY(g->split_col.ensure(sizeof(int32_t) * (g->E + 1)));
+  GB_TRY(g->split_w.ensure(sizeof(uint32_t) * (g->E + 1)));
+  GB_TRY(g->split_le.ensure(sizeof(int32_t) * (g->V + 1)));
+  if (g->V > 0) {
+    hipLaunchKernelGGL((separate_rows_kernel<uint32_t, LightU32>), dim3(rows_grid(g->V)),
+                       dim3(256), 0, st, g->V, g->rp, g->col, g->w, LightU32{light_max},
+                       g->split_col.as<int32_t>(), g->split_w.as<uint32_t>(),
+                       g->split_le.as<int32_t>());
+    GB_HIP(hipGetLastError());
+  }
+  g->split_valid = true;
+  g->split_light_max = light_max;
+  return GB_OK;
+}
+
+gb_status gb_graph_split64(gb_graph g, double light_max, hipStream_t st) {
+  if (g->split64_valid && g->split64_delta == light_max) return GB_OK;
+  if (!g->w64 && g->E > 0) return gb_fail(GB_ERR_INVALID, "graph has no f64 weights");
+  GB_TRY(g->split64_col.ensure(sizeof(int32_t) * (g->E + 1)));
+  GB_TRY(g->split64_w.ensure(sizeof(double) * (g->E + 1)));
+  GB_TRY(g->split64_le.ensure(sizeof(int32_t) * (g->V + 1)));
+  if (g->V > 0) {
+    hipLaunchKernelGGL((separate_rows_kernel<double, LightF64>), dim3(rows_grid(g->V)), dim3(256),
+                       0, st, g->V, g->rp, g->col, g->w64, LightF64{light_max},
+                       g->split64_col.as<int32_t>(), g->split64_w.as<double>(),
+                       g->split64_le.as<int32_t>());
+    GB_HIP(hipGetLastError());
+  }
+  g->split64_valid = true;
+  g->split64_delta = light_max;
+  return GB_OK;
+}
+
+extern "C" gb_status gb_separate(gb_graph g, uint32_t light_max, int32_t *d_col_out,
+                                 uint32_t *d_w_out, int32_t *d_light_end, void *stream) {
+  if (!g || !g->w || !d_col_out || !d_w_out || !d_light_end)
+    return gb_fail(GB_ERR_INVALID, "gb_separate: bad arguments");
+  hipStream_t st = gb_stream(stream);
+  if (g->V > 0) {
+    hipLaunchKernelGGL((separate_rows_kernel<uint32_t, LightU32>), dim3(rows_grid(g->V)),
+                       dim3(256), 0, st, g->V, g->rp, g->col, g->w, LightU32{light_max}, d_col_out,
+                       d_w_out, d_light_end);
+    GB_HIP(hipGetLastError());
+  }
+  return GB_OK;
+}

@@ -1,0 +1,545 @@
+// gb_sssp.hip -- delta-stepping SSSP for gfx950.
+//
+// Replaces run_sspath_arr (sssp/csr_sspath_array.c:149-363) and the distance
+// part of run_sspath (sssp/csr_sspath.c:38-122).  The reference's per-bucket
+// pipeline request_arr -> merge_sort -> reorder_arr -> relax_arr becomes:
+//   light phase  one wave per queue entry (a vertex, or a 1024-edge chunk of
+//                a hub's light edges); lanes relax consecutive edges with
+//                atomicMin on the distance; an improved target whose new
+//                distance is still in the current bucket is appended to the
+//                next near queue (dedup by a per-vertex generation stamp),
+//                otherwise to the far pile (dedup per far epoch).  Queue
+//                slots are reserved with one atomicAdd per wave.
+//   heavy phase  the same over the bucket's settled set S (vertices expanded
+//                in the bucket's light phases, remember_arr 1044-1098), heavy
+//                edges only (w > light_max, separate_csr graph/csr.c:75-130).
+//   far scan     when the bucket is exhausted, one pass over the far pile
+//                moves entries of the next non-empty bucket to the near queue
+//                and keeps the rest.
+// Sorting by target and first-of-run relaxation are not needed: atomicMin
+// resolves concurrent requests to the same target.
+// Many tasks -- sources of one graph, or whole graphs -- run together: every
+// queue entry carries its task id and all tasks step through bucket indices
+// in lockstep, so one launch serves all of them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gb_device.hpp"
+#include "gb_internal.hpp"
+
+namespace {
+
+constexpr int SS_BLOCK = 256;
+constexpr int CHUNK_SHIFT = 10;  // 1024 edges per queue entry
+constexpr int32_t CHUNK = 1 << CHUNK_SHIFT;
+
+__device__ __host__ __forceinline__ uint32_t nchunks(int32_t deg) {
+  return (uint32_t)((deg + CHUNK - 1) >> CHUNK_SHIFT);
+}
+__device__ __forceinline__ uint64_t mk_entry(uint32_t t, uint32_t ch, uint32_t v) {
+  return ((uint64_t)t << 48) | ((uint64_t)(ch & 0xFFFFu) << 32) | v;
+}
+
+// distance/weight traits
+struct TU32 {
+  using D = uint32_t;
+  using W = uint32_t;
+  using DT = uint64_t;
+  static __device__ __forceinline__ D inf() { return 0xFFFFFFFFu; }
+  static __device__ __forceinline__ D add(D du, W w, bool &ovf) {
+    uint64_t nd = (uint64_t)du + w;
+    if (nd >= 0xFFFFFFFFull) {
+      ovf = true;
+      nd = 0xFFFFFFFEull;
+    }
+    return (D)nd;
+  }
+  static __device__ __forceinline__ D amin(D *p, D v) { return atomicMin(p, v); }
+  static __device__ __forceinline__ uint64_t bucket(D d, DT delta) { return (uint64_t)d / delta; }
+};
+struct TF64 {
+  using D = double;
+  using W = double;
+  using DT = double;
+  static __device__ __forceinline__ D inf() { return __longlong_as_double(0x7FF0000000000000ll); }
+  static __device__ __forceinline__ D add(D du, W w, bool &) { return __dadd_rn(du, w); }
+  static __device__ __forceinline__ D amin(D *p, D v) {
+    // non-negative doubles order like their bit patterns
+    unsigned long long old =
+        atomicMin((unsigned long long *)p, (unsigned long long)__double_as_longlong(v));
+    return __longlong_as_double((long long)old);
+  }
+  static __device__ __forceinline__ uint64_t bucket(D d, DT delta) {
+    return (uint64_t)(long long)(d / delta);
+  }
+};
+
+template <class W>
+struct SGraph {
+  const int32_t *rp;
+  const int32_t *le;  // light end per vertex (light edges first in each row)
+  const int32_t *col;
+  const W *w;
+  int32_t V;
+  int32_t pad;
+  int64_t E;
+};
+
+template <class Tr>
+struct STask {
+  typename Tr::D *dist;
+  int64_t off;  // offset of this task's stamps
+  typename Tr::DT delta;
+  int32_t graph;
+  int32_t src;
+};
+
+// counters (device, u64): near A/B, S, far A/B, min bucket, relaxations, overflow
+enum { C_NEAR0 = 0, C_NEAR1 = 1, C_S = 2, C_FAR0 = 3, C_FAR1 = 4, C_MINB = 5, C_RELAX = 6,
+       C_OVF = 7, C_N = 8 };
+
+template <class Tr>
+struct Ctx {
+  const SGraph<typename Tr::W> *graphs;
+  const STask<Tr> *tasks;
+  uint32_t *nst;  // near generation stamp
+  uint32_t *sst;  // settled-set round stamp
+  uint32_t *fst;  // far epoch stamp
+  unsigned long long *cnt;
+};
+
+template <class Tr>
+__global__ __launch_bounds__(SS_BLOCK) void sssp_fill_kernel(Ctx<Tr> c, int32_t ntasks) {
+  // grid.y = task
+  const int32_t t = blockIdx.y;
+  if (t >= ntasks) return;
+  const STask<Tr> tk = c.tasks[t];
+  const int32_t V = c.graphs[tk.graph].V;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    tk.dist[v] = Tr::inf();
+    c.nst[tk.off + v] = 0;
+    c.sst[tk.off + v] = 0;
+    c.fst[tk.off + v] = 0;
+  }
+}
+
+template <class Tr>
+__global__ void sssp_seed_kernel(Ctx<Tr> c, int32_t ntasks, uint64_t *near_out,
+                                 unsigned long long *n_near, uint32_t gen) {
+  const int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= ntasks) return;
+  const STask<Tr> tk = c.tasks[t];
+  const SGraph<typename Tr::W> G = c.graphs[tk.graph];
+  const int32_t s = tk.src;
+  if (s < 0 || s >= G.V) return;
+  tk.dist[s] = (typename Tr::D)0;
+  c.nst[tk.off + s] = gen;
+  uint32_t k = nchunks(G.le[s] - G.rp[s]);
+  if (k == 0) k = 1;
+  unsigned long long b = atomicAdd(n_near, (unsigned long long)k);
+  for (uint32_t i = 0; i < k; i++) near_out[b + i] = mk_entry((uint32_t)t, i, (uint32_t)s);
+}
+
+// Light (HEAVY=false) or heavy (HEAVY=true) expansion of one queue.
+template <class Tr, bool HEAVY>
+__global__ __launch_bounds__(SS_BLOCK) void sssp_expand_kernel(
+    Ctx<Tr> c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
+    uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ s_out,
+    unsigned long long *n_s, uint64_t *__restrict__ far_out, unsigned long long *n_far,
+    uint64_t bucket, uint32_t gen_next, uint32_t sround, uint32_t fepoch) {
+  using D = typename Tr::D;
+  const int lane = gb_lane();
+  const uint64_t n = *n_in;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  uint32_t relax = 0;
+  bool ovf = false;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    const uint64_t ent = in[i];
+    const uint32_t t = (uint32_t)(ent >> 48);
+    const uint32_t ch = (uint32_t)(ent >> 32) & 0xFFFFu;
+    const uint32_t u = (uint32_t)ent;
+    const STask<Tr> tk = c.tasks[t];
+    const SGraph<typename Tr::W> G = c.graphs[tk.graph];
+    const D du = tk.dist[u];
+    const int32_t rs = G.rp[u], le = G.le[u], re = G.rp[u + 1];
+    int32_t beg, end;
+    if (!HEAVY) {
+      beg = rs + (int32_t)(ch << CHUNK_SHIFT);
+      end = min(le, beg + CHUNK);
+      // remember_arr: the first chunk's wave records u in S (heavy edges)
+      uint32_t scnt = 0;
+      if (ch == 0 && lane == 0 && re > le) {
+        if (atomicExch(&c.sst[tk.off + u], sround) != sround) scnt = nchunks(re - le);
+      }
+      uint64_t sb = gb_wave_reserve(n_s, scnt);
+      for (uint32_t k = 0; k < scnt; k++) s_out[sb + k] = mk_entry(t, k, u);
+    } else {
+      beg = le + (int32_t)(ch << CHUNK_SHIFT);
+      end = min(re, beg + CHUNK);
+    }
+    for (int32_t b0 = beg; b0 < end; b0 += GB_WAVE) {
+      const int32_t e = b0 + lane;
+      uint32_t ncnt = 0;
+      bool far = false;
+      uint32_t v = 0;
+      if (e < end) {
+        v = (uint32_t)G.col[e];
+        const D nd = Tr::add(du, G.w[e], ovf);
+        relax++;
+        D *dv = tk.dist + v;
+        if (nd < *dv) {
+          const D old = Tr::amin(dv, nd);
+          if (nd < old) {
+            if (Tr::bucket(nd, tk.delta) <= bucket) {
+              if (atomicExch(&c.nst[tk.off + v], gen_next) != gen_next) {
+                ncnt = nchunks(G.le[v] - G.rp[v]);
+                if (ncnt == 0) ncnt = 1;
+              }
+            } else {
+              if (atomicExch(&c.fst[tk.off + v], fepoch) != fepoch) far = true;
+            }
+          }
+        }
+      }
+      const uint64_t nb = gb_wave_reserve(n_near, ncnt);
+      for (uint32_t k = 0; k < ncnt; k++) near_out[nb + k] = mk_entry(t, k, v);
+      const uint64_t fb = gb_wave_reserve(n_far, far ? 1u : 0u);
+      if (far) far_out[fb] = mk_entry(t, 0, v);
+    }
+  }
+  // per-wave statistics
+  uint32_t tot;
+  (void)gb_wave_excl_sum(relax, &tot);
+  if (lane == 0 && tot) atomicAdd(&c.cnt[C_RELAX], (unsigned long long)tot);
+  if (__ballot(ovf) && lane == 0) atomicOr(&c.cnt[C_OVF], 1ull);
+}
+
+template <class Tr>
+__global__ __launch_bounds__(SS_BLOCK) void sssp_far_scan_kernel(
+    Ctx<Tr> c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
+    uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ far_out,
+    unsigned long long *n_far, uint64_t bnext, uint32_t gen_next, uint32_t fepoch_next) {
+  const int lane = gb_lane();
+  const uint64_t n = *n_in;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  uint64_t minb = ~0ull;
+  for (uint64_t b0 = wave * GB_WAVE; b0 < n; b0 += nwaves * GB_WAVE) {
+    const uint64_t i = b0 + lane;
+    uint32_t ncnt = 0;
+    bool far = false;
+    uint32_t t = 0, v = 0;
+    if (i < n) {
+      const uint64_t ent = in[i];
+      t = (uint32_t)(ent >> 48);
+      v = (uint32_t)ent;
+      const STask<Tr> tk = c.tasks[t];
+      const uint64_t bk = Tr::bucket(tk.dist[v], tk.delta);
+      if (bk == bnext) {
+        if (atomicExch(&c.nst[tk.off + v], gen_next) != gen_next) {
+          const SGraph<typename Tr::W> G = c.graphs[tk.graph];
+          ncnt = nchunks(G.le[v] - G.rp[v]);
+          if (ncnt == 0) ncnt = 1;
+        }
+      } else if (bk > bnext) {
+        if (atomicExch(&c.fst[tk.off + v], fepoch_next) != fepoch_next) far = true;
+        minb = bk < minb ? bk : minb;
+      }
+    }
+    const uint64_t nb = gb_wave_reserve(n_near, ncnt);
+    for (uint32_t k = 0; k < ncnt; k++) near_out[nb + k] = mk_entry(t, k, v);
+    const uint64_t fb = gb_wave_reserve(n_far, far ? 1u : 0u);
+    if (far) far_out[fb] = mk_entry(t, 0, v);
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    uint64_t o = gb_shfl_u64(minb, lane ^ d);
+    minb = o < minb ? o : minb;
+  }
+  if (lane == 0 && minb != ~0ull) atomicMin(&c.cnt[C_MINB], (unsigned long long)minb);
+}
+
+// traversed edges: sum of out-degrees of reached (task, vertex) pairs
+template <class Tr>
+__global__ __launch_bounds__(SS_BLOCK) void sssp_count_kernel(Ctx<Tr> c, int32_t ntasks,
+                                                               unsigned long long *out) {
+  const int32_t t = blockIdx.y;
+  if (t >= ntasks) return;
+  const STask<Tr> tk = c.tasks[t];
+  const SGraph<typename Tr::W> G = c.graphs[tk.graph];
+  unsigned long long reached = 0, edges = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < G.V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (tk.dist[v] != Tr::inf()) {
+      reached++;
+      edges += (unsigned long long)(G.rp[v + 1] - G.rp[v]);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    reached += __shfl_xor(reached, d, 64);
+    edges += __shfl_xor(edges, d, 64);
+  }
+  if (gb_lane() == 0) {
+    atomicAdd(&out[0], reached);
+    atomicAdd(&out[1], edges);
+  }
+}
+
+struct SsspWork {
+  DevBuf graphs, tasks, nst, sst, fst, q[5], cnt, counts;
+  void release() {
+    for (DevBuf *b : {&graphs, &tasks, &nst, &sst, &fst, &cnt, &counts}) b->release();
+    for (auto &b : q) b.release();
+  }
+};
+SsspWork g_ssw;
+
+int expand_grid(uint64_t n_entries) {
+  // one wave per entry, capped at a few resident waves per CU
+  uint64_t waves = n_entries;
+  uint64_t blocks = (waves + 3) / 4;
+  uint64_t cap = (uint64_t)gb_num_cus() * 16;
+  if (blocks > cap) blocks = cap;
+  return blocks < 1 ? 1 : (int)blocks;
+}
+
+template <class Tr>
+gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
+                      std::vector<STask<Tr>> &ht, int max_phases, gb_sssp_stats *stats,
+                      hipStream_t st) {
+  using G = SGraph<typename Tr::W>;
+  const int32_t ntasks = (int32_t)ht.size();
+  if (ntasks <= 0) return GB_OK;
+  if (ntasks > 65535) return gb_fail(GB_ERR_INVALID, "sssp: at most 65535 tasks per call");
+  int64_t total = 0, qcap = 0;
+  int32_t maxV = 0;
+  for (auto &t : ht) {
+    const G &g = hg[t.graph];
+    t.off = total;
+    total += g.V;
+    maxV = std::max(maxV, g.V);
+  }
+  // Queue capacity: a (task, vertex) pair enters a queue at most once per
+  // generation, with ceil(deg / CHUNK) >= 1 entries, so per task at most
+  // V + E / CHUNK + 1 entries.
+  for (auto &t : ht) qcap += (int64_t)hg[t.graph].V + hg[t.graph].E / CHUNK + 1;
+  SsspWork &w = g_ssw;
+  GB_TRY(w.graphs.ensure(sizeof(G) * hg.size()));
+  GB_TRY(w.tasks.ensure(sizeof(STask<Tr>) * ht.size()));
+  GB_TRY(w.nst.ensure(sizeof(uint32_t) * total));
+  GB_TRY(w.sst.ensure(sizeof(uint32_t) * total));
+  GB_TRY(w.fst.ensure(sizeof(uint32_t) * total));
+  for (int k = 0; k < 5; k++) GB_TRY(w.q[k].ensure(sizeof(uint64_t) * qcap));
+  GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * C_N));
+  GB_HIP(hipMemcpyAsync(w.graphs.p, hg.data(), sizeof(G) * hg.size(), hipMemcpyHostToDevice, st));
+  GB_HIP(hipMemcpyAsync(w.tasks.p, ht.data(), sizeof(STask<Tr>) * ht.size(),
+                        hipMemcpyHostToDevice, st));
+  unsigned long long *cnt = w.cnt.as<unsigned long long>();
+  GB_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * C_N, st));
+  Ctx<Tr> c{w.graphs.as<G>(), w.tasks.as<STask<Tr>>(), w.nst.as<uint32_t>(), w.sst.as<uint32_t>(),
+            w.fst.as<uint32_t>(), cnt};
+  uint64_t *near[2] = {w.q[0].as<uint64_t>(), w.q[1].as<uint64_t>()};
+  uint64_t *S = w.q[2].as<uint64_t>();
+  uint64_t *far[2] = {w.q[3].as<uint64_t>(), w.q[4].as<uint64_t>()};
+
+  hipEvent_t e0, e1;
+  GB_HIP(hipEventCreate(&e0));
+  GB_HIP(hipEventCreate(&e1));
+  GB_HIP(hipEventRecord(e0, st));
+  {
+    int gx = (int)std::min<int64_t>((maxV + SS_BLOCK - 1) / SS_BLOCK, 1024);
+    if (gx < 1) gx = 1;
+    hipLaunchKernelGGL(sssp_fill_kernel<Tr>, dim3(gx, ntasks), dim3(SS_BLOCK), 0, st, c, ntasks);
+  }
+  uint32_t gen = 1, sround = 0, fepoch = 1;
+  hipLaunchKernelGGL(sssp_seed_kernel<Tr>, dim3((ntasks + 255) / 256), dim3(256), 0, st, c, ntasks,
+                     near[0], cnt + C_NEAR0, gen);
+  GB_HIP(hipGetLastError());
+  unsigned long long h[C_N];
+  auto read = [&]() -> gb_status {
+    GB_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    return GB_OK;
+  };
+  auto zero = [&](int k) -> gb_status {
+    GB_HIP(hipMemsetAsync(cnt + k, 0, sizeof(unsigned long long), st));
+    return GB_OK;
+  };
+  GB_TRY(read());
+  int cur = 0, fcur = 0;
+  uint64_t b = 0;
+  int64_t phases = 0, buckets = 0;
+  const int64_t cap = max_phases > 0 ? max_phases : (int64_t)1 << 40;
+  gb_status status = GB_OK;
+  for (;;) {
+    buckets++;
+    bool again;
+    do {
+      again = false;
+      sround++;
+      GB_TRY(zero(C_S));
+      while (h[C_NEAR0 + cur] > 0) {
+        if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
+        gen++;
+        GB_TRY(zero(C_NEAR0 + (cur ^ 1)));
+        hipLaunchKernelGGL((sssp_expand_kernel<Tr, false>), dim3(expand_grid(h[C_NEAR0 + cur])),
+                           dim3(SS_BLOCK), 0, st, c, near[cur], cnt + C_NEAR0 + cur, near[cur ^ 1],
+                           cnt + C_NEAR0 + (cur ^ 1), S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur,
+                           b, gen, sround, fepoch);
+        GB_HIP(hipGetLastError());
+        cur ^= 1;
+        GB_TRY(read());
+      }
+      if (h[C_S] > 0) {
+        if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
+        gen++;
+        GB_TRY(zero(C_NEAR0 + cur));
+        hipLaunchKernelGGL((sssp_expand_kernel<Tr, true>), dim3(expand_grid(h[C_S])),
+                           dim3(SS_BLOCK), 0, st, c, S, cnt + C_S, near[cur], cnt + C_NEAR0 + cur,
+                           S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur, b, gen, sround, fepoch);
+        GB_HIP(hipGetLastError());
+        GB_TRY(read());
+        again = h[C_NEAR0 + cur] > 0;  // a heavy edge landed back in bucket b
+      }
+    } while (again);
+    if (h[C_FAR0 + fcur] == 0) break;
+    uint64_t bnext = b + 1;
+    for (;;) {
+      if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
+      gen++;
+      GB_TRY(zero(C_NEAR0 + cur));
+      GB_TRY(zero(C_FAR0 + (fcur ^ 1)));
+      GB_HIP(hipMemsetAsync(cnt + C_MINB, 0xFF, sizeof(unsigned long long), st));
+      hipLaunchKernelGGL(sssp_far_scan_kernel<Tr>, dim3(expand_grid(h[C_FAR0 + fcur] / 64 + 1)),
+                         dim3(SS_BLOCK), 0, st, c, far[fcur], cnt + C_FAR0 + fcur, near[cur],
+                         cnt + C_NEAR0 + cur, far[fcur ^ 1], cnt + C_FAR0 + (fcur ^ 1), bnext, gen,
+                         fepoch + 1);
+      GB_HIP(hipGetLastError());
+      fepoch++;
+      fcur ^= 1;
+      GB_TRY(read());
+      if (h[C_NEAR0 + cur] > 0 || h[C_FAR0 + fcur] == 0) break;
+      bnext = h[C_MINB];
+    }
+    if (h[C_NEAR0 + cur] == 0) break;
+    b = bnext;
+  }
+  GB_HIP(hipEventRecord(e1, st));
+  GB_TRY(read());
+  if (h[C_OVF]) status = gb_fail(GB_ERR_OVERFLOW, "sssp: uint32 distance overflow");
+  float ms = 0.0f;
+  GB_HIP(hipEventSynchronize(e1));
+  GB_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (stats) {
+    stats->buckets = buckets;
+    stats->phases = phases;
+    stats->relax_edges = (int64_t)h[C_RELAX];
+    stats->ms = ms;
+    GB_TRY(w.counts.ensure(2 * sizeof(unsigned long long)));
+    GB_HIP(hipMemsetAsync(w.counts.p, 0, 2 * sizeof(unsigned long long), st));
+    int gx = (int)std::min<int64_t>((maxV + SS_BLOCK - 1) / SS_BLOCK, 1024);
+    if (gx < 1) gx = 1;
+    hipLaunchKernelGGL(sssp_count_kernel<Tr>, dim3(gx, ntasks), dim3(SS_BLOCK), 0, st, c, ntasks,
+                       w.counts.as<unsigned long long>());
+    unsigned long long hc[2];
+    GB_HIP(hipMemcpyAsync(hc, w.counts.p, sizeof(hc), hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    stats->reached = (int64_t)hc[0];
+    stats->traversed = (int64_t)hc[1];
+  }
+  return status;
+}
+
+uint32_t plain_delta(uint32_t max_w, int32_t V, int64_t E) {
+  // max_weight / average degree, as an integer bucket width >= 1
+  if (E <= 0 || V <= 0) return 1;
+  double d = (double)max_w * (double)V / (double)E;
+  uint32_t r = (uint32_t)std::floor(d);
+  return r < 1 ? 1 : r;
+}
+
+}  // namespace
+
+void gb_sssp_release() { g_ssw.release(); }
+
+extern "C" gb_status gb_sssp(gb_graph g, const gb_sssp_params *p, const int32_t *h_sources,
+                             int32_t nsrc, uint32_t *d_dist, gb_sssp_stats *stats,
+                             void *stream) {
+  if (!g || !h_sources || nsrc < 0 || !d_dist) return gb_fail(GB_ERR_INVALID, "gb_sssp: null");
+  if (!g->w && g->E > 0) return gb_fail(GB_ERR_INVALID, "gb_sssp: graph has no uint32 weights");
+  for (int32_t i = 0; i < nsrc; i++)
+    if (h_sources[i] < 0 || h_sources[i] >= g->V)
+      return gb_fail(GB_ERR_INVALID, "gb_sssp: source %d out of range", h_sources[i]);
+  hipStream_t st = gb_stream(stream);
+  uint32_t delta = p ? p->delta : 0, lmax = p ? p->light_max : 0;
+  if (delta == 0) {
+    uint32_t mw = 0;
+    GB_TRY(gb_graph_max_w(g, &mw, st));
+    delta = plain_delta(mw, g->V, g->E);
+  }
+  if (lmax == 0) lmax = delta;
+  GB_TRY(gb_graph_split(g, lmax, st));
+  std::vector<SGraph<uint32_t>> hg(1);
+  hg[0] = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
+                           g->split_w.as<uint32_t>(), g->V, 0, g->E};
+  std::vector<STask<TU32>> ht(nsrc);
+  for (int32_t i = 0; i < nsrc; i++)
+    ht[i] = STask<TU32>{d_dist + (int64_t)i * g->V, 0, (uint64_t)delta, 0, h_sources[i]};
+  return sssp_engine<TU32>(hg, ht, p ? p->max_phases : 0, stats, st);
+}
+
+extern "C" gb_status gb_sssp_f64(gb_graph g, double delta, const int32_t *h_sources,
+                                 int32_t nsrc, double *d_dist, gb_sssp_stats *stats,
+                                 void *stream) {
+  if (!g || !h_sources || nsrc < 0 || !d_dist) return gb_fail(GB_ERR_INVALID, "gb_sssp_f64: null");
+  if (!g->w64 && g->E > 0) return gb_fail(GB_ERR_INVALID, "gb_sssp_f64: graph has no f64 weights");
+  if (!(delta > 0.0) || !std::isfinite(delta))
+    return gb_fail(GB_ERR_INVALID, "gb_sssp_f64: delta must be positive");
+  for (int32_t i = 0; i < nsrc; i++)
+    if (h_sources[i] < 0 || h_sources[i] >= g->V)
+      return gb_fail(GB_ERR_INVALID, "gb_sssp_f64: source %d out of range", h_sources[i]);
+  hipStream_t st = gb_stream(stream);
+  GB_TRY(gb_graph_split64(g, delta, st));
+  std::vector<SGraph<double>> hg(1);
+  hg[0] = SGraph<double>{g->rp, g->split64_le.as<int32_t>(), g->split64_col.as<int32_t>(),
+                         g->split64_w.as<double>(), g->V, 0, g->E};
+  std::vector<STask<TF64>> ht(nsrc);
+  for (int32_t i = 0; i < nsrc; i++)
+    ht[i] = STask<TF64>{d_dist + (int64_t)i * g->V, 0, delta, 0, h_sources[i]};
+  return sssp_engine<TF64>(hg, ht, 0, stats, st);
+}
+
+extern "C" gb_status gb_sssp_multi(int32_t ngraphs, const gb_graph *gs, const int32_t *sources,
+                                   const uint32_t *deltas, const uint32_t *light_max,
+                                   uint32_t *const *d_dist, gb_sssp_stats *stats, void *stream) {
+  if (ngraphs < 0 || (ngraphs > 0 && (!gs || !sources || !d_dist)))
+    return gb_fail(GB_ERR_INVALID, "gb_sssp_multi: null");
+  hipStream_t st = gb_stream(stream);
+  std::vector<SGraph<uint32_t>> hg(ngraphs);
+  std::vector<STask<TU32>> ht(ngraphs);
+  for (int32_t i = 0; i < ngraphs; i++) {
+    gb_graph g = gs[i];
+    if (!g || (!g->w && g->E > 0) || sources[i] < 0 || sources[i] >= g->V || !d_dist[i])
+      return gb_fail(GB_ERR_INVALID, "gb_sssp_multi: bad graph %d", i);
+    uint32_t delta = deltas ? deltas[i] : 0;
+    if (delta == 0) {
+      uint32_t mw = 0;
+      GB_TRY(gb_graph_max_w(g, &mw, st));
+      delta = plain_delta(mw, g->V, g->E);
+    }
+    uint32_t lm = light_max && light_max[i] ? light_max[i] : delta;
+    GB_TRY(gb_graph_split(g, lm, st));
+    hg[i] = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
+                             g->split_w.as<uint32_t>(), g->V, 0, g->E};
+    ht[i] = STask<TU32>{d_dist[i], 0, (uint64_t)delta, i, sources[i]};
+  }
+  return sssp_engine<TU32>(hg, ht, 0, stats, st);
+}

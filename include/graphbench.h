@@ -1,0 +1,227 @@
+/*
+ * graphbench.h -- native C-ABI of the MI355X CSR analytics library
+ * (libgraphbench.so).  Plain C types only: device pointers are passed as
+ * pointers, streams as void* (a hipStream_t, NULL = the default stream).
+ *
+ * Graph layout on the device (one graph):
+ *   row_ptr[V+1]  int32   CSR offsets, row_ptr[0] = 0, E = row_ptr[V] < 2^31
+ *   col[E]        int32   destination ids (source ids for the reverse CSR)
+ *   w[E]          uint32  edge weights (SSSP only; may be NULL otherwise)
+ * The reference's t_csr keeps end offsets vet_idx[V] (graph/csr.c:194-200);
+ * row_ptr[i+1] == vet_idx[i].  The legacy t_csr entry points live in
+ * gb_legacy.h and convert into this layout.
+ *
+ * Which reference function each call replaces:
+ *   gb_sssp            run_sspath_arr   sssp/csr_sspath_array.c:149-363
+ *                      run_sspath       sssp/csr_sspath.c:38-83 (distances)
+ *   gb_pagerank        run_pagerank_csr_mpi local gather + update,
+ *                      pagerank/pagerank_mpi.c:833-846, 995-1020 (config form)
+ *   gb_pagerank_ref    the same loop with the reference constants, FP64
+ *   gb_bfs / gb_bc     run_bc, btwcentral/csr_bc.c:16-130 (unweighted form)
+ *   gb_bc_weighted     run_bc on weighted graphs (csr_bc.c:40-63 + run_sspath)
+ *   gb_merge_path_split comp_distrib_csr partition/csr_partition.c:5-30 and
+ *                      bin_search utils/utils.c:222-239 (edge-balanced split)
+ *   gb_separate        separate_csr graph/csr.c:75-130 (light/heavy split)
+ */
+#ifndef GRAPHBENCH_H
+#define GRAPHBENCH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gb_status {
+  GB_OK = 0,
+  GB_ERR_INVALID = 1,       /* bad argument or graph */
+  GB_ERR_NOMEM = 2,         /* device allocation failed or above gb_set_mem_cap */
+  GB_ERR_HIP = 3,           /* HIP runtime error (see gb_last_error) */
+  GB_ERR_OVERFLOW = 4,      /* uint32 distance overflow */
+  GB_ERR_NOT_CONVERGED = 5, /* iteration cap reached (result still written) */
+  GB_ERR_NO_DEVICE = 6,     /* no HIP device */
+  GB_ERR_UNSUPPORTED = 7
+} gb_status;
+
+#define GB_DIST_INF 0xFFFFFFFFu
+
+const char *gb_status_string(gb_status s);
+/* Text of the last error on this thread (never NULL). */
+const char *gb_last_error(void);
+/* Select the device and create per-device state; idempotent. */
+gb_status gb_init(int32_t device);
+/* Cap on bytes the library itself allocates (0 = no cap). */
+void gb_set_mem_cap(uint64_t bytes);
+/* Release cached workspaces. */
+void gb_release_workspaces(void);
+/* Library version string. */
+const char *gb_version(void);
+
+/* ---- graph handles --------------------------------------------------------
+ * A handle borrows device arrays (no copy) or owns uploaded copies.  It also
+ * caches derived device structures (merge-path tiles, 1/outdeg, light/heavy
+ * orderings).  The reverse CSR (in-edges) is needed by PageRank and by the
+ * sigma pull of weighted Brandes on directed graphs; for a symmetric graph
+ * pass the forward arrays again. */
+typedef struct gb_graph_s *gb_graph;
+gb_status gb_graph_wrap(gb_graph *out, int32_t V, int64_t E,
+                        const int32_t *d_row_ptr, const int32_t *d_col,
+                        const uint32_t *d_w, const int32_t *d_rrow_ptr,
+                        const int32_t *d_rcol, const uint32_t *d_rw);
+gb_status gb_graph_upload(gb_graph *out, int32_t V, int64_t E,
+                          const int32_t *row_ptr, const int32_t *col,
+                          const uint32_t *w, const int32_t *rrow_ptr,
+                          const int32_t *rcol, const uint32_t *rw);
+/* f64 weights (reference t_csr edge_weight); stored alongside, used by
+ * gb_sssp_f64 / gb_bc_weighted. d_w64/d_rw64 are device pointers. */
+gb_status gb_graph_set_f64_weights(gb_graph g, const double *d_w64,
+                                   const double *d_rw64);
+gb_status gb_graph_destroy(gb_graph g);
+
+/* ---- merge-path split -----------------------------------------------------
+ * Splits the merge of row ends and edges into ntiles tiles of `items` items
+ * (rows + edges).  tile_row/tile_edge have ntiles+1 entries (device). */
+gb_status gb_merge_path_split(int32_t V, const int32_t *d_row_ptr, int64_t items,
+                              int64_t ntiles, int32_t *d_tile_row,
+                              int32_t *d_tile_edge, void *stream);
+/* comp_distrib_csr on the device: p parts, distrib[k] = last vertex of part k
+ * (same cut rule as the reference).  d_distrib has p entries (int64). */
+gb_status gb_edge_balanced_parts(int32_t V, const int32_t *d_row_ptr,
+                                 int32_t p, int64_t *d_distrib, void *stream);
+
+/* ---- light/heavy split ----------------------------------------------------
+ * Stable per-row partition: light edges (w <= light_max) first.  Writes
+ * d_col_out/d_w_out (E each) and d_light_end[V] (absolute offset where the
+ * heavy edges of the row begin). */
+gb_status gb_separate(gb_graph g, uint32_t light_max, int32_t *d_col_out,
+                      uint32_t *d_w_out, int32_t *d_light_end, void *stream);
+
+/* ---- SSSP (delta-stepping) --------------------------------------------------*/
+typedef struct gb_sssp_params {
+  uint32_t delta;     /* bucket width (0 = plain rule max_w / avg_degree) */
+  uint32_t light_max; /* light/heavy threshold (0 = delta) */
+  int32_t max_phases; /* safety cap on phases (0 = none) */
+  int32_t reserved;
+} gb_sssp_params;
+
+typedef struct gb_sssp_stats {
+  int64_t buckets;      /* lockstep buckets processed */
+  int64_t phases;       /* light + heavy + far-scan launches */
+  int64_t relax_edges;  /* edge relaxations attempted */
+  int64_t reached;      /* (source, vertex) pairs with finite distance */
+  int64_t traversed;    /* out-edges of reached vertices, summed over sources */
+  double ms;            /* device time of the solve */
+} gb_sssp_stats;
+
+/* Distances from nsrc sources, all processed together in lockstep buckets.
+ * d_dist: nsrc*V uint32 (device), row-major by source; GB_DIST_INF when
+ * unreached.  Returns GB_ERR_OVERFLOW if some distance exceeds 2^32-2. */
+gb_status gb_sssp(gb_graph g, const gb_sssp_params *p, const int32_t *h_sources,
+                  int32_t nsrc, uint32_t *d_dist, gb_sssp_stats *st,
+                  void *stream);
+/* Same on the f64 weights (reference semantics: d_dist f64, +inf unreached).
+ * delta is a double here. */
+gb_status gb_sssp_f64(gb_graph g, double delta, const int32_t *h_sources,
+                      int32_t nsrc, double *d_dist, gb_sssp_stats *st,
+                      void *stream);
+
+/* Batched SSSP over many independent graphs: graph i is gs[i] with source
+ * sources[i] and bucket width deltas[i] (0 = plain rule), light threshold
+ * light_max[i] (0 = delta); distances to d_dist[i] (device, V_i entries each).
+ * All graphs advance in one lockstep bucket loop. */
+gb_status gb_sssp_multi(int32_t ngraphs, const gb_graph *gs,
+                        const int32_t *sources, const uint32_t *deltas,
+                        const uint32_t *light_max, uint32_t *const *d_dist,
+                        gb_sssp_stats *st, void *stream);
+
+/* ---- PageRank ----------------------------------------------------------------*/
+typedef struct gb_pr_params {
+  double damping;    /* 0.85 */
+  double tol;        /* stop when L1 residual < tol */
+  int32_t max_iter;  /* 100 */
+  int32_t items_per_thread; /* merge-path items per thread (0 = autotuned) */
+  int32_t use_graph; /* capture iteration chunks in a hipGraph (1) or not (0) */
+  int32_t time_kernels; /* record per-launch gather times with HIP events */
+  const float *d_x0; /* optional warm start (device, V; NULL = 1/N) */
+} gb_pr_params;
+
+typedef struct gb_pr_result {
+  int32_t iterations;
+  int32_t converged;
+  double residual;       /* last L1 residual (FP64) */
+  double dangling;       /* dangling mass of the result */
+  double gather_ms;      /* summed gather-kernel time (time_kernels=1) */
+  int32_t gather_launches;
+  int32_t tiles;
+} gb_pr_result;
+
+/* Configured PageRank: in-edge pull gather over the reverse CSR, FP32 ranks
+ * with per-row compensated sums, FP64 residual and dangling reductions.
+ * d_x: V floats (device) receives the ranks. */
+gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
+                      gb_pr_result *res, void *stream);
+/* Reference-semantics PageRank (RDM_JMP 0.3, THRESH 1e-6, x0 = 1, delta
+ * form, FP64, per-row sums in in-CSR order: bit-identical to the reference's
+ * loop).  d_recip: V doubles (device, the reference's recip values).
+ * iterations receives the count, or -1 when max_iter is hit. */
+gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t max_iter,
+                          double *d_x, int32_t *iterations, void *stream);
+
+/* ---- BFS / Brandes ---------------------------------------------------------*/
+/* BFS levels (-1 unreached) from each source; d_level nsrc*V int32. */
+gb_status gb_bfs(gb_graph g, const int32_t *h_sources, int32_t nsrc,
+                 int32_t *d_level, void *stream);
+
+typedef struct gb_bc_stats {
+  int32_t max_depth;
+  int32_t batches;
+  double ms;
+} gb_bc_stats;
+
+/* Unweighted Brandes betweenness (sigma, delta FP64; sums not halved, as in
+ * csr_bc.c:57-61).  d_bc: V doubles (device). batch = sources per launch
+ * group (0 = all at once). */
+gb_status gb_bc(gb_graph g, const int32_t *h_sources, int32_t nsrc,
+                int32_t batch, double *d_bc, gb_bc_stats *st, void *stream);
+/* Weighted Brandes over the f64 weights (gb_graph_set_f64_weights). */
+gb_status gb_bc_weighted(gb_graph g, const int32_t *h_sources, int32_t nsrc,
+                         int32_t batch, double *d_bc, gb_bc_stats *st,
+                         void *stream);
+
+/* ---- synthetic graphs (host) -------------------------------------------------
+ * Deterministic, counter-based generators used by tests and benchmarks.
+ * Rows are sorted by (destination, weight).  Output arrays are host memory
+ * allocated by the library (free with gb_host_free). */
+typedef struct gb_host_csr {
+  int32_t V;
+  int64_t E;
+  int32_t *row_ptr; /* V+1 */
+  int32_t *col;     /* E */
+  uint32_t *w;      /* E (NULL if unweighted) */
+} gb_host_csr;
+
+/* R-MAT (a,b,c; d = 1-a-b-c), 2^scale vertices, edge_factor*2^scale draws,
+ * self-loops dropped, duplicates kept.  weight_mode: 0 none, 1 uniform
+ * [wlo,whi], 2 50/50 mix of uniform [wlo,whi] and log-uniform [1,65535].
+ * symmetric != 0 adds the reverse of every edge. nverts (0 = 2^scale) allows
+ * non-power-of-two V (endpoints >= nverts are rejected and redrawn). */
+gb_status gb_gen_rmat(gb_host_csr *out, int32_t scale, int64_t nverts,
+                      double edge_factor, double a, double b, double c,
+                      uint64_t seed, int32_t weight_mode, uint32_t wlo,
+                      uint32_t whi, int32_t symmetric);
+/* Erdos-Renyi G(n, m = n*avg_degree), u != v. */
+gb_status gb_gen_er(gb_host_csr *out, int64_t n, double avg_degree,
+                    uint64_t seed, int32_t weight_mode, uint32_t wlo,
+                    uint32_t whi);
+/* Reverse (transpose) CSR, rows sorted by (source, weight). */
+gb_status gb_transpose(const gb_host_csr *in, gb_host_csr *out);
+/* k distinct sources uniform among vertices with out-degree > 0. */
+gb_status gb_pick_sources(const gb_host_csr *g, int32_t k, uint64_t seed,
+                          int32_t *out);
+void gb_host_csr_free(gb_host_csr *g);
+/* SplitMix64 finaliser used by the generators (exposed for tests). */
+uint64_t gb_splitmix64(uint64_t x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,134 @@
+/*
+ * oracle.h -- CPU oracle for the CSR analytics path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is the checker, never the product: only tests/, the smoke()
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load it.
+ * The shipped library (graphbench_amd/libgraphbench.so) never links or calls it.
+ *
+ * Every function restates one part of the reference algorithm and names the
+ * reference file:line it follows.  Graph arguments use the compact CSR that
+ * the device path also uses:
+ *   row_ptr[V+1] (int32, row_ptr[0] == 0), col[E] (int32), w[E] (uint32 or f64).
+ * The reference t_csr stores *end* offsets vet_idx[V] with an implicit leading
+ * 0 (graph/csr.c:194-200); row_ptr[i+1] == vet_idx[i].
+ */
+#ifndef GB_ORACLE_H
+#define GB_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_UINF 0xFFFFFFFFu
+
+/* ---- single-source shortest paths --------------------------------------- */
+/* Binary-heap Dijkstra (restates run_djkstra, sssp/csr_sspath.c:5-36: strict
+ * '<' relaxation from the popped minimum).  Unreached vertices get ORC_UINF /
+ * +inf.  Distances saturate at ORC_UINF-1 on uint32 overflow (never happens
+ * for the configured weight ranges; reported through the return value). */
+int orc_dijkstra_u32(int32_t V, const int32_t *rp, const int32_t *col,
+                     const uint32_t *w, int32_t src, uint32_t *dist);
+int orc_dijkstra_f64(int32_t V, const int32_t *rp, const int32_t *col,
+                     const double *w, int32_t src, double *dist);
+
+/* Delta-stepping (restates run_sspath_arr, sssp/csr_sspath_array.c:149-363,
+ * with request_arr 872-939, remember_arr 1044-1098 and relax_arr 625-826; the
+ * light/heavy split is separate_csr, graph/csr.c:75-130: light iff w <= light_max).
+ * Fixes applied (the reference defects A2-A8): every request is relaxed
+ * with min(), bucket array is unbounded, iteration counter reset per call.
+ * stats (nullable, 4 entries): buckets processed, light phases, relaxations
+ * attempted, relaxations that improved a distance. */
+int orc_delta_stepping_u32(int32_t V, const int32_t *rp, const int32_t *col,
+                           const uint32_t *w, int32_t src, uint32_t delta,
+                           uint32_t light_max, uint32_t *dist, int64_t *stats);
+/* List delta-stepping distances on f64 weights (run_sspath,
+ * sssp/csr_sspath.c:38-122; bucket = (lint)(d/delta)). */
+int orc_delta_stepping_f64(int32_t V, const int32_t *rp, const int32_t *col,
+                           const double *w, int32_t src, double delta,
+                           double *dist);
+
+/* ---- breadth-first levels ------------------------------------------------ */
+void orc_bfs_levels(int32_t V, const int32_t *rp, const int32_t *col,
+                    int32_t src, int32_t *level);
+
+/* ---- Brandes betweenness (btwcentral/csr_bc.c:16-130) --------------------- */
+/* Unweighted Brandes on BFS levels, sigma and delta in FP64.
+ *   sigma[v] = sum over in-edges (u,v), level[u]==level[v]-1, in in-CSR order
+ *   delta[v] = sum over out-edges (v,w), level[w]==level[v]+1, in CSR order, of
+ *              ((1+delta[w])*sigma[v])/sigma[w]     (csr_bc.c:116 operand order)
+ *   bc[v]   += delta[v] for v != s, sources in the given order (csr_bc.c:57-61,
+ *              not halved for undirected graphs).
+ * Parallel edges each count as a separate shortest-path edge. For an undirected
+ * (symmetric) graph pass the same CSR as forward and reverse. */
+void orc_brandes_bfs(int32_t V, const int32_t *rp, const int32_t *col,
+                     const int32_t *rrp, const int32_t *rcol,
+                     const int32_t *sources, int32_t nsrc, double *bc);
+/* Weighted Brandes with f64 weights (the run_bc + run_sspath pairing,
+ * csr_bc.c:40-63, csr_sspath.c:101-118): predecessors are the tight edges
+ * fl(d[u]+w)==d[v] of the exact distances, each counted once (defect A9 fixed).
+ * Vertices are processed by tight-DAG depth; w must be > 0. */
+void orc_brandes_weighted_f64(int32_t V, const int32_t *rp, const int32_t *col,
+                              const double *w, const int32_t *rrp,
+                              const int32_t *rcol, const double *rw,
+                              const int32_t *sources, int32_t nsrc, double *bc);
+
+/* ---- PageRank --------------------------------------------------------------*/
+/* Reference semantics (pagerank/pagerank_mpi.c:667-1044 with one rank):
+ *   x=1.0 (63-68,740); diff=x*recip (752-754); wb=RDM_JMP=0.3 (808-809);
+ *   loop: wb[v] += (1-0.3)*diff[u] over in-edges in order (833-846);
+ *         diff=(wb-x)*recip; stop when no |diff|>1e-6; x=wb (995-1002).
+ * recip[v] is the value the reference reads (1/outdeg, possibly %f-rounded).
+ * Returns the iteration count, or -1 when max_iter is reached first. */
+int orc_pagerank_ref(int32_t V, const int32_t *irp, const int32_t *icol,
+                     const double *recip, int32_t max_iter, double *x);
+/* Power iteration of the configured form: damping d, dangling mass spread
+ * uniformly, x0 = 1/N, stop when the L1 residual < tol or after max_iter.
+ * All FP64, per-row sums in in-CSR order.  Returns iterations; resid (nullable)
+ * receives the last L1 residual. */
+int orc_pagerank_cfg(int32_t V, const int32_t *irp, const int32_t *icol,
+                     const int32_t *outdeg, double d, double tol,
+                     int32_t max_iter, double *x, double *resid);
+
+/* ---- work split ------------------------------------------------------------*/
+/* comp_distrib_csr (partition/csr_partition.c:5-30), SEND form: walk vertices,
+ * add deg[i] to the current part, cut at i once the part's sum exceeds
+ * floor(E/p); distrib[p-1] = V-1.  distrib[k] is the last vertex of part k. */
+void orc_comp_distrib(int64_t V, const int64_t *deg, int64_t E, int32_t p,
+                      int64_t *distrib);
+/* bin_search (utils/utils.c:222-239): part owning vertex val. */
+int64_t orc_bin_search(const int64_t *distrib, int64_t val, int64_t start,
+                       int64_t end);
+/* Merge-path split of the (row ends, edges) merge into equal diagonals:
+ * tile t starts at diagonal t*items; tile_row[t], tile_edge[t] are the rows and
+ * edges consumed before it (t = 0..ntiles, the last entry is (V, E)). */
+void orc_merge_path_split(int32_t V, const int32_t *rp, int64_t items,
+                          int64_t ntiles, int32_t *tile_row,
+                          int32_t *tile_edge);
+/* separate_csr (graph/csr.c:75-130): stable partition of every row into light
+ * (w <= light_max) and heavy edges.  Outputs two CSRs of the same V. */
+void orc_separate_csr(int32_t V, const int32_t *rp, const int32_t *col,
+                      const uint32_t *w, uint32_t light_max, int32_t *lrp,
+                      int32_t *lcol, uint32_t *lw, int32_t *hrp, int32_t *hcol,
+                      uint32_t *hw);
+
+/* ---- learned-part reference ------------------------------------------------*/
+/* FP64 forward + backward of the 2-hidden-layer ReLU MLP used for the learned
+ * delta / warm start: y = W3 relu(W2 relu(W1 x + b1) + b2) + b3, loss =
+ * mean over rows of 0.5*||y - t||^2.  Weights are row-major [out][in], FP32
+ * inputs, FP64 accumulation.  Gradients written to g* (same shapes). */
+double orc_mlp_loss_grad(int32_t n, int32_t din, int32_t h, int32_t dout,
+                         const float *X, const float *T, const float *W1,
+                         const float *b1, const float *W2, const float *b2,
+                         const float *W3, const float *b3, double *gW1,
+                         double *gb1, double *gW2, double *gb2, double *gW3,
+                         double *gb3);
+
+/* Threads used by the OpenMP loops above (0 = library default). */
+void orc_set_threads(int32_t n);
+int32_t orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

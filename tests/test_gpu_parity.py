@@ -1,0 +1,330 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Bars: SSSP distances and BFS levels bit-exact; configured PageRank within
+1e-6 L1 of the FP64 oracle; reference-semantics PageRank bit-exact with the
+FP64 restatement of pagerank_mpi.c; Brandes BC within 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+BC_RTOL = 1e-9
+PR_L1 = 1e-6
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def small_graph(seed, V=400, E=3000, wlo=1, whi=255, self_loops=False):
+    import graphbench_amd as gb
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, V, E)
+    dst = rng.integers(0, V, E)
+    if not self_loops:
+        m = src != dst
+        src, dst = src[m], dst[m]
+    w = rng.integers(wlo, whi + 1, len(src)).astype(np.uint32)
+    return gb.HostCSR.from_edges(V, src, dst, w)
+
+
+# --------------------------------------------------------------------- SSSP
+
+@pytest.mark.parametrize("seed,delta,light", [(1, 0, 0), (2, 16, 16), (3, 1, 1), (4, 300, 300),
+                                              (5, 16, 4), (6, 8, 64)])
+def test_sssp_small_bit_exact(gpu, seed, delta, light):
+    gb = gpu
+    h = small_graph(seed)
+    _say(f"sssp small seed={seed} delta={delta} light={light}")
+    g = gb.Graph.from_host(h, need_reverse=False)
+    srcs = [0, 7, 123, 399]
+    d = gb.u32(gb.sssp(g, srcs, delta=delta, light_max=light))
+    for i, s in enumerate(srcs):
+        ref = oracle.dijkstra(h.row_ptr, h.col, h.w, s)
+        assert np.array_equal(d[i], ref), f"source {s}"
+
+
+def test_sssp_config1_rmat16(gpu):
+    """R-MAT scale 16, edge factor 16, w in [1,255], seed 42, 8 sources, plain delta."""
+    gb = gpu
+    h = gb.gen_rmat(16, 16, seed=42, weights="uniform", wlo=1, whi=255)
+    srcs = gb.pick_sources(h, 8, 42)
+    _say(f"sssp config1 V={h.V} E={h.E}")
+    g = gb.Graph.from_host(h, need_reverse=False)
+    d, st = gb.sssp(g, srcs, stats=True)
+    d = gb.u32(d)
+    for i, s in enumerate(srcs):
+        ref = oracle.dijkstra(h.row_ptr, h.col, h.w, int(s))
+        assert np.array_equal(d[i], ref), f"source {s}"
+    deg = h.outdeg().astype(np.int64)
+    reached = d != oracle.UINF
+    assert st["reached"] == int(reached.sum())
+    assert st["traversed"] == int(sum(deg[r].sum() for r in reached))
+
+
+def test_sssp_edge_cases(gpu):
+    gb = gpu
+    _say("sssp edge cases")
+    # isolated source, unreachable vertices, zero-weight edges, parallel edges
+    V = 10
+    src = [0, 0, 1, 1, 2, 5, 5, 6]
+    dst = [1, 1, 2, 3, 3, 6, 7, 5]
+    w = [5, 3, 0, 7, 0, 1, 2, 0]
+    h = gb.HostCSR.from_edges(V, src, dst, w)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    for s in range(V):
+        d = gb.u32(gb.sssp(g, [s], delta=2))[0]
+        assert np.array_equal(d, oracle.dijkstra(h.row_ptr, h.col, h.w, s)), s
+    # single vertex, no edges
+    h1 = gb.HostCSR(np.zeros(2, np.int32), np.zeros(0, np.int32), np.zeros(0, np.uint32))
+    g1 = gb.Graph.from_host(h1, need_reverse=False)
+    d1 = gb.u32(gb.sssp(g1, [0], delta=1))
+    assert d1.tolist() == [[0]]
+
+
+def test_sssp_hub_chunks(gpu):
+    """A star with 5000 out-edges (more than one 1024-edge chunk) plus a tail."""
+    gb = gpu
+    _say("sssp hub chunks")
+    rng = np.random.default_rng(9)
+    V = 6000
+    src = np.concatenate([np.zeros(5000, np.int64), rng.integers(1, V, 20000)])
+    dst = np.concatenate([np.arange(1, 5001), rng.integers(0, V, 20000)])
+    m = src != dst
+    w = rng.integers(1, 100, m.sum()).astype(np.uint32)
+    h = gb.HostCSR.from_edges(V, src[m], dst[m], w)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    for delta, lm in [(10, 10), (50, 5), (1000, 1000)]:
+        d = gb.u32(gb.sssp(g, [0, 17], delta=delta, light_max=lm))
+        assert np.array_equal(d[0], oracle.dijkstra(h.row_ptr, h.col, h.w, 0))
+        assert np.array_equal(d[1], oracle.dijkstra(h.row_ptr, h.col, h.w, 17))
+
+
+def test_sssp_f64_matches_dijkstra(gpu):
+    import torch
+    gb = gpu
+    _say("sssp f64")
+    h = small_graph(11, V=300, E=2000)
+    rng = np.random.default_rng(3)
+    w64 = rng.random(h.E) * 0.9 + 0.1
+    g = gb.Graph.from_host(h, need_reverse=False, w64=w64)
+    for s in [0, 5, 77]:
+        d = gb.sssp_f64(g, [s], delta=0.25).cpu().numpy()[0]
+        ref = oracle.dijkstra(h.row_ptr, h.col, w64, s)
+        assert np.array_equal(d, ref), s
+
+
+def test_sssp_multi_graphs(gpu):
+    gb = gpu
+    _say("sssp multi graphs")
+    hs = [gb.gen_rmat(10, 8, seed=100 + i, weights="uniform", wlo=1, whi=1023) for i in range(3)]
+    hs += [gb.gen_er(2000 + 500 * i, 8 + i, seed=200 + i, weights="uniform", wlo=1, whi=1023)
+           for i in range(3)]
+    gs = [gb.Graph.from_host(h, need_reverse=False) for h in hs]
+    srcs = [int(gb.pick_sources(h, 1, 5 + i)[0]) for i, h in enumerate(hs)]
+    outs = gb.sssp_multi(gs, srcs, deltas=[0, 64, 200, 0, 16, 1000])
+    for h, s, o in zip(hs, srcs, outs):
+        assert np.array_equal(gb.u32(o), oracle.dijkstra(h.row_ptr, h.col, h.w, s))
+
+
+def test_sssp_overflow_reported(gpu):
+    gb = gpu
+    _say("sssp overflow")
+    V = 4
+    h = gb.HostCSR.from_edges(V, [0, 1, 2], [1, 2, 3], [0xFFFFFFF0, 0xFFFFFFF0, 5])
+    g = gb.Graph.from_host(h, need_reverse=False)
+    with pytest.raises(gb.GBError) as ei:
+        gb.sssp(g, [0], delta=1 << 30)
+    assert ei.value.status == 4
+
+
+# --------------------------------------------------------------------- BFS / BC
+
+def test_bfs_levels_bit_exact(gpu):
+    gb = gpu
+    h = gb.gen_rmat(14, 8, seed=3, weights="none", symmetric=True)
+    srcs = gb.pick_sources(h, 5, 1)
+    _say(f"bfs V={h.V} E={h.E}")
+    g = gb.Graph.from_host(h, need_reverse=False)
+    lv = gb.bfs(g, srcs).cpu().numpy()
+    for i, s in enumerate(srcs):
+        assert np.array_equal(lv[i], oracle.bfs_levels(h.row_ptr, h.col, int(s)))
+
+
+@pytest.mark.parametrize("batch", [0, 3])
+def test_bc_rmat_undirected(gpu, batch):
+    gb = gpu
+    h = gb.gen_rmat(12, 8, seed=11, weights="none", symmetric=True)
+    srcs = gb.pick_sources(h, 16, 11)
+    _say(f"bc V={h.V} E={h.E} batch={batch}")
+    g = gb.Graph.from_host(h, need_reverse=False)
+    bc = gb.bc(g, srcs, batch=batch).cpu().numpy()
+    ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
+    assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+
+
+def test_bc_directed(gpu):
+    gb = gpu
+    h = small_graph(21, V=300, E=1500)
+    r = h.transpose()
+    srcs = list(range(0, 300, 7))
+    _say("bc directed")
+    g = gb.Graph.from_host(h, reverse=r)
+    bc = gb.bc(g, srcs).cpu().numpy()
+    ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs, r.row_ptr, r.col)
+    assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+
+
+def test_bc_weighted(gpu):
+    gb = gpu
+    h = small_graph(31, V=250, E=1500, wlo=1, whi=20)
+    r = h.transpose()
+    _say("bc weighted")
+    w64 = h.w.astype(np.float64) / 8.0
+    rw64 = r.w.astype(np.float64) / 8.0
+    g = gb.Graph.from_host(h, reverse=r, w64=w64, rw64=rw64)
+    srcs = list(range(0, 250, 5))
+    bc = gb.bc(g, srcs, weighted=True).cpu().numpy()
+    ref = oracle.brandes_weighted(h.row_ptr, h.col, w64, r.row_ptr, r.col, rw64, srcs)
+    assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+
+
+# --------------------------------------------------------------------- PageRank
+
+def _pr_case(gb, h, **kw):
+    r = h.transpose()
+    g = gb.Graph.from_host(h, reverse=r)
+    x, res = gb.pagerank(g, **kw)
+    x = x.cpu().numpy().astype(np.float64)
+    ref, it, rr = oracle.pagerank_cfg(r.row_ptr, r.col, h.outdeg(), kw.get("damping", 0.85),
+                                      kw.get("tol", 1e-7), kw.get("max_iter", 100))
+    return x, res, ref, it
+
+
+@pytest.mark.parametrize("ipt", [4, 8, 16])
+def test_pagerank_rmat(gpu, ipt):
+    gb = gpu
+    h = gb.gen_rmat(15, 16, seed=7, weights="none")
+    _say(f"pagerank rmat15 ipt={ipt}")
+    x, res, ref, it = _pr_case(gb, h, items_per_thread=ipt)
+    l1 = np.abs(x - ref).sum()
+    assert l1 <= PR_L1, l1
+    assert abs(res["iterations"] - it) <= 1
+    assert abs(x.sum() - 1.0) < 1e-5
+
+
+def test_pagerank_hubs_span_tiles(gpu):
+    """In-degree hubs far larger than a tile exercise the cross-tile fix-up."""
+    gb = gpu
+    rng = np.random.default_rng(5)
+    V = 3000
+    src = np.concatenate([rng.integers(0, V, 40000), rng.integers(0, V, 5000)])
+    dst = np.concatenate([np.full(40000, 17), rng.integers(0, V, 5000)])
+    m = src != dst
+    h = gb.HostCSR.from_edges(V, src[m], dst[m])
+    _say("pagerank hubs")
+    for ipt in (4, 8):
+        x, res, ref, it = _pr_case(gb, h, items_per_thread=ipt)
+        assert np.abs(x - ref).sum() <= PR_L1
+
+
+def test_pagerank_graph_vs_eager(gpu):
+    gb = gpu
+    h = gb.gen_rmat(12, 8, seed=9, weights="none")
+    r = h.transpose()
+    g = gb.Graph.from_host(h, reverse=r)
+    _say("pagerank graph vs eager")
+    xs = []
+    for ug in (True, False, True):
+        x, res = gb.pagerank(g, use_graph=ug)
+        xs.append(x.cpu().numpy())
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[1], xs[2])
+
+
+def test_pagerank_edge_cases(gpu):
+    gb = gpu
+    _say("pagerank edge cases")
+    # all vertices dangling (no edges): stays uniform
+    h = gb.HostCSR(np.zeros(6, np.int32), np.zeros(0, np.int32))
+    x, res, ref, it = _pr_case(gb, h)
+    assert np.allclose(x, 0.2, atol=1e-7)
+    # directed cycle: uniform fixed point
+    V = 50
+    h = gb.HostCSR.from_edges(V, np.arange(V), (np.arange(V) + 1) % V)
+    x, res, ref, it = _pr_case(gb, h)
+    assert np.abs(x - 1.0 / V).sum() < 1e-6 and res["iterations"] == 1
+
+
+def test_pagerank_warm_start(gpu):
+    import torch
+    gb = gpu
+    h = gb.gen_rmat(13, 16, seed=4, weights="none")
+    r = h.transpose()
+    g = gb.Graph.from_host(h, reverse=r)
+    _say("pagerank warm start")
+    x, res = gb.pagerank(g, tol=1e-7)
+    x2, res2 = gb.pagerank(g, tol=1e-7, x0=x)
+    assert res2["iterations"] <= 2
+    ref, it, _ = oracle.pagerank_cfg(r.row_ptr, r.col, h.outdeg(), 0.85, 1e-7, 100)
+    assert np.abs(x2.cpu().numpy() - ref).sum() <= PR_L1
+
+
+def test_pagerank_ref_bit_exact(gpu):
+    import torch
+    gb = gpu
+    h = gb.gen_rmat(11, 8, seed=2, weights="none")
+    r = h.transpose()
+    g = gb.Graph.from_host(h, reverse=r)
+    od = h.outdeg().astype(np.float64)
+    with np.errstate(divide="ignore"):
+        recip = 1.0 / od
+    _say("pagerank ref")
+    x, it = gb.pagerank_ref(g, torch.from_numpy(recip).cuda())
+    xr, itr = oracle.pagerank_ref(r.row_ptr, r.col, recip)
+    assert it == itr
+    assert np.array_equal(x.cpu().numpy(), xr)
+
+
+# --------------------------------------------------------------------- split
+
+def test_merge_path_split(gpu):
+    import torch
+    gb = gpu
+    h = gb.gen_rmat(12, 16, seed=1, weights="none")
+    rp = torch.from_numpy(h.row_ptr).cuda()
+    _say("merge path split")
+    for items in (7, 256, 2048):
+        nt = (h.V + h.E + items - 1) // items
+        tr, te = gb.merge_path_split(rp, items, nt)
+        rr, re = oracle.merge_path_split(h.row_ptr, items, nt)
+        assert np.array_equal(tr.cpu().numpy(), rr) and np.array_equal(te.cpu().numpy(), re)
+
+
+def test_edge_balanced_parts(gpu):
+    import torch
+    gb = gpu
+    h = gb.gen_rmat(12, 16, seed=1, weights="none")
+    rp = torch.from_numpy(h.row_ptr).cuda()
+    _say("edge balanced parts")
+    for p in (1, 2, 3, 8, 64):
+        d = gb.edge_balanced_parts(rp, p).cpu().numpy()
+        ref = oracle.comp_distrib(h.outdeg(), h.E, p)
+        assert np.array_equal(d, ref), p
+
+
+def test_separate(gpu):
+    gb = gpu
+    h = small_graph(41)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    _say("separate")
+    col, w, le = gb.separate(g, 100)
+    (lrp, lcol, lw), (hrp, hcol, hw) = oracle.separate_csr(h.row_ptr, h.col, h.w, 100)
+    col, w, le = col.cpu().numpy(), w.cpu().numpy().view(np.uint32), le.cpu().numpy()
+    for v in range(h.V):
+        a, b, c = h.row_ptr[v], le[v], h.row_ptr[v + 1]
+        assert np.array_equal(col[a:b], lcol[lrp[v]:lrp[v + 1]])
+        assert np.array_equal(w[a:b], lw[lrp[v]:lrp[v + 1]])
+        assert np.array_equal(col[b:c], hcol[hrp[v]:hrp[v + 1]])

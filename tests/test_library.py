@@ -1,0 +1,159 @@
+"""CPU checks of the shipped library: it loads, exports every symbol the
+headers declare, and its host-side generators are deterministic.
+No GPU compute is called here."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DECL = re.compile(r"^\s*(?:[A-Za-z_][\w\s\*]*?[\s\*])((?:gb|run|scan|read|separate)_?\w*)\s*\(",
+                  re.M)
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for line in txt.split(";"):
+            line = line.strip()
+            if not line or line.startswith("typedef") or "{" in line:
+                # declarations only (skip struct bodies)
+                pass
+            m = DECL.search(line + "\n")
+            if m and "typedef" not in line.split("(")[0]:
+                names.add(m.group(1))
+    return names
+
+
+def test_library_loads_and_exports_all_declared_symbols():
+    import graphbench_amd as gb
+    L = gb.lib()
+    names = _declared()
+    assert len(names) >= 25
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_status_strings():
+    import graphbench_amd as gb
+    L = gb.lib()
+    assert L.gb_status_string(0) == b"ok"
+    assert b"overflow" in L.gb_status_string(4)
+    assert L.gb_version().startswith(b"graphbench")
+
+
+def test_library_has_gfx950_code_object():
+    import graphbench_amd as gb
+    data = open(gb.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_splitmix64_known_values():
+    import graphbench_amd as gb
+    # SplitMix64 finaliser of 0 and 1 (Steele, Lea, Flood 2014 reference outputs
+    # of the generator seeded with 0: first two outputs)
+    assert gb.splitmix64(0) == 0xE220A8397B1DCDAF
+    assert gb.splitmix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+
+
+def test_rmat_deterministic_and_thread_independent(monkeypatch):
+    import graphbench_amd as gb
+    a = gb.gen_rmat(12, 16, seed=42, weights="uniform")
+    monkeypatch.setenv("GB_GEN_THREADS", "1")
+    b = gb.gen_rmat(12, 16, seed=42, weights="uniform")
+    assert np.array_equal(a.row_ptr, b.row_ptr)
+    assert np.array_equal(a.col, b.col) and np.array_equal(a.w, b.w)
+    c = gb.gen_rmat(12, 16, seed=43, weights="uniform")
+    assert not np.array_equal(a.col, c.col)
+
+
+def test_rmat_properties():
+    import graphbench_amd as gb
+    h = gb.gen_rmat(16, 16, seed=42, weights="uniform", wlo=1, whi=255)
+    assert h.V == 65536
+    draws = 16 * 65536
+    # self-loops dropped: about sum over levels of P(same bit) ... small fraction
+    assert draws - 2000 < h.E <= draws
+    src = np.repeat(np.arange(h.V), np.diff(h.row_ptr))
+    assert not np.any(src == h.col)
+    assert h.w.min() >= 1 and h.w.max() <= 255
+    # rows sorted by (dst, w)
+    key = h.col.astype(np.int64) * 1024 + h.w
+    for v in range(0, h.V, 997):
+        seg = key[h.row_ptr[v]:h.row_ptr[v + 1]]
+        assert np.all(np.diff(seg) >= 0)
+    # quadrant frequencies of the top bit: P(src top=0) = a+b = 0.76
+    top_src = (src >> 15) & 1
+    top_dst = (h.col >> 15) & 1
+    assert abs((top_src == 0).mean() - 0.76) < 0.01
+    assert abs(((top_src == 0) & (top_dst == 0)).mean() - 0.57) < 0.01
+    # uniform weights: mean of U[1,255] is 128
+    assert abs(h.w.mean() - 128) < 1.0
+
+
+def test_rmat_nonpow2_and_symmetric():
+    import graphbench_amd as gb
+    h = gb.gen_rmat(11, 8, seed=5, weights="none", nverts=1500)
+    assert h.V == 1500 and h.col.max() < 1500
+    s = gb.gen_rmat(10, 4, seed=5, weights="none", symmetric=True)
+    src = np.repeat(np.arange(s.V), np.diff(s.row_ptr))
+    fwd = np.sort(src.astype(np.int64) * s.V + s.col)
+    bwd = np.sort(s.col.astype(np.int64) * s.V + src)
+    assert np.array_equal(fwd, bwd)
+
+
+def test_mixed_weights():
+    import graphbench_amd as gb
+    h = gb.gen_rmat(12, 16, seed=99, weights="mix", wlo=1, whi=255)
+    assert h.w.min() >= 1 and h.w.max() <= 65535
+    big = (h.w > 255).mean()
+    # log-uniform half exceeds 255 with probability 1 - 8/16 = 0.5 -> about 25% overall
+    assert 0.22 < big < 0.28
+
+
+def test_er_generator():
+    import graphbench_amd as gb
+    h = gb.gen_er(5000, 10, seed=3)
+    assert h.V == 5000 and h.E == 50000
+    src = np.repeat(np.arange(h.V), np.diff(h.row_ptr))
+    assert not np.any(src == h.col)
+
+
+def test_transpose_roundtrip():
+    import graphbench_amd as gb
+    h = gb.gen_rmat(10, 8, seed=2, weights="uniform")
+    t = h.transpose()
+    tt = t.transpose()
+    assert np.array_equal(tt.row_ptr, h.row_ptr)
+    assert np.array_equal(tt.col, h.col) and np.array_equal(tt.w, h.w)
+
+
+def test_pick_sources():
+    import graphbench_amd as gb
+    h = gb.gen_rmat(12, 4, seed=1, weights="none")
+    s = gb.pick_sources(h, 64, 7)
+    assert len(set(s.tolist())) == 64
+    assert np.all(np.diff(h.row_ptr)[s] > 0)
+    assert np.array_equal(s, gb.pick_sources(h, 64, 7))
+
+
+def test_invalid_arguments_raise():
+    import graphbench_amd as gb
+    with pytest.raises(gb.GBError):
+        gb.gen_rmat(0, 16)
+    with pytest.raises(gb.GBError):
+        gb.gen_er(1, 4)
+
+
+def test_no_gpu_compute_without_device():
+    """On a CPU-only host gb_init reports the missing device instead of faulting."""
+    import torch
+    import graphbench_amd as gb
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    assert gb.lib().gb_init(0) == 6
