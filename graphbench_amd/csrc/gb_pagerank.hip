@@ -444,8 +444,11 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
 
   const int32_t *trow = g->pr_tile_row.as<int32_t>();
   const int32_t *tedge = g->pr_tile_edge.as<int32_t>();
-  auto one_iter = [&](const float *x_old, const float *c_old, float *x_new, float *c_new) {
+  // One iteration; ev (optional) brackets the gather launch.
+  auto one_iter = [&](const float *x_old, const float *c_old, float *x_new, float *c_new,
+                      hipEvent_t ev0, hipEvent_t ev1) {
     RowOut out{0.0f, (float)p->damping, x_old, inv, x_new, c_new};
+    if (ev0) (void)hipEventRecord(ev0, st);
     switch (ipt) {
       case 4:
         launch_gather<4>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
@@ -467,6 +470,7 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
                          w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
                          w.pdang.as<double>(), state);
     }
+    if (ev1) (void)hipEventRecord(ev1, st);
     hipLaunchKernelGGL(pr_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, ntiles,
                        g->rrp, trow, tedge, w.tail.as<F2>(), w.head.as<F2>(), out,
                        w.fres.as<double>(), w.fdang.as<double>(), state);
@@ -485,9 +489,9 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     GB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < CH; k++) {
       if (k % 2 == 0)
-        one_iter(xa, ca, xb, cb);
+        one_iter(xa, ca, xb, cb, nullptr, nullptr);
       else
-        one_iter(xb, cb, xa, ca);
+        one_iter(xb, cb, xa, ca, nullptr, nullptr);
     }
     GB_HIP(hipStreamEndCapture(st, &graph));
     GB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -500,62 +504,37 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
   } else {
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // eager launches; with time_kernels every gather is bracketed by its own
+    // event pair and the stop flag is read every CH iterations only
+    std::vector<hipEvent_t> evs;
     if (p->time_kernels) {
-      GB_HIP(hipEventCreate(&ev0));
-      GB_HIP(hipEventCreate(&ev1));
+      evs.resize(2 * (size_t)p->max_iter, nullptr);
+      for (auto &e : evs) GB_HIP(hipEventCreate(&e));
     }
     for (int k = 0; k < p->max_iter; k++) {
       const float *xo = (k % 2 == 0) ? xa : xb;
       const float *co = (k % 2 == 0) ? ca : cb;
       float *xn = (k % 2 == 0) ? xb : xa;
       float *cn = (k % 2 == 0) ? cb : ca;
-      if (p->time_kernels) {
-        // time the gather alone: same launch as one_iter's first kernel
-        RowOut out{0.0f, (float)p->damping, xo, inv, xn, cn};
-        GB_HIP(hipEventRecord(ev0, st));
-        switch (ipt) {
-          case 4:
-            launch_gather<4>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
-                             w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
-            break;
-          case 12:
-            launch_gather<12>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
-                              w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
-            break;
-          case 16:
-            launch_gather<16>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
-                              w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
-            break;
-          default:
-            launch_gather<8>(ntiles, st, g->rrp, g->rcol, trow, tedge, co, out, w.tail.as<F2>(),
-                             w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
-        }
-        GB_HIP(hipEventRecord(ev1, st));
-        hipLaunchKernelGGL(pr_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, ntiles,
-                           g->rrp, trow, tedge, w.tail.as<F2>(), w.head.as<F2>(), out,
-                           w.fres.as<double>(), w.fdang.as<double>(), state);
-        hipLaunchKernelGGL(pr_finalize_kernel, dim3(1), dim3(1024), 0, st, ntiles,
-                           w.pres.as<double>(), w.pdang.as<double>(), nfix, w.fres.as<double>(),
-                           w.fdang.as<double>(), state);
+      one_iter(xo, co, xn, cn, p->time_kernels ? evs[2 * k] : nullptr,
+               p->time_kernels ? evs[2 * k + 1] : nullptr);
+      if ((k + 1) % CH == 0 || k + 1 == p->max_iter) {
         GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
         GB_HIP(hipStreamSynchronize(st));
-        float ms = 0.0f;
-        GB_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-        gather_ms += ms;
-        gather_launches++;
         if (host_state.done) break;
-      } else {
-        one_iter(xo, co, xn, cn);
-        if ((k + 1) % CH == 0 || k + 1 == p->max_iter) {
-          GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
-          GB_HIP(hipStreamSynchronize(st));
-          if (host_state.done) break;
-        }
       }
     }
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
+    if (p->time_kernels) {
+      GB_HIP(hipStreamSynchronize(st));
+      // launches after the stop flag exit at once: count only working ones
+      for (int k = 0; k < host_state.iter && k < p->max_iter; k++) {
+        float ms = 0.0f;
+        GB_HIP(hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
+        gather_ms += ms;
+        gather_launches++;
+      }
+      for (auto &e : evs) (void)hipEventDestroy(e);
+    }
   }
   GB_HIP(hipGetLastError());
   GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
