@@ -227,7 +227,13 @@ struct BcWork {
     for (DevBuf *b : {&level, &sigma, &delta, &order, &src, &cnt, &indeg, &dist}) b->release();
   }
 };
-BcWork g_bcw;
+// Heap singleton, never destroyed: device memory must not be freed from a
+// static destructor, which can run after the HIP runtime has shut down.
+BcWork &g_bcw_ref() {
+  static BcWork *w = new BcWork();
+  return *w;
+}
+#define g_bcw (g_bcw_ref())
 
 int wave_grid(uint64_t n) {
   uint64_t blocks = (n + 3) / 4;

@@ -145,8 +145,24 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
   const int32_t r0 = tile_row[t], e0 = tile_edge[t];
   const int32_t r1 = tile_row[t + 1], e1 = tile_edge[t + 1];
   const int nrows = r1 - r0, nedges = e1 - e0;
-  for (int k = tid; k < nrows; k += PR_BLOCK) s_end[k] = irp[r0 + k + 1] - e0;
-  for (int k = tid; k < nedges; k += PR_BLOCK) s_val[k] = contrib[icol[e0 + k]];
+  // Stage the tile: IPT independent column loads per thread (streamed past the
+  // caches), then IPT independent gathers, so many requests are in flight.
+  int32_t cidx[IPT];
+#pragma unroll
+  for (int m = 0; m < IPT; ++m) {
+    const int k = tid + m * PR_BLOCK;
+    cidx[m] = k < nedges ? __builtin_nontemporal_load(icol + e0 + k) : 0;
+  }
+#pragma unroll
+  for (int m = 0; m < IPT; ++m) {
+    const int k = tid + m * PR_BLOCK;
+    if (k < nedges) s_val[k] = contrib[cidx[m]];
+  }
+#pragma unroll
+  for (int m = 0; m < IPT; ++m) {
+    const int k = tid + m * PR_BLOCK;
+    if (k < nrows) s_end[k] = irp[r0 + k + 1] - e0;
+  }
   __syncthreads();
 
   // this thread's start on the merge path, searched in LDS
@@ -222,12 +238,18 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
   }
 }
 
+// Finishes rows that began in an earlier tile, then the last block to finish
+// reduces every partial (gather tiles first, then fix-up blocks, fixed order)
+// and sets the next base and the stop flag.
 __global__ __launch_bounds__(PR_BLOCK) void pr_fixup_kernel(
     int64_t ntiles, const int32_t *__restrict__ irp, const int32_t *__restrict__ tile_row,
     const int32_t *__restrict__ tile_edge, const F2 *__restrict__ tail,
-    const F2 *__restrict__ head, RowOut out, double *__restrict__ fix_res,
-    double *__restrict__ fix_dang, const PrState *__restrict__ st) {
+    const F2 *__restrict__ head, RowOut out, const double *__restrict__ p_res,
+    const double *__restrict__ p_dang, double *__restrict__ fix_res,
+    double *__restrict__ fix_dang, unsigned int *__restrict__ done_blocks,
+    PrState *__restrict__ st) {
   __shared__ double s_red[PR_BLOCK / 64];
+  __shared__ int s_last;
   if (st->done) return;
   out.base = st->base;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -251,34 +273,33 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_fixup_kernel(
   if (threadIdx.x == 0) {
     fix_res[blockIdx.x] = rsum;
     fix_dang[blockIdx.x] = dsum;
+    __threadfence();
+    s_last = atomicAdd(done_blocks, 1u) == gridDim.x - 1;
   }
-}
-
-__global__ void pr_finalize_kernel(int64_t n1, const double *__restrict__ p_res,
-                                   const double *__restrict__ p_dang, int64_t n2,
-                                   const double *__restrict__ f_res,
-                                   const double *__restrict__ f_dang, PrState *__restrict__ st) {
-  __shared__ double s_red[1024 / 64];
-  if (st->done) return;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int64_t n2 = gridDim.x;
   double a = 0.0, b = 0.0;
-  for (int64_t i = threadIdx.x; i < n1; i += blockDim.x) {
+  for (int64_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
     a += p_res[i];
     b += p_dang[i];
   }
   for (int64_t i = threadIdx.x; i < n2; i += blockDim.x) {
-    a += f_res[i];
-    b += f_dang[i];
+    a += __builtin_nontemporal_load(fix_res + i);
+    b += __builtin_nontemporal_load(fix_dang + i);
   }
-  double res = block_sum_f64(a, s_red);
-  double dang = block_sum_f64(b, s_red);
+  const double rtot = block_sum_f64(a, s_red);
+  const double dtot = block_sum_f64(b, s_red);
   if (threadIdx.x == 0) {
     const double N = (double)st->V;
     const double d = (double)st->damping;
     st->iter += 1;
-    st->residual = res;
-    st->dangling = dang;
-    st->base = (float)((1.0 - d) / N + d * dang / N);
-    st->done = (res < st->tol || st->iter >= st->max_iter) ? 1 : 0;
+    st->residual = rtot;
+    st->dangling = dtot;
+    st->base = (float)((1.0 - d) / N + d * dtot / N);
+    st->done = (rtot < st->tol || st->iter >= st->max_iter) ? 1 : 0;
+    *done_blocks = 0;  // ready for the next iteration
   }
 }
 
@@ -321,13 +342,19 @@ __global__ void pr_ref_update_kernel(int32_t V, const double *__restrict__ recip
 }
 
 struct PrWork {
-  DevBuf xb, ca, cb, tail, head, pres, pdang, fres, fdang, state;
+  DevBuf xb, ca, cb, tail, head, pres, pdang, fres, fdang, state, counter;
   void release() {
-    for (DevBuf *b : {&xb, &ca, &cb, &tail, &head, &pres, &pdang, &fres, &fdang, &state})
+    for (DevBuf *b : {&xb, &ca, &cb, &tail, &head, &pres, &pdang, &fres, &fdang, &state, &counter})
       b->release();
   }
 };
-PrWork g_prw;
+// Heap singleton, never destroyed: device memory must not be freed from a
+// static destructor, which can run after the HIP runtime has shut down.
+PrWork &g_prw_ref() {
+  static PrWork *w = new PrWork();
+  return *w;
+}
+#define g_prw (g_prw_ref())
 
 int default_ipt(gb_graph g) {
   (void)g;
@@ -426,6 +453,8 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
   GB_TRY(w.fres.ensure(sizeof(double) * nfix));
   GB_TRY(w.fdang.ensure(sizeof(double) * nfix));
   GB_TRY(w.state.ensure(sizeof(PrState)));
+  GB_TRY(w.counter.ensure(sizeof(unsigned int)));
+  GB_HIP(hipMemsetAsync(w.counter.p, 0, sizeof(unsigned int), st));
   PrState hs{};
   hs.tol = p->tol;
   hs.damping = (float)p->damping;
@@ -473,10 +502,8 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     if (ev1) (void)hipEventRecord(ev1, st);
     hipLaunchKernelGGL(pr_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, ntiles,
                        g->rrp, trow, tedge, w.tail.as<F2>(), w.head.as<F2>(), out,
-                       w.fres.as<double>(), w.fdang.as<double>(), state);
-    hipLaunchKernelGGL(pr_finalize_kernel, dim3(1), dim3(1024), 0, st, ntiles,
-                       w.pres.as<double>(), w.pdang.as<double>(), nfix, w.fres.as<double>(),
-                       w.fdang.as<double>(), state);
+                       w.pres.as<double>(), w.pdang.as<double>(), w.fres.as<double>(),
+                       w.fdang.as<double>(), w.counter.as<unsigned int>(), state);
   };
 
   PrState host_state{};

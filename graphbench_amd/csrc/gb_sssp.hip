@@ -298,7 +298,13 @@ struct SsspWork {
     for (auto &b : q) b.release();
   }
 };
-SsspWork g_ssw;
+// Heap singleton, never destroyed: device memory must not be freed from a
+// static destructor, which can run after the HIP runtime has shut down.
+SsspWork &g_ssw_ref() {
+  static SsspWork *w = new SsspWork();
+  return *w;
+}
+#define g_ssw (g_ssw_ref())
 
 int expand_grid(uint64_t n_entries) {
   // one wave per entry, capped at a few resident waves per CU
