@@ -92,6 +92,20 @@ gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st);
 // Light-first ordering of the forward CSR for a threshold (cached per handle).
 gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st);
 gb_status gb_graph_split64(gb_graph g, double light_max, hipStream_t st);
+// Reference-form PageRank with a hook called at the top of every iteration
+// with the device wb array (before that iteration's gather).
+typedef void (*gb_pr_ref_hook)(int32_t iter, const double *d_wb, void *ctx);
+gb_status gb_pagerank_ref_impl(gb_graph g, const double *d_recip, int32_t max_iter, double *d_x,
+                               int32_t *iterations, hipStream_t st, gb_pr_ref_hook hook,
+                               void *ctx);
+// comp_distrib_csr on the device: distrib (p, int64) and, when non-null, the
+// per-part degree sums of two degree prefix arrays (send and recv).
+gb_status gb_edge_balanced_parts_ex(int32_t V, const int32_t *d_cut_rp, const int32_t *d_send_rp,
+                                    const int32_t *d_recv_rp, int32_t p, int64_t *d_distrib,
+                                    int64_t *d_send_size, int64_t *d_recv_size, hipStream_t st);
+// init_node_id on the device: part of every edge target.
+gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib, int32_t p,
+                      int64_t *d_out, hipStream_t st);
 // per-module workspace release (gb_release_workspaces)
 void gb_pr_release();
 void gb_sssp_release();

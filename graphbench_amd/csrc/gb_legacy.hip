@@ -1,0 +1,667 @@
+// gb_legacy.hip -- the reference's own C entry points over the GPU library.
+//
+// run_sspath_arr / run_sspath / run_pagerank_csr_mpi / run_bc and the
+// readers keep the reference's signatures, t_csr layout, file formats,
+// printed output and exit(123) error handling (include/gb_legacy.h).  Each
+// converts t_csr (int64 end offsets, AoS vertex/edge info) into the device
+// layout once, calls the native gb_* path and writes results back into the
+// caller's t_csr.
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "gb_internal.hpp"
+#include "gb_legacy.h"
+
+static_assert(sizeof(t_vi) == 32, "t_vi layout (graph.h:95-102)");
+static_assert(sizeof(t_e) == 16, "t_e layout (graph.h:105-110)");
+static_assert(sizeof(t_csr) == 72, "t_csr layout (graph.h:113-125)");
+static_assert(sizeof(t_list) == 32, "t_list layout (list.h:3-9)");
+
+// ---- the reference's globals (main/main.c:8-22); weak so that a caller's
+// own definitions win --------------------------------------------------------
+extern "C" {
+__attribute__((weak)) sint num_threads = 1;
+__attribute__((weak)) sint num_nodes = 1;
+__attribute__((weak)) lint g_v_num = 1;
+__attribute__((weak)) char *par_extend = (char *)"parex/";
+__attribute__((weak)) char *data_extend = (char *)"datex/";
+__attribute__((weak)) char *idx_extend = (char *)"idxex/";
+__attribute__((weak)) char *base_dir = (char *)"./";
+__attribute__((weak)) char *nodes_extension = nullptr;
+__attribute__((weak)) int bin = 0;
+__attribute__((weak)) int buck_size = 10;
+__attribute__((weak)) sint use_partition = 0;
+__attribute__((weak)) int buff_size = 10000;
+__attribute__((weak)) lint prefetch_dis = 10;
+}
+
+namespace {
+
+// ERROR_PRINT (graph.h:74): message, then exit(123)
+[[noreturn]] void fatal(const char *what) {
+  printf("Error: %s\n", what);
+  const char *e = gb_last_error();
+  if (e && *e) printf("  %s\n", e);
+  fflush(stdout);
+  exit(123);
+}
+
+void check(gb_status s, const char *what) {
+  if (s != GB_OK) fatal(what);
+}
+
+void hipcheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    gb_fail(GB_ERR_HIP, "%s", hipGetErrorString(e));
+    fatal(what);
+  }
+}
+
+void *amalloc(size_t n) {  // _mm_malloc(n, 64) equivalent
+  void *p = nullptr;
+  if (posix_memalign(&p, 64, n ? n : 64) != 0) fatal("host allocation failed");
+  return p;
+}
+
+// Whitespace token reader matching the reference's fscanf("%s") loop.
+struct Tok {
+  FILE *f;
+  bool next(char *buf, size_t n) { return fscanf(f, "%99s", buf) == 1; }
+  bool num(long long *v) { return fscanf(f, "%lld", v) == 1; }
+};
+
+FILE *open_in(const char *dir, const char *file, const char *mode) {
+  char buf[4096];
+  snprintf(buf, sizeof(buf), "%s%s", dir, file);
+  FILE *f = fopen(buf, mode);
+  if (!f) {
+    printf("the file %s you input does not exist!\n", buf);
+    fatal("cannot open graph file");
+  }
+  return f;
+}
+
+// Device copy of a t_csr (forward and optional reverse), f64 and u32 weights.
+struct DevGraph {
+  gb_graph g = nullptr;
+  DevBuf rp, col, w, w64, rrp, rcol, rw64;
+  int32_t V = 0;
+  int64_t E = 0;
+  bool integral = false;
+  ~DevGraph() {
+    if (g) gb_graph_destroy(g);
+  }
+};
+
+template <class T>
+void upload(DevBuf &b, const std::vector<T> &v) {
+  check(b.ensure(sizeof(T) * (v.size() + 1)), "device allocation");
+  if (!v.empty())
+    hipcheck(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), "upload");
+}
+
+// t_csr -> row_ptr/col (int32), weights f64 (and u32 when all are integers in
+// [0, 2^32)); optional reverse CSR with weights (stable counting sort).
+void to_device(const t_csr *gs, DevGraph &dg, bool reverse, bool weights) {
+  if (!gs || gs->v_size <= 0 || gs->v_size >= (1ll << 31) || gs->e_size < 0 ||
+      gs->e_size >= (1ll << 31))
+    fatal("graph size outside the int32 device layout");
+  const int32_t V = (int32_t)gs->v_size;
+  const int64_t E = gs->e_size;
+  std::vector<int32_t> rp(V + 1), col(E);
+  rp[0] = 0;
+  for (int32_t i = 0; i < V; i++) {
+    lint end = gs->vet_idx[i];
+    if (end < rp[i] || end > E) fatal("vet_idx is not a non-decreasing end-offset array");
+    rp[i + 1] = (int32_t)end;
+  }
+  if (rp[V] != E) fatal("vet_idx[V-1] != e_size");
+  for (int64_t e = 0; e < E; e++) {
+    lint c = gs->edge_idx[e];
+    if (c < 0 || c >= V) fatal("edge_idx out of range");
+    col[e] = (int32_t)c;
+  }
+  std::vector<double> w64;
+  std::vector<uint32_t> w32;
+  bool integral = true;
+  if (weights) {
+    w64.resize(E);
+    for (int64_t e = 0; e < E; e++) {
+      double x = gs->edge_info[e].edge_weight;
+      if (!(x >= 0.0) || std::isinf(x)) fatal("edge weights must be finite and non-negative");
+      w64[e] = x;
+      if (integral && !(x == std::floor(x) && x < 4294967296.0)) integral = false;
+    }
+    if (integral) {
+      w32.resize(E);
+      for (int64_t e = 0; e < E; e++) w32[e] = (uint32_t)w64[e];
+    }
+  }
+  dg.V = V;
+  dg.E = E;
+  dg.integral = weights && integral;
+  upload(dg.rp, rp);
+  upload(dg.col, col);
+  if (dg.integral) upload(dg.w, w32);
+  if (weights) upload(dg.w64, w64);
+  if (reverse) {
+    std::vector<int32_t> rrp(V + 1, 0), rcol(E);
+    std::vector<double> rw(weights ? E : 0);
+    for (int64_t e = 0; e < E; e++) rrp[col[e] + 1]++;
+    for (int32_t i = 0; i < V; i++) rrp[i + 1] += rrp[i];
+    std::vector<int32_t> pos(rrp.begin(), rrp.end() - 1);
+    for (int32_t u = 0; u < V; u++)
+      for (int32_t e = rp[u]; e < rp[u + 1]; e++) {
+        int32_t p = pos[col[e]]++;
+        rcol[p] = u;
+        if (weights) rw[p] = w64[e];
+      }
+    upload(dg.rrp, rrp);
+    upload(dg.rcol, rcol);
+    if (weights) upload(dg.rw64, rw);
+  }
+  check(gb_graph_wrap(&dg.g, V, E, dg.rp.as<int32_t>(), dg.col.as<int32_t>(),
+                      dg.integral ? dg.w.as<uint32_t>() : nullptr,
+                      reverse ? dg.rrp.as<int32_t>() : nullptr,
+                      reverse ? dg.rcol.as<int32_t>() : nullptr, nullptr),
+        "graph handle");
+  if (weights)
+    check(gb_graph_set_f64_weights(dg.g, dg.w64.as<double>(),
+                                   reverse ? dg.rw64.as<double>() : nullptr),
+          "f64 weights");
+}
+
+void sssp_into(t_csr *gs, lint source, double delta, DevGraph &dg, std::vector<double> &dist) {
+  if (source < 0 || source >= gs->v_size) fatal("source out of range");
+  const int32_t src = (int32_t)source;
+  const int32_t V = dg.V;
+  dist.assign(V, INFINITY);
+  if (dg.integral) {
+    DevBuf d;
+    check(d.ensure(sizeof(uint32_t) * V), "device allocation");
+    gb_sssp_params p{};
+    double fl = std::floor(delta);
+    p.delta = (uint32_t)(fl >= 1.0 ? (fl < 4294967295.0 ? fl : 4294967295.0) : 1.0);
+    p.light_max = p.delta;
+    check(gb_sssp(dg.g, &p, &src, 1, d.as<uint32_t>(), nullptr, nullptr), "sssp");
+    std::vector<uint32_t> h(V);
+    hipcheck(hipMemcpy(h.data(), d.p, sizeof(uint32_t) * V, hipMemcpyDeviceToHost), "download");
+    for (int32_t v = 0; v < V; v++) dist[v] = h[v] == GB_DIST_INF ? INFINITY : (double)h[v];
+  } else {
+    DevBuf d;
+    check(d.ensure(sizeof(double) * V), "device allocation");
+    check(gb_sssp_f64(dg.g, delta > 0 ? delta : 1.0, &src, 1, d.as<double>(), nullptr, nullptr),
+          "sssp");
+    hipcheck(hipMemcpy(dist.data(), d.p, sizeof(double) * V, hipMemcpyDeviceToHost), "download");
+  }
+}
+
+// tight in-edges of the shortest-path DAG: d[u] + w == d[v] (csr_sspath.c:101)
+__global__ void tight_edges_kernel(int32_t V, const int32_t *__restrict__ rp,
+                                   const int32_t *__restrict__ col, const double *__restrict__ w,
+                                   const double *__restrict__ dist, uint8_t *__restrict__ tight) {
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < V;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const double du = dist[u];
+    for (int32_t e = rp[u]; e < rp[u + 1]; e++)
+      tight[e] = (du != INFINITY && __dadd_rn(du, w[e]) == dist[col[e]]) ? 1 : 0;
+  }
+}
+
+void add_list_one(p_list list, lint val) {  // list.c:11-26 (prepend)
+  p_list elem = (p_list)malloc(sizeof(t_list));
+  if (!elem) fatal("host allocation failed");
+  elem->int_val = val;
+  elem->int_val1 = -1;
+  elem->double_val = -1;
+  elem->next = list->next;
+  list->next = elem;
+}
+
+void pr_print_hook(int32_t iter, const double *d_wb, void *ctx) {
+  const t_csr *gr = (const t_csr *)ctx;
+  const int n = gr->v_size < 10 ? (int)gr->v_size : 10;
+  double wb[10];
+  if (hipMemcpy(wb, d_wb, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    fatal("download");
+  printf("ITER %d ", iter);  // pagerank_mpi.c:823-830
+  for (int i = 0; i < n; i++) printf("%f|%f ", wb[i], gr->vet_info[i].recip);
+  printf("\n");
+  fflush(stdout);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- readers (graph/csr.c) --------------------------------------------------
+
+void scan_csr_idx(t_csr *graph, char *file, char *dir, sint type) {
+  FILE *f = open_in(dir, file, "r");
+  char str[128];
+  Tok t{f};
+  long long v_num = 0, e_num = 0, offset = 0, g_v_size = 0, u, v, w;
+  long long read = 0;
+  bool header = false;
+  while (t.next(str, sizeof(str))) {
+    if (str[0] == 'p' && str[1] == '\0') {
+      char kind[128];
+      if (!t.next(kind, sizeof(kind)) || !t.num(&v_num) || !t.num(&e_num) || !t.num(&offset) ||
+          !t.num(&g_v_size))
+        fatal("bad 'p' line");
+      graph->offset = offset;
+      graph->v_size = v_num;
+      graph->e_size = e_num;
+      graph->global_v_size = g_v_size;
+      graph->vet_idx = (lint *)amalloc(sizeof(lint) * v_num);
+      graph->vet_info = (t_vi *)amalloc(sizeof(t_vi) * v_num);
+      for (long long i = 0; i < v_num; i++) {
+        graph->vet_info[i].weight = 1.0;
+        graph->vet_info[i].wb = 0;
+        graph->vet_info[i].recip = 0;
+        graph->vet_info[i].vet_deg = 0;
+      }
+      header = true;
+    } else if (str[0] == 'a' && str[1] == '\0') {
+      if (!header) fatal("'a' line before the 'p' line");
+      if (!t.num(&u) || !t.num(&v) || !t.num(&w)) fatal("bad 'a' line");
+      if (type == GB_LEGACY_RECV) {
+        u -= 1;
+        v -= offset;
+        if (v < 0 || v >= v_num) fatal("edge target out of range");
+        graph->vet_info[v].vet_deg++;
+      } else {
+        u -= offset;
+        v -= 1;
+        if (u < 0 || u >= v_num) fatal("edge source out of range");
+        graph->vet_info[u].vet_deg++;
+      }
+      if (++read == e_num) break;
+    }
+  }
+  fclose(f);
+  if (!header) fatal("no 'p' line");
+  lint sum = 0;
+  for (long long i = 0; i < v_num; i++) {  // csr.c:194-200
+    sum += graph->vet_info[i].vet_deg;
+    graph->vet_info[i].recip = 1.0 / (double)graph->vet_info[i].vet_deg;
+    graph->vet_idx[i] = sum;
+  }
+}
+
+void read_graph_csr(t_csr *graph, char *file, char *dir, sint type) {
+  FILE *f = open_in(dir, file, "r");
+  char str[128];
+  Tok t{f};
+  graph->edge_idx = (lint *)amalloc(sizeof(lint) * graph->e_size);
+  graph->edge_info = (t_e *)amalloc(sizeof(t_e) * graph->e_size);
+  long long v_num = 0, e_num = 0, offset = 0, g_v_size = 0, u, v, w;
+  long long read = 0;
+  std::vector<lint> pos;
+  while (t.next(str, sizeof(str))) {
+    if (str[0] == 'p' && str[1] == '\0') {
+      char kind[128];
+      if (!t.next(kind, sizeof(kind)) || !t.num(&v_num) || !t.num(&e_num) || !t.num(&offset) ||
+          !t.num(&g_v_size))
+        fatal("bad 'p' line");
+      pos.assign(v_num, 0);
+      for (long long i = 1; i < v_num; i++) pos[i] = graph->vet_idx[i - 1];
+    } else if (str[0] == 'a' && str[1] == '\0') {
+      if (pos.empty()) fatal("'a' line before the 'p' line");
+      if (!t.num(&u) || !t.num(&v) || !t.num(&w)) fatal("bad 'a' line");
+      // csr.c:42: the weight of the read-th edge is v_to / v_num (file order)
+      if (read < graph->e_size) graph->edge_info[read].edge_weight = (double)v / (double)v_num;
+      if (type == GB_LEGACY_RECV) {
+        u -= 1;
+        v -= offset;
+        graph->edge_idx[pos[v]++] = u;
+      } else {
+        u -= offset;
+        v -= 1;
+        graph->edge_idx[pos[u]++] = v;
+      }
+      if (++read == e_num) break;
+    }
+  }
+  fclose(f);
+}
+
+void read_csr_bin(t_csr *graph, char *file, char *dir, sint type, int myrank) {
+  (void)myrank;
+  FILE *f = open_in(dir, file, "rb");
+  lint hdr[4];
+  if (fread(hdr, sizeof(lint), 4, f) != 4) fatal("short binary header");
+  const lint v_num = hdr[0], e_num = hdr[1], offset = hdr[2];
+  std::vector<lint> edges(2 * e_num);
+  if ((lint)fread(edges.data(), sizeof(lint), 2 * e_num, f) != 2 * e_num) fatal("short edge list");
+  fclose(f);
+  const lint OFFSET = 1;  // graph.h:28-32 without USE_GRAPH500
+  graph->offset = offset;
+  graph->v_size = v_num;
+  graph->e_size = e_num;
+  graph->global_v_size = hdr[3];
+  graph->vet_idx = (lint *)amalloc(sizeof(lint) * v_num);
+  graph->vet_info = (t_vi *)amalloc(sizeof(t_vi) * v_num);
+  graph->edge_idx = (lint *)amalloc(sizeof(lint) * e_num);
+  graph->edge_info = (t_e *)amalloc(sizeof(t_e) * e_num);
+  for (lint i = 0; i < v_num; i++) {
+    graph->vet_info[i].weight = INFINITY;
+    graph->vet_info[i].wb = 0;
+    graph->vet_info[i].recip = 0;
+    graph->vet_info[i].vet_deg = 0;
+  }
+  std::vector<lint> cnt(v_num, 0);
+  for (lint i = 0; i < e_num; i++) {
+    lint key = type == GB_LEGACY_RECV ? edges[2 * i + 1] - offset : edges[2 * i] - offset;
+    if (key < 0 || key >= v_num) fatal("binary edge out of range");
+    cnt[key]++;
+  }
+  lint sum = 0;
+  for (lint i = 0; i < v_num; i++) {
+    graph->vet_idx[i] = cnt[i] + sum;
+    sum += cnt[i];
+  }
+  std::vector<lint> pos(v_num, 0);
+  for (lint i = 1; i < v_num; i++) pos[i] = graph->vet_idx[i - 1];
+  for (lint i = 0; i < e_num; i++) {
+    lint from = edges[2 * i], to = edges[2 * i + 1];
+    if (type == GB_LEGACY_RECV)
+      graph->edge_idx[pos[to - offset]++] = from - OFFSET;
+    else
+      graph->edge_idx[pos[from - offset]++] = to - OFFSET;
+  }
+  for (lint i = 0; i < e_num; i++) graph->edge_info[i].edge_weight = (i % 2 == 0) ? 0.1 : 0.2;
+}
+
+// ---- SSSP -------------------------------------------------------------------
+
+void run_sspath_arr(t_csr *gs, p_list *predecesor, lint source, double delta) {
+  (void)predecesor;
+  DevGraph dg;
+  to_device(gs, dg, false, true);
+  std::vector<double> dist;
+  sssp_into(gs, source, delta, dg, dist);
+  for (lint v = 0; v < gs->v_size; v++) gs->vet_info[v].weight = dist[v];
+  g_v_num = gs->v_size;
+}
+
+void run_sspath(t_csr *gs, p_list *predecesor, lint source, double delta) {
+  DevGraph dg;
+  to_device(gs, dg, false, true);
+  std::vector<double> dist;
+  sssp_into(gs, source, delta, dg, dist);
+  for (lint v = 0; v < gs->v_size; v++) gs->vet_info[v].weight = dist[v];
+  if (!predecesor) return;
+  const int32_t V = dg.V;
+  const int64_t E = dg.E;
+  DevBuf dd, tight;
+  check(dd.ensure(sizeof(double) * V), "device allocation");
+  check(tight.ensure(E + 1), "device allocation");
+  hipcheck(hipMemcpy(dd.p, dist.data(), sizeof(double) * V, hipMemcpyHostToDevice), "upload");
+  if (E > 0) {
+    int blocks = (V + 255) / 256;
+    hipLaunchKernelGGL(tight_edges_kernel, dim3(blocks), dim3(256), 0, 0, V, dg.rp.as<int32_t>(),
+                       dg.col.as<int32_t>(), dg.w64.as<double>(), dd.as<double>(),
+                       tight.as<uint8_t>());
+    hipcheck(hipGetLastError(), "tight edges");
+  }
+  std::vector<uint8_t> h(E);
+  if (E) hipcheck(hipMemcpy(h.data(), tight.p, E, hipMemcpyDeviceToHost), "download");
+  lint e = 0;
+  for (lint u = 0; u < gs->v_size; u++)
+    for (; e < gs->vet_idx[u]; e++)
+      if (h[e]) add_list_one(predecesor[gs->edge_idx[e]], u);
+}
+
+// ---- PageRank (pagerank_mpi.c:667-1044, one rank) ---------------------------
+
+void run_pagerank_csr_mpi(char *file, char *dir) {
+  if (num_nodes != 1)
+    fatal("run_pagerank_csr_mpi: this build runs on one GPU (num_nodes must be 1)");
+  const int myrank = 0;
+  char f_buf[4096];
+  t_csr gr_s{};
+  t_csr *gr = &gr_s;
+  snprintf(f_buf, sizeof(f_buf), "%s%s_rc_%d_%d", data_extend, file, myrank, num_nodes);
+  if (!bin) {
+    scan_csr_idx(gr, f_buf, dir, GB_LEGACY_RECV);
+    read_graph_csr(gr, f_buf, dir, GB_LEGACY_RECV);
+  } else {
+    read_csr_bin(gr, f_buf, dir, GB_LEGACY_RECV, myrank);
+  }
+  // read_recip (csr_partition.c:166-182 / read_recip_bin for `bin`)
+  std::vector<double> recip(gr->v_size);
+  {
+    if (!bin) {
+      snprintf(f_buf, sizeof(f_buf), "%s%s%s_recip_%d_%s", dir, par_extend, file, myrank,
+               nodes_extension ? nodes_extension : "(null)");
+      FILE *r = fopen(f_buf, "r");
+      if (!r) {
+        printf("the file %s does not exists!\n", f_buf);
+        fatal("missing recip file");
+      }
+      for (lint i = 0; i < gr->v_size; i++)
+        if (fscanf(r, "%lf", &recip[i]) != 1) fatal("short recip file");
+      fclose(r);
+    } else {
+      snprintf(f_buf, sizeof(f_buf), "%s%s%s_recip_%d_%s", dir, par_extend, file, myrank,
+               nodes_extension ? nodes_extension : "(null)");
+      FILE *r = fopen(f_buf, "rb");
+      if (!r) {
+        printf("the file %s does not exists!\n", f_buf);
+        fatal("missing recip file");
+      }
+      if ((lint)fread(recip.data(), sizeof(double), gr->v_size, r) != gr->v_size)
+        fatal("short recip file");
+      fclose(r);
+    }
+    for (lint i = 0; i < gr->v_size; i++) gr->vet_info[i].recip = recip[i];
+    if (!bin) printf("%1.10f\n", gr->vet_info[0].recip);
+  }
+  // the RECV CSR is the in-CSR: wrap it as the reverse graph of the handle
+  DevGraph dg;
+  {
+    const int32_t V = (int32_t)gr->v_size;
+    const int64_t E = gr->e_size;
+    std::vector<int32_t> rp(V + 1), col(E);
+    rp[0] = 0;
+    for (int32_t i = 0; i < V; i++) rp[i + 1] = (int32_t)gr->vet_idx[i];
+    for (int64_t e = 0; e < E; e++) {
+      if (gr->edge_idx[e] < 0 || gr->edge_idx[e] >= V) fatal("edge_idx out of range");
+      col[e] = (int32_t)gr->edge_idx[e];
+    }
+    upload(dg.rrp, rp);
+    upload(dg.rcol, col);
+    check(gb_graph_wrap(&dg.g, V, E, nullptr, nullptr, nullptr, dg.rrp.as<int32_t>(),
+                        dg.rcol.as<int32_t>(), nullptr),
+          "graph handle");
+    dg.V = V;
+    dg.E = E;
+  }
+  DevBuf drecip, dx;
+  upload(drecip, recip);
+  check(dx.ensure(sizeof(double) * (gr->v_size + 1)), "device allocation");
+  int32_t iters = 0;
+  hipEvent_t e0, e1;
+  hipcheck(hipEventCreate(&e0), "event");
+  hipcheck(hipEventCreate(&e1), "event");
+  hipcheck(hipEventRecord(e0, 0), "event");
+  gb_status s = gb_pagerank_ref_impl(dg.g, drecip.as<double>(), 0, dx.as<double>(), &iters, 0,
+                                     pr_print_hook, gr);
+  hipcheck(hipEventRecord(e1, 0), "event");
+  check(s, "pagerank");
+  std::vector<double> x(gr->v_size);
+  hipcheck(hipMemcpy(x.data(), dx.p, sizeof(double) * gr->v_size, hipMemcpyDeviceToHost),
+           "download");
+  for (lint i = 0; i < gr->v_size; i++) {
+    gr->vet_info[i].weight = x[i];
+    gr->vet_info[i].wb = x[i];
+  }
+  float ms = 0;
+  hipcheck(hipEventSynchronize(e1), "event");
+  hipcheck(hipEventElapsedTime(&ms, e0, e1), "event");
+  printf("Final pagerank values: \n");  // pagerank_mpi.c:1024-1030
+  for (lint ii = 0; ii < 10 && ii < gr->v_size; ii++)
+    printf("%lld|%lf ", ii + gr->offset, gr->vet_info[ii].weight);
+  printf("\n");
+  printf("iterations taken: %d\n", iters);
+  printf("gpu time: %f ms (%lld in-edges, %d iterations)\n", ms, gr->e_size, iters);
+  fflush(stdout);
+}
+
+// ---- Brandes (csr_bc.c:16-68) --------------------------------------------------
+
+void run_bc(char *file, char *dir) {
+  t_csr gs_s{};
+  t_csr *gs = &gs_s;
+  if (!bin) {
+    scan_csr_idx(gs, file, dir, GB_LEGACY_SEND);
+    read_graph_csr(gs, file, dir, GB_LEGACY_SEND);
+  } else {
+    read_csr_bin(gs, file, dir, GB_LEGACY_SEND, 0);
+  }
+  DevGraph dg;
+  to_device(gs, dg, true, true);
+  const int32_t V = dg.V;
+  std::vector<int32_t> sources(V);
+  for (int32_t i = 0; i < V; i++) sources[i] = i;  // every vertex (csr_bc.c:40)
+  DevBuf dbc;
+  check(dbc.ensure(sizeof(double) * V), "device allocation");
+  int64_t batch = ((int64_t)64 << 20) / (V > 0 ? V : 1);
+  if (batch < 1) batch = 1;
+  if (batch > 65535) batch = 65535;
+  check(gb_bc_weighted(dg.g, sources.data(), V, (int32_t)batch, dbc.as<double>(), nullptr,
+                       nullptr),
+        "betweenness");
+  std::vector<double> bc(V);
+  hipcheck(hipMemcpy(bc.data(), dbc.p, sizeof(double) * V, hipMemcpyDeviceToHost), "download");
+  printf("the final result for betweeness centrality is:\n");
+  for (int32_t i = 0; i < V; i++) printf("%lld:%lf ", (lint)(i + 1), bc[i]);
+  printf("\n");
+  fflush(stdout);
+}
+
+// ---- light/heavy split and partitioning ------------------------------------------
+
+void separate_csr(t_csr *graph, t_csr *gl, t_csr *gh, double delta) {
+  DevGraph dg;
+  to_device(graph, dg, false, true);
+  check(gb_graph_split64(dg.g, delta, nullptr), "separate");
+  const int32_t V = dg.V;
+  const int64_t E = dg.E;
+  std::vector<int32_t> col(E), le(V);
+  std::vector<double> w(E);
+  if (E) {
+    hipcheck(hipMemcpy(col.data(), dg.g->split64_col.p, sizeof(int32_t) * E, hipMemcpyDeviceToHost),
+             "download");
+    hipcheck(hipMemcpy(w.data(), dg.g->split64_w.p, sizeof(double) * E, hipMemcpyDeviceToHost),
+             "download");
+  }
+  hipcheck(hipMemcpy(le.data(), dg.g->split64_le.p, sizeof(int32_t) * V, hipMemcpyDeviceToHost),
+           "download");
+  lint nl = 0, nh = 0;
+  for (int32_t v = 0; v < V; v++) {
+    lint s = v ? graph->vet_idx[v - 1] : 0;
+    nl += le[v] - s;
+    nh += graph->vet_idx[v] - le[v];
+  }
+  for (t_csr *g : {gl, gh}) {
+    g->v_size = V;
+    g->vet_idx = (lint *)amalloc(sizeof(lint) * V);
+    g->vet_info = (t_vi *)amalloc(sizeof(t_vi) * V);
+    memset(g->vet_info, 0, sizeof(t_vi) * V);
+  }
+  gl->e_size = nl;
+  gh->e_size = nh;
+  gl->edge_idx = (lint *)amalloc(sizeof(lint) * (nl ? nl : 1));
+  gh->edge_idx = (lint *)amalloc(sizeof(lint) * (nh ? nh : 1));
+  gl->edge_info = (t_e *)amalloc(sizeof(t_e) * (nl ? nl : 1));
+  gh->edge_info = (t_e *)amalloc(sizeof(t_e) * (nh ? nh : 1));
+  lint pl = 0, ph = 0;
+  for (int32_t v = 0; v < V; v++) {
+    lint s = v ? graph->vet_idx[v - 1] : 0, end = graph->vet_idx[v];
+    for (lint e = s; e < end; e++) {
+      t_csr *g = e < le[v] ? gl : gh;
+      lint &p = e < le[v] ? pl : ph;
+      g->edge_idx[p] = col[e];
+      g->edge_info[p].edge_weight = w[e];
+      g->edge_info[p].node_id = 0;
+      g->vet_info[v].vet_deg++;
+      p++;
+    }
+    gl->vet_idx[v] = pl;
+    gh->vet_idx[v] = ph;
+  }
+}
+
+void comp_distrib_csr(t_csr *graph_send, t_csr *graph_recv, lint *distrib, lint *send_size,
+                      lint *recv_size, sint type) {
+  const int32_t V = (int32_t)graph_send->v_size;
+  const int32_t p = num_nodes;
+  if (p <= 0 || V <= 0) fatal("comp_distrib_csr: empty graph or num_nodes <= 0");
+  auto prefix = [&](const t_csr *g, DevBuf &b) {
+    // degrees from vet_deg (the reference's walk reads vet_info[i].vet_deg)
+    std::vector<int32_t> rp(V + 1, 0);
+    for (int32_t i = 0; i < V; i++) rp[i + 1] = rp[i] + (int32_t)g->vet_info[i].vet_deg;
+    upload(b, rp);
+  };
+  DevBuf srp, rrp, dd, ds, dr;
+  prefix(graph_send, srp);
+  prefix(graph_recv, rrp);
+  check(dd.ensure(sizeof(int64_t) * p), "device allocation");
+  check(ds.ensure(sizeof(int64_t) * p), "device allocation");
+  check(dr.ensure(sizeof(int64_t) * p), "device allocation");
+  // the cut threshold is gr->e_size / num_nodes in both modes (csr_partition.c:9)
+  check(gb_edge_balanced_parts_ex(V, type == GB_LEGACY_RECV ? rrp.as<int32_t>() : srp.as<int32_t>(),
+                                  srp.as<int32_t>(), rrp.as<int32_t>(), p, dd.as<int64_t>(),
+                                  ds.as<int64_t>(), dr.as<int64_t>(), nullptr),
+        "comp_distrib_csr");
+  hipcheck(hipMemcpy(distrib, dd.p, sizeof(int64_t) * p, hipMemcpyDeviceToHost), "download");
+  hipcheck(hipMemcpy(send_size, ds.p, sizeof(int64_t) * p, hipMemcpyDeviceToHost), "download");
+  hipcheck(hipMemcpy(recv_size, dr.p, sizeof(int64_t) * p, hipMemcpyDeviceToHost), "download");
+}
+
+void init_node_id(t_csr *graph, lint *distrib, sint nparts) {
+  const int64_t E = graph->e_size;
+  if (E == 0) return;
+  if (nparts <= 0) fatal("init_node_id: no parts");
+  std::vector<int32_t> col(E);
+  for (int64_t e = 0; e < E; e++) col[e] = (int32_t)graph->edge_idx[e];
+  DevBuf dc, dd, dout;
+  upload(dc, col);
+  std::vector<int64_t> dist(distrib, distrib + nparts);
+  upload(dd, dist);
+  check(dout.ensure(sizeof(int64_t) * E), "device allocation");
+  check(gb_node_ids(E, dc.as<int32_t>(), dd.as<int64_t>(), nparts, dout.as<int64_t>(), nullptr),
+        "init_node_id");
+  std::vector<int64_t> out(E);
+  hipcheck(hipMemcpy(out.data(), dout.p, sizeof(int64_t) * E, hipMemcpyDeviceToHost), "download");
+  for (int64_t e = 0; e < E; e++) graph->edge_info[e].node_id = out[e];
+}
+
+lint bin_search(lint *distrib, lint val, lint start, lint end) {
+  // utils/utils.c:222-239
+  for (;;) {
+    lint mid = start + (end - start) / 2;
+    if (start == end) return start;
+    if (start == end - 1) return val <= distrib[start] ? start : end;
+    if (val <= distrib[mid] && val > distrib[mid - 1]) return mid;
+    if (val > distrib[mid]) {
+      start = mid;
+      continue;
+    }
+    if (val <= distrib[mid - 1]) {
+      end = mid;
+      continue;
+    }
+    return -1;
+  }
+}
+
+}  // extern "C"

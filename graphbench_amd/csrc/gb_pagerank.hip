@@ -582,12 +582,12 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
   return GB_OK;
 }
 
-extern "C" gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t max_iter,
-                                     double *d_x, int32_t *iterations, void *stream) {
+gb_status gb_pagerank_ref_impl(gb_graph g, const double *d_recip, int32_t max_iter, double *d_x,
+                               int32_t *iterations, hipStream_t st, gb_pr_ref_hook hook,
+                               void *ctx) {
   if (!g || !d_recip || !d_x) return gb_fail(GB_ERR_INVALID, "gb_pagerank_ref: null argument");
   if (!g->rrp || (!g->rcol && g->E > 0))
     return gb_fail(GB_ERR_INVALID, "gb_pagerank_ref: no reverse CSR");
-  hipStream_t st = gb_stream(stream);
   const int32_t V = g->V;
   if (V <= 0) return gb_fail(GB_ERR_INVALID, "gb_pagerank_ref: empty graph");
   DevBuf diff, wb, flag;
@@ -605,6 +605,10 @@ extern "C" gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t 
       break;
     }
     it++;
+    if (hook) {
+      GB_HIP(hipStreamSynchronize(st));
+      hook(it, wb.as<double>(), ctx);  // pagerank_mpi.c:823-830 prints wb here
+    }
     GB_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(pr_ref_gather_kernel, dim3(blocks), dim3(256), 0, st, V, g->rrp, g->rcol,
                        diff.as<double>(), wb.as<double>(), pj);
@@ -618,4 +622,10 @@ extern "C" gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t 
   GB_HIP(hipGetLastError());
   if (iterations) *iterations = it;
   return it < 0 ? GB_ERR_NOT_CONVERGED : GB_OK;
+}
+
+extern "C" gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t max_iter,
+                                     double *d_x, int32_t *iterations, void *stream) {
+  return gb_pagerank_ref_impl(g, d_recip, max_iter, d_x, iterations, gb_stream(stream), nullptr,
+                              nullptr);
 }

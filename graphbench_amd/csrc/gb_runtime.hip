@@ -240,17 +240,28 @@ extern "C" gb_status gb_merge_path_split(int32_t V, const int32_t *d_row_ptr, in
 // the vertex after the previous cut; it is cut at the first vertex i where the
 // part's degree sum exceeds q = floor(E/p).  With prefix sums this is one
 // binary search per part; the cut chain is sequential, so one thread walks it.
-__global__ void edge_balanced_parts_kernel(int32_t V, const int32_t *__restrict__ rp, int32_t p,
-                                           int64_t *__restrict__ distrib) {
+// Vertices after the last cut form part ncuts; later parts stay empty, and
+// distrib[p-1] is V-1 (csr_partition.c:29).
+__global__ void edge_balanced_parts_kernel(int32_t V, const int32_t *__restrict__ rp,
+                                           const int32_t *__restrict__ srp,
+                                           const int32_t *__restrict__ rrp, int32_t p,
+                                           int64_t *__restrict__ distrib,
+                                           int64_t *__restrict__ ssize,
+                                           int64_t *__restrict__ rsize) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const int64_t E = rp[V];
   const int64_t q = E / p;
-  for (int32_t k = 0; k < p; k++) distrib[k] = 0;
+  for (int32_t k = 0; k < p; k++) {
+    distrib[k] = 0;
+    if (ssize) ssize[k] = 0;
+    if (rsize) rsize[k] = 0;
+  }
   int64_t start = 0;  // first vertex of the current part
-  for (int32_t j = 0; j < p; j++) {
+  int32_t j = 0;
+  while (j < p && start < V) {
     // smallest i >= start with rp[i+1] - rp[start] > q
-    int64_t base = rp[start];
-    int64_t lo = start, hi = V;  // answer in [start, V) or V = none
+    const int64_t base = rp[start];
+    int64_t lo = start, hi = V;
     while (lo < hi) {
       int64_t mid = (lo + hi) >> 1;
       if ((int64_t)rp[mid + 1] - base > q)
@@ -258,20 +269,70 @@ __global__ void edge_balanced_parts_kernel(int32_t V, const int32_t *__restrict_
       else
         lo = mid + 1;
     }
-    if (lo >= V) break;
+    const int64_t end = lo < V ? lo : V - 1;  // last vertex of part j
+    if (ssize) ssize[j] = (int64_t)srp[end + 1] - srp[start];
+    if (rsize) rsize[j] = (int64_t)rrp[end + 1] - rrp[start];
+    if (lo >= V) break;  // no cut: part j runs to the last vertex
     distrib[j] = lo;
     start = lo + 1;
-    if (start >= V) break;
+    j++;
   }
   distrib[p - 1] = (int64_t)V - 1;
 }
 
+gb_status gb_edge_balanced_parts_ex(int32_t V, const int32_t *d_cut_rp, const int32_t *d_send_rp,
+                                    const int32_t *d_recv_rp, int32_t p, int64_t *d_distrib,
+                                    int64_t *d_send_size, int64_t *d_recv_size, hipStream_t st) {
+  if (V <= 0 || p <= 0 || !d_cut_rp || !d_distrib)
+    return gb_fail(GB_ERR_INVALID, "gb_edge_balanced_parts: bad arguments");
+  hipLaunchKernelGGL(edge_balanced_parts_kernel, dim3(1), dim3(64), 0, st, V, d_cut_rp,
+                     d_send_rp ? d_send_rp : d_cut_rp, d_recv_rp ? d_recv_rp : d_cut_rp, p,
+                     d_distrib, d_send_size, d_recv_size);
+  GB_HIP(hipGetLastError());
+  return GB_OK;
+}
+
 extern "C" gb_status gb_edge_balanced_parts(int32_t V, const int32_t *d_row_ptr, int32_t p,
                                             int64_t *d_distrib, void *stream) {
-  if (V <= 0 || p <= 0 || !d_row_ptr || !d_distrib)
-    return gb_fail(GB_ERR_INVALID, "gb_edge_balanced_parts: bad arguments");
-  hipLaunchKernelGGL(edge_balanced_parts_kernel, dim3(1), dim3(64), 0, gb_stream(stream), V,
-                     d_row_ptr, p, d_distrib);
+  return gb_edge_balanced_parts_ex(V, d_row_ptr, nullptr, nullptr, p, d_distrib, nullptr, nullptr,
+                                   gb_stream(stream));
+}
+
+// bin_search (utils/utils.c:222-239), same branch structure, iterative.
+__device__ __host__ inline int64_t gb_bin_search(const int64_t *distrib, int64_t val,
+                                                 int64_t start, int64_t end) {
+  for (;;) {
+    const int64_t mid = start + (end - start) / 2;
+    if (start == end) return start;
+    if (start == end - 1) return val <= distrib[start] ? start : end;
+    if (val <= distrib[mid] && val > distrib[mid - 1]) return mid;
+    if (val > distrib[mid]) {
+      start = mid;
+      continue;
+    }
+    if (val <= distrib[mid - 1]) {
+      end = mid;
+      continue;
+    }
+    return -1;
+  }
+}
+
+__global__ void node_ids_kernel(int64_t E, const int32_t *__restrict__ col,
+                                const int64_t *__restrict__ distrib, int32_t p,
+                                int64_t *__restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = gb_bin_search(distrib, col[e], 0, p - 1);
+}
+
+gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib, int32_t p,
+                      int64_t *d_out, hipStream_t st) {
+  if (E == 0) return GB_OK;
+  int64_t blocks = (E + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(node_ids_kernel, dim3((unsigned)blocks), dim3(256), 0, st, E, d_col,
+                     d_distrib, p, d_out);
   GB_HIP(hipGetLastError());
   return GB_OK;
 }

@@ -61,6 +61,7 @@ def _declare(L):
         "orc_bin_search": (_i64, [_p, _i64, _i64, _i64]),
         "orc_merge_path_split": (None, [_i32, _p, _i64, _i64, _p, _p]),
         "orc_separate_csr": (None, [_i32, _p, _p, _p, _u32, _p, _p, _p, _p, _p, _p]),
+        "orc_separate_csr_f64": (None, [_i32, _p, _p, _p, _f64, _p, _p, _p, _p, _p, _p]),
         "orc_mlp_loss_grad": (_f64, [_i32, _i32, _i32, _i32] + [_p] * 14),
         "orc_set_threads": (None, [_i32]),
         "orc_get_threads": (_i32, []),
@@ -191,6 +192,8 @@ def merge_path_split(rp, items, ntiles):
 
 
 def separate_csr(rp, col, w, light_max):
+    if np.asarray(w).dtype.kind == "f":
+        return _separate_f64(rp, col, w, light_max)
     rp, col, w = _c(rp, np.int32), _c(col, np.int32), _c(w, np.uint32)
     V = len(rp) - 1
     E = len(col)
@@ -202,6 +205,22 @@ def separate_csr(rp, col, w, light_max):
     hw = np.empty(E, np.uint32)
     lib().orc_separate_csr(V, _ptr(rp), _ptr(col), _ptr(w), int(light_max), _ptr(lrp),
                            _ptr(lcol), _ptr(lw), _ptr(hrp), _ptr(hcol), _ptr(hw))
+    nl, nh = int(lrp[-1]), int(hrp[-1])
+    return (lrp, lcol[:nl], lw[:nl]), (hrp, hcol[:nh], hw[:nh])
+
+
+def _separate_f64(rp, col, w, delta):
+    rp, col, w = _c(rp, np.int32), _c(col, np.int32), _c(w, np.float64)
+    V = len(rp) - 1
+    E = len(col)
+    lrp = np.empty(V + 1, np.int32)
+    hrp = np.empty(V + 1, np.int32)
+    lcol = np.empty(max(E, 1), np.int32)
+    hcol = np.empty(max(E, 1), np.int32)
+    lw = np.empty(max(E, 1), np.float64)
+    hw = np.empty(max(E, 1), np.float64)
+    lib().orc_separate_csr_f64(V, _ptr(rp), _ptr(col), _ptr(w), float(delta), _ptr(lrp),
+                               _ptr(lcol), _ptr(lw), _ptr(hrp), _ptr(hcol), _ptr(hw))
     nl, nh = int(lrp[-1]), int(hrp[-1])
     return (lrp, lcol[:nl], lw[:nl]), (hrp, hcol[:nh], hw[:nh])
 

@@ -678,6 +678,26 @@ void orc_separate_csr(int32_t V, const int32_t *rp, const int32_t *col,
   }
 }
 
+void orc_separate_csr_f64(int32_t V, const int32_t *rp, const int32_t *col,
+                          const double *w, double delta, int32_t *lrp, int32_t *lcol,
+                          double *lw, int32_t *hrp, int32_t *hcol, double *hw) {
+  int32_t nl = 0, nh = 0;
+  lrp[0] = hrp[0] = 0;
+  for (int32_t v = 0; v < V; v++) {
+    for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
+      if (w[e] <= delta) { /* csr.c:85, 118 */
+        lcol[nl] = col[e];
+        lw[nl++] = w[e];
+      } else {
+        hcol[nh] = col[e];
+        hw[nh++] = w[e];
+      }
+    }
+    lrp[v + 1] = nl;
+    hrp[v + 1] = nh;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 double orc_mlp_loss_grad(int32_t n, int32_t din, int32_t h, int32_t dout,
                          const float *X, const float *T, const float *W1,

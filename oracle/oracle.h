@@ -112,6 +112,11 @@ void orc_separate_csr(int32_t V, const int32_t *rp, const int32_t *col,
                       int32_t *lcol, uint32_t *lw, int32_t *hrp, int32_t *hcol,
                       uint32_t *hw);
 
+/* Same on f64 weights (the reference's edge_weight type). */
+void orc_separate_csr_f64(int32_t V, const int32_t *rp, const int32_t *col,
+                          const double *w, double delta, int32_t *lrp, int32_t *lcol,
+                          double *lw, int32_t *hrp, int32_t *hcol, double *hw);
+
 /* ---- learned-part reference ------------------------------------------------*/
 /* FP64 forward + backward of the 2-hidden-layer ReLU MLP used for the learned
  * delta / warm start: y = W3 relu(W2 relu(W1 x + b1) + b2) + b3, loss =

@@ -1,0 +1,272 @@
+"""The reference's own entry points (include/gb_legacy.h) through the C-ABI.
+
+CPU part: the readers (host I/O, no GPU).  GPU part (marked gpu): run_sspath_arr,
+run_sspath, run_pagerank_csr_mpi, run_bc, separate_csr, comp_distrib_csr,
+init_node_id against the oracle and the reference's golden vectors.
+"""
+import ctypes as C
+import json
+import os
+import struct
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)["cases"]
+
+
+def legacy():
+    from graphbench_amd import legacy as L
+    return L
+
+
+class CaptureStdout:
+    """Captures what C code writes to fd 1 (printf) in this process."""
+
+    def __enter__(self):
+        import sys
+        sys.stdout.flush()
+        self.tmp = tempfile.TemporaryFile(mode="w+b")
+        self.saved = os.dup(1)
+        os.dup2(self.tmp.fileno(), 1)
+        return self
+
+    def __exit__(self, *exc):
+        libc = C.CDLL(None)
+        libc.fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        self.tmp.seek(0)
+        self.text = self.tmp.read().decode()
+        self.tmp.close()
+
+
+def write_text_graph(path, V, u, v, w=None):
+    with open(path, "w") as f:
+        f.write(f"p sp {V} {len(u)} 1 {V}\n")
+        for i, (a, b) in enumerate(zip(u, v)):
+            f.write(f"a {a + 1} {b + 1} {1 if w is None else w[i]}\n")
+
+
+# ------------------------------------------------------------------ CPU: readers
+
+def test_text_readers_send_and_recv(tmp_path):
+    L = legacy()
+    lib = L.lib()
+    V = 6
+    u = [0, 0, 1, 3, 3, 4, 5, 5]
+    v = [1, 2, 2, 2, 4, 5, 3, 0]
+    write_text_graph(tmp_path / "g.txt", V, u, v)
+    d = (str(tmp_path) + "/").encode()
+    g = L.TCsr()
+    lib.scan_csr_idx(C.byref(g), b"g.txt", d, L.SEND)
+    lib.read_graph_csr(C.byref(g), b"g.txt", d, L.SEND)
+    rp, col, w = L.csr_arrays(g)
+    assert rp.tolist() == [0, 2, 3, 3, 5, 6, 8]
+    assert col.tolist() == [1, 2, 2, 2, 4, 5, 3, 0]
+    # graph/csr.c:42: weight of the k-th edge read = v_to / V (file order)
+    assert np.allclose(w, (np.array(v) + 1) / V)
+    deg = [g.vet_info[i].vet_deg for i in range(V)]
+    assert deg == [2, 1, 0, 2, 1, 2]
+    assert g.vet_info[0].recip == 0.5 and g.vet_info[2].recip == float("inf")
+    assert g.vet_info[0].weight == 1.0
+    r = L.TCsr()
+    lib.scan_csr_idx(C.byref(r), b"g.txt", d, L.RECV)
+    lib.read_graph_csr(C.byref(r), b"g.txt", d, L.RECV)
+    rrp, rcol, _ = L.csr_arrays(r)
+    assert rrp.tolist() == [0, 1, 2, 5, 6, 7, 8]
+    assert rcol.tolist() == [5, 0, 0, 1, 3, 5, 3, 4]
+
+
+def test_binary_reader(tmp_path):
+    L = legacy()
+    V, off = 5, 1
+    u = [1, 1, 2, 4, 5]
+    v = [2, 3, 3, 5, 1]
+    with open(tmp_path / "g.bin", "wb") as f:
+        f.write(struct.pack("4q", V, len(u), off, V))
+        for a, b in zip(u, v):
+            f.write(struct.pack("2q", a, b))
+    g = L.TCsr()
+    L.lib().read_csr_bin(C.byref(g), b"g.bin", (str(tmp_path) + "/").encode(), L.SEND, 0)
+    rp, col, w = L.csr_arrays(g)
+    assert rp.tolist() == [0, 2, 3, 3, 4, 5]
+    assert col.tolist() == [1, 2, 2, 4, 0]
+    assert w.tolist() == [0.1, 0.2, 0.1, 0.2, 0.1]  # csr.c:293-298
+    assert all(g.vet_info[i].weight == float("inf") for i in range(V))
+
+
+def test_bin_search_host():
+    L = legacy()
+    lib = L.lib()
+    for case in load("distrib.json")[:6]:
+        d = np.array(case["distrib"], np.int64)
+        p = (C.c_longlong * len(d))(*d.tolist())
+        parts = [lib.bin_search(p, vv, 0, len(d) - 1) for vv in range(case["V"])]
+        assert parts == case["part"]
+
+
+# ------------------------------------------------------------------ GPU
+
+def _csr_sorted(V, src, dst, w):
+    src, dst, w = np.asarray(src), np.asarray(dst), np.asarray(w, np.float64)
+    o = np.argsort(src, kind="stable")
+    rp = np.zeros(V + 1, np.int64)
+    np.add.at(rp, src + 1, 1)
+    return np.cumsum(rp), dst[o], w[o]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("frac", [False, True])
+def test_run_sspath_arr(gpu, frac):
+    L = legacy()
+    rng = np.random.default_rng(3)
+    V, E = 500, 4000
+    s, d = rng.integers(0, V, E), rng.integers(0, V, E)
+    m = s != d
+    w = rng.random(m.sum()) + 0.01 if frac else rng.integers(1, 100, m.sum()).astype(float)
+    rp, col, w = _csr_sorted(V, s[m], d[m], w)
+    g = L.LegacyGraph(rp, col, w)
+    heads, arr = L.new_pred_lists(V)
+    for src in (0, 123):
+        L.lib().run_sspath_arr(C.byref(g.csr), arr, src, 0.4 if frac else 8.0)
+        ref = oracle.dijkstra(rp.astype(np.int32), col, w, src)
+        assert np.array_equal(g.weights_out(), ref)
+
+
+@pytest.mark.gpu
+def test_run_sspath_distances_and_predecessors(gpu):
+    """Against the reference's golden run_sspath output: distances where its
+    first-parallel-edge defect cannot fire, tight-edge sets (deduplicated)."""
+    L = legacy()
+    for case in load("sssp_list.json")[:15]:
+        V = case["V"]
+        rp, col, w = _csr_sorted(V, case["src"], case["dst"], case["w"])
+        g = L.LegacyGraph(rp, col, w)
+        heads, arr = L.new_pred_lists(V)
+        L.lib().run_sspath(C.byref(g.csr), arr, case["source"], case["delta"])
+        dist = g.weights_out()
+        ref_true = oracle.dijkstra(rp.astype(np.int32), col, w, case["source"])
+        assert np.array_equal(dist, ref_true)
+        pairs = set(zip(np.repeat(np.arange(V), np.diff(rp)).tolist(), col.tolist()))
+        if len(pairs) == len(col):  # no parallel edges: reference distances are exact
+            assert np.array_equal(dist, np.array(case["dist"]))
+            for v in range(V):
+                if v != case["source"]:
+                    assert sorted(L.list_values(heads[v])) == sorted(set(case["pred_raw"][v]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", load("pagerank.json"), ids=lambda c: f"V{c['V']}")
+def test_run_pagerank_csr_mpi_vs_reference(gpu, case, tmp_path):
+    L = legacy()
+    d = str(tmp_path) + "/"
+    os.makedirs(d + "datex")
+    os.makedirs(d + "parex")
+    with open(d + "datex/g.txt_rc_0_1", "w") as f:
+        f.write(case["rc_file"])
+    with open(d + "parex/g.txt_recip_0_1", "w") as f:
+        f.write(case["recip_file"])
+    lib = L.lib()
+    C.c_char_p.in_dll(lib, "nodes_extension").value = b"1"
+    with CaptureStdout() as cap:
+        lib.run_pagerank_csr_mpi(b"g.txt", d.encode())
+    out = cap.text
+    assert out.splitlines()[0] == case["recip0_line"]
+    it = int(out.split("iterations taken:")[1].split()[0])
+    assert it == case["iterations"]
+    fin = out.split("Final pagerank values:")[1].splitlines()[1].split()
+    vals = [float(t.split("|")[1]) for t in fin]
+    ref = case["final"]
+    assert np.max(np.abs(np.array(vals[:len(ref)]) - np.array(ref))) <= 1.01e-6
+    iters = [ln for ln in out.splitlines() if ln.startswith("ITER")]
+    assert len(iters) == case["iterations"]
+    n = min(10, case["V"])
+    for k, ln in enumerate(iters):
+        wb = [float(t.split("|")[0]) for t in ln.split()[2:]]
+        assert np.max(np.abs(np.array(wb[:n]) - np.array(case["iter_wb"][k][:n]))) <= 1.01e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", load("bc.json"), ids=lambda c: f"V{c['V']}E{len(c['src'])}")
+def test_run_bc_vs_reference_and_oracle(gpu, case, tmp_path):
+    L = legacy()
+    V = case["V"]
+    u, v = np.array(case["src"]), np.array(case["dst"])
+    write_text_graph(tmp_path / "g.txt", V, u, v)
+    with CaptureStdout() as cap:
+        L.lib().run_bc(b"g.txt", (str(tmp_path) + "/").encode())
+    vals = cap.text.split("betweeness centrality is:")[1].split()
+    bc = np.array([float(t.split(":")[1]) for t in vals])
+    w = (v + 1) / V
+    rp, col, ww = _csr_sorted(V, u, v, w)
+    src = np.repeat(np.arange(V), np.diff(rp))
+    o = np.argsort(col, kind="stable")
+    rrp = np.zeros(V + 1, np.int64)
+    np.add.at(rrp, col + 1, 1)
+    rrp = np.cumsum(rrp)
+    ref = oracle.brandes_weighted(rp.astype(np.int32), col, ww, rrp.astype(np.int32), src[o],
+                                  ww[o], np.arange(V))
+    assert np.allclose(bc, ref, atol=1e-6)  # printed with %lf
+    if V <= 6:
+        assert np.allclose(bc, case["bc"], atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_separate_csr_vs_reference(gpu):
+    L = legacy()
+    for case in load("separate.json"):
+        V = case["V"]
+        rp, col, w = _csr_sorted(V, case["src"], case["dst"], case["w"])
+        g = L.LegacyGraph(rp, col, w)
+        gl, gh = L.TCsr(), L.TCsr()
+        L.lib().separate_csr(C.byref(g.csr), C.byref(gl), C.byref(gh), case["delta"])
+        lrp, lcol, lw = L.csr_arrays(gl)
+        hrp, hcol, hw = L.csr_arrays(gh)
+        assert lrp[1:].tolist() == case["light_idx"] and hrp[1:].tolist() == case["heavy_idx"]
+        assert lcol.tolist() == case["light_col"] and hcol.tolist() == case["heavy_col"]
+        assert lw.tolist() == case["light_w"] and hw.tolist() == case["heavy_w"]
+
+
+@pytest.mark.gpu
+def test_comp_distrib_and_init_node_id_vs_reference(gpu):
+    L = legacy()
+    lib = L.lib()
+    for case in load("distrib.json"):
+        V, p = case["V"], case["p"]
+        C.c_int.in_dll(lib, "num_nodes").value = p
+        gs, gr = L.TCsr(), L.TCsr()
+        vs = (L.TVi * V)()
+        vr = (L.TVi * V)()
+        for i in range(V):
+            vs[i].vet_deg = case["send_deg"][i]
+            vr[i].vet_deg = case["recv_deg"][i]
+        gs.vet_info = C.cast(vs, C.POINTER(L.TVi))
+        gr.vet_info = C.cast(vr, C.POINTER(L.TVi))
+        gs.v_size = gr.v_size = V
+        gs.e_size = gr.e_size = int(sum(case["send_deg"]))
+        dist = (C.c_longlong * p)()
+        ss = (C.c_longlong * p)()
+        rs = (C.c_longlong * p)()
+        lib.comp_distrib_csr(C.byref(gs), C.byref(gr), dist, ss, rs, case["type"])
+        assert list(dist) == case["distrib"], case
+        assert list(ss) == case["send"] and list(rs) == case["recv"]
+        # init_node_id: owner of every edge target
+        E = V
+        edge_idx = np.arange(V, dtype=np.int64)
+        einfo = (L.TE * E)()
+        g = L.TCsr()
+        g.edge_idx = edge_idx.ctypes.data_as(C.POINTER(C.c_longlong))
+        g.edge_info = C.cast(einfo, C.POINTER(L.TE))
+        g.e_size = E
+        lib.init_node_id(C.byref(g), dist, p)
+        assert [einfo[e].node_id for e in range(E)] == case["part"]
+    C.c_int.in_dll(lib, "num_nodes").value = 1
