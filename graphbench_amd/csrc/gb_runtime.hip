@@ -124,7 +124,11 @@ extern "C" gb_status gb_graph_wrap(gb_graph *out, int32_t V, int64_t E,
                                    const int32_t *d_row_ptr, const int32_t *d_col,
                                    const uint32_t *d_w, const int32_t *d_rrow_ptr,
                                    const int32_t *d_rcol, const uint32_t *d_rw) {
-  if (!out || V < 0 || E < 0 || E >= (1ll << 31) || (!d_row_ptr && V > 0) || (!d_col && E > 0))
+  // the forward CSR may be absent when only pull kernels (reverse CSR) are used
+  const bool fwd = d_row_ptr && (d_col || E == 0);
+  const bool rev = d_rrow_ptr && (d_rcol || E == 0);
+  if (!out || V < 0 || E < 0 || E >= (1ll << 31) || (V > 0 && !fwd && !rev) ||
+      (d_row_ptr && !d_col && E > 0) || (d_rrow_ptr && !d_rcol && E > 0))
     return gb_fail(GB_ERR_INVALID, "gb_graph_wrap: bad graph (V=%d E=%lld)", V, (long long)E);
   if (g_device < 0) GB_TRY(gb_init(0));
   gb_graph g = new gb_graph_s();

@@ -28,25 +28,27 @@ def legacy():
     return L
 
 
-class CaptureStdout:
-    """Captures what C code writes to fd 1 (printf) in this process."""
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
-    def __enter__(self):
-        import sys
-        sys.stdout.flush()
-        self.tmp = tempfile.TemporaryFile(mode="w+b")
-        self.saved = os.dup(1)
-        os.dup2(self.tmp.fileno(), 1)
-        return self
 
-    def __exit__(self, *exc):
-        libc = C.CDLL(None)
-        libc.fflush(None)
-        os.dup2(self.saved, 1)
-        os.close(self.saved)
-        self.tmp.seek(0)
-        self.text = self.tmp.read().decode()
-        self.tmp.close()
+def run_printing_entry(func, file, dirname, nodes_extension=None):
+    """Runs run_pagerank_csr_mpi / run_bc in a child process: they print their
+    results (like the reference) and exit(123) on fatal errors, which must not
+    end the test runner.  Returns (returncode, stdout)."""
+    import subprocess
+    import sys
+    code = (
+        "import ctypes as C, sys\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "from graphbench_amd import legacy as L\n"
+        "lib = L.lib()\n"
+        + (f"ext = C.create_string_buffer({nodes_extension!r})\n"
+           "C.c_void_p.in_dll(lib, 'nodes_extension').value = C.addressof(ext)\n"
+           if nodes_extension else "")
+        + f"lib.{func}({file!r}, {dirname!r})\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    print(f"[case] {func} rc={r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}", flush=True)
+    return r.returncode, r.stdout
 
 
 def write_text_graph(path, V, u, v, w=None):
@@ -175,11 +177,8 @@ def test_run_pagerank_csr_mpi_vs_reference(gpu, case, tmp_path):
         f.write(case["rc_file"])
     with open(d + "parex/g.txt_recip_0_1", "w") as f:
         f.write(case["recip_file"])
-    lib = L.lib()
-    C.c_char_p.in_dll(lib, "nodes_extension").value = b"1"
-    with CaptureStdout() as cap:
-        lib.run_pagerank_csr_mpi(b"g.txt", d.encode())
-    out = cap.text
+    rc, out = run_printing_entry("run_pagerank_csr_mpi", b"g.txt", d.encode(), b"1")
+    assert rc == 0
     assert out.splitlines()[0] == case["recip0_line"]
     it = int(out.split("iterations taken:")[1].split()[0])
     assert it == case["iterations"]
@@ -202,9 +201,9 @@ def test_run_bc_vs_reference_and_oracle(gpu, case, tmp_path):
     V = case["V"]
     u, v = np.array(case["src"]), np.array(case["dst"])
     write_text_graph(tmp_path / "g.txt", V, u, v)
-    with CaptureStdout() as cap:
-        L.lib().run_bc(b"g.txt", (str(tmp_path) + "/").encode())
-    vals = cap.text.split("betweeness centrality is:")[1].split()
+    rc, out = run_printing_entry("run_bc", b"g.txt", (str(tmp_path) + "/").encode())
+    assert rc == 0
+    vals = out.split("betweeness centrality is:")[1].split()
     bc = np.array([float(t.split(":")[1]) for t in vals])
     w = (v + 1) / V
     rp, col, ww = _csr_sorted(V, u, v, w)
