@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--only", choices=["sssp", "bc"], default=None,
+                    help="time one secondary workload alone (diagnostics; no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -159,6 +161,10 @@ def main():
     import oracle
 
     gb.init(0)
+    if args.only:
+        fn = {"sssp": run_sssp_c1, "bc": run_bc_c3}[args.only]
+        print(json.dumps(fn(gb, oracle, torch, np)), flush=True)
+        return
     log("pagerank R-MAT 22")
     h, r, pr, x = run_pagerank(gb, torch, np, args)
     log(f"pagerank: {pr['step_s']*1e3:.2f} ms/step, {pr['gteps']:.1f} GTEPS, gather "

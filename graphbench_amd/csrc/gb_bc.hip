@@ -390,39 +390,6 @@ gb_status bc_run(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t bat
 
 void gb_bc_release() { g_bcw.release(); }
 
-extern "C" gb_status gb_bfs(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t *d_level,
-                            void *stream) {
-  if (!g || (!h_sources && nsrc > 0) || nsrc < 0 || !d_level)
-    return gb_fail(GB_ERR_INVALID, "gb_bfs: null argument");
-  const int32_t V = g->V;
-  for (int32_t i = 0; i < nsrc; i++)
-    if (h_sources[i] < 0 || h_sources[i] >= V)
-      return gb_fail(GB_ERR_INVALID, "gb_bfs: source %d out of range", h_sources[i]);
-  if (nsrc > 65535) return gb_fail(GB_ERR_INVALID, "gb_bfs: at most 65535 sources per call");
-  hipStream_t st = gb_stream(stream);
-  BcWork &w = g_bcw;
-  const int64_t n = (int64_t)nsrc * V;
-  if (n == 0) return GB_OK;
-  GB_TRY(w.order.ensure(sizeof(uint64_t) * (n + 1)));
-  GB_TRY(w.src.ensure(sizeof(int32_t) * nsrc));
-  GB_TRY(w.cnt.ensure(sizeof(unsigned long long)));
-  GB_HIP(hipMemcpyAsync(w.src.p, h_sources, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(bc_fill_kernel, dim3(fill_grid(n)), dim3(256), 0, st, n, d_level,
-                     (double *)nullptr, (double *)nullptr);
-  hipLaunchKernelGGL(bc_seed_kernel, dim3((nsrc + 255) / 256), dim3(256), 0, st, V, nsrc,
-                     w.src.as<int32_t>(), d_level, (double *)nullptr, w.order.as<uint64_t>());
-  GB_HIP(hipGetLastError());
-  std::vector<uint64_t> off;
-  gb_status s = forward_levels(g, nsrc, false, st, off, false, d_level);
-  GB_HIP(hipStreamSynchronize(st));
-  return s;
-}
-
-extern "C" gb_status gb_bc(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t batch,
-                           double *d_bc, gb_bc_stats *st, void *stream) {
-  return bc_run(g, h_sources, nsrc, batch, d_bc, st, stream, false);
-}
-
 extern "C" gb_status gb_bc_weighted(gb_graph g, const int32_t *h_sources, int32_t nsrc,
                                     int32_t batch, double *d_bc, gb_bc_stats *st, void *stream) {
   return bc_run(g, h_sources, nsrc, batch, d_bc, st, stream, true);

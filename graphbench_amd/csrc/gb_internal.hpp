@@ -110,5 +110,6 @@ gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib,
 void gb_pr_release();
 void gb_sssp_release();
 void gb_bc_release();
+void gb_msbc_release();
 
 #endif

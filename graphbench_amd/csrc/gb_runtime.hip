@@ -92,6 +92,7 @@ void gb_release_workspaces(void) {
   gb_pr_release();
   gb_sssp_release();
   gb_bc_release();
+  gb_msbc_release();
 }
 
 void gb_set_mem_cap(uint64_t bytes) {

@@ -164,6 +164,32 @@ def test_bc_rmat_undirected(gpu, batch):
     bc = gb.bc(g, srcs, batch=batch).cpu().numpy()
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
     assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+    # lanes sum in CSR order and sources accumulate in order: identical bits
+    assert np.array_equal(bc, ref)
+
+
+def test_bc_many_sources_and_duplicates(gpu):
+    """More than 64 sources (several lane groups), a repeated source, batch split."""
+    gb = gpu
+    h = gb.gen_rmat(11, 8, seed=12, weights="none", symmetric=True)
+    srcs = list(gb.pick_sources(h, 150, 3)) + [int(gb.pick_sources(h, 1, 3)[0])]
+    _say(f"bc many sources n={len(srcs)}")
+    g = gb.Graph.from_host(h, need_reverse=False)
+    ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
+    for batch in (0, 64, 100):
+        bc = gb.bc(g, srcs, batch=batch).cpu().numpy()
+        assert np.array_equal(bc, ref), batch
+
+
+def test_bfs_many_sources(gpu):
+    gb = gpu
+    h = gb.gen_rmat(12, 8, seed=4, weights="none")
+    srcs = list(gb.pick_sources(h, 130, 9)) + [0, 0]
+    _say("bfs many sources")
+    g = gb.Graph.from_host(h)
+    lv = gb.bfs(g, srcs).cpu().numpy()
+    for i, s in enumerate(srcs):
+        assert np.array_equal(lv[i], oracle.bfs_levels(h.row_ptr, h.col, int(s))), i
 
 
 def test_bc_directed(gpu):
@@ -176,6 +202,7 @@ def test_bc_directed(gpu):
     bc = gb.bc(g, srcs).cpu().numpy()
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs, r.row_ptr, r.col)
     assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+    assert np.array_equal(bc, ref)
 
 
 def test_bc_weighted(gpu):
