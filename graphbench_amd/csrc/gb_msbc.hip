@@ -32,17 +32,118 @@ namespace {
 
 constexpr int MB = 256;  // threads per block (4 waves)
 
-__global__ void mbc_cand_init_kernel(int32_t V, const int32_t *__restrict__ rrp,
+__global__ void mbc_cand_init_kernel(int32_t V, const int32_t *__restrict__ rrp, int32_t hub,
                                      int32_t *__restrict__ cand, unsigned long long *count) {
-  // vertices with at least one in-edge (others can never be reached)
+  // vertices with 1..hub in-edges (none: never reached; more: the hub kernels)
   const int lane = gb_lane();
   for (int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; b0 < V;
        b0 += (int64_t)gridDim.x * blockDim.x) {
     const int64_t v = b0 + lane;
-    const bool keep = v < V && rrp[v + 1] > rrp[v];
+    const int32_t deg = v < V ? rrp[v + 1] - rrp[v] : 0;
+    const bool keep = deg > 0 && deg <= hub;
     const uint64_t pos = gb_wave_reserve(count, keep ? 1u : 0u);
     if (keep) cand[pos] = (int32_t)v;
   }
+}
+
+// vertices with more than `hub` edges in a CSR, and the largest such degree
+__global__ void mbc_hub_list_kernel(int32_t V, const int32_t *__restrict__ rp, int32_t hub,
+                                    int32_t *__restrict__ list, unsigned long long *count,
+                                    unsigned int *maxdeg) {
+  const int lane = gb_lane();
+  for (int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; b0 < V;
+       b0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = b0 + lane;
+    const int32_t deg = v < V ? rp[v + 1] - rp[v] : 0;
+    const bool keep = deg > hub;
+    const uint64_t pos = gb_wave_reserve(count, keep ? 1u : 0u);
+    if (keep) {
+      list[pos] = (int32_t)v;
+      atomicMax(maxdeg, (unsigned int)deg);
+    }
+  }
+}
+
+// Pull over in-edges [beg, end) of one vertex: lane s adds sigma[u][s] for every
+// in-edge u whose level-d mask has bit s within `need`, in CSR order, 8 loads
+// in flight.  Without SIGMA only the union of matching bits is collected.
+template <bool SIGMA>
+__device__ __forceinline__ void pull_in(int lane, int32_t beg, int32_t end,
+                                        const int32_t *__restrict__ rcol,
+                                        const unsigned long long *__restrict__ fmd, uint64_t need,
+                                        const double *__restrict__ sigma, int64_t gV, double &acc,
+                                        uint64_t &any) {
+  for (int32_t e0 = beg; e0 < end; e0 += GB_WAVE) {
+    const int32_t e = e0 + lane;
+    int32_t u = 0;
+    uint64_t m = 0;
+    if (e < end) {
+      u = rcol[e];
+      m = fmd[u] & need;
+    }
+    if (!SIGMA) {
+#pragma unroll
+      for (int k = 32; k >= 1; k >>= 1) m |= gb_shfl_u64(m, lane ^ k);
+      any |= m;
+      continue;
+    }
+    uint64_t hit = __ballot(m != 0);
+    while (hit) {
+      double val[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const bool ok = hit != 0;
+        const int j = ok ? __ffsll((unsigned long long)hit) - 1 : 0;
+        hit &= hit - 1;
+        const int32_t uj = __shfl(u, j, GB_WAVE);
+        const uint64_t mj = ok ? gb_shfl_u64(m, j) : 0ull;
+        val[k] = ((mj >> lane) & 1ull) ? sigma[((int64_t)gV + uj) * 64 + lane] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) acc += val[k];  // +0.0 leaves acc unchanged
+    }
+  }
+}
+
+// Pull over out-edges [beg, end): lane s adds ((1+delta[w][s])*sv)/sigma[w][s]
+// for every successor w one level deeper for source s, in CSR order.
+__device__ __forceinline__ double pull_out(int lane, int32_t beg, int32_t end,
+                                           const int32_t *__restrict__ col,
+                                           const unsigned long long *__restrict__ fmd1,
+                                           uint64_t mine, double sv,
+                                           const double *__restrict__ sigma,
+                                           const double *__restrict__ delta, int64_t gV) {
+  double acc = 0.0;
+  for (int32_t e0 = beg; e0 < end; e0 += GB_WAVE) {
+    const int32_t e = e0 + lane;
+    int32_t w = 0;
+    uint64_t m = 0;
+    if (e < end) {
+      w = col[e];
+      m = fmd1[w] & mine;
+    }
+    uint64_t hit = __ballot(m != 0);
+    while (hit) {
+      double dw[8], sw[8];
+      bool on[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const bool ok = hit != 0;
+        const int j = ok ? __ffsll((unsigned long long)hit) - 1 : 0;
+        hit &= hit - 1;
+        const int32_t wj = __shfl(w, j, GB_WAVE);
+        const uint64_t mj = ok ? gb_shfl_u64(m, j) : 0ull;
+        on[k] = (mj >> lane) & 1ull;
+        const int64_t r = ((int64_t)gV + wj) * 64 + lane;
+        dw[k] = on[k] ? delta[r] : 0.0;
+        sw[k] = on[k] ? sigma[r] : 1.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (on[k]) acc += ((1.0 + dw[k]) * sv) / sw[k];
+    }
+  }
+  return acc;
 }
 
 // copy the shared candidate list to every group
@@ -104,37 +205,7 @@ __global__ __launch_bounds__(MB) void mbc_forward_kernel(
     if (need == 0) continue;  // dropped: every source has reached x
     double acc = 0.0;
     uint64_t any = 0;
-    const int32_t beg = rrp[x], end = rrp[x + 1];
-    for (int32_t e0 = beg; e0 < end; e0 += GB_WAVE) {
-      const int32_t e = e0 + lane;
-      int32_t u = 0;
-      uint64_t m = 0;
-      if (e < end) {
-        u = rcol[e];
-        m = fmd[u] & need;
-      }
-      uint64_t hit = __ballot(m != 0);
-      if (!SIGMA) {
-#pragma unroll
-        for (int k = 32; k >= 1; k >>= 1) m |= gb_shfl_u64(m, lane ^ k);
-        any |= m;
-        continue;
-      }
-      while (hit) {  // matching in-edges, 8 loads in flight, added in CSR order
-        double val[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const bool ok = hit != 0;
-          const int j = ok ? __ffsll((unsigned long long)hit) - 1 : 0;
-          hit &= hit - 1;
-          const int32_t uj = __shfl(u, j, GB_WAVE);
-          const uint64_t mj = ok ? gb_shfl_u64(m, j) : 0ull;
-          val[k] = ((mj >> lane) & 1ull) ? sigma[((int64_t)gV + uj) * 64 + lane] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; k++) acc += val[k];  // +0.0 leaves acc unchanged
-      }
-    }
+    pull_in<SIGMA>(lane, rrp[x], rrp[x + 1], rcol, fmd, need, sigma, gV, acc, any);
     const uint64_t found = SIGMA ? (__ballot(acc > 0.0) & need) : (any & need);
     if (found) {
       if ((found >> lane) & 1ull) {
@@ -156,7 +227,7 @@ __global__ __launch_bounds__(MB) void mbc_forward_kernel(
 }
 
 __global__ __launch_bounds__(MB) void mbc_backward_kernel(
-    int32_t V, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+    int32_t V, int32_t hub, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
     const int32_t *__restrict__ list_d, const unsigned long long *__restrict__ list_d_count,
     const unsigned long long *__restrict__ fm_d, const unsigned long long *__restrict__ fm_d1,
     const double *__restrict__ sigma, double *__restrict__ delta) {
@@ -166,45 +237,126 @@ __global__ __launch_bounds__(MB) void mbc_backward_kernel(
   const uint64_t n = list_d_count[g];
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const unsigned long long *fmd1 = fm_d1 + gV;
   for (uint64_t i = wave; i < n; i += nwaves) {
     const int32_t v = list_d[gV + i];
+    const int32_t beg = rp[v], end = rp[v + 1];
+    if (end - beg > hub) continue;  // split over several waves by the hub kernels
     const uint64_t mine = fm_d[gV + v];
     const bool me = (mine >> lane) & 1ull;
     const double sv = me ? sigma[(gV + v) * 64 + lane] : 0.0;
-    double acc = 0.0;
-    const int32_t beg = rp[v], end = rp[v + 1];
-    for (int32_t e0 = beg; e0 < end; e0 += GB_WAVE) {
-      const int32_t e = e0 + lane;
-      int32_t w = 0;
-      uint64_t m = 0;
-      if (e < end) {
-        w = col[e];
-        m = fmd1[w] & mine;
-      }
-      uint64_t hit = __ballot(m != 0);
-      while (hit) {  // successors one level deeper: 8 at a time, added in CSR order
-        double dw[8], sw[8];
-        bool on[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const bool ok = hit != 0;
-          const int j = ok ? __ffsll((unsigned long long)hit) - 1 : 0;
-          hit &= hit - 1;
-          const int32_t wj = __shfl(w, j, GB_WAVE);
-          const uint64_t mj = ok ? gb_shfl_u64(m, j) : 0ull;
-          on[k] = (mj >> lane) & 1ull;
-          const int64_t r = ((int64_t)gV + wj) * 64 + lane;
-          dw[k] = on[k] ? delta[r] : 0.0;
-          sw[k] = on[k] ? sigma[r] : 1.0;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-          if (on[k]) acc += ((1.0 + dw[k]) * sv) / sw[k];
-      }
-    }
+    const double acc = pull_out(lane, beg, end, col, fm_d1 + gV, mine, sv, sigma, delta, gV);
     if (me) delta[(gV + v) * 64 + lane] = acc;
   }
+}
+
+// Forward step for in-degree hubs: one wave per (hub, chunk of `hub` edges).
+// Chunk sums of sigma are integers, so atomicAdd keeps them exact.
+template <bool SIGMA>
+__global__ __launch_bounds__(MB) void mbc_forward_hub_kernel(
+    int32_t V, int32_t hub, const int32_t *__restrict__ rrp, const int32_t *__restrict__ rcol,
+    const int32_t *__restrict__ hubs, int32_t nhubs, int32_t maxch,
+    const unsigned long long *__restrict__ fm_d, unsigned long long *__restrict__ fm_d1,
+    const unsigned long long *__restrict__ visited, const unsigned long long *__restrict__ active,
+    double *__restrict__ sigma, int32_t *__restrict__ list_d1, unsigned long long *list_d1_count) {
+  const int32_t g = blockIdx.y;
+  const int64_t gV = (int64_t)g * V;
+  const int lane = gb_lane();
+  const uint64_t items = (uint64_t)nhubs * maxch;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t act = active[g];
+  for (uint64_t it = wave; it < items; it += nwaves) {
+    const int32_t x = hubs[it / maxch];
+    const int32_t k = (int32_t)(it % maxch);
+    const int32_t beg = rrp[x] + k * hub;
+    const int32_t end = min(rrp[x + 1], beg + hub);
+    if (beg >= end) continue;
+    const uint64_t need = ~visited[gV + x] & act;
+    if (need == 0) continue;
+    double acc = 0.0;
+    uint64_t any = 0;
+    pull_in<SIGMA>(lane, beg, end, rcol, fm_d + gV, need, sigma, gV, acc, any);
+    const uint64_t found = SIGMA ? (__ballot(acc > 0.0) & need) : (any & need);
+    if (found) {
+      if (SIGMA && ((found >> lane) & 1ull)) atomicAdd(&sigma[(gV + x) * 64 + lane], acc);
+      if (lane == 0) {
+        const unsigned long long old = atomicOr(&fm_d1[gV + x], found);
+        if (old == 0) {  // the first chunk to reach x enters it in level d+1
+          const unsigned long long p = atomicAdd(&list_d1_count[g], 1ull);
+          list_d1[gV + p] = x;
+        }
+      }
+    }
+  }
+}
+
+// After the hub chunks of a level: record the hubs' new sources as visited
+// and, for BFS, write their levels.
+__global__ void mbc_hub_finalize_kernel(int32_t V, int32_t d, const int32_t *__restrict__ hubs,
+                                        int32_t nhubs,
+                                        const unsigned long long *__restrict__ fm_d1,
+                                        unsigned long long *__restrict__ visited,
+                                        int32_t *__restrict__ level_out) {
+  const int32_t g = blockIdx.y;
+  const int64_t gV = (int64_t)g * V;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nhubs) return;
+  const int32_t x = hubs[i];
+  const uint64_t f = fm_d1[gV + x];
+  if (!f) return;
+  visited[gV + x] |= f;
+  if (level_out)
+    for (int s = 0; s < 64; s++)
+      if ((f >> s) & 1ull) level_out[((int64_t)g * 64 + s) * V + x] = d + 1;
+}
+
+// Backward step for out-degree hubs: per-chunk partials ...
+__global__ __launch_bounds__(MB) void mbc_backward_hub_kernel(
+    int32_t V, int32_t hub, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ hubs, int32_t nhubs, int32_t maxch,
+    const unsigned long long *__restrict__ fm_d, const unsigned long long *__restrict__ fm_d1,
+    const double *__restrict__ sigma, const double *__restrict__ delta,
+    double *__restrict__ part) {
+  const int32_t g = blockIdx.y;
+  const int64_t gV = (int64_t)g * V;
+  const int lane = gb_lane();
+  const uint64_t items = (uint64_t)nhubs * maxch;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t it = wave; it < items; it += nwaves) {
+    const int32_t x = hubs[it / maxch];
+    const int32_t k = (int32_t)(it % maxch);
+    const int32_t beg = rp[x] + k * hub;
+    const int32_t end = min(rp[x + 1], beg + hub);
+    if (beg >= end) continue;
+    const uint64_t mine = fm_d[gV + x];
+    if (mine == 0) continue;
+    const bool me = (mine >> lane) & 1ull;
+    const double sv = me ? sigma[(gV + x) * 64 + lane] : 0.0;
+    const double acc = pull_out(lane, beg, end, col, fm_d1 + gV, mine, sv, sigma, delta, gV);
+    part[((int64_t)g * items + it) * 64 + lane] = acc;
+  }
+}
+
+// ... summed in chunk order (deterministic), one wave per hub.
+__global__ __launch_bounds__(MB) void mbc_backward_combine_kernel(
+    int32_t V, int32_t hub, const int32_t *__restrict__ rp, const int32_t *__restrict__ hubs,
+    int32_t nhubs, int32_t maxch, const unsigned long long *__restrict__ fm_d,
+    const double *__restrict__ part, double *__restrict__ delta) {
+  const int32_t g = blockIdx.y;
+  const int64_t gV = (int64_t)g * V;
+  const int lane = gb_lane();
+  const uint64_t items = (uint64_t)nhubs * maxch;
+  const int32_t i = (int32_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= nhubs) return;
+  const int32_t x = hubs[i];
+  const uint64_t mine = fm_d[gV + x];
+  if (!((mine >> lane) & 1ull)) return;
+  const int32_t nch = (rp[x + 1] - rp[x] + hub - 1) / hub;
+  double acc = 0.0;
+  for (int32_t k = 0; k < nch; k++)
+    acc += part[((int64_t)g * items + (int64_t)i * maxch + k) * 64 + lane];
+  delta[(gV + x) * 64 + lane] = acc;
 }
 
 // bc[v] += delta[v][s] (v != source s) over groups and lanes in source order
@@ -229,11 +381,12 @@ struct LevelBuf {
 };
 
 struct MsWork {
-  DevBuf sigma, delta, visited, cand[2], counts, src, active, allcand;
+  DevBuf sigma, delta, visited, cand[2], counts, src, active, allcand, hub_in, hub_out, hub_meta,
+      part;
   std::vector<LevelBuf *> levels;
   void release() {
     for (DevBuf *b : {&sigma, &delta, &visited, &cand[0], &cand[1], &counts, &src, &active,
-                      &allcand})
+                      &allcand, &hub_in, &hub_out, &hub_meta, &part})
       b->release();
     for (auto *l : levels) delete l;
     levels.clear();
@@ -242,6 +395,13 @@ struct MsWork {
 MsWork &msw() {
   static MsWork *w = new MsWork();  // never destroyed (no frees at exit)
   return *w;
+}
+
+// edges per wave for high-degree vertices (GB_BC_HUB overrides)
+int32_t hub_edges() {
+  const char *e = getenv("GB_BC_HUB");
+  const int v = e ? atoi(e) : 0;
+  return v >= 64 ? v : 1024;
 }
 
 int wgrid(uint64_t n) {
@@ -297,9 +457,30 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
   unsigned long long *cnt = w.counts.as<unsigned long long>();
   GB_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * CNT_STRIDE * G, st));
   GB_TRY(level(0));
-  // candidates: vertices with in-edges, the same list for every group
-  hipLaunchKernelGGL(mbc_cand_init_kernel, dim3(std::max(1, std::min((V + 255) / 256, 2048))),
-                     dim3(256), 0, st, V, rrp, w.cand[0].as<int32_t>(), cnt);
+  // vertices above `hub` edges are split into chunks of `hub` edges, one wave
+  // each (in-degree for the forward pulls, out-degree for the backward ones)
+  const int32_t hub = hub_edges();
+  const int vgrid = std::max(1, std::min((V + 255) / 256, 2048));
+  GB_TRY(w.hub_in.ensure(sizeof(int32_t) * (V + 1)));
+  GB_TRY(w.hub_out.ensure(sizeof(int32_t) * (V + 1)));
+  GB_TRY(w.hub_meta.ensure(sizeof(unsigned long long) * 4));
+  GB_HIP(hipMemsetAsync(w.hub_meta.p, 0, sizeof(unsigned long long) * 4, st));
+  unsigned long long *hm = w.hub_meta.as<unsigned long long>();
+  hipLaunchKernelGGL(mbc_hub_list_kernel, dim3(vgrid), dim3(256), 0, st, V, rrp, hub,
+                     w.hub_in.as<int32_t>(), hm + 0, (unsigned int *)(hm + 1));
+  hipLaunchKernelGGL(mbc_hub_list_kernel, dim3(vgrid), dim3(256), 0, st, V, rp, hub,
+                     w.hub_out.as<int32_t>(), hm + 2, (unsigned int *)(hm + 3));
+  unsigned long long hmh[4];
+  GB_HIP(hipMemcpyAsync(hmh, hm, sizeof(hmh), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  const int32_t nhub_in = (int32_t)hmh[0], nhub_out = (int32_t)hmh[2];
+  const int32_t maxch_in = ((int32_t)(uint32_t)hmh[1] + hub - 1) / hub;
+  const int32_t maxch_out = ((int32_t)(uint32_t)hmh[3] + hub - 1) / hub;
+  if (with_sigma && nhub_out > 0)
+    GB_TRY(w.part.ensure(sizeof(double) * 64 * (size_t)G * nhub_out * maxch_out));
+  // candidates: vertices with 1..hub in-edges, the same list for every group
+  hipLaunchKernelGGL(mbc_cand_init_kernel, dim3(vgrid), dim3(256), 0, st, V, rrp, hub,
+                     w.cand[0].as<int32_t>(), cnt);
   unsigned long long nc = 0;
   GB_HIP(hipMemcpyAsync(&nc, cnt, sizeof(nc), hipMemcpyDeviceToHost, st));
   GB_HIP(hipStreamSynchronize(st));
@@ -329,11 +510,11 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
   for (;;) {
     uint64_t maxc = 0;
     for (int32_t k = 0; k < G; k++) maxc = std::max<uint64_t>(maxc, h[k]);
-    if (maxc == 0) break;
+    if (maxc == 0 && nhub_in == 0) break;
     GB_TRY(level(d + 1));
     // zero cand_out and list counts for the next level
     GB_HIP(hipMemsetAsync(cnt + G, 0, sizeof(unsigned long long) * 2 * G, st));
-    dim3 grid(wgrid(maxc), G);
+    dim3 grid(wgrid(std::max<uint64_t>(maxc, 1)), G);
     if (with_sigma)
       hipLaunchKernelGGL(mbc_forward_kernel<true>, grid, dim3(MB), 0, st, V, d, rrp, rcol,
                          w.cand[cur].as<int32_t>(), cnt, w.cand[cur ^ 1].as<int32_t>(), cnt + G,
@@ -351,6 +532,28 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
                          (double *)nullptr, w.levels[d + 1]->list.as<int32_t>(), cnt + 2 * G,
                          d_level);
     GB_HIP(hipGetLastError());
+    if (nhub_in > 0) {
+      dim3 hgrid(wgrid((uint64_t)nhub_in * maxch_in), G);
+      if (with_sigma)
+        hipLaunchKernelGGL(mbc_forward_hub_kernel<true>, hgrid, dim3(MB), 0, st, V, hub, rrp, rcol,
+                           w.hub_in.as<int32_t>(), nhub_in, maxch_in,
+                           w.levels[d]->fm.as<unsigned long long>(),
+                           w.levels[d + 1]->fm.as<unsigned long long>(),
+                           w.visited.as<unsigned long long>(), w.active.as<unsigned long long>(),
+                           w.sigma.as<double>(), w.levels[d + 1]->list.as<int32_t>(), cnt + 2 * G);
+      else
+        hipLaunchKernelGGL(mbc_forward_hub_kernel<false>, hgrid, dim3(MB), 0, st, V, hub, rrp,
+                           rcol, w.hub_in.as<int32_t>(), nhub_in, maxch_in,
+                           w.levels[d]->fm.as<unsigned long long>(),
+                           w.levels[d + 1]->fm.as<unsigned long long>(),
+                           w.visited.as<unsigned long long>(), w.active.as<unsigned long long>(),
+                           (double *)nullptr, w.levels[d + 1]->list.as<int32_t>(), cnt + 2 * G);
+      hipLaunchKernelGGL(mbc_hub_finalize_kernel, dim3((nhub_in + 255) / 256, G), dim3(256), 0, st,
+                         V, d, w.hub_in.as<int32_t>(), nhub_in,
+                         w.levels[d + 1]->fm.as<unsigned long long>(),
+                         w.visited.as<unsigned long long>(), d_level);
+      GB_HIP(hipGetLastError());
+    }
     GB_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(unsigned long long) * CNT_STRIDE * G,
                           hipMemcpyDeviceToHost, st));
     GB_HIP(hipStreamSynchronize(st));
@@ -376,12 +579,24 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
     if (maxn == 0) continue;
     GB_HIP(hipMemcpyAsync(cnt + 2 * G, list_n[dd].data(), sizeof(unsigned long long) * G,
                           hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(mbc_backward_kernel, dim3(wgrid(maxn), G), dim3(MB), 0, st, V, rp, col,
-                       w.levels[dd]->list.as<int32_t>(), cnt + 2 * G,
+    hipLaunchKernelGGL(mbc_backward_kernel, dim3(wgrid(maxn), G), dim3(MB), 0, st, V, hub, rp,
+                       col, w.levels[dd]->list.as<int32_t>(), cnt + 2 * G,
                        w.levels[dd]->fm.as<unsigned long long>(),
                        w.levels[dd + 1]->fm.as<unsigned long long>(), w.sigma.as<double>(),
                        w.delta.as<double>());
     GB_HIP(hipGetLastError());
+    if (nhub_out > 0) {
+      hipLaunchKernelGGL(mbc_backward_hub_kernel, dim3(wgrid((uint64_t)nhub_out * maxch_out), G),
+                         dim3(MB), 0, st, V, hub, rp, col, w.hub_out.as<int32_t>(), nhub_out,
+                         maxch_out, w.levels[dd]->fm.as<unsigned long long>(),
+                         w.levels[dd + 1]->fm.as<unsigned long long>(), w.sigma.as<double>(),
+                         w.delta.as<double>(), w.part.as<double>());
+      hipLaunchKernelGGL(mbc_backward_combine_kernel, dim3((nhub_out + 3) / 4, G), dim3(MB), 0, st,
+                         V, hub, rp, w.hub_out.as<int32_t>(), nhub_out, maxch_out,
+                         w.levels[dd]->fm.as<unsigned long long>(), w.part.as<double>(),
+                         w.delta.as<double>());
+      GB_HIP(hipGetLastError());
+    }
   }
   hipLaunchKernelGGL(mbc_accum_kernel, dim3((V + 255) / 256), dim3(256), 0, st, V, G, nsrc,
                      w.src.as<int32_t>(), w.delta.as<double>(), d_bc);

@@ -165,7 +165,34 @@ def test_bc_rmat_undirected(gpu, batch):
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
     assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
     # lanes sum in CSR order and sources accumulate in order: identical bits
-    assert np.array_equal(bc, ref)
+    # (vertices above the hub threshold sum their delta in 1024-edge chunks)
+    if h.outdeg().max() <= 1024:
+        assert np.array_equal(bc, ref)
+
+
+def test_bc_and_bfs_hub_split(gpu, monkeypatch):
+    """Force the hub path (every vertex above 64 edges is split across waves)."""
+    gb = gpu
+    monkeypatch.setenv("GB_BC_HUB", "64")
+    h = gb.gen_rmat(12, 16, seed=21, weights="none", symmetric=True)
+    assert h.outdeg().max() > 64 * 4
+    srcs = gb.pick_sources(h, 70, 2)
+    _say("bc hub split")
+    g = gb.Graph.from_host(h, need_reverse=False)
+    bc = gb.bc(g, srcs).cpu().numpy()
+    ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
+    assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+    lv = gb.bfs(g, srcs[:5]).cpu().numpy()
+    for i, s in enumerate(srcs[:5]):
+        assert np.array_equal(lv[i], oracle.bfs_levels(h.row_ptr, h.col, int(s)))
+    # directed, distinct in/out hubs
+    hd = gb.gen_rmat(11, 16, seed=22, weights="none")
+    r = hd.transpose()
+    gd = gb.Graph.from_host(hd, reverse=r)
+    s2 = gb.pick_sources(hd, 40, 3)
+    bcd = gb.bc(gd, s2).cpu().numpy()
+    refd = oracle.brandes_bfs(hd.row_ptr, hd.col, s2, r.row_ptr, r.col)
+    assert np.allclose(bcd, refd, rtol=BC_RTOL, atol=1e-12)
 
 
 def test_bc_many_sources_and_duplicates(gpu):
@@ -176,9 +203,12 @@ def test_bc_many_sources_and_duplicates(gpu):
     _say(f"bc many sources n={len(srcs)}")
     g = gb.Graph.from_host(h, need_reverse=False)
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
+    exact = h.outdeg().max() <= 1024
     for batch in (0, 64, 100):
         bc = gb.bc(g, srcs, batch=batch).cpu().numpy()
-        assert np.array_equal(bc, ref), batch
+        assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12), batch
+        if exact:
+            assert np.array_equal(bc, ref), batch
 
 
 def test_bfs_many_sources(gpu):
@@ -202,7 +232,7 @@ def test_bc_directed(gpu):
     bc = gb.bc(g, srcs).cpu().numpy()
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs, r.row_ptr, r.col)
     assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
-    assert np.array_equal(bc, ref)
+    assert np.array_equal(bc, ref)  # no vertex above the hub threshold here
 
 
 def test_bc_weighted(gpu):
