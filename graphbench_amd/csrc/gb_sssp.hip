@@ -145,12 +145,28 @@ __global__ void sssp_seed_kernel(Ctx<Tr> c, int32_t ntasks, uint64_t *near_out,
 }
 
 // Light (HEAVY=false) or heavy (HEAVY=true) expansion of one queue.
+// Each wave takes 64 queue entries at a time (one per lane) and spreads their
+// edges over its lanes: a wave prefix sum of the entries' edge counts gives
+// every edge slot k an owner lane, found by a binary search over the lanes'
+// offsets with cross-lane reads.  Lanes therefore stay busy whether entries
+// hold 3 edges or 1024.
+template <class T>
+__device__ __forceinline__ T gb_shfl_any(T v, int src);
+template <>
+__device__ __forceinline__ uint32_t gb_shfl_any<uint32_t>(uint32_t v, int src) {
+  return __shfl(v, src, GB_WAVE);
+}
+template <>
+__device__ __forceinline__ double gb_shfl_any<double>(double v, int src) {
+  return __longlong_as_double((long long)gb_shfl_u64((uint64_t)__double_as_longlong(v), src));
+}
+
 template <class Tr, bool HEAVY>
 __global__ __launch_bounds__(SS_BLOCK) void sssp_expand_kernel(
     Ctx<Tr> c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
     uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ s_out,
     unsigned long long *n_s, uint64_t *__restrict__ far_out, unsigned long long *n_far,
-    uint64_t bucket, uint32_t gen_next, uint32_t sround, uint32_t fepoch) {
+    uint64_t bucket, uint32_t gen_next, uint32_t sround, uint32_t fepoch, uint32_t epw) {
   using D = typename Tr::D;
   const int lane = gb_lane();
   const uint64_t n = *n_in;
@@ -158,58 +174,89 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_expand_kernel(
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   uint32_t relax = 0;
   bool ovf = false;
-  for (uint64_t i = wave; i < n; i += nwaves) {
-    const uint64_t ent = in[i];
-    const uint32_t t = (uint32_t)(ent >> 48);
-    const uint32_t ch = (uint32_t)(ent >> 32) & 0xFFFFu;
-    const uint32_t u = (uint32_t)ent;
-    const STask<Tr> tk = c.tasks[t];
-    const SGraph<typename Tr::W> G = c.graphs[tk.graph];
-    const D du = tk.dist[u];
-    const int32_t rs = G.rp[u], le = G.le[u], re = G.rp[u + 1];
-    int32_t beg, end;
-    if (!HEAVY) {
-      beg = rs + (int32_t)(ch << CHUNK_SHIFT);
-      end = min(le, beg + CHUNK);
-      // remember_arr: the first chunk's wave records u in S (heavy edges)
-      uint32_t scnt = 0;
-      if (ch == 0 && lane == 0 && re > le) {
-        if (atomicExch(&c.sst[tk.off + u], sround) != sround) scnt = nchunks(re - le);
+  // epw (1..64) entries per wave and round: small queues use fewer entries per
+  // wave so that more waves run
+  for (uint64_t b0 = wave * epw; b0 < n; b0 += nwaves * epw) {
+    // ---- this lane's entry
+    const uint64_t i = b0 + lane;
+    uint32_t t = 0, u = 0, scnt = 0;
+    int32_t beg = 0, end = 0;
+    D du = (D)0;
+    if ((uint32_t)lane < epw && i < n) {
+      const uint64_t ent = in[i];
+      t = (uint32_t)(ent >> 48);
+      const uint32_t ch = (uint32_t)(ent >> 32) & 0xFFFFu;
+      u = (uint32_t)ent;
+      const STask<Tr> tk = c.tasks[t];
+      const SGraph<typename Tr::W> G = c.graphs[tk.graph];
+      du = tk.dist[u];
+      const int32_t rs = G.rp[u], le = G.le[u], re = G.rp[u + 1];
+      if (!HEAVY) {
+        beg = rs + (int32_t)(ch << CHUNK_SHIFT);
+        end = min(le, beg + CHUNK);
+        // remember_arr: the first chunk records u in S (heavy edges pending)
+        if (ch == 0 && re > le && atomicExch(&c.sst[tk.off + u], sround) != sround)
+          scnt = nchunks(re - le);
+      } else {
+        beg = le + (int32_t)(ch << CHUNK_SHIFT);
+        end = min(re, beg + CHUNK);
       }
-      uint64_t sb = gb_wave_reserve(n_s, scnt);
-      for (uint32_t k = 0; k < scnt; k++) s_out[sb + k] = mk_entry(t, k, u);
-    } else {
-      beg = le + (int32_t)(ch << CHUNK_SHIFT);
-      end = min(re, beg + CHUNK);
     }
-    for (int32_t b0 = beg; b0 < end; b0 += GB_WAVE) {
-      const int32_t e = b0 + lane;
+    if (!HEAVY && __ballot(scnt != 0)) {
+      const uint64_t sb = gb_wave_reserve(n_s, scnt);
+      for (uint32_t k = 0; k < scnt; k++) s_out[sb + k] = mk_entry(t, k, u);
+    }
+    // ---- spread the 64 entries' edges over the lanes
+    uint32_t total;
+    const uint32_t off = gb_wave_excl_sum((uint32_t)max(end - beg, 0), &total);
+    for (uint32_t k0 = 0; k0 < total; k0 += GB_WAVE) {
+      const uint32_t k = k0 + lane;
+      // owner: the last lane j with off_j <= k (offsets are non-decreasing)
+      int j = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const int cand = j + step;
+        const uint32_t oc = __shfl(off, cand < GB_WAVE ? cand : GB_WAVE - 1, GB_WAVE);
+        if (cand < GB_WAVE && oc <= k) j = cand;
+      }
+      const uint32_t tj = __shfl(t, j, GB_WAVE);
+      const int32_t e = __shfl(beg, j, GB_WAVE) + (int32_t)(k - __shfl(off, j, GB_WAVE));
+      const D duj = gb_shfl_any<D>(du, j);
       uint32_t ncnt = 0;
       bool far = false;
       uint32_t v = 0;
-      if (e < end) {
+      if (k < total) {
+        const STask<Tr> tk = c.tasks[tj];
+        const SGraph<typename Tr::W> G = c.graphs[tk.graph];
         v = (uint32_t)G.col[e];
-        const D nd = Tr::add(du, G.w[e], ovf);
+        const D nd = Tr::add(duj, G.w[e], ovf);
         relax++;
         D *dv = tk.dist + v;
         if (nd < *dv) {
           const D old = Tr::amin(dv, nd);
           if (nd < old) {
+            // dedup stamps: a plain read first, the exchange only when unset
             if (Tr::bucket(nd, tk.delta) <= bucket) {
-              if (atomicExch(&c.nst[tk.off + v], gen_next) != gen_next) {
+              uint32_t *ns = &c.nst[tk.off + v];
+              if (*ns != gen_next && atomicExch(ns, gen_next) != gen_next) {
                 ncnt = nchunks(G.le[v] - G.rp[v]);
                 if (ncnt == 0) ncnt = 1;
               }
             } else {
-              if (atomicExch(&c.fst[tk.off + v], fepoch) != fepoch) far = true;
+              uint32_t *fs = &c.fst[tk.off + v];
+              if (*fs != fepoch && atomicExch(fs, fepoch) != fepoch) far = true;
             }
           }
         }
       }
-      const uint64_t nb = gb_wave_reserve(n_near, ncnt);
-      for (uint32_t k = 0; k < ncnt; k++) near_out[nb + k] = mk_entry(t, k, v);
-      const uint64_t fb = gb_wave_reserve(n_far, far ? 1u : 0u);
-      if (far) far_out[fb] = mk_entry(t, 0, v);
+      if (__ballot(ncnt != 0)) {
+        const uint64_t nb = gb_wave_reserve(n_near, ncnt);
+        for (uint32_t q = 0; q < ncnt; q++) near_out[nb + q] = mk_entry(tj, q, v);
+      }
+      if (__ballot(far)) {
+        const uint64_t fb = gb_wave_reserve(n_far, far ? 1u : 0u);
+        if (far) far_out[fb] = mk_entry(tj, 0, v);
+      }
     }
   }
   // per-wave statistics
@@ -251,10 +298,14 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_far_scan_kernel(
         minb = bk < minb ? bk : minb;
       }
     }
-    const uint64_t nb = gb_wave_reserve(n_near, ncnt);
-    for (uint32_t k = 0; k < ncnt; k++) near_out[nb + k] = mk_entry(t, k, v);
-    const uint64_t fb = gb_wave_reserve(n_far, far ? 1u : 0u);
-    if (far) far_out[fb] = mk_entry(t, 0, v);
+    if (__ballot(ncnt != 0)) {
+      const uint64_t nb = gb_wave_reserve(n_near, ncnt);
+      for (uint32_t k = 0; k < ncnt; k++) near_out[nb + k] = mk_entry(t, k, v);
+    }
+    if (__ballot(far)) {
+      const uint64_t fb = gb_wave_reserve(n_far, far ? 1u : 0u);
+      if (far) far_out[fb] = mk_entry(t, 0, v);
+    }
   }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
@@ -306,9 +357,17 @@ SsspWork &g_ssw_ref() {
 }
 #define g_ssw (g_ssw_ref())
 
-int expand_grid(uint64_t n_entries) {
-  // one wave per entry, capped at a few resident waves per CU
-  uint64_t waves = n_entries;
+// Entries per wave for an expansion of n queue entries: enough waves to
+// cover the GPU (32 per CU) before packing up to 64 entries into a wave.
+uint32_t entries_per_wave(uint64_t n) {
+  const uint64_t target = (uint64_t)gb_num_cus() * 32;
+  uint64_t e = (n + target - 1) / target;
+  return (uint32_t)(e < 1 ? 1 : (e > GB_WAVE ? GB_WAVE : e));
+}
+
+int expand_grid(uint64_t n_entries, uint32_t epw = GB_WAVE) {
+  // one wave per epw queue entries, capped at 16 blocks (64 waves) per CU
+  uint64_t waves = (n_entries + epw - 1) / epw;
   uint64_t blocks = (waves + 3) / 4;
   uint64_t cap = (uint64_t)gb_num_cus() * 16;
   if (blocks > cap) blocks = cap;
@@ -394,10 +453,12 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
         if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
         gen++;
         GB_TRY(zero(C_NEAR0 + (cur ^ 1)));
-        hipLaunchKernelGGL((sssp_expand_kernel<Tr, false>), dim3(expand_grid(h[C_NEAR0 + cur])),
-                           dim3(SS_BLOCK), 0, st, c, near[cur], cnt + C_NEAR0 + cur, near[cur ^ 1],
-                           cnt + C_NEAR0 + (cur ^ 1), S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur,
-                           b, gen, sround, fepoch);
+        const uint32_t epw = entries_per_wave(h[C_NEAR0 + cur]);
+        hipLaunchKernelGGL((sssp_expand_kernel<Tr, false>),
+                           dim3(expand_grid(h[C_NEAR0 + cur], epw)), dim3(SS_BLOCK), 0, st, c,
+                           near[cur], cnt + C_NEAR0 + cur, near[cur ^ 1], cnt + C_NEAR0 + (cur ^ 1),
+                           S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur, b, gen, sround, fepoch,
+                           epw);
         GB_HIP(hipGetLastError());
         cur ^= 1;
         GB_TRY(read());
@@ -406,9 +467,11 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
         if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
         gen++;
         GB_TRY(zero(C_NEAR0 + cur));
-        hipLaunchKernelGGL((sssp_expand_kernel<Tr, true>), dim3(expand_grid(h[C_S])),
+        const uint32_t epw = entries_per_wave(h[C_S]);
+        hipLaunchKernelGGL((sssp_expand_kernel<Tr, true>), dim3(expand_grid(h[C_S], epw)),
                            dim3(SS_BLOCK), 0, st, c, S, cnt + C_S, near[cur], cnt + C_NEAR0 + cur,
-                           S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur, b, gen, sround, fepoch);
+                           S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur, b, gen, sround, fepoch,
+                           epw);
         GB_HIP(hipGetLastError());
         GB_TRY(read());
         again = h[C_NEAR0 + cur] > 0;  // a heavy edge landed back in bucket b
@@ -422,7 +485,7 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
       GB_TRY(zero(C_NEAR0 + cur));
       GB_TRY(zero(C_FAR0 + (fcur ^ 1)));
       GB_HIP(hipMemsetAsync(cnt + C_MINB, 0xFF, sizeof(unsigned long long), st));
-      hipLaunchKernelGGL(sssp_far_scan_kernel<Tr>, dim3(expand_grid(h[C_FAR0 + fcur] / 64 + 1)),
+      hipLaunchKernelGGL(sssp_far_scan_kernel<Tr>, dim3(expand_grid(h[C_FAR0 + fcur])),
                          dim3(SS_BLOCK), 0, st, c, far[fcur], cnt + C_FAR0 + fcur, near[cur],
                          cnt + C_NEAR0 + cur, far[fcur ^ 1], cnt + C_FAR0 + (fcur ^ 1), bnext, gen,
                          fepoch + 1);
