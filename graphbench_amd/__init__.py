@@ -109,6 +109,13 @@ _SIGS = {
     "gb_pick_sources": (_i32, [C.POINTER(_HostCSR), _i32, _u64, _p]),
     "gb_host_csr_free": (None, [C.POINTER(_HostCSR)]),
     "gb_splitmix64": (_u64, [_u64]),
+    "gb_mlp_param_count": (_i32, [_i32, _i32]),
+    "gb_mlp_forward": (_i32, [_p, _i32, _i32, _p, _i64, _p, _p]),
+    "gb_mlp_loss_grad": (_i32, [_p, _i32, _i32, _p, _p, _i64, _p, C.POINTER(_f64), _p]),
+    "gb_mlp_adam": (_i32, [_p, _p, _p, _p, _i32, C.c_float, C.c_float, C.c_float, C.c_float,
+                           _i32, _p]),
+    "gb_graph_features": (_i32, [_p, _i32, _p, _p]),
+    "gb_pr_warm_start": (_i32, [_p, _p, _p, _p, _p]),
 }
 
 

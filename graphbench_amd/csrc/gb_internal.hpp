@@ -111,5 +111,6 @@ void gb_pr_release();
 void gb_sssp_release();
 void gb_bc_release();
 void gb_msbc_release();
+void gb_mlp_release();
 
 #endif

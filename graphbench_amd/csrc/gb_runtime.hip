@@ -93,6 +93,7 @@ void gb_release_workspaces(void) {
   gb_sssp_release();
   gb_bc_release();
   gb_msbc_release();
+  gb_mlp_release();
 }
 
 void gb_set_mem_cap(uint64_t bytes) {

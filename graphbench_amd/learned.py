@@ -1,0 +1,325 @@
+"""Learned delta / light threshold and PageRank warm start.
+
+Two small MLPs (2 hidden ReLU layers of 64, FP32), trained and evaluated on the
+GPU by the library's own kernels (gb_mlp_loss_grad, gb_mlp_adam,
+gb_mlp_forward), on data this module generates with the library itself:
+
+* delta model: graph statistics (gb_graph_features: log2 V, log2 average
+  degree, log2 max / mean weight, log2 degree skew, log2 sources) ->
+  (log2 delta, log2 light_max/delta), the fastest setting found by sweeping
+  batched delta-stepping over generated R-MAT and Erdos-Renyi graphs;
+* warm-start model: per-vertex (log2(1+indeg), log2(1+outdeg), dangling) ->
+  standardised log2(N * converged rank), from PageRank solved to 1e-9.
+
+The plain rules stay the fallback: delta = max_w / avg_degree and x0 = 1/N.
+Delta-stepping is exact for every delta, and the warm-started power iteration
+is judged by the same residual, so the models change speed only.
+
+Train on a GPU:  python -m graphbench_amd.learned train --out graphbench_amd/learned_models.json
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from typing import Optional
+
+import numpy as np
+
+from . import (GBError, Graph, HostCSR, _check, _ptr, _stream, gen_er, gen_rmat, lib,
+               pagerank, pick_sources, sssp)
+
+MODELS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_models.json")
+HIDDEN = 64
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class MLP:
+    """din -> 64 -> 64 -> dout ReLU MLP; parameters live on the GPU."""
+
+    def __init__(self, din: int, dout: int, seed: int = 0, params: Optional[np.ndarray] = None):
+        torch = _torch()
+        self.din, self.dout = din, dout
+        self.total = int(lib().gb_mlp_param_count(din, dout))
+        if self.total <= 0:
+            raise GBError(1, f"unsupported MLP shape {din}->{dout}")
+        if params is None:
+            params = self._init(seed)
+        self.params = torch.from_numpy(np.asarray(params, np.float32).copy()).cuda()
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grad = torch.zeros_like(self.params)
+        self.step = 0
+
+    def _init(self, seed):
+        rng = np.random.default_rng(seed)
+        H, din, dout = HIDDEN, self.din, self.dout
+        parts = [rng.uniform(-1, 1, (H, din)) * math.sqrt(6 / din), np.zeros(H),
+                 rng.uniform(-1, 1, (H, H)) * math.sqrt(6 / H), np.zeros(H),
+                 rng.uniform(-1, 1, (dout, H)) * math.sqrt(3 / H), np.zeros(dout)]
+        return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+    def split(self, flat=None):
+        """Parameters as (W1, b1, W2, b2, W3, b3) numpy arrays."""
+        a = self.params.cpu().numpy() if flat is None else np.asarray(flat)
+        H, din, dout = HIDDEN, self.din, self.dout
+        shapes = [(H, din), (H,), (H, H), (H,), (dout, H), (dout,)]
+        out, o = [], 0
+        for sh in shapes:
+            n = int(np.prod(sh))
+            out.append(a[o:o + n].reshape(sh))
+            o += n
+        return out
+
+    def loss_grad(self, X, T) -> float:
+        loss = C.c_double(0.0)
+        _check(lib().gb_mlp_loss_grad(_ptr(self.params), self.din, self.dout, _ptr(X), _ptr(T),
+                                      X.shape[0], _ptr(self.grad), C.byref(loss), _stream()))
+        return float(loss.value)
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.step += 1
+        _check(lib().gb_mlp_adam(_ptr(self.params), _ptr(self.grad), _ptr(self.m), _ptr(self.v),
+                                 self.total, lr, b1, b2, eps, self.step, _stream()))
+
+    def forward(self, X):
+        torch = _torch()
+        Y = torch.empty((X.shape[0], self.dout), dtype=torch.float32, device="cuda")
+        _check(lib().gb_mlp_forward(_ptr(self.params), self.din, self.dout, _ptr(X), X.shape[0],
+                                    _ptr(Y), _stream()))
+        return Y
+
+    def fit(self, X, T, steps=500, lr=3e-3, log_every=0):
+        losses = []
+        for k in range(steps):
+            loss = self.loss_grad(X, T)
+            self.adam(lr=lr)
+            losses.append(loss)
+            if log_every and k % log_every == 0:
+                print(f"  step {k} loss {loss:.5f}", flush=True)
+        return losses
+
+
+# --------------------------------------------------------------------- data
+
+def graph_features(g: Graph, nsrc: int) -> np.ndarray:
+    f = np.zeros(6, np.float32)
+    _check(lib().gb_graph_features(g.handle, nsrc, f.ctypes.data, _stream()))
+    return f
+
+
+def plain_delta(h: HostCSR) -> int:
+    """max_weight / average degree (the reference's rule), as an integer width."""
+    return max(1, int(math.floor(float(h.w.max()) * h.V / max(h.E, 1))))
+
+
+def _time_sssp(g, srcs, delta, light, reps=1):
+    torch = _torch()
+    best = float("inf")
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sssp(g, srcs, delta=delta, light_max=light)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t)
+    return best
+
+
+def delta_sweep(h: HostCSR, g: Graph, nsrc: int, seed: int, kmax: int = 18, kmin: int = 2):
+    """Times batched delta-stepping over delta = 2^k (descending, stopping once
+    a run is 3x slower than the best) and then light_max / delta in {1/2, 1/4}
+    around the best k.  Returns the features, the soft-best target (mean of the
+    settings within 5% of the fastest) and the timings."""
+    srcs = pick_sources(h, nsrc, seed)
+    _time_sssp(g, srcs[:1], 1 << 10, 1 << 10)  # warm-up (workspaces, code objects)
+    reps = 2 if h.E < (1 << 22) else 1
+    rows = []
+    best = float("inf")
+    for k in range(kmax, kmin - 1, -1):
+        t = _time_sssp(g, srcs, 1 << k, 1 << k, reps)
+        rows.append((k, 0.0, t))
+        best = min(best, t)
+        if t > 3 * best:
+            break
+    kb = min(rows, key=lambda r: r[2])[0]
+    for k in (kb - 1, kb, kb + 1):
+        for lr in (-1.0, -2.0):
+            if kmin <= k <= kmax:
+                lm = max(1, int((1 << k) * 2.0 ** lr))
+                rows.append((k, lr, _time_sssp(g, srcs, 1 << k, lm, reps)))
+    best = min(t for _, _, t in rows)
+    good = [(k, lr) for k, lr, t in rows if t <= 1.05 * best]
+    target = np.array([np.mean([k for k, _ in good]), np.mean([lr for _, lr in good])], np.float32)
+    feats = graph_features(g, nsrc)
+    plain = plain_delta(h)
+    t_plain = _time_sssp(g, srcs, plain, plain, reps)
+    return feats, target, dict(best_s=best, plain_s=t_plain, sweep=rows)
+
+
+def vertex_features_host(h: HostCSR, r: HostCSR) -> np.ndarray:
+    """Same formula as the device kernel of gb_pr_warm_start (before scaling)."""
+    indeg = np.diff(r.row_ptr).astype(np.float32)
+    outdeg = np.diff(h.row_ptr).astype(np.float32)
+    return np.stack([np.log2(1 + indeg), np.log2(1 + outdeg),
+                     (outdeg == 0).astype(np.float32)], axis=1)
+
+
+def pr_samples(h: HostCSR, n_samples: int, seed: int):
+    r = h.transpose()
+    g = Graph.from_host(h, reverse=r)
+    x, res = pagerank(g, damping=0.85, tol=1e-9, max_iter=500)
+    x = x.cpu().numpy().astype(np.float64)
+    g.close()
+    X = vertex_features_host(h, r)
+    Y = np.log2(np.maximum(x * h.V, 1e-30)).astype(np.float32)[:, None]
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(h.V, size=min(n_samples, h.V), replace=False)
+    return X[idx], Y[idx]
+
+
+# --------------------------------------------------------------------- models
+
+class LearnedModels:
+    def __init__(self, data: dict):
+        self.data = data
+        d = data["delta"]
+        self.delta_mlp = MLP(d["din"], d["dout"], params=np.array(d["params"], np.float32))
+        p = data["warm_start"]
+        self.pr_mlp = MLP(p["din"], p["dout"], params=np.array(p["params"], np.float32))
+
+    @staticmethod
+    def load(path: str = MODELS_PATH) -> "LearnedModels":
+        with open(path) as f:
+            return LearnedModels(json.load(f))
+
+    def predict_delta(self, g: Graph, nsrc: int):
+        """(delta, light_max) for a batch of nsrc sources on g."""
+        torch = _torch()
+        d = self.data["delta"]
+        f = graph_features(g, nsrc)
+        xs = (f - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
+        y = self.delta_mlp.forward(torch.from_numpy(xs[None, :].astype(np.float32)).cuda())
+        y = y.cpu().numpy()[0] * np.array(d["y_std"]) + np.array(d["y_mean"])
+        delta = int(round(2.0 ** float(np.clip(y[0], 0, 24))))
+        ratio = 2.0 ** float(np.clip(y[1], -3, 3))
+        return max(1, delta), max(1, int(round(delta * ratio)))
+
+    def warm_start(self, g: Graph):
+        """x0 (V floats on the GPU, sum 1) for gb_pagerank."""
+        torch = _torch()
+        p = self.data["warm_start"]
+        norm = np.array(list(p["x_mean"]) + list(p["x_std"]) + [p["y_mean"][0], p["y_std"][0]],
+                        np.float32)
+        x0 = torch.empty(g.V, dtype=torch.float32, device="cuda")
+        _check(lib().gb_pr_warm_start(g.handle, _ptr(self.pr_mlp.params), norm.ctypes.data,
+                                      _ptr(x0), _stream()))
+        return x0
+
+
+def _standardise(A):
+    m = A.mean(axis=0)
+    s = A.std(axis=0)
+    s = np.where(s > 1e-6, s, 1.0)
+    return m, s
+
+
+def train_all(out_path: str, quick: bool = False, log=print):
+    """Generates the training data on the GPU and fits both models."""
+    torch = _torch()
+    t0 = time.time()
+    # ---- delta model: sweeps over generated graphs, 1 / 8 / 32 sources each
+    specs = []
+    scales = [12, 15, 18] if quick else [12, 14, 16, 18, 20]
+    for sc in scales:
+        for ef in ([16] if quick else [8, 16, 32]):
+            for wmode, whi in [("uniform", 255), ("uniform", 65535), ("mix", 255)]:
+                specs.append(("rmat", sc, ef, wmode, whi, 1000 + len(specs)))
+    for sc in ([14] if quick else [13, 16, 19]):
+        for deg in [6, 24]:
+            specs.append(("er", sc, deg, "uniform", 1023, 2000 + len(specs)))
+    feats, targets, info = [], [], []
+    for i, (kind, sc, ef, wmode, whi, seed) in enumerate(specs):
+        if kind == "rmat":
+            h = gen_rmat(sc, ef, seed=seed, weights=wmode, wlo=1, whi=whi)
+        else:
+            h = gen_er(1 << sc, ef, seed=seed, weights=wmode, wlo=1, whi=whi)
+        g = Graph.from_host(h, need_reverse=False)
+        for nsrc in (1, 8, 32):
+            f, t, meta = delta_sweep(h, g, nsrc, seed)
+            feats.append(f)
+            targets.append(t)
+            info.append(dict(kind=kind, scale=sc, ef=ef, weights=wmode, whi=whi, nsrc=nsrc,
+                             target=t.tolist(), best_ms=meta["best_s"] * 1e3,
+                             plain_ms=meta["plain_s"] * 1e3))
+            log(f"[delta {i+1}/{len(specs)}] {kind} s{sc} ef{ef} {wmode}{whi} nsrc{nsrc}: "
+                f"log2 delta*={t[0]:.2f} ratio={t[1]:.2f} best {meta['best_s']*1e3:.1f} ms "
+                f"plain {meta['plain_s']*1e3:.1f} ms ({time.time()-t0:.0f}s)")
+        g.close()
+        del h, g
+    F = np.array(feats, np.float32)
+    Tt = np.array(targets, np.float32)
+    xm, xs = _standardise(F)
+    ym, ys = _standardise(Tt)
+    Xd = torch.from_numpy(((F - xm) / xs).astype(np.float32)).cuda()
+    Td = torch.from_numpy(((Tt - ym) / ys).astype(np.float32)).cuda()
+    dm = MLP(6, 2, seed=1)
+    losses = dm.fit(Xd, Td, steps=400 if quick else 1500, lr=3e-3)
+    log(f"delta model: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    delta_json = dict(din=6, dout=2, params=dm.params.cpu().numpy().tolist(),
+                      x_mean=xm.tolist(), x_std=xs.tolist(), y_mean=ym.tolist(),
+                      y_std=ys.tolist(), final_loss=losses[-1], samples=info)
+    # ---- warm-start model: converged ranks
+    Xs, Ys = [], []
+    pr_specs = [("rmat", 14, 16, 10), ("rmat", 16, 8, 11), ("rmat", 18, 16, 12),
+                ("rmat", 19, 32, 13), ("er", 16, 16, 14), ("rmat", 20, 16, 15)]
+    if quick:
+        pr_specs = pr_specs[:3]
+    for kind, sc, ef, seed in pr_specs:
+        h = gen_rmat(sc, ef, seed=seed, weights="none") if kind == "rmat" else \
+            gen_er(1 << sc, ef, seed=seed, weights="none")
+        X, Y = pr_samples(h, 60000, seed)
+        Xs.append(X)
+        Ys.append(Y)
+        log(f"[warm start] {kind} s{sc} ef{ef}: {len(X)} samples ({time.time()-t0:.0f}s)")
+        del h
+    X = np.concatenate(Xs)
+    Y = np.concatenate(Ys)
+    xm2, xs2 = _standardise(X)
+    ym2, ys2 = _standardise(Y)
+    Xd = torch.from_numpy(((X - xm2) / xs2).astype(np.float32)).cuda()
+    Td = torch.from_numpy(((Y - ym2) / ys2).astype(np.float32)).cuda()
+    pm = MLP(3, 1, seed=2)
+    losses2 = pm.fit(Xd, Td, steps=300 if quick else 1000, lr=3e-3)
+    log(f"warm-start model: loss {losses2[0]:.4f} -> {losses2[-1]:.4f}")
+    pr_json = dict(din=3, dout=1, params=pm.params.cpu().numpy().tolist(), x_mean=xm2.tolist(),
+                   x_std=xs2.tolist(), y_mean=ym2.tolist(), y_std=ys2.tolist(),
+                   final_loss=losses2[-1], samples=int(len(X)))
+    data = dict(format="graphbench-mi355x learned models v1", delta=delta_json,
+                warm_start=pr_json, train_seconds=time.time() - t0)
+    with open(out_path, "w") as f:
+        json.dump(data, f)
+    log(f"wrote {out_path} in {time.time()-t0:.0f}s")
+    return data
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("--out", default=MODELS_PATH)
+    ap.add_argument("--quick", action="store_true")
+    a = ap.parse_args(argv)
+    from . import init
+    init(0)
+    train_all(a.out, quick=a.quick, log=lambda m: print(m, flush=True))
+
+
+if __name__ == "__main__":
+    main()

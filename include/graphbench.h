@@ -187,6 +187,34 @@ gb_status gb_bc_weighted(gb_graph g, const int32_t *h_sources, int32_t nsrc,
                          int32_t batch, double *d_bc, gb_bc_stats *st,
                          void *stream);
 
+/* ---- learned delta and PageRank warm start --------------------------------------
+ * A 2-hidden-layer ReLU MLP (width 64, FP32) trained and evaluated by the
+ * library's own kernels.  Parameters are one float array (device):
+ * W1[64][din] b1[64] W2[64][64] b2[64] W3[dout][64] b3[dout]; din <= 16,
+ * dout <= 4. */
+int32_t gb_mlp_param_count(int32_t din, int32_t dout);
+/* Y[n][dout] = MLP(X[n][din]) */
+gb_status gb_mlp_forward(const float *d_params, int32_t din, int32_t dout,
+                         const float *d_X, int64_t n, float *d_Y, void *stream);
+/* Mean squared-error loss (0.5*||y-t||^2 averaged over rows) and its gradient
+ * (same layout as the parameters), summed in a fixed order. */
+gb_status gb_mlp_loss_grad(const float *d_params, int32_t din, int32_t dout,
+                           const float *d_X, const float *d_T, int64_t n,
+                           float *d_grad, double *loss, void *stream);
+/* One Adam step (step counts from 1). */
+gb_status gb_mlp_adam(float *d_params, const float *d_grad, float *d_m, float *d_v,
+                      int32_t total, float lr, float beta1, float beta2, float eps,
+                      int32_t step, void *stream);
+/* Inputs of the delta model (6 floats, host): log2 V, log2 avg degree,
+ * log2 max weight, log2 mean weight, log2(max out-degree / avg degree),
+ * log2 nsrc. */
+gb_status gb_graph_features(gb_graph g, int32_t nsrc, float *h_feat, void *stream);
+/* PageRank warm start from the warm-start model (din 3, dout 1): h_norm holds
+ * the feature means[3], stds[3] and the target mean, std.  d_x0: V floats,
+ * normalised to sum 1 (pass it as gb_pr_params.d_x0). */
+gb_status gb_pr_warm_start(gb_graph g, const float *d_params, const float *h_norm,
+                           float *d_x0, void *stream);
+
 /* ---- synthetic graphs (host) -------------------------------------------------
  * Deterministic, counter-based generators used by tests and benchmarks.
  * Rows are sorted by (destination, weight).  Output arrays are host memory
