@@ -139,6 +139,150 @@ def run_bc_c3(gb, oracle, torch, np):
                 achieved_gbs=256 * B / s / 1e9, max_rel_err_vs_oracle_4src=rel)
 
 
+def _timed(torch, fn):
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t
+
+
+def run_learned_c5(gb, oracle, torch, np, cpu_budget):
+    """R-MAT 24 (seed 99, weights 50/50 uniform [1,255] / log-uniform
+    [1,65535]), 32 SSSP sources + PageRank to 1e-7: the plain GPU form (one
+    source per solve, delta = max_w / avg_degree; PageRank from 1/N) against
+    the learned + batched form (one 32-source solve with the predicted delta
+    and light threshold; PageRank from the predicted warm start).  Model
+    inference is inside the learned timings."""
+    from graphbench_amd import learned
+    models = learned.LearnedModels.load()
+    t0 = time.time()
+    h = gb.gen_rmat(24, 16, seed=99, weights="mix", wlo=1, whi=255)
+    r = h.transpose()
+    log(f"c5: generated R-MAT 24: V={h.V} E={h.E} in {time.time() - t0:.1f}s")
+    srcs = gb.pick_sources(h, 32, 99)
+    g = gb.Graph.from_host(h, reverse=r)
+    V, E = h.V, h.E
+    plain = learned.plain_delta(h)
+    gb.sssp(g, srcs[:1], delta=plain, light_max=plain)  # warm-up
+    d_plain = torch.empty((len(srcs), V), dtype=torch.int32, device="cuda")
+
+    def plain_sssp():
+        tr = 0
+        for i, s in enumerate(srcs):
+            _, st = gb.sssp(g, [s], delta=plain, light_max=plain, out=d_plain[i:i + 1], stats=True)
+            tr += st["traversed"]
+        return tr
+    traversed, t_sp = _timed(torch, plain_sssp)
+
+    def learned_sssp():
+        dl, lm = models.predict_delta(g, len(srcs))
+        out, st = gb.sssp(g, srcs, delta=dl, light_max=lm, stats=True)
+        return out, st, dl, lm
+    (d_learn, st_l, dl, lm), t_sl = _timed(torch, learned_sssp)
+    exact = bool(torch.equal(d_plain, d_learn))
+    del d_plain, d_learn
+    pr_kw = dict(damping=0.85, tol=1e-7, max_iter=100)
+    gb.pagerank(g, max_iter=3)  # warm-up (autotunes the tile size)
+    (x_p, res_p), t_pp = _timed(torch, lambda: gb.pagerank(g, **pr_kw))
+
+    def learned_pr():
+        x0 = models.warm_start(g)
+        return gb.pagerank(g, x0=x0, **pr_kw)
+    (x_w, res_w), t_pw = _timed(torch, learned_pr)
+    xp = x_p.cpu().numpy().astype(np.float64)
+    xw = x_w.cpu().numpy().astype(np.float64)
+    l1 = {"plain_vs_warm": float(np.abs(xp - xw).sum())}
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    tq = time.perf_counter()
+    oracle.pagerank_cfg(r.row_ptr, r.col, h.outdeg(), 0.85, 1e-10, 1)
+    if (time.perf_counter() - tq) * 40 <= 2 * cpu_budget:
+        ref, _, _ = oracle.pagerank_cfg(r.row_ptr, r.col, h.outdeg(), 0.85, 1e-10, 150)
+        l1["plain_vs_fp64_oracle"] = float(np.abs(xp - ref).sum())
+        l1["warm_vs_fp64_oracle"] = float(np.abs(xw - ref).sum())
+    work_pr = E * res_p["iterations"]  # the plain solve's edge work, credited to both forms
+    plain_s = t_sp + t_pp
+    learned_s = t_sl + t_pw
+    return dict(
+        workload="learned_rmat24_mix_32src_plus_pagerank", V=V, E=E,
+        sssp={"plain_ms": t_sp * 1e3, "learned_batched_ms": t_sl * 1e3,
+              "plain_delta": plain, "learned_delta": dl, "learned_light_max": lm,
+              "plain_gteps": traversed / t_sp / 1e9, "learned_batched_gteps": traversed / t_sl / 1e9,
+              "speedup": t_sp / t_sl, "distances_bit_identical": exact,
+              "buckets": st_l["buckets"], "phases": st_l["phases"]},
+        pagerank={"plain_ms": t_pp * 1e3, "warm_ms": t_pw * 1e3,
+                  "plain_iterations": res_p["iterations"], "warm_iterations": res_w["iterations"],
+                  "plain_gteps": work_pr / t_pp / 1e9, "warm_effective_gteps": work_pr / t_pw / 1e9,
+                  "speedup": t_pp / t_pw, "l1": l1},
+        combined={"plain_gteps": (traversed + work_pr) / plain_s / 1e9,
+                  "learned_batched_gteps": (traversed + work_pr) / learned_s / 1e9,
+                  "speedup": plain_s / learned_s})
+
+
+def c4_specs(n=512, seed=2024):
+    """512 directed graphs, V = round(2^(10 + 10 U)) (log-uniform in [1024,
+    1048576]); even indices R-MAT with edge factor U{8..32}, odd indices
+    Erdos-Renyi with average degree U{8..32}; weights uniform [1, 1023]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        V = int(round(2.0 ** (10 + 10 * rng.random())))
+        deg = int(rng.integers(8, 33))
+        out.append(("rmat" if i % 2 == 0 else "er", V, deg, 5000 + i))
+    return out
+
+
+def run_batched_c4(gb, oracle, torch, np):
+    """512 uneven graphs, one source each: one gb_sssp_multi solve with a
+    learned delta per graph, against 512 plain single-graph solves (delta =
+    max_w / avg_degree).  Distances of the two forms must be identical."""
+    from graphbench_amd import learned
+    models = learned.LearnedModels.load()
+    t0 = time.time()
+    hs = []
+    for kind, V, deg, seed in c4_specs():
+        if kind == "rmat":
+            sc = max(1, (V - 1).bit_length())
+            hs.append(gb.gen_rmat(sc, deg, seed=seed, weights="uniform", wlo=1, whi=1023, nverts=V))
+        else:
+            hs.append(gb.gen_er(V, deg, seed=seed, weights="uniform", wlo=1, whi=1023))
+    sumV = sum(h.V for h in hs)
+    sumE = sum(h.E for h in hs)
+    log(f"c4: generated 512 graphs, sum V={sumV} sum E={sumE} in {time.time() - t0:.1f}s")
+    gs = [gb.Graph.from_host(h, need_reverse=False) for h in hs]
+    srcs = [int(gb.pick_sources(h, 1, 7 + i)[0]) for i, h in enumerate(hs)]
+    outs_p = [torch.empty(h.V, dtype=torch.int32, device="cuda") for h in hs]
+    outs_l = [torch.empty(h.V, dtype=torch.int32, device="cuda") for h in hs]
+    gb.sssp(gs[0], [srcs[0]], out=outs_p[0].view(1, -1))  # warm-up
+
+    def plain():
+        tr = 0
+        for g, s, o in zip(gs, srcs, outs_p):
+            _, st = gb.sssp(g, [s], out=o.view(1, -1), stats=True)
+            tr += st["traversed"]
+        return tr
+    traversed, t_p = _timed(torch, plain)
+
+    def batched():
+        dl = models.predict_deltas(gs, 1)
+        _, st = gb.sssp_multi(gs, srcs, deltas=[d for d, _ in dl], light_max=[l for _, l in dl],
+                              outs=outs_l, stats=True)
+        return st
+    st_l, t_l = _timed(torch, batched)
+    exact = all(bool(torch.equal(a, b)) for a, b in zip(outs_p, outs_l))
+    small = sorted(range(len(hs)), key=lambda i: hs[i].E)[:4]
+    bad = sum(int((gb.u32(outs_l[i]) != oracle.dijkstra(hs[i].row_ptr, hs[i].col, hs[i].w,
+                                                          srcs[i])).sum()) for i in small)
+    B = sum(12 * h.E + 12 * h.V + 4 for h in hs)
+    return dict(workload="sssp_512_uneven_graphs", graphs=len(hs), sum_V=sumV, sum_E=sumE,
+                plain_ms=t_p * 1e3, batched_learned_ms=t_l * 1e3,
+                plain_gteps=traversed / t_p / 1e9, batched_learned_gteps=traversed / t_l / 1e9,
+                speedup=t_p / t_l, distances_bit_identical=exact,
+                mismatches_vs_dijkstra_4_smallest=bad, buckets=st_l["buckets"],
+                phases=st_l["phases"], achieved_gbs=B / t_l / 1e9)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,7 +292,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
-    ap.add_argument("--only", choices=["sssp", "bc"], default=None,
+    ap.add_argument("--only", choices=["sssp", "bc", "c4", "c5"], default=None,
                     help="time one secondary workload alone (diagnostics; no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -162,7 +306,10 @@ def main():
 
     gb.init(0)
     if args.only:
-        fn = {"sssp": run_sssp_c1, "bc": run_bc_c3}[args.only]
+        if args.only == "c5":
+            print(json.dumps(run_learned_c5(gb, oracle, torch, np, args.cpu_budget)), flush=True)
+            return
+        fn = {"sssp": run_sssp_c1, "bc": run_bc_c3, "c4": run_batched_c4}[args.only]
         print(json.dumps(fn(gb, oracle, torch, np)), flush=True)
         return
     log("pagerank R-MAT 22")
@@ -182,6 +329,10 @@ def main():
         extra["sssp_c1"] = run_sssp_c1(gb, oracle, torch, np)
         log("bc config 3")
         extra["bc_c3"] = run_bc_c3(gb, oracle, torch, np)
+        log("512 uneven graphs")
+        extra["batched_c4"] = run_batched_c4(gb, oracle, torch, np)
+        log("learned + batched vs plain, R-MAT 24")
+        extra["learned_c5"] = run_learned_c5(gb, oracle, torch, np, args.cpu_budget)
     line = {
         "metric": "GTEPS + % HBM peak (SSSP/PR/BC); learned-delta+batched vs plain, same GPU",
         "value": pr["gteps"],

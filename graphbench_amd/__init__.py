@@ -125,6 +125,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run `python -m graphbench_amd.build`")
+        # torch ships its own libamdhip64 (same soname).  Loading torch first
+        # makes the library bind to that runtime, so the process holds one HIP
+        # runtime and torch's device pointers and streams are valid here.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

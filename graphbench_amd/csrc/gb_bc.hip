@@ -1,24 +1,24 @@
-// gb_bc.hip -- BFS levels and Brandes betweenness for gfx950.
+// gb_bc.hip -- weighted Brandes betweenness (f64 weights) for gfx950.
 //
-// Replaces run_bc (btwcentral/csr_bc.c:16-130): the per-source SSSP with
-// predecessor lists, the distance sort (q_sort_double) and the sequential
-// comp_num_spath / comp_delta loops become level-synchronous kernels over a
-// batch of sources at once.  Every (source slot, vertex) pair is an entry of
-// one `order` array, grouped by level, which is also Brandes' stack.
-//   expand    one wave per level-d entry, lanes over its out-edges: the first
-//             CAS of level[x] from -1 to d+1 appends (slot, x) to level d+1.
-//   sigma     one wave per level-(d+1) entry, pull over its in-edges:
-//             sigma[x] = sum of sigma[u] with level[u] == d (FP64; integer
-//             values are exact below 2^53, so the order cannot matter).
-//   delta     backwards, one wave per level-d entry, pull over out-edges:
-//             delta[v] = sum ((1+delta[w])*sigma[v])/sigma[w], level[w]==d+1
-//             (the operand order of csr_bc.c:116), fixed-order wave sum.
+// Replaces the weighted form of run_bc (btwcentral/csr_bc.c:16-130): the
+// per-source run_sspath with predecessor lists, the distance sort
+// (q_sort_double) and the comp_num_spath / comp_delta loops become kernels over
+// a batch of sources on the tight-edge DAG of the exact distances.  An edge
+// u->x is tight when fl(d[u] + w) == d[x] (the '<=' predecessor rule of
+// csr_sspath.c:101-118 with each edge recorded once).  Every (source slot,
+// vertex) pair is an entry of one `order` array, grouped by DAG depth, which
+// is also Brandes' stack.
+//   indeg     count of tight in-edges per (slot, vertex)
+//   expand    one wave per depth-d entry, lanes over its out-edges: the last
+//             tight in-edge to be processed assigns depth d+1 (Kahn order).
+//   sigma     one wave per depth-(d+1) entry, pull over its tight in-edges
+//             (FP64; integer values are exact below 2^53).
+//   delta     backwards, one wave per depth-d entry, pull over tight out-edges:
+//             delta[v] = sum ((1+delta[w])*sigma[v])/sigma[w] (the operand
+//             order of csr_bc.c:116), fixed-order wave sum.
 //   accum     bc[v] += delta[slot][v] for v != source, slots in order
 //             (csr_bc.c:57-61, sums not halved).
-// Weighted graphs (f64 weights) use the same kernels on the tight-edge DAG of
-// the exact distances: depth replaces the BFS level, and an edge counts when
-// fl(d[u] + w) == d[x] (the '<=' predecessor rule of csr_sspath.c:101-118 with
-// each edge recorded once).
+// Unweighted BC and BFS levels are the batched kernels of gb_msbc.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,34 +50,6 @@ __global__ void bc_seed_kernel(int32_t V, int32_t nslots, const int32_t *__restr
   level[base + src[s]] = 0;
   if (sigma) sigma[base + src[s]] = 1.0;
   order[s] = ((uint64_t)s << 32) | (uint32_t)src[s];
-}
-
-// BFS discovery from level-d entries [lo, hi).
-__global__ __launch_bounds__(BC_BLOCK) void bc_expand_kernel(
-    int32_t V, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
-    uint64_t *__restrict__ order, uint64_t lo, uint64_t hi, int32_t d, int32_t *__restrict__ level,
-    unsigned long long *n_order) {
-  const int lane = gb_lane();
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = lo + wave; i < hi; i += nwaves) {
-    const uint64_t ent = order[i];
-    const uint32_t s = (uint32_t)(ent >> 32);
-    const int32_t u = (int32_t)(uint32_t)ent;
-    int32_t *lv = level + (int64_t)s * V;
-    const int32_t beg = rp[u], end = rp[u + 1];
-    for (int32_t b0 = beg; b0 < end; b0 += GB_WAVE) {
-      const int32_t e = b0 + lane;
-      bool push = false;
-      int32_t x = 0;
-      if (e < end) {
-        x = col[e];
-        if (lv[x] == -1) push = atomicCAS(&lv[x], -1, d + 1) == -1;
-      }
-      const uint64_t pb = gb_wave_reserve(n_order, push ? 1u : 0u);
-      if (push) order[pb] = ((uint64_t)s << 32) | (uint32_t)x;
-    }
-  }
 }
 
 // Tight-DAG discovery (weighted): decrement the count of unprocessed tight
@@ -138,17 +110,6 @@ __global__ __launch_bounds__(BC_BLOCK) void bcw_indeg_kernel(
   }
 }
 
-struct PredLevel {
-  const int32_t *level;
-  __device__ __forceinline__ bool in_edge(int64_t base, int32_t u, int32_t v, int32_t e,
-                                          int32_t d) const {
-    return level[base + u] == d - 1;
-  }
-  __device__ __forceinline__ bool out_edge(int64_t base, int32_t v, int32_t x, int32_t e,
-                                           int32_t d) const {
-    return level[base + x] == d + 1;
-  }
-};
 struct PredTight {
   const double *dist;
   const double *w;   // forward weights
@@ -250,7 +211,7 @@ int fill_grid(int64_t n) {
 }
 
 // Forward levels for nslots sources; returns level offsets (size depth+2).
-gb_status forward_levels(gb_graph g, int32_t nslots, bool weighted, hipStream_t st,
+gb_status forward_levels(gb_graph g, int32_t nslots, hipStream_t st,
                          std::vector<uint64_t> &off, bool with_sigma, int32_t *level) {
   BcWork &w = g_bcw;
   const int32_t V = g->V;
@@ -262,16 +223,11 @@ gb_status forward_levels(gb_graph g, int32_t nslots, bool weighted, hipStream_t 
   unsigned long long h = nslots;
   GB_HIP(hipMemcpyAsync(cnt, &h, sizeof(h), hipMemcpyHostToDevice, st));
   int32_t d = 0;
-  PredLevel pl{level};
   PredTight pt{w.dist.as<double>(), g->w64, g->rw64};
   for (;;) {
     const uint64_t lo = off[d], hi = off[d + 1];
     if (hi == lo) break;
-    if (!weighted)
-      hipLaunchKernelGGL(bc_expand_kernel, dim3(wave_grid(hi - lo)), dim3(BC_BLOCK), 0, st, V,
-                         g->rp, g->col, order, lo, hi, d, level, cnt);
-    else
-      hipLaunchKernelGGL(bcw_expand_kernel, dim3(wave_grid(hi - lo)), dim3(BC_BLOCK), 0, st, V,
+    hipLaunchKernelGGL(bcw_expand_kernel, dim3(wave_grid(hi - lo)), dim3(BC_BLOCK), 0, st, V,
                          g->rp, g->col, g->w64, w.dist.as<double>(), w.indeg.as<int32_t>(), order,
                          lo, hi, d, level, cnt);
     GB_HIP(hipGetLastError());
@@ -282,12 +238,8 @@ gb_status forward_levels(gb_graph g, int32_t nslots, bool weighted, hipStream_t 
     if (with_sigma && h > hi) {
       const int32_t *rrp = g->rrp ? g->rrp : g->rp;
       const int32_t *rcol = g->rcol ? g->rcol : g->col;
-      if (!weighted)
-        hipLaunchKernelGGL(bc_sigma_kernel<PredLevel>, dim3(wave_grid(h - hi)), dim3(BC_BLOCK), 0,
-                           st, V, rrp, rcol, order, hi, (uint64_t)h, d, pl, sigma);
-      else
-        hipLaunchKernelGGL(bc_sigma_kernel<PredTight>, dim3(wave_grid(h - hi)), dim3(BC_BLOCK), 0,
-                           st, V, rrp, rcol, order, hi, (uint64_t)h, d, pt, sigma);
+      hipLaunchKernelGGL(bc_sigma_kernel<PredTight>, dim3(wave_grid(h - hi)), dim3(BC_BLOCK), 0,
+                         st, V, rrp, rcol, order, hi, (uint64_t)h, d, pt, sigma);
       GB_HIP(hipGetLastError());
     }
   }
@@ -295,14 +247,14 @@ gb_status forward_levels(gb_graph g, int32_t nslots, bool weighted, hipStream_t 
 }
 
 gb_status bc_run(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t batch, double *d_bc,
-                 gb_bc_stats *stats, void *stream, bool weighted) {
+                 gb_bc_stats *stats, void *stream) {
   if (!g || (!h_sources && nsrc > 0) || nsrc < 0 || !d_bc)
     return gb_fail(GB_ERR_INVALID, "gb_bc: null argument");
   const int32_t V = g->V;
   for (int32_t i = 0; i < nsrc; i++)
     if (h_sources[i] < 0 || h_sources[i] >= V)
       return gb_fail(GB_ERR_INVALID, "gb_bc: source %d out of range", h_sources[i]);
-  if (weighted && (!g->w64 || (g->rrp && !g->rw64)))
+  if (!g->w64 || (g->rrp && !g->rw64))
     return gb_fail(GB_ERR_INVALID, "gb_bc_weighted: graph needs f64 weights (forward and reverse)");
   hipStream_t st = gb_stream(stream);
   if (batch <= 0 || batch > nsrc) batch = nsrc;
@@ -315,10 +267,8 @@ gb_status bc_run(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t bat
   GB_TRY(w.order.ensure(sizeof(uint64_t) * (n + 1)));
   GB_TRY(w.src.ensure(sizeof(int32_t) * batch));
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long)));
-  if (weighted) {
-    GB_TRY(w.indeg.ensure(sizeof(int32_t) * (n + 1)));
-    GB_TRY(w.dist.ensure(sizeof(double) * (n + 1)));
-  }
+  GB_TRY(w.indeg.ensure(sizeof(int32_t) * (n + 1)));
+  GB_TRY(w.dist.ensure(sizeof(double) * (n + 1)));
   hipEvent_t e0, e1;
   GB_HIP(hipEventCreate(&e0));
   GB_HIP(hipEventCreate(&e1));
@@ -330,41 +280,33 @@ gb_status bc_run(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t bat
     const int32_t ns = std::min(batch, nsrc - b0);
     const int64_t nn = (int64_t)ns * V;
     GB_HIP(hipMemcpyAsync(w.src.p, h_sources + b0, sizeof(int32_t) * ns, hipMemcpyHostToDevice, st));
-    if (weighted) {
-      // exact distances of the batch, then tight in-degree counts
-      // bucket width: the last f64 split of this graph, else the reference's
-      // BC constant (run_sspath(gs, pred, source, 0.1), csr_bc.c:43)
-      GB_TRY(gb_sssp_f64(g, g->split64_valid ? g->split64_delta : 0.1, h_sources + b0, ns,
-                         w.dist.as<double>(), nullptr, stream));
-      const int32_t *rrp = g->rrp ? g->rrp : g->rp;
-      const int32_t *rcol = g->rcol ? g->rcol : g->col;
-      const double *rw = g->rrp ? g->rw64 : g->w64;
-      hipLaunchKernelGGL(bcw_indeg_kernel, dim3(fill_grid(nn)), dim3(256), 0, st, V, ns, rrp, rcol,
-                         rw, w.dist.as<double>(), w.indeg.as<int32_t>());
-      GB_HIP(hipGetLastError());
-    }
+    // exact distances of the batch, then tight in-degree counts; bucket
+    // width: the last f64 split of this graph, else the reference's BC
+    // constant (run_sspath(gs, pred, source, 0.1), csr_bc.c:43)
+    GB_TRY(gb_sssp_f64(g, g->split64_valid ? g->split64_delta : 0.1, h_sources + b0, ns,
+                       w.dist.as<double>(), nullptr, stream));
+    const int32_t *rrp = g->rrp ? g->rrp : g->rp;
+    const int32_t *rcol = g->rcol ? g->rcol : g->col;
+    const double *rw = g->rrp ? g->rw64 : g->w64;
+    hipLaunchKernelGGL(bcw_indeg_kernel, dim3(fill_grid(nn)), dim3(256), 0, st, V, ns, rrp, rcol,
+                       rw, w.dist.as<double>(), w.indeg.as<int32_t>());
+    GB_HIP(hipGetLastError());
     hipLaunchKernelGGL(bc_fill_kernel, dim3(fill_grid(nn)), dim3(256), 0, st, nn,
                        w.level.as<int32_t>(), w.sigma.as<double>(), w.delta.as<double>());
     hipLaunchKernelGGL(bc_seed_kernel, dim3((ns + 255) / 256), dim3(256), 0, st, V, ns,
                        w.src.as<int32_t>(), w.level.as<int32_t>(), w.sigma.as<double>(),
                        w.order.as<uint64_t>());
     GB_HIP(hipGetLastError());
-    GB_TRY(forward_levels(g, ns, weighted, st, off, true, w.level.as<int32_t>()));
+    GB_TRY(forward_levels(g, ns, st, off, true, w.level.as<int32_t>()));
     const int32_t depth = (int32_t)off.size() - 2;  // number of non-empty levels
     max_depth = std::max(max_depth, depth);
-    PredLevel pl{w.level.as<int32_t>()};
     PredTight pt{w.dist.as<double>(), g->w64, g->rw64};
     for (int32_t d = depth - 1; d >= 0; d--) {
       const uint64_t lo = off[d], hi = off[d + 1];
       if (hi == lo) continue;
-      if (!weighted)
-        hipLaunchKernelGGL(bc_delta_kernel<PredLevel>, dim3(wave_grid(hi - lo)), dim3(BC_BLOCK), 0,
-                           st, V, g->rp, g->col, w.order.as<uint64_t>(), lo, hi, d, pl,
-                           w.sigma.as<double>(), w.delta.as<double>());
-      else
-        hipLaunchKernelGGL(bc_delta_kernel<PredTight>, dim3(wave_grid(hi - lo)), dim3(BC_BLOCK), 0,
-                           st, V, g->rp, g->col, w.order.as<uint64_t>(), lo, hi, d, pt,
-                           w.sigma.as<double>(), w.delta.as<double>());
+      hipLaunchKernelGGL(bc_delta_kernel<PredTight>, dim3(wave_grid(hi - lo)), dim3(BC_BLOCK), 0,
+                         st, V, g->rp, g->col, w.order.as<uint64_t>(), lo, hi, d, pt,
+                         w.sigma.as<double>(), w.delta.as<double>());
       GB_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(bc_accum_kernel, dim3((V + 255) / 256), dim3(256), 0, st, V, ns,
@@ -392,5 +334,5 @@ void gb_bc_release() { g_bcw.release(); }
 
 extern "C" gb_status gb_bc_weighted(gb_graph g, const int32_t *h_sources, int32_t nsrc,
                                     int32_t batch, double *d_bc, gb_bc_stats *st, void *stream) {
-  return bc_run(g, h_sources, nsrc, batch, d_bc, st, stream, true);
+  return bc_run(g, h_sources, nsrc, batch, d_bc, st, stream);
 }

@@ -74,6 +74,7 @@ struct gb_graph_s {
   bool max_w_valid = false;
   DevBuf inv_outdeg;  // float[V]
   bool inv_outdeg_valid = false;
+  int32_t pr_tuned_ipt = 0;  // items per thread picked by the autotuner (0 = not yet)
   int32_t pr_tile_items = 0;
   int64_t pr_ntiles = 0;
   DevBuf pr_tile_row, pr_tile_edge;

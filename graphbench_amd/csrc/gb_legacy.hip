@@ -241,8 +241,11 @@ void pr_print_hook(int32_t iter, const double *d_wb, void *ctx) {
 extern "C" {
 
 // ---- readers (graph/csr.c) --------------------------------------------------
+// The library's own entry points call the *_impl functions (internal
+// linkage), so a caller that also links the reference's graph/csr.o, whose
+// exported readers then take precedence, does not change what they run.
 
-void scan_csr_idx(t_csr *graph, char *file, char *dir, sint type) {
+static void scan_csr_idx_impl(t_csr *graph, char *file, char *dir, sint type) {
   FILE *f = open_in(dir, file, "r");
   char str[128];
   Tok t{f};
@@ -295,7 +298,7 @@ void scan_csr_idx(t_csr *graph, char *file, char *dir, sint type) {
   }
 }
 
-void read_graph_csr(t_csr *graph, char *file, char *dir, sint type) {
+static void read_graph_csr_impl(t_csr *graph, char *file, char *dir, sint type) {
   FILE *f = open_in(dir, file, "r");
   char str[128];
   Tok t{f};
@@ -332,7 +335,7 @@ void read_graph_csr(t_csr *graph, char *file, char *dir, sint type) {
   fclose(f);
 }
 
-void read_csr_bin(t_csr *graph, char *file, char *dir, sint type, int myrank) {
+static void read_csr_bin_impl(t_csr *graph, char *file, char *dir, sint type, int myrank) {
   (void)myrank;
   FILE *f = open_in(dir, file, "rb");
   lint hdr[4];
@@ -377,6 +380,16 @@ void read_csr_bin(t_csr *graph, char *file, char *dir, sint type, int myrank) {
       graph->edge_idx[pos[from - offset]++] = to - OFFSET;
   }
   for (lint i = 0; i < e_num; i++) graph->edge_info[i].edge_weight = (i % 2 == 0) ? 0.1 : 0.2;
+}
+
+void scan_csr_idx(t_csr *graph, char *file, char *dir, sint type) {
+  scan_csr_idx_impl(graph, file, dir, type);
+}
+void read_graph_csr(t_csr *graph, char *file, char *dir, sint type) {
+  read_graph_csr_impl(graph, file, dir, type);
+}
+void read_csr_bin(t_csr *graph, char *file, char *dir, sint type, int myrank) {
+  read_csr_bin_impl(graph, file, dir, type, myrank);
 }
 
 // ---- SSSP -------------------------------------------------------------------
@@ -430,10 +443,10 @@ void run_pagerank_csr_mpi(char *file, char *dir) {
   t_csr *gr = &gr_s;
   snprintf(f_buf, sizeof(f_buf), "%s%s_rc_%d_%d", data_extend, file, myrank, num_nodes);
   if (!bin) {
-    scan_csr_idx(gr, f_buf, dir, GB_LEGACY_RECV);
-    read_graph_csr(gr, f_buf, dir, GB_LEGACY_RECV);
+    scan_csr_idx_impl(gr, f_buf, dir, GB_LEGACY_RECV);
+    read_graph_csr_impl(gr, f_buf, dir, GB_LEGACY_RECV);
   } else {
-    read_csr_bin(gr, f_buf, dir, GB_LEGACY_RECV, myrank);
+    read_csr_bin_impl(gr, f_buf, dir, GB_LEGACY_RECV, myrank);
   }
   // read_recip (csr_partition.c:166-182 / read_recip_bin for `bin`)
   std::vector<double> recip(gr->v_size);
@@ -521,10 +534,10 @@ void run_bc(char *file, char *dir) {
   t_csr gs_s{};
   t_csr *gs = &gs_s;
   if (!bin) {
-    scan_csr_idx(gs, file, dir, GB_LEGACY_SEND);
-    read_graph_csr(gs, file, dir, GB_LEGACY_SEND);
+    scan_csr_idx_impl(gs, file, dir, GB_LEGACY_SEND);
+    read_graph_csr_impl(gs, file, dir, GB_LEGACY_SEND);
   } else {
-    read_csr_bin(gs, file, dir, GB_LEGACY_SEND, 0);
+    read_csr_bin_impl(gs, file, dir, GB_LEGACY_SEND, 0);
   }
   DevGraph dg;
   to_device(gs, dg, true, true);

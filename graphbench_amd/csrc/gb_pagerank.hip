@@ -356,12 +356,7 @@ PrWork &g_prw_ref() {
 }
 #define g_prw (g_prw_ref())
 
-int default_ipt(gb_graph g) {
-  (void)g;
-  const char *e = getenv("GB_PR_IPT");
-  if (e) return atoi(e);
-  return 8;
-}
+bool valid_ipt(int ipt) { return ipt == 4 || ipt == 8 || ipt == 12 || ipt == 16; }
 
 template <int IPT>
 void launch_gather(int64_t ntiles, hipStream_t st, const int32_t *irp, const int32_t *icol,
@@ -400,6 +395,44 @@ static gb_status ensure_inv_outdeg(gb_graph g, hipStream_t st) {
   return GB_OK;
 }
 
+// Items per thread for a graph: GB_PR_IPT if set, else the autotuner's pick.
+// The autotuner runs once per graph handle: three timed iterations for each
+// candidate tile size (256 x {4, 8, 12, 16} items), keeping the fastest mean
+// gather.  Tile size trades LDS staging and scan work (small tiles) against
+// occupancy and the cost of rows that span threads (large tiles).
+static gb_status pick_ipt(gb_graph g, const gb_pr_params *p, float *d_x, void *stream, int *out) {
+  const char *e = getenv("GB_PR_IPT");
+  if (e && valid_ipt(atoi(e))) {
+    *out = atoi(e);
+    return GB_OK;
+  }
+  if (g->pr_tuned_ipt) {
+    *out = g->pr_tuned_ipt;
+    return GB_OK;
+  }
+  int best = 8;
+  double best_ms = 1e300;
+  for (int cand : {4, 8, 12, 16}) {
+    gb_pr_params q = *p;
+    q.items_per_thread = cand;
+    q.max_iter = 3;
+    q.tol = 0.0;
+    q.use_graph = 0;
+    q.time_kernels = 1;
+    q.d_x0 = nullptr;
+    gb_pr_result r{};
+    GB_TRY(gb_pagerank(g, &q, d_x, &r, stream));
+    const double ms = r.gather_launches ? r.gather_ms / r.gather_launches : 1e300;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cand;
+    }
+  }
+  g->pr_tuned_ipt = best;
+  *out = best;
+  return GB_OK;
+}
+
 extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
                                  gb_pr_result *res, void *stream) {
   if (!g || !p || !d_x) return gb_fail(GB_ERR_INVALID, "gb_pagerank: null argument");
@@ -426,8 +459,10 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     }
   } guard{own};
   const int32_t V = g->V;
-  int ipt = p->items_per_thread > 0 ? p->items_per_thread : default_ipt(g);
-  if (ipt != 4 && ipt != 8 && ipt != 12 && ipt != 16) ipt = 8;
+  int ipt = p->items_per_thread;
+  if (ipt == 0) GB_TRY(pick_ipt(g, p, d_x, stream, &ipt));
+  if (!valid_ipt(ipt))
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: items_per_thread must be 0, 4, 8, 12 or 16");
   const int64_t tile = (int64_t)PR_BLOCK * ipt;
   GB_TRY(ensure_inv_outdeg(g, st));
   if (g->pr_tile_items != tile) {

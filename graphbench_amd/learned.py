@@ -33,7 +33,9 @@ import numpy as np
 from . import (GBError, Graph, HostCSR, _check, _ptr, _stream, gen_er, gen_rmat, lib,
                pagerank, pick_sources, sssp)
 
-MODELS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_models.json")
+MODELS_PATH = os.environ.get(
+    "GB_LEARNED_MODELS",
+    os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_models.json"))
 HIDDEN = 64
 
 
@@ -135,17 +137,19 @@ def _time_sssp(g, srcs, delta, light, reps=1):
 
 def delta_sweep(h: HostCSR, g: Graph, nsrc: int, seed: int, kmax: int = 18, kmin: int = 2):
     """Times batched delta-stepping over delta = 2^k (descending, stopping once
-    a run is 3x slower than the best) and then light_max / delta in {1/2, 1/4}
-    around the best k.  Returns the features, the soft-best target (mean of the
-    settings within 5% of the fastest) and the timings."""
+    a run is 3x slower than the best), then light_max / delta in {1/2, 1/4}
+    around the best k, and the plain rule.  The target is the mean
+    (log2 delta, log2 light_max/delta) of the settings within 5% of the fastest
+    and within 1.5 of its log2 delta (the time-vs-delta curve can have two
+    separate minima; averaging across them would land between them)."""
     srcs = pick_sources(h, nsrc, seed)
     _time_sssp(g, srcs[:1], 1 << 10, 1 << 10)  # warm-up (workspaces, code objects)
-    reps = 2 if h.E < (1 << 22) else 1
+    reps = 2 if h.E < (1 << 25) else 1
     rows = []
     best = float("inf")
     for k in range(kmax, kmin - 1, -1):
         t = _time_sssp(g, srcs, 1 << k, 1 << k, reps)
-        rows.append((k, 0.0, t))
+        rows.append((float(k), 0.0, t))
         best = min(best, t)
         if t > 3 * best:
             break
@@ -153,14 +157,15 @@ def delta_sweep(h: HostCSR, g: Graph, nsrc: int, seed: int, kmax: int = 18, kmin
     for k in (kb - 1, kb, kb + 1):
         for lr in (-1.0, -2.0):
             if kmin <= k <= kmax:
-                lm = max(1, int((1 << k) * 2.0 ** lr))
-                rows.append((k, lr, _time_sssp(g, srcs, 1 << k, lm, reps)))
-    best = min(t for _, _, t in rows)
-    good = [(k, lr) for k, lr, t in rows if t <= 1.05 * best]
-    target = np.array([np.mean([k for k, _ in good]), np.mean([lr for _, lr in good])], np.float32)
-    feats = graph_features(g, nsrc)
+                lm = max(1, int((1 << int(k)) * 2.0 ** lr))
+                rows.append((k, lr, _time_sssp(g, srcs, 1 << int(k), lm, reps)))
     plain = plain_delta(h)
     t_plain = _time_sssp(g, srcs, plain, plain, reps)
+    rows.append((math.log2(plain), 0.0, t_plain))
+    kb, _, best = min(rows, key=lambda r: r[2])
+    good = [(k, lr) for k, lr, t in rows if t <= 1.05 * best and abs(k - kb) <= 1.5]
+    target = np.array([np.mean([k for k, _ in good]), np.mean([lr for _, lr in good])], np.float32)
+    feats = graph_features(g, nsrc)
     return feats, target, dict(best_s=best, plain_s=t_plain, sweep=rows)
 
 
@@ -212,6 +217,21 @@ class LearnedModels:
         ratio = 2.0 ** float(np.clip(y[1], -3, 3))
         return max(1, delta), max(1, int(round(delta * ratio)))
 
+    def predict_deltas(self, graphs, nsrc: int = 1):
+        """(delta, light_max) per graph, with one batched forward pass."""
+        torch = _torch()
+        d = self.data["delta"]
+        F = np.stack([graph_features(g, nsrc) for g in graphs])
+        xs = (F - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
+        Y = self.delta_mlp.forward(torch.from_numpy(xs.astype(np.float32)).cuda()).cpu().numpy()
+        Y = Y * np.array(d["y_std"]) + np.array(d["y_mean"])
+        out = []
+        for y in Y:
+            delta = max(1, int(round(2.0 ** float(np.clip(y[0], 0, 24)))))
+            ratio = 2.0 ** float(np.clip(y[1], -3, 3))
+            out.append((delta, max(1, int(round(delta * ratio)))))
+        return out
+
     def warm_start(self, g: Graph):
         """x0 (V floats on the GPU, sum 1) for gb_pagerank."""
         torch = _torch()
@@ -224,6 +244,32 @@ class LearnedModels:
         return x0
 
 
+def _fit_with_holdout(din, dout, X, T, groups, max_steps, seed, log, lr=3e-3, every=50):
+    """Chooses the number of Adam steps on a 20% hold-out of whole graphs
+    (``groups``), then trains a fresh model on all rows for that many steps."""
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    ug = np.unique(groups)
+    val_g = rng.choice(ug, size=max(1, len(ug) // 5), replace=False)
+    val = np.isin(groups, val_g)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+    Xt, Tt, Xv, Tv = dev(X[~val]), dev(T[~val]), dev(X[val]), dev(T[val])
+    m = MLP(din, dout, seed=seed)
+    curve = []
+    for k in range(1, max_steps + 1):
+        m.loss_grad(Xt, Tt)
+        m.adam(lr=lr)
+        if k % every == 0:
+            curve.append((k, m.loss_grad(Xv, Tv)))
+    best_k, best_v = min(curve, key=lambda c: c[1])
+    log(f"  hold-out ({int(val.sum())} of {len(X)} rows): best {best_v:.4f} at step {best_k} "
+        f"(first {curve[0][1]:.4f}, last {curve[-1][1]:.4f})")
+    m = MLP(din, dout, seed=seed)
+    losses = m.fit(dev(X), dev(T), steps=best_k, lr=lr)
+    return m, losses, dict(val_rows=int(val.sum()), val_curve=curve, steps=best_k,
+                           val_loss=best_v)
+
+
 def _standardise(A):
     m = A.mean(axis=0)
     s = A.std(axis=0)
@@ -231,53 +277,87 @@ def _standardise(A):
     return m, s
 
 
-def train_all(out_path: str, quick: bool = False, log=print):
-    """Generates the training data on the GPU and fits both models."""
-    torch = _torch()
-    t0 = time.time()
-    # ---- delta model: sweeps over generated graphs, 1 / 8 / 32 sources each
+def delta_specs(quick: bool = False):
+    """The generated graphs the delta model is fitted on: R-MAT at several
+    scales / edge factors / weight laws, plus Erdos-Renyi."""
     specs = []
     scales = [12, 15, 18] if quick else [12, 14, 16, 18, 20]
+    laws = [("uniform", 255), ("uniform", 65535), ("mix", 255)]
     for sc in scales:
         for ef in ([16] if quick else [8, 16, 32]):
-            for wmode, whi in [("uniform", 255), ("uniform", 65535), ("mix", 255)]:
+            for wmode, whi in laws:
                 specs.append(("rmat", sc, ef, wmode, whi, 1000 + len(specs)))
+    if not quick:
+        for wmode, whi in laws:
+            specs.append(("rmat", 22, 16, wmode, whi, 1000 + len(specs)))
     for sc in ([14] if quick else [13, 16, 19]):
         for deg in [6, 24]:
             specs.append(("er", sc, deg, "uniform", 1023, 2000 + len(specs)))
-    feats, targets, info = [], [], []
-    for i, (kind, sc, ef, wmode, whi, seed) in enumerate(specs):
-        if kind == "rmat":
-            h = gen_rmat(sc, ef, seed=seed, weights=wmode, wlo=1, whi=whi)
-        else:
-            h = gen_er(1 << sc, ef, seed=seed, weights=wmode, wlo=1, whi=whi)
-        g = Graph.from_host(h, need_reverse=False)
-        for nsrc in (1, 8, 32):
-            f, t, meta = delta_sweep(h, g, nsrc, seed)
-            feats.append(f)
-            targets.append(t)
-            info.append(dict(kind=kind, scale=sc, ef=ef, weights=wmode, whi=whi, nsrc=nsrc,
-                             target=t.tolist(), best_ms=meta["best_s"] * 1e3,
-                             plain_ms=meta["plain_s"] * 1e3))
-            log(f"[delta {i+1}/{len(specs)}] {kind} s{sc} ef{ef} {wmode}{whi} nsrc{nsrc}: "
-                f"log2 delta*={t[0]:.2f} ratio={t[1]:.2f} best {meta['best_s']*1e3:.1f} ms "
-                f"plain {meta['plain_s']*1e3:.1f} ms ({time.time()-t0:.0f}s)")
-        g.close()
-        del h, g
+    return specs
+
+
+def collect_delta_data(specs, cache_path: Optional[str], log=print):
+    """Sweeps every spec for 1 / 8 / 32 sources.  Records are appended to
+    ``cache_path`` (JSON lines) as they are made, and specs already there are
+    not swept again, so an interrupted collection resumes."""
+    done = {}
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            for line in f:
+                r = json.loads(line)
+                done[(tuple(r["spec"]), r["nsrc"])] = r
+    t0 = time.time()
+    out = []
+    for i, spec in enumerate(specs):
+        kind, sc, ef, wmode, whi, seed = spec
+        todo = [n for n in (1, 8, 32) if (tuple(spec), n) not in done]
+        if todo:
+            if kind == "rmat":
+                h = gen_rmat(sc, ef, seed=seed, weights=wmode, wlo=1, whi=whi)
+            else:
+                h = gen_er(1 << sc, ef, seed=seed, weights=wmode, wlo=1, whi=whi)
+            g = Graph.from_host(h, need_reverse=False)
+            for nsrc in todo:
+                f, t, meta = delta_sweep(h, g, nsrc, seed)
+                r = dict(spec=list(spec), group=i, nsrc=nsrc, features=f.tolist(),
+                         target=t.tolist(), best_ms=meta["best_s"] * 1e3,
+                         plain_ms=meta["plain_s"] * 1e3,
+                         sweep=[[k, lr, ms * 1e3] for k, lr, ms in meta["sweep"]])
+                done[(tuple(spec), nsrc)] = r
+                if cache_path:
+                    with open(cache_path, "a") as fo:
+                        fo.write(json.dumps(r) + "\n")
+                log(f"[delta {i+1}/{len(specs)}] {kind} s{sc} ef{ef} {wmode}{whi} nsrc{nsrc}: "
+                    f"log2 delta*={t[0]:.2f} ratio={t[1]:.2f} best {r['best_ms']:.1f} ms "
+                    f"plain {r['plain_ms']:.1f} ms ({time.time()-t0:.0f}s)")
+            g.close()
+            del h, g
+        out.extend(done[(tuple(spec), n)] for n in (1, 8, 32))
+    return out
+
+
+def train_all(out_path: str, quick: bool = False, log=print, cache_path: Optional[str] = None):
+    """Generates the training data on the GPU and fits both models."""
+    torch = _torch()
+    t0 = time.time()
+    recs = collect_delta_data(delta_specs(quick), cache_path, log)
+    feats = [r["features"] for r in recs]
+    targets = [r["target"] for r in recs]
+    groups = [r["group"] for r in recs]
+    info = [dict(spec=r["spec"], nsrc=r["nsrc"], target=r["target"], best_ms=r["best_ms"],
+                 plain_ms=r["plain_ms"]) for r in recs]
     F = np.array(feats, np.float32)
     Tt = np.array(targets, np.float32)
     xm, xs = _standardise(F)
     ym, ys = _standardise(Tt)
-    Xd = torch.from_numpy(((F - xm) / xs).astype(np.float32)).cuda()
-    Td = torch.from_numpy(((Tt - ym) / ys).astype(np.float32)).cuda()
-    dm = MLP(6, 2, seed=1)
-    losses = dm.fit(Xd, Td, steps=400 if quick else 1500, lr=3e-3)
+    dm, losses, hold = _fit_with_holdout(6, 2, (F - xm) / xs, (Tt - ym) / ys, np.array(groups),
+                                         600 if quick else 2000, 1, log)
     log(f"delta model: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     delta_json = dict(din=6, dout=2, params=dm.params.cpu().numpy().tolist(),
                       x_mean=xm.tolist(), x_std=xs.tolist(), y_mean=ym.tolist(),
-                      y_std=ys.tolist(), final_loss=losses[-1], samples=info)
+                      y_std=ys.tolist(), final_loss=losses[-1], holdout=hold, samples=info)
     # ---- warm-start model: converged ranks
-    Xs, Ys = [], []
+    Xs, Ys, Gs = [], [], []
     pr_specs = [("rmat", 14, 16, 10), ("rmat", 16, 8, 11), ("rmat", 18, 16, 12),
                 ("rmat", 19, 32, 13), ("er", 16, 16, 14), ("rmat", 20, 16, 15)]
     if quick:
@@ -288,20 +368,19 @@ def train_all(out_path: str, quick: bool = False, log=print):
         X, Y = pr_samples(h, 60000, seed)
         Xs.append(X)
         Ys.append(Y)
+        Gs.append(np.full(len(X), len(Gs)))
         log(f"[warm start] {kind} s{sc} ef{ef}: {len(X)} samples ({time.time()-t0:.0f}s)")
         del h
     X = np.concatenate(Xs)
     Y = np.concatenate(Ys)
     xm2, xs2 = _standardise(X)
     ym2, ys2 = _standardise(Y)
-    Xd = torch.from_numpy(((X - xm2) / xs2).astype(np.float32)).cuda()
-    Td = torch.from_numpy(((Y - ym2) / ys2).astype(np.float32)).cuda()
-    pm = MLP(3, 1, seed=2)
-    losses2 = pm.fit(Xd, Td, steps=300 if quick else 1000, lr=3e-3)
+    pm, losses2, hold2 = _fit_with_holdout(3, 1, (X - xm2) / xs2, (Y - ym2) / ys2,
+                                           np.concatenate(Gs), 400 if quick else 1200, 2, log)
     log(f"warm-start model: loss {losses2[0]:.4f} -> {losses2[-1]:.4f}")
     pr_json = dict(din=3, dout=1, params=pm.params.cpu().numpy().tolist(), x_mean=xm2.tolist(),
                    x_std=xs2.tolist(), y_mean=ym2.tolist(), y_std=ys2.tolist(),
-                   final_loss=losses2[-1], samples=int(len(X)))
+                   final_loss=losses2[-1], holdout=hold2, samples=int(len(X)))
     data = dict(format="graphbench-mi355x learned models v1", delta=delta_json,
                 warm_start=pr_json, train_seconds=time.time() - t0)
     with open(out_path, "w") as f:
@@ -315,10 +394,11 @@ def main(argv=None):
     ap.add_argument("cmd", choices=["train"])
     ap.add_argument("--out", default=MODELS_PATH)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--cache", default=None, help="JSON-lines file of sweep records (resumable)")
     a = ap.parse_args(argv)
     from . import init
     init(0)
-    train_all(a.out, quick=a.quick, log=lambda m: print(m, flush=True))
+    train_all(a.out, quick=a.quick, log=lambda m: print(m, flush=True), cache_path=a.cache)
 
 
 if __name__ == "__main__":

@@ -261,7 +261,7 @@ def _pr_case(gb, h, **kw):
     return x, res, ref, it
 
 
-@pytest.mark.parametrize("ipt", [4, 8, 16])
+@pytest.mark.parametrize("ipt", [0, 4, 8, 12, 16])
 def test_pagerank_rmat(gpu, ipt):
     gb = gpu
     h = gb.gen_rmat(15, 16, seed=7, weights="none")
@@ -271,6 +271,14 @@ def test_pagerank_rmat(gpu, ipt):
     assert l1 <= PR_L1, l1
     assert abs(res["iterations"] - it) <= 1
     assert abs(x.sum() - 1.0) < 1e-5
+
+
+def test_pagerank_bad_items_per_thread_rejected(gpu):
+    gb = gpu
+    h = gb.gen_rmat(8, 8, seed=3, weights="none")
+    g = gb.Graph.from_host(h, reverse=h.transpose())
+    with pytest.raises(gb.GBError):
+        gb.pagerank(g, items_per_thread=5)
 
 
 def test_pagerank_hubs_span_tiles(gpu):
