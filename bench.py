@@ -114,7 +114,7 @@ def run_sssp_c1(gb, oracle, torch, np):
               for i, x in enumerate(srcs))
     return dict(workload="sssp_rmat16_8src", V=h.V, E=h.E, ms=s * 1e3,
                 gteps=st["traversed"] / s / 1e9, buckets=st["buckets"], phases=st["phases"],
-                mismatches_vs_dijkstra=bad)
+                relaxations=st["relax_edges"], mismatches_vs_dijkstra=bad)
 
 
 def run_bc_c3(gb, oracle, torch, np):
@@ -209,7 +209,8 @@ def run_learned_c5(gb, oracle, torch, np, cpu_budget):
               "plain_delta": plain, "learned_delta": dl, "learned_light_max": lm,
               "plain_gteps": traversed / t_sp / 1e9, "learned_batched_gteps": traversed / t_sl / 1e9,
               "speedup": t_sp / t_sl, "distances_bit_identical": exact,
-              "buckets": st_l["buckets"], "phases": st_l["phases"]},
+              "buckets": st_l["buckets"], "phases": st_l["phases"],
+              "relaxations": st_l["relax_edges"], "traversed": traversed},
         pagerank={"plain_ms": t_pp * 1e3, "warm_ms": t_pw * 1e3,
                   "plain_iterations": res_p["iterations"], "warm_iterations": res_w["iterations"],
                   "plain_gteps": work_pr / t_pp / 1e9, "warm_effective_gteps": work_pr / t_pw / 1e9,
@@ -280,7 +281,8 @@ def run_batched_c4(gb, oracle, torch, np):
                 plain_gteps=traversed / t_p / 1e9, batched_learned_gteps=traversed / t_l / 1e9,
                 speedup=t_p / t_l, distances_bit_identical=exact,
                 mismatches_vs_dijkstra_4_smallest=bad, buckets=st_l["buckets"],
-                phases=st_l["phases"], achieved_gbs=B / t_l / 1e9)
+                phases=st_l["phases"], relaxations=st_l["relax_edges"],
+                achieved_gbs=B / t_l / 1e9)
 
 
 def main():

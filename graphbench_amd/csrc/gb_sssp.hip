@@ -29,6 +29,7 @@
 
 #include "gb_device.hpp"
 #include "gb_internal.hpp"
+#include "gb_sssp_ctl.hpp"
 
 namespace {
 
@@ -97,9 +98,6 @@ struct STask {
   int32_t src;
 };
 
-// counters (device, u64): near A/B, S, far A/B, min bucket, relaxations, overflow
-enum { C_NEAR0 = 0, C_NEAR1 = 1, C_S = 2, C_FAR0 = 3, C_FAR1 = 4, C_MINB = 5, C_RELAX = 6,
-       C_OVF = 7, C_N = 8 };
 
 template <class Tr>
 struct Ctx {
@@ -162,8 +160,8 @@ __device__ __forceinline__ double gb_shfl_any<double>(double v, int src) {
 }
 
 template <class Tr, bool HEAVY>
-__global__ __launch_bounds__(SS_BLOCK) void sssp_expand_kernel(
-    Ctx<Tr> c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
+__device__ __forceinline__ void expand_body(
+    const Ctx<Tr> &c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
     uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ s_out,
     unsigned long long *n_s, uint64_t *__restrict__ far_out, unsigned long long *n_far,
     uint64_t bucket, uint32_t gen_next, uint32_t sround, uint32_t fepoch, uint32_t epw) {
@@ -267,8 +265,8 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_expand_kernel(
 }
 
 template <class Tr>
-__global__ __launch_bounds__(SS_BLOCK) void sssp_far_scan_kernel(
-    Ctx<Tr> c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
+__device__ __forceinline__ void far_scan_body(
+    const Ctx<Tr> &c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
     uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ far_out,
     unsigned long long *n_far, uint64_t bnext, uint32_t gen_next, uint32_t fepoch_next) {
   const int lane = gb_lane();
@@ -315,6 +313,48 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_far_scan_kernel(
   if (lane == 0 && minb != ~0ull) atomicMin(&c.cnt[C_MINB], (unsigned long long)minb);
 }
 
+// ---- device-driven phase loop ----------------------------------------------
+// The bucket loop runs as a state machine on the device, so that the host
+// neither reads queue sizes nor chooses launches: sssp_ctrl_kernel (one lane)
+// looks at the counters left by the previous phase, advances the loop state,
+// clears the counters the next phase appends to, and writes the next action;
+// sssp_step_kernel (fixed grid) performs that action.  Pairs of the two are
+// captured in a hipGraph and replayed until the state says done.
+// Queues: 0/1 near (double-buffered), 2 settled set S, 3/4 far (double-
+// buffered); queue k's length is counter k.
+struct Queues {
+  uint64_t *q[5];
+};
+
+__global__ void sssp_ctrl_kernel(unsigned long long *__restrict__ cnt, SsspCtl *__restrict__ ctl,
+                                 uint32_t cus) {
+  if (threadIdx.x != 0) return;
+  SsspCtl s = *ctl;
+  sssp_ctl_advance(s, cnt, cus);
+  *ctl = s;
+}
+
+template <class Tr>
+__global__ __launch_bounds__(SS_BLOCK) void sssp_step_kernel(Ctx<Tr> c, Queues q,
+                                                              const SsspCtl *__restrict__ ctl) {
+  const int act = ctl->action;
+  if (act == ACT_NONE) return;
+  unsigned long long *cnt = c.cnt;
+  const int in = ctl->a_in, nr = ctl->a_near, fr = ctl->a_far;
+  if (act == ACT_FAR) {
+    far_scan_body<Tr>(c, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[fr], cnt + fr,
+                      ctl->a_bucket, ctl->a_gen, ctl->a_fepoch);
+  } else if (act == ACT_LIGHT) {
+    expand_body<Tr, false>(c, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[C_S], cnt + C_S,
+                           q.q[fr], cnt + fr, ctl->a_bucket, ctl->a_gen, ctl->a_sround,
+                           ctl->a_fepoch, ctl->a_epw);
+  } else {
+    expand_body<Tr, true>(c, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[C_S], cnt + C_S,
+                          q.q[fr], cnt + fr, ctl->a_bucket, ctl->a_gen, ctl->a_sround,
+                          ctl->a_fepoch, ctl->a_epw);
+  }
+}
+
 // traversed edges: sum of out-degrees of reached (task, vertex) pairs
 template <class Tr>
 __global__ __launch_bounds__(SS_BLOCK) void sssp_count_kernel(Ctx<Tr> c, int32_t ntasks,
@@ -343,9 +383,9 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_count_kernel(Ctx<Tr> c, int32_t
 }
 
 struct SsspWork {
-  DevBuf graphs, tasks, nst, sst, fst, q[5], cnt, counts;
+  DevBuf graphs, tasks, nst, sst, fst, q[5], cnt, counts, ctl;
   void release() {
-    for (DevBuf *b : {&graphs, &tasks, &nst, &sst, &fst, &cnt, &counts}) b->release();
+    for (DevBuf *b : {&graphs, &tasks, &nst, &sst, &fst, &cnt, &counts, &ctl}) b->release();
     for (auto &b : q) b.release();
   }
 };
@@ -356,23 +396,6 @@ SsspWork &g_ssw_ref() {
   return *w;
 }
 #define g_ssw (g_ssw_ref())
-
-// Entries per wave for an expansion of n queue entries: enough waves to
-// cover the GPU (32 per CU) before packing up to 64 entries into a wave.
-uint32_t entries_per_wave(uint64_t n) {
-  const uint64_t target = (uint64_t)gb_num_cus() * 32;
-  uint64_t e = (n + target - 1) / target;
-  return (uint32_t)(e < 1 ? 1 : (e > GB_WAVE ? GB_WAVE : e));
-}
-
-int expand_grid(uint64_t n_entries, uint32_t epw = GB_WAVE) {
-  // one wave per epw queue entries, capped at 16 blocks (64 waves) per CU
-  uint64_t waves = (n_entries + epw - 1) / epw;
-  uint64_t blocks = (waves + 3) / 4;
-  uint64_t cap = (uint64_t)gb_num_cus() * 16;
-  if (blocks > cap) blocks = cap;
-  return blocks < 1 ? 1 : (int)blocks;
-}
 
 template <class Tr>
 gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
@@ -402,6 +425,7 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   GB_TRY(w.fst.ensure(sizeof(uint32_t) * total));
   for (int k = 0; k < 5; k++) GB_TRY(w.q[k].ensure(sizeof(uint64_t) * qcap));
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * C_N));
+  GB_TRY(w.ctl.ensure(sizeof(SsspCtl)));
   GB_HIP(hipMemcpyAsync(w.graphs.p, hg.data(), sizeof(G) * hg.size(), hipMemcpyHostToDevice, st));
   GB_HIP(hipMemcpyAsync(w.tasks.p, ht.data(), sizeof(STask<Tr>) * ht.size(),
                         hipMemcpyHostToDevice, st));
@@ -409,9 +433,6 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   GB_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * C_N, st));
   Ctx<Tr> c{w.graphs.as<G>(), w.tasks.as<STask<Tr>>(), w.nst.as<uint32_t>(), w.sst.as<uint32_t>(),
             w.fst.as<uint32_t>(), cnt};
-  uint64_t *near[2] = {w.q[0].as<uint64_t>(), w.q[1].as<uint64_t>()};
-  uint64_t *S = w.q[2].as<uint64_t>();
-  uint64_t *far[2] = {w.q[3].as<uint64_t>(), w.q[4].as<uint64_t>()};
 
   hipEvent_t e0, e1;
   GB_HIP(hipEventCreate(&e0));
@@ -422,85 +443,95 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
     if (gx < 1) gx = 1;
     hipLaunchKernelGGL(sssp_fill_kernel<Tr>, dim3(gx, ntasks), dim3(SS_BLOCK), 0, st, c, ntasks);
   }
-  uint32_t gen = 1, sround = 0, fepoch = 1;
+  const uint32_t gen = 1;
   hipLaunchKernelGGL(sssp_seed_kernel<Tr>, dim3((ntasks + 255) / 256), dim3(256), 0, st, c, ntasks,
-                     near[0], cnt + C_NEAR0, gen);
+                     w.q[0].as<uint64_t>(), cnt + C_NEAR0, gen);
   GB_HIP(hipGetLastError());
-  unsigned long long h[C_N];
-  auto read = [&]() -> gb_status {
-    GB_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
-    GB_HIP(hipStreamSynchronize(st));
-    return GB_OK;
+  // the loop state starts at bucket 0 with the seeds in near queue 0
+  SsspCtl hc{};
+  hc.gen = gen;
+  hc.sround = 0;
+  hc.fepoch = 1;
+  hc.max_phases = max_phases > 0 ? max_phases : ((int64_t)1 << 40);
+  hc.state = ST_BUCKET;
+  GB_HIP(hipMemcpyAsync(w.ctl.p, &hc, sizeof(hc), hipMemcpyHostToDevice, st));
+  SsspCtl *ctl = w.ctl.as<SsspCtl>();
+  Queues qs{{w.q[0].as<uint64_t>(), w.q[1].as<uint64_t>(), w.q[2].as<uint64_t>(),
+             w.q[3].as<uint64_t>(), w.q[4].as<uint64_t>()}};
+  const int grid = gb_num_cus() * 16;
+  const uint32_t cus = (uint32_t)gb_num_cus();
+  auto one_phase = [&]() {
+    hipLaunchKernelGGL(sssp_ctrl_kernel, dim3(1), dim3(64), 0, st, cnt, ctl, cus);
+    hipLaunchKernelGGL(sssp_step_kernel<Tr>, dim3(grid), dim3(SS_BLOCK), 0, st, c, qs,
+                       (const SsspCtl *)ctl);
   };
-  auto zero = [&](int k) -> gb_status {
-    GB_HIP(hipMemsetAsync(cnt + k, 0, sizeof(unsigned long long), st));
-    return GB_OK;
-  };
-  GB_TRY(read());
-  int cur = 0, fcur = 0;
-  uint64_t b = 0;
-  int64_t phases = 0, buckets = 0;
-  const int64_t cap = max_phases > 0 ? max_phases : (int64_t)1 << 40;
-  gb_status status = GB_OK;
-  for (;;) {
-    buckets++;
-    bool again;
-    do {
-      again = false;
-      sround++;
-      GB_TRY(zero(C_S));
-      while (h[C_NEAR0 + cur] > 0) {
-        if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
-        gen++;
-        GB_TRY(zero(C_NEAR0 + (cur ^ 1)));
-        const uint32_t epw = entries_per_wave(h[C_NEAR0 + cur]);
-        hipLaunchKernelGGL((sssp_expand_kernel<Tr, false>),
-                           dim3(expand_grid(h[C_NEAR0 + cur], epw)), dim3(SS_BLOCK), 0, st, c,
-                           near[cur], cnt + C_NEAR0 + cur, near[cur ^ 1], cnt + C_NEAR0 + (cur ^ 1),
-                           S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur, b, gen, sround, fepoch,
-                           epw);
-        GB_HIP(hipGetLastError());
-        cur ^= 1;
-        GB_TRY(read());
-      }
-      if (h[C_S] > 0) {
-        if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
-        gen++;
-        GB_TRY(zero(C_NEAR0 + cur));
-        const uint32_t epw = entries_per_wave(h[C_S]);
-        hipLaunchKernelGGL((sssp_expand_kernel<Tr, true>), dim3(expand_grid(h[C_S], epw)),
-                           dim3(SS_BLOCK), 0, st, c, S, cnt + C_S, near[cur], cnt + C_NEAR0 + cur,
-                           S, cnt + C_S, far[fcur], cnt + C_FAR0 + fcur, b, gen, sround, fepoch,
-                           epw);
-        GB_HIP(hipGetLastError());
-        GB_TRY(read());
-        again = h[C_NEAR0 + cur] > 0;  // a heavy edge landed back in bucket b
-      }
-    } while (again);
-    if (h[C_FAR0 + fcur] == 0) break;
-    uint64_t bnext = b + 1;
-    for (;;) {
-      if (++phases > cap) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
-      gen++;
-      GB_TRY(zero(C_NEAR0 + cur));
-      GB_TRY(zero(C_FAR0 + (fcur ^ 1)));
-      GB_HIP(hipMemsetAsync(cnt + C_MINB, 0xFF, sizeof(unsigned long long), st));
-      hipLaunchKernelGGL(sssp_far_scan_kernel<Tr>, dim3(expand_grid(h[C_FAR0 + fcur])),
-                         dim3(SS_BLOCK), 0, st, c, far[fcur], cnt + C_FAR0 + fcur, near[cur],
-                         cnt + C_NEAR0 + cur, far[fcur ^ 1], cnt + C_FAR0 + (fcur ^ 1), bnext, gen,
-                         fepoch + 1);
-      GB_HIP(hipGetLastError());
-      fepoch++;
-      fcur ^= 1;
-      GB_TRY(read());
-      if (h[C_NEAR0 + cur] > 0 || h[C_FAR0 + fcur] == 0) break;
-      bnext = h[C_MINB];
-    }
-    if (h[C_NEAR0 + cur] == 0) break;
-    b = bnext;
+  // Chunks of PH phases are captured once per call and replayed until the
+  // state is done; a replay past the end costs two near-empty launches per
+  // phase.  The null stream cannot be captured: a blocking stream, ordered
+  // with it, is used for the replays instead.
+  constexpr int PH = 16;
+  hipStream_t cs = st;
+  hipStream_t own = nullptr;
+  if (!st) {
+    GB_HIP(hipStreamCreate(&own));
+    cs = own;
+    hipEvent_t ready;
+    GB_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    GB_HIP(hipEventRecord(ready, st));
+    GB_HIP(hipStreamWaitEvent(cs, ready, 0));
+    (void)hipEventDestroy(ready);
   }
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  gb_status gst = GB_OK;
+  {
+    hipStream_t keep = st;
+    st = cs;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (int k = 0; k < PH; k++) one_phase();
+      e = hipStreamEndCapture(cs, &graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    st = keep;
+    if (e != hipSuccess) gst = gb_fail(GB_ERR_HIP, "sssp: graph capture failed: %s", hipGetErrorString(e));
+  }
+  SsspCtl hs{};
+  int64_t last_phases = -1;
+  while (gst == GB_OK) {
+    if (hipGraphLaunch(exec, cs) != hipSuccess ||
+        hipMemcpyAsync(&hs, ctl, sizeof(hs), hipMemcpyDeviceToHost, cs) != hipSuccess ||
+        hipStreamSynchronize(cs) != hipSuccess) {
+      gst = gb_fail(GB_ERR_HIP, "sssp: phase replay failed");
+      break;
+    }
+    if (hs.done) break;
+    // every replay that is not done issues PH actions; guard the host loop
+    if (hs.phases == last_phases) {
+      gst = gb_fail(GB_ERR_HIP, "sssp: bucket loop made no progress");
+      break;
+    }
+    last_phases = hs.phases;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (own) {
+    (void)hipStreamSynchronize(own);
+    (void)hipStreamDestroy(own);
+  }
+  if (gst == GB_OK && hs.capped) gst = gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
+  if (gst != GB_OK) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return gst;
+  }
+  GB_HIP(hipGetLastError());
+  gb_status status = GB_OK;
+  const int64_t phases = hs.phases, buckets = hs.buckets;
   GB_HIP(hipEventRecord(e1, st));
-  GB_TRY(read());
+  unsigned long long h[C_N];
+  GB_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
   if (h[C_OVF]) status = gb_fail(GB_ERR_OVERFLOW, "sssp: uint32 distance overflow");
   float ms = 0.0f;
   GB_HIP(hipEventSynchronize(e1));
