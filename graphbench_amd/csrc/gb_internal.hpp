@@ -74,10 +74,18 @@ struct gb_graph_s {
   bool max_w_valid = false;
   DevBuf inv_outdeg;  // float[V]
   bool inv_outdeg_valid = false;
-  int32_t pr_tuned_ipt = 0;  // items per thread picked by the autotuner (0 = not yet)
+  // PageRank gather shape {items per thread, hot-set size} picked by the
+  // autotuner, per (fixed items per thread or none, relabelled or not);
+  // items per thread 0 = not tuned yet
+  int32_t pr_tuned[10][2] = {};
   int32_t pr_tile_items = 0;
+  bool pr_tile_relabeled = false;
   int64_t pr_ntiles = 0;
   DevBuf pr_tile_row, pr_tile_edge;
+  // PageRank relabelling: vertices renumbered by out-degree (descending), the
+  // in-CSR and 1/outdeg in the new numbering (gb_graph_pr_relabel)
+  bool rl_valid = false;
+  DevBuf rl_perm, rl_rank, rl_rrp, rl_rcol, rl_inv;
   uint32_t split_light_max = 0;
   bool split_valid = false;
   DevBuf split_col, split_w, split_le;
@@ -87,6 +95,11 @@ struct gb_graph_s {
 };
 
 hipStream_t gb_stream(void *s);
+// Builds (once per handle) the out-degree relabelling used by the PageRank
+// gather; needs the reverse CSR.  Sets g->rl_*.
+gb_status gb_graph_pr_relabel(gb_graph g, hipStream_t st);
+// Out-degrees (int32[V]) from the forward CSR, or counted from the reverse.
+gb_status gb_graph_outdeg(gb_graph g, int32_t *d_deg, hipStream_t st);
 int gb_num_cus();
 // Max weight of the graph's uint32 weights (cached).
 gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st);

@@ -73,14 +73,16 @@ __device__ __forceinline__ double block_sum_f64(double v, double *s_red) {
   return t;
 }
 
+// perm (optional): gather numbering -> caller's numbering, for x0
 __global__ __launch_bounds__(PR_BLOCK) void pr_init_kernel(
-    int32_t V, const float *__restrict__ x0, const float *__restrict__ inv_od,
-    float *__restrict__ x, float *__restrict__ contrib, double *__restrict__ part_dang) {
+    int32_t V, const float *__restrict__ x0, const int32_t *__restrict__ perm,
+    const float *__restrict__ inv_od, float *__restrict__ x, float *__restrict__ contrib,
+    double *__restrict__ part_dang) {
   __shared__ double s_red[PR_BLOCK / 64];
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double dang = 0.0;
   if (v < V) {
-    float xv = x0 ? x0[v] : (float)(1.0 / (double)V);
+    float xv = x0 ? x0[perm ? perm[v] : v] : (float)(1.0 / (double)V);
     float io = inv_od[v];
     x[v] = xv;
     contrib[v] = xv * io;
@@ -107,6 +109,13 @@ __global__ void pr_init_finalize_kernel(int32_t nparts, const double *__restrict
   }
 }
 
+// ranks back to the caller's numbering: x[old] = x_gather[rank[old]]
+__global__ void pr_unpermute_kernel(int32_t V, const int32_t *__restrict__ rank,
+                                    const float *__restrict__ xg, float *__restrict__ x) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < V) x[v] = xg[rank[v]];
+}
+
 struct RowOut {
   float base, damping;
   const float *__restrict__ x_old;
@@ -125,23 +134,34 @@ struct RowOut {
   }
 };
 
-template <int IPT>
+// HOT > 0: the contributions of vertices [0, HOT) -- the highest out-degrees
+// once the graph is relabelled -- are copied into LDS once per block, and the
+// block walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...; gathers of hot
+// sources are served from LDS instead of the L2.
+template <int IPT, int HOT>
 __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
     const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
     const int32_t *__restrict__ tile_row, const int32_t *__restrict__ tile_edge,
     const float *__restrict__ contrib, RowOut out, F2 *__restrict__ tail,
     F2 *__restrict__ head, double *__restrict__ part_res, double *__restrict__ part_dang,
-    const PrState *__restrict__ st) {
+    const PrState *__restrict__ st, int64_t ntiles) {
   constexpr int TILE = PR_BLOCK * IPT;
   __shared__ int32_t s_end[TILE];
   __shared__ float s_val[TILE];
   __shared__ F2 s_scan[PR_BLOCK];
   __shared__ int32_t s_flag[PR_BLOCK];
   __shared__ double s_red[PR_BLOCK / 64];
+  __shared__ float s_hot[HOT > 0 ? HOT : 1];
   if (st->done) return;
   out.base = st->base;
-  const int t = blockIdx.x;
   const int tid = threadIdx.x;
+  if (HOT > 0) {
+    const int nh = min((int64_t)HOT, (int64_t)st->V);
+    for (int k = tid; k < nh; k += PR_BLOCK) s_hot[k] = contrib[k];
+    __syncthreads();
+  }
+  for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+  const int64_t t = tt;
   const int32_t r0 = tile_row[t], e0 = tile_edge[t];
   const int32_t r1 = tile_row[t + 1], e1 = tile_edge[t + 1];
   const int nrows = r1 - r0, nedges = e1 - e0;
@@ -156,7 +176,10 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
 #pragma unroll
   for (int m = 0; m < IPT; ++m) {
     const int k = tid + m * PR_BLOCK;
-    if (k < nedges) s_val[k] = contrib[cidx[m]];
+    if (k < nedges) {
+      const int32_t c = cidx[m];
+      s_val[k] = (HOT > 0 && c < HOT) ? s_hot[c] : contrib[c];
+    }
   }
 #pragma unroll
   for (int m = 0; m < IPT; ++m) {
@@ -235,6 +258,8 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
   if (tid == 0) {
     part_res[t] = rs;
     part_dang[t] = ds;
+  }
+  __syncthreads();  // the next tile reuses the LDS staging
   }
 }
 
@@ -342,9 +367,10 @@ __global__ void pr_ref_update_kernel(int32_t V, const double *__restrict__ recip
 }
 
 struct PrWork {
-  DevBuf xb, ca, cb, tail, head, pres, pdang, fres, fdang, state, counter;
+  DevBuf xa, xb, ca, cb, tail, head, pres, pdang, fres, fdang, state, counter;
   void release() {
-    for (DevBuf *b : {&xb, &ca, &cb, &tail, &head, &pres, &pdang, &fres, &fdang, &state, &counter})
+    for (DevBuf *b :
+         {&xa, &xb, &ca, &cb, &tail, &head, &pres, &pdang, &fres, &fdang, &state, &counter})
       b->release();
   }
 };
@@ -358,12 +384,51 @@ PrWork &g_prw_ref() {
 
 bool valid_ipt(int ipt) { return ipt == 4 || ipt == 8 || ipt == 12 || ipt == 16; }
 
-template <int IPT>
+// Blocks of a persistent (HOT > 0) gather: as many as are resident at once.
+template <int IPT, int HOT>
+int64_t gather_grid(int64_t ntiles) {
+  if (HOT == 0) return ntiles;
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pr_gather_kernel<IPT, HOT>, PR_BLOCK,
+                                                     0) != hipSuccess ||
+        nb < 1)
+      nb = 1;
+    per_cu = nb;
+  }
+  return std::min<int64_t>(ntiles, (int64_t)gb_num_cus() * per_cu);
+}
+
+template <int IPT, int HOT>
 void launch_gather(int64_t ntiles, hipStream_t st, const int32_t *irp, const int32_t *icol,
                    const int32_t *trow, const int32_t *tedge, const float *contrib, RowOut out,
                    F2 *tail, F2 *head, double *pres, double *pdang, const PrState *state) {
-  hipLaunchKernelGGL(pr_gather_kernel<IPT>, dim3((unsigned)ntiles), dim3(PR_BLOCK), 0, st, irp,
-                     icol, trow, tedge, contrib, out, tail, head, pres, pdang, state);
+  const int64_t grid = gather_grid<IPT, HOT>(ntiles);
+  hipLaunchKernelGGL((pr_gather_kernel<IPT, HOT>), dim3((unsigned)std::max<int64_t>(grid, 1)),
+                     dim3(PR_BLOCK), 0, st, irp, icol, trow, tedge, contrib, out, tail, head, pres,
+                     pdang, state, ntiles);
+}
+
+// gather variant by (items per thread, hot-set size)
+template <int IPT>
+void launch_gather_hot(int hot, int64_t ntiles, hipStream_t st, const int32_t *irp,
+                       const int32_t *icol, const int32_t *trow, const int32_t *tedge,
+                       const float *contrib, RowOut out, F2 *tail, F2 *head, double *pres,
+                       double *pdang, const PrState *state) {
+  switch (hot) {
+    case 4096:
+      launch_gather<IPT, 4096>(ntiles, st, irp, icol, trow, tedge, contrib, out, tail, head, pres,
+                               pdang, state);
+      break;
+    case 8192:
+      launch_gather<IPT, 8192>(ntiles, st, irp, icol, trow, tedge, contrib, out, tail, head, pres,
+                               pdang, state);
+      break;
+    default:
+      launch_gather<IPT, 0>(ntiles, st, irp, icol, trow, tedge, contrib, out, tail, head, pres,
+                            pdang, state);
+  }
 }
 
 }  // namespace
@@ -395,52 +460,90 @@ static gb_status ensure_inv_outdeg(gb_graph g, hipStream_t st) {
   return GB_OK;
 }
 
-// Items per thread for a graph: GB_PR_IPT if set, else the autotuner's pick.
-// The autotuner runs once per graph handle: three timed iterations for each
-// candidate tile size (256 x {4, 8, 12, 16} items), keeping the fastest mean
-// gather.  Tile size trades LDS staging and scan work (small tiles) against
-// occupancy and the cost of rows that span threads (large tiles).
-static gb_status pick_ipt(gb_graph g, const gb_pr_params *p, float *d_x, void *stream, int *out) {
-  const char *e = getenv("GB_PR_IPT");
-  if (e && valid_ipt(atoi(e))) {
-    *out = atoi(e);
+static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_pr_result *res,
+                              void *stream, int ipt, int hot);
+
+static bool use_relabel(gb_graph g, const gb_pr_params *p) {
+  return p->relabel > 0 || (p->relabel == 0 && g->E >= ((int64_t)1 << 20));
+}
+
+// Gather shape for a graph: items per thread (tile = 256 x ipt items) and
+// the LDS hot-set size.  GB_PR_IPT / GB_PR_HOT override; otherwise the
+// autotuner runs once per graph handle and layout: three timed iterations
+// for each candidate, keeping the fastest mean gather.  Small tiles cost
+// staging and scan work per item, large ones occupancy and rows that span
+// threads; a hot set (relabelled graphs only) takes the most-read sources off
+// the L2 but its LDS lowers occupancy.
+static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void *stream,
+                            int *ipt_out, int *hot_out) {
+  const bool rl = use_relabel(g, p);
+  const char *ei = getenv("GB_PR_IPT");
+  const char *eh = getenv("GB_PR_HOT");
+  // fixed parts of the shape (0 / -1 = left to the tuner)
+  int fixed_ipt = p->items_per_thread;
+  if (!fixed_ipt && ei && valid_ipt(atoi(ei))) fixed_ipt = atoi(ei);
+  int fixed_hot = -1;
+  if (eh && (atoi(eh) == 0 || atoi(eh) == 4096 || atoi(eh) == 8192)) fixed_hot = atoi(eh);
+  if (!rl) fixed_hot = 0;
+  if (fixed_ipt && fixed_hot >= 0) {
+    *ipt_out = fixed_ipt;
+    *hot_out = fixed_hot;
     return GB_OK;
   }
-  if (g->pr_tuned_ipt) {
-    *out = g->pr_tuned_ipt;
+  // cached per (fixed ipt or none, layout)
+  const int key = (fixed_ipt ? fixed_ipt / 4 : 0) * 2 + (rl ? 1 : 0);
+  if (g->pr_tuned[key][0]) {
+    *ipt_out = g->pr_tuned[key][0];
+    *hot_out = g->pr_tuned[key][1];
     return GB_OK;
   }
-  int best = 8;
+  std::vector<int> ipts = fixed_ipt ? std::vector<int>{fixed_ipt} : std::vector<int>{4, 8, 12, 16};
+  std::vector<int> hots =
+      fixed_hot >= 0 ? std::vector<int>{fixed_hot} : std::vector<int>{0, 4096, 8192};
+  int best_i = ipts[0], best_h = hots[0];
   double best_ms = 1e300;
-  for (int cand : {4, 8, 12, 16}) {
-    gb_pr_params q = *p;
-    q.items_per_thread = cand;
-    q.max_iter = 3;
-    q.tol = 0.0;
-    q.use_graph = 0;
-    q.time_kernels = 1;
-    q.d_x0 = nullptr;
-    gb_pr_result r{};
-    GB_TRY(gb_pagerank(g, &q, d_x, &r, stream));
-    const double ms = r.gather_launches ? r.gather_ms / r.gather_launches : 1e300;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = cand;
+  for (int ci : ipts)
+    for (int ch : hots) {
+      gb_pr_params q = *p;
+      q.max_iter = 3;
+      q.tol = 0.0;
+      q.use_graph = 0;
+      q.time_kernels = 1;
+      q.d_x0 = nullptr;
+      gb_pr_result r{};
+      GB_TRY(pagerank_run(g, &q, d_x, &r, stream, ci, ch));
+      const double ms = r.gather_launches ? r.gather_ms / r.gather_launches : 1e300;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_i = ci;
+        best_h = ch;
+      }
     }
-  }
-  g->pr_tuned_ipt = best;
-  *out = best;
+  g->pr_tuned[key][0] = best_i;
+  g->pr_tuned[key][1] = best_h;
+  *ipt_out = best_i;
+  *hot_out = best_h;
   return GB_OK;
 }
 
 extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
                                  gb_pr_result *res, void *stream) {
   if (!g || !p || !d_x) return gb_fail(GB_ERR_INVALID, "gb_pagerank: null argument");
+  if (p->items_per_thread != 0 && !valid_ipt(p->items_per_thread))
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: items_per_thread must be 0, 4, 8, 12 or 16");
   if (!g->rrp || (!g->rcol && g->E > 0))
     return gb_fail(GB_ERR_INVALID, "gb_pagerank: graph has no reverse CSR");
   if (g->V <= 0) return gb_fail(GB_ERR_INVALID, "gb_pagerank: empty graph");
   if (!(p->damping >= 0.0 && p->damping <= 1.0) || p->max_iter < 0)
     return gb_fail(GB_ERR_INVALID, "gb_pagerank: bad damping/max_iter");
+  int ipt = 8, hot = 0;
+  GB_TRY(ensure_inv_outdeg(g, gb_stream(stream)));
+  GB_TRY(pick_shape(g, p, d_x, stream, &ipt, &hot));
+  return pagerank_run(g, p, d_x, res, stream, ipt, hot);
+}
+
+static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_pr_result *res,
+                              void *stream, int ipt, int hot) {
   hipStream_t st = gb_stream(stream);
   hipStream_t own = nullptr;
   if (!st && p->use_graph && !p->time_kernels) {
@@ -459,25 +562,30 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     }
   } guard{own};
   const int32_t V = g->V;
-  int ipt = p->items_per_thread;
-  if (ipt == 0) GB_TRY(pick_ipt(g, p, d_x, stream, &ipt));
-  if (!valid_ipt(ipt))
-    return gb_fail(GB_ERR_INVALID, "gb_pagerank: items_per_thread must be 0, 4, 8, 12 or 16");
   const int64_t tile = (int64_t)PR_BLOCK * ipt;
   GB_TRY(ensure_inv_outdeg(g, st));
-  if (g->pr_tile_items != tile) {
+  const bool relabel = use_relabel(g, p);
+  if (relabel) GB_TRY(gb_graph_pr_relabel(g, st));
+  if (!relabel) hot = 0;
+  // the in-CSR and 1/outdeg the gather runs on
+  const int32_t *irp = relabel ? g->rl_rrp.as<int32_t>() : g->rrp;
+  const int32_t *icol = relabel ? g->rl_rcol.as<int32_t>() : g->rcol;
+  const float *inv = relabel ? g->rl_inv.as<float>() : g->inv_outdeg.as<float>();
+  if (g->pr_tile_items != tile || g->pr_tile_relabeled != relabel) {
     const int64_t ntiles = ((int64_t)V + g->E + tile - 1) / tile;
     GB_TRY(g->pr_tile_row.ensure(sizeof(int32_t) * (ntiles + 1)));
     GB_TRY(g->pr_tile_edge.ensure(sizeof(int32_t) * (ntiles + 1)));
-    GB_TRY(gb_merge_path_split(V, g->rrp, tile, ntiles, g->pr_tile_row.as<int32_t>(),
+    GB_TRY(gb_merge_path_split(V, irp, tile, ntiles, g->pr_tile_row.as<int32_t>(),
                                g->pr_tile_edge.as<int32_t>(), st));
     g->pr_tile_items = (int32_t)tile;
+    g->pr_tile_relabeled = relabel;
     g->pr_ntiles = ntiles;
   }
   const int64_t ntiles = g->pr_ntiles;
   const int64_t nfix = (ntiles + PR_BLOCK - 1) / PR_BLOCK;
   const int64_t vblocks = ((int64_t)V + PR_BLOCK - 1) / PR_BLOCK;
   PrWork &w = g_prw;
+  GB_TRY(w.xa.ensure(sizeof(float) * V));
   GB_TRY(w.xb.ensure(sizeof(float) * V));
   GB_TRY(w.ca.ensure(sizeof(float) * V));
   GB_TRY(w.cb.ensure(sizeof(float) * V));
@@ -497,11 +605,11 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
   hs.V = V;
   GB_HIP(hipMemcpyAsync(w.state.p, &hs, sizeof(hs), hipMemcpyHostToDevice, st));
   PrState *state = w.state.as<PrState>();
-  float *xa = d_x, *xb = w.xb.as<float>();
+  float *xa = w.xa.as<float>(), *xb = w.xb.as<float>();
   float *ca = w.ca.as<float>(), *cb = w.cb.as<float>();
-  const float *inv = g->inv_outdeg.as<float>();
   hipLaunchKernelGGL(pr_init_kernel, dim3((unsigned)vblocks), dim3(PR_BLOCK), 0, st, V, p->d_x0,
-                     inv, xa, ca, w.pdang.as<double>());
+                     relabel ? (const int32_t *)g->rl_perm.as<int32_t>() : nullptr, inv, xa, ca,
+                     w.pdang.as<double>());
   hipLaunchKernelGGL(pr_init_finalize_kernel, dim3(1), dim3(1024), 0, st, (int32_t)vblocks,
                      w.pdang.as<double>(), state);
   GB_HIP(hipGetLastError());
@@ -515,28 +623,26 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     if (ev0) (void)hipEventRecord(ev0, st);
     switch (ipt) {
       case 4:
-        launch_gather<4>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
-                         w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
-                         w.pdang.as<double>(), state);
+        launch_gather_hot<4>(hot, ntiles, st, irp, icol, trow, tedge, c_old, out, w.tail.as<F2>(),
+                             w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
         break;
       case 12:
-        launch_gather<12>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
-                          w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
-                          w.pdang.as<double>(), state);
+        launch_gather_hot<12>(hot, ntiles, st, irp, icol, trow, tedge, c_old, out,
+                              w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
+                              w.pdang.as<double>(), state);
         break;
       case 16:
-        launch_gather<16>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
-                          w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
-                          w.pdang.as<double>(), state);
+        launch_gather_hot<16>(hot, ntiles, st, irp, icol, trow, tedge, c_old, out,
+                              w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
+                              w.pdang.as<double>(), state);
         break;
       default:
-        launch_gather<8>(ntiles, st, g->rrp, g->rcol, trow, tedge, c_old, out,
-                         w.tail.as<F2>(), w.head.as<F2>(), w.pres.as<double>(),
-                         w.pdang.as<double>(), state);
+        launch_gather_hot<8>(hot, ntiles, st, irp, icol, trow, tedge, c_old, out, w.tail.as<F2>(),
+                             w.head.as<F2>(), w.pres.as<double>(), w.pdang.as<double>(), state);
     }
     if (ev1) (void)hipEventRecord(ev1, st);
     hipLaunchKernelGGL(pr_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, ntiles,
-                       g->rrp, trow, tedge, w.tail.as<F2>(), w.head.as<F2>(), out,
+                       irp, trow, tedge, w.tail.as<F2>(), w.head.as<F2>(), out,
                        w.pres.as<double>(), w.pdang.as<double>(), w.fres.as<double>(),
                        w.fdang.as<double>(), w.counter.as<unsigned int>(), state);
   };
@@ -601,8 +707,13 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
   GB_HIP(hipGetLastError());
   GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
   GB_HIP(hipStreamSynchronize(st));
-  if (host_state.iter % 2 == 1)
-    GB_HIP(hipMemcpyAsync(d_x, xb, sizeof(float) * V, hipMemcpyDeviceToDevice, st));
+  const float *xf = host_state.iter % 2 == 1 ? xb : xa;
+  if (relabel)
+    hipLaunchKernelGGL(pr_unpermute_kernel, dim3((unsigned)vblocks), dim3(PR_BLOCK), 0, st, V,
+                       g->rl_rank.as<int32_t>(), xf, d_x);
+  else
+    GB_HIP(hipMemcpyAsync(d_x, xf, sizeof(float) * V, hipMemcpyDeviceToDevice, st));
+  GB_HIP(hipGetLastError());
   GB_HIP(hipStreamSynchronize(st));
   const bool conv = host_state.residual < p->tol;
   if (res) {
@@ -613,6 +724,10 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     res->gather_ms = gather_ms;
     res->gather_launches = gather_launches;
     res->tiles = (int32_t)ntiles;
+    res->items_per_thread = ipt;
+    res->hot = hot;
+    res->relabeled = relabel ? 1 : 0;
+    res->reserved = 0;
   }
   return GB_OK;
 }

@@ -142,6 +142,9 @@ typedef struct gb_pr_params {
   int32_t use_graph; /* capture iteration chunks in a hipGraph (1) or not (0) */
   int32_t time_kernels; /* record per-launch gather times with HIP events */
   const float *d_x0; /* optional warm start (device, V; NULL = 1/N) */
+  int32_t relabel;   /* gather in out-degree order: 0 auto (E >= 2^20), 1 on, -1 off;
+                        built once per graph handle, results in the caller's ids */
+  int32_t reserved;
 } gb_pr_params;
 
 typedef struct gb_pr_result {
@@ -152,6 +155,10 @@ typedef struct gb_pr_result {
   double gather_ms;      /* summed gather-kernel time (time_kernels=1) */
   int32_t gather_launches;
   int32_t tiles;
+  int32_t items_per_thread; /* gather shape used: tile = 256 x items_per_thread */
+  int32_t hot;              /* sources served from LDS (0 = none) */
+  int32_t relabeled;        /* 1 if the gather ran in out-degree order */
+  int32_t reserved;
 } gb_pr_result;
 
 /* Configured PageRank: in-edge pull gather over the reverse CSR, FP32 ranks

@@ -273,6 +273,25 @@ def test_pagerank_rmat(gpu, ipt):
     assert abs(x.sum() - 1.0) < 1e-5
 
 
+@pytest.mark.parametrize("relabel", [-1, 1])
+def test_pagerank_relabel_matches_oracle(gpu, relabel):
+    """The out-degree relabelled gather gives the oracle's ranks in the
+    caller's ids, with and without a warm start, on skewed and small graphs."""
+    gb = gpu
+    for h in (gb.gen_rmat(14, 16, seed=21, weights="none"),
+              gb.gen_er(3000, 5, seed=22, weights="none"),
+              gb.HostCSR.from_edges(6, [0, 1, 2, 3], [1, 2, 0, 0])):
+        _say(f"pagerank relabel={relabel} V={h.V} E={h.E}")
+        x, res, ref, it = _pr_case(gb, h, relabel=relabel)
+        assert np.abs(x - ref).sum() <= PR_L1
+        r = h.transpose()
+        g = gb.Graph.from_host(h, reverse=r)
+        x0 = gb.to_device(np.asarray(ref, np.float32))
+        xw, rw = gb.pagerank(g, x0=x0, relabel=relabel)
+        assert rw["iterations"] <= 2
+        assert np.abs(xw.cpu().numpy().astype(np.float64) - ref).sum() <= PR_L1
+
+
 def test_pagerank_bad_items_per_thread_rejected(gpu):
     gb = gpu
     h = gb.gen_rmat(8, 8, seed=3, weights="none")
