@@ -71,6 +71,7 @@ struct gb_graph_s {
   DevBuf own_rp, own_col, own_w, own_rrp, own_rcol, own_rw;
   // cached derived data
   uint32_t max_w = 0;  // 0 = not computed
+  int32_t max_outdeg = -1;  // -1 = not computed
   bool max_w_valid = false;
   DevBuf inv_outdeg;  // float[V]
   bool inv_outdeg_valid = false;
@@ -101,6 +102,8 @@ gb_status gb_graph_pr_relabel(gb_graph g, hipStream_t st);
 // Out-degrees (int32[V]) from the forward CSR, or counted from the reverse.
 gb_status gb_graph_outdeg(gb_graph g, int32_t *d_deg, hipStream_t st);
 int gb_num_cus();
+// Largest out-degree (cached; needs the forward CSR or the reverse).
+gb_status gb_graph_max_outdeg(gb_graph g, int32_t *out, hipStream_t st);
 // Max weight of the graph's uint32 weights (cached).
 gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st);
 // Light-first ordering of the forward CSR for a threshold (cached per handle).

@@ -4,19 +4,23 @@
 // (pagerank/pagerank_mpi.c:833-846 and 995-1020) on one GPU.
 //
 // Configured form (gb_pagerank): x_k[v] = base_k + d * sum_{u->v} x_{k-1}[u]/outdeg[u],
-// base_k = (1-d)/N + d * dangling_{k-1}/N.  One iteration = three launches:
+// base_k = (1-d)/N + d * dangling_{k-1}/N.  One iteration = two launches:
 //   pr_gather  merge-path tiles over the in-CSR (rows + edges split evenly),
 //              edge values gathered into LDS with coalesced col reads, each
 //              thread consumes IPT items with a compensated FP32 sum, a block
 //              segmented scan joins rows split between threads, the epilogue
 //              writes x_k, contrib_k = x_k/outdeg and FP64 residual/dangling
-//              block partials.  A row that began in an earlier tile is left to
-//   pr_fixup   which adds the earlier tiles' tail partials in tile order.
-//   pr_finalize one block: FP64 sums of the partials in a fixed order, the
-//              next base, and the stop flag (L1 residual < tol or max_iter).
+//              tile partials.  A row that began in an earlier tile is left to
+//   pr_fixup   which adds the earlier tiles' tail partials in tile order and
+//              sums the tile partials per block; the last block to finish
+//              reduces the block sums in a fixed order, sets the next base
+//              and the stop flag (L1 residual < tol or max_iter).
 // Every reduction has a fixed order, so results are run-to-run identical.
 // Kernels read the stop flag first, so chunks of iterations are captured in
 // one hipGraph and replayed until the flag is set.
+// Large graphs run in out-degree order (gb_relabel.hip): the hot sources'
+// contributions then share cache lines.  The gather shape (items per thread,
+// an optional LDS copy of the hottest contributions) is autotuned per graph.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -263,9 +267,9 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
   }
 }
 
-// Finishes rows that began in an earlier tile, then the last block to finish
-// reduces every partial (gather tiles first, then fix-up blocks, fixed order)
-// and sets the next base and the stop flag.
+// Finishes rows that began in an earlier tile and sums, per block, the gather
+// partials of its tiles with its own; the last block to finish reduces the
+// block sums (fixed order) and sets the next base and the stop flag.
 __global__ __launch_bounds__(PR_BLOCK) void pr_fixup_kernel(
     int64_t ntiles, const int32_t *__restrict__ irp, const int32_t *__restrict__ tile_row,
     const int32_t *__restrict__ tile_edge, const F2 *__restrict__ tail,
@@ -280,6 +284,10 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_fixup_kernel(
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.0, dang = 0.0;
   if (t < ntiles) {
+    // this tile's gather partials join the fix-up block's sum, so the last
+    // block reduces one value per fix-up block instead of one per tile
+    res = p_res[t];
+    dang = p_dang[t];
     const int32_t r0 = tile_row[t], e0 = tile_edge[t], r1 = tile_row[t + 1];
     if (r1 > r0) {
       const int32_t rs = irp[r0];
@@ -306,10 +314,6 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_fixup_kernel(
   __threadfence();
   const int64_t n2 = gridDim.x;
   double a = 0.0, b = 0.0;
-  for (int64_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
-    a += p_res[i];
-    b += p_dang[i];
-  }
   for (int64_t i = threadIdx.x; i < n2; i += blockDim.x) {
     a += __builtin_nontemporal_load(fix_res + i);
     b += __builtin_nontemporal_load(fix_dang + i);

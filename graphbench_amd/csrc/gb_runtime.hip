@@ -359,6 +359,32 @@ __global__ void max_u32_kernel(const uint32_t *__restrict__ a, int64_t n, unsign
   if (gb_lane() == 0) atomicMax(out, m);
 }
 
+gb_status gb_graph_max_outdeg(gb_graph g, int32_t *out, hipStream_t st) {
+  if (g->max_outdeg >= 0) {
+    *out = g->max_outdeg;
+    return GB_OK;
+  }
+  if (g->E == 0 || g->V == 0) {
+    *out = g->max_outdeg = 0;
+    return GB_OK;
+  }
+  DevBuf deg, tmp;
+  GB_TRY(deg.ensure(sizeof(int32_t) * g->V));
+  GB_TRY(tmp.ensure(sizeof(unsigned int)));
+  GB_TRY(gb_graph_outdeg(g, deg.as<int32_t>(), st));
+  GB_HIP(hipMemsetAsync(tmp.p, 0, sizeof(unsigned int), st));
+  int blocks = (int)std::min<int64_t>((g->V + 255) / 256, 4096);
+  hipLaunchKernelGGL(max_u32_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st,
+                     (const uint32_t *)deg.as<int32_t>(), (int64_t)g->V, tmp.as<unsigned int>());
+  GB_HIP(hipGetLastError());
+  unsigned int h = 0;
+  GB_HIP(hipMemcpyAsync(&h, tmp.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  g->max_outdeg = (int32_t)h;
+  *out = g->max_outdeg;
+  return GB_OK;
+}
+
 gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st) {
   if (g->max_w_valid) {
     *out = g->max_w;

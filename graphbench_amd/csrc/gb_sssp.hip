@@ -3,7 +3,7 @@
 // Replaces run_sspath_arr (sssp/csr_sspath_array.c:149-363) and the distance
 // part of run_sspath (sssp/csr_sspath.c:38-122).  The reference's per-bucket
 // pipeline request_arr -> merge_sort -> reorder_arr -> relax_arr becomes:
-//   light phase  one wave per queue entry (a vertex, or a 1024-edge chunk of
+//   light phase  waves take queue entries (a vertex, or a 256-edge chunk of
 //                a hub's light edges); lanes relax consecutive edges with
 //                atomicMin on the distance; an improved target whose new
 //                distance is still in the current bucket is appended to the
@@ -34,7 +34,7 @@
 namespace {
 
 constexpr int SS_BLOCK = 256;
-constexpr int CHUNK_SHIFT = 10;  // 1024 edges per queue entry
+constexpr int CHUNK_SHIFT = 8;  // 256 edges per queue entry at most
 constexpr int32_t CHUNK = 1 << CHUNK_SHIFT;
 
 __device__ __host__ __forceinline__ uint32_t nchunks(int32_t deg) {
@@ -59,7 +59,12 @@ struct TU32 {
     return (D)nd;
   }
   static __device__ __forceinline__ D amin(D *p, D v) { return atomicMin(p, v); }
-  static __device__ __forceinline__ uint64_t bucket(D d, DT delta) { return (uint64_t)d / delta; }
+  // floor(d / delta) as a multiply-high by magic = floor((2^64 - 1) / delta) + 1,
+  // exact for d, delta < 2^32 (delta = 1: magic = 0, the quotient is d)
+  static __device__ __forceinline__ uint64_t bucket(D d, DT delta, uint64_t magic) {
+    return magic ? __umul64hi((uint64_t)d, magic) : (uint64_t)d;
+  }
+  static uint64_t host_magic(DT delta) { return delta > 1 ? (~0ull / delta) + 1 : 0; }
 };
 struct TF64 {
   using D = double;
@@ -73,9 +78,10 @@ struct TF64 {
         atomicMin((unsigned long long *)p, (unsigned long long)__double_as_longlong(v));
     return __longlong_as_double((long long)old);
   }
-  static __device__ __forceinline__ uint64_t bucket(D d, DT delta) {
+  static __device__ __forceinline__ uint64_t bucket(D d, DT delta, uint64_t) {
     return (uint64_t)(long long)(d / delta);
   }
+  static uint64_t host_magic(DT) { return 0; }
 };
 
 template <class W>
@@ -94,6 +100,7 @@ struct STask {
   typename Tr::D *dist;
   int64_t off;  // offset of this task's stamps
   typename Tr::DT delta;
+  uint64_t magic;  // bucket division constant of delta
   int32_t graph;
   int32_t src;
 };
@@ -107,35 +114,74 @@ struct Ctx {
   uint32_t *sst;  // settled-set round stamp
   uint32_t *fst;  // far epoch stamp
   unsigned long long *cnt;
+  // one graph and one delta, task t's distances at dist0 + t * V (sources of a
+  // single graph): task data then follow from t without per-edge lookups
+  typename Tr::D *dist0;
+  typename Tr::DT delta0;
+  uint64_t magic0;
 };
 
-template <class Tr>
+// What the kernels need of task t: its distances and stamps, bucket width
+// and graph.  With SINGLE the graph is the kernel-wide G0.
+template <class Tr, bool SINGLE>
+struct TaskView {
+  typename Tr::D *dist;
+  int64_t off;
+  typename Tr::DT delta;
+  uint64_t magic;
+  SGraph<typename Tr::W> G;
+  __device__ __forceinline__ TaskView(const Ctx<Tr> &c, const SGraph<typename Tr::W> &G0,
+                                      uint32_t t) {
+    if constexpr (SINGLE) {
+      off = (int64_t)t * G0.V;
+      dist = c.dist0 + off;
+      delta = c.delta0;
+      magic = c.magic0;
+      G = G0;
+    } else {
+      const STask<Tr> tk = c.tasks[t];
+      dist = tk.dist;
+      off = tk.off;
+      delta = tk.delta;
+      magic = tk.magic;
+      G = c.graphs[tk.graph];
+    }
+  }
+  __device__ __forceinline__ typename Tr::D *d(uint32_t v) const { return dist + v; }
+  __device__ __forceinline__ int64_t s(uint32_t v) const { return off + v; }
+};
+
+template <class Tr, bool SINGLE>
 __global__ __launch_bounds__(SS_BLOCK) void sssp_fill_kernel(Ctx<Tr> c, int32_t ntasks) {
   // grid.y = task
   const int32_t t = blockIdx.y;
   if (t >= ntasks) return;
-  const STask<Tr> tk = c.tasks[t];
-  const int32_t V = c.graphs[tk.graph].V;
+  SGraph<typename Tr::W> G0{};
+  if constexpr (SINGLE) G0 = c.graphs[0];
+  const TaskView<Tr, SINGLE> tk(c, G0, t);
+  const int32_t V = tk.G.V;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
        v += (int64_t)gridDim.x * blockDim.x) {
-    tk.dist[v] = Tr::inf();
-    c.nst[tk.off + v] = 0;
-    c.sst[tk.off + v] = 0;
-    c.fst[tk.off + v] = 0;
+    *tk.d(v) = Tr::inf();
+    c.nst[tk.s(v)] = 0;
+    c.sst[tk.s(v)] = 0;
+    c.fst[tk.s(v)] = 0;
   }
 }
 
-template <class Tr>
+template <class Tr, bool SINGLE>
 __global__ void sssp_seed_kernel(Ctx<Tr> c, int32_t ntasks, uint64_t *near_out,
                                  unsigned long long *n_near, uint32_t gen) {
   const int32_t t = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= ntasks) return;
-  const STask<Tr> tk = c.tasks[t];
-  const SGraph<typename Tr::W> G = c.graphs[tk.graph];
-  const int32_t s = tk.src;
+  SGraph<typename Tr::W> G0{};
+  if constexpr (SINGLE) G0 = c.graphs[0];
+  const TaskView<Tr, SINGLE> tk(c, G0, t);
+  const SGraph<typename Tr::W> &G = tk.G;
+  const int32_t s = c.tasks[t].src;
   if (s < 0 || s >= G.V) return;
-  tk.dist[s] = (typename Tr::D)0;
-  c.nst[tk.off + s] = gen;
+  *tk.d(s) = (typename Tr::D)0;
+  c.nst[tk.s(s)] = gen;
   uint32_t k = nchunks(G.le[s] - G.rp[s]);
   if (k == 0) k = 1;
   unsigned long long b = atomicAdd(n_near, (unsigned long long)k);
@@ -147,7 +193,7 @@ __global__ void sssp_seed_kernel(Ctx<Tr> c, int32_t ntasks, uint64_t *near_out,
 // edges over its lanes: a wave prefix sum of the entries' edge counts gives
 // every edge slot k an owner lane, found by a binary search over the lanes'
 // offsets with cross-lane reads.  Lanes therefore stay busy whether entries
-// hold 3 edges or 1024.
+// hold 3 edges or 256.
 template <class T>
 __device__ __forceinline__ T gb_shfl_any(T v, int src);
 template <>
@@ -159,9 +205,10 @@ __device__ __forceinline__ double gb_shfl_any<double>(double v, int src) {
   return __longlong_as_double((long long)gb_shfl_u64((uint64_t)__double_as_longlong(v), src));
 }
 
-template <class Tr, bool HEAVY>
+template <class Tr, bool HEAVY, bool SINGLE>
 __device__ __forceinline__ void expand_body(
-    const Ctx<Tr> &c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
+    const Ctx<Tr> &c, const SGraph<typename Tr::W> &G0, const uint64_t *__restrict__ in,
+    const unsigned long long *__restrict__ n_in,
     uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ s_out,
     unsigned long long *n_s, uint64_t *__restrict__ far_out, unsigned long long *n_far,
     uint64_t bucket, uint32_t gen_next, uint32_t sround, uint32_t fepoch, uint32_t epw) {
@@ -185,15 +232,14 @@ __device__ __forceinline__ void expand_body(
       t = (uint32_t)(ent >> 48);
       const uint32_t ch = (uint32_t)(ent >> 32) & 0xFFFFu;
       u = (uint32_t)ent;
-      const STask<Tr> tk = c.tasks[t];
-      const SGraph<typename Tr::W> G = c.graphs[tk.graph];
-      du = tk.dist[u];
-      const int32_t rs = G.rp[u], le = G.le[u], re = G.rp[u + 1];
+      const TaskView<Tr, SINGLE> tk(c, G0, t);
+      du = *tk.d(u);
+      const int32_t rs = tk.G.rp[u], le = tk.G.le[u], re = tk.G.rp[u + 1];
       if (!HEAVY) {
         beg = rs + (int32_t)(ch << CHUNK_SHIFT);
         end = min(le, beg + CHUNK);
         // remember_arr: the first chunk records u in S (heavy edges pending)
-        if (ch == 0 && re > le && atomicExch(&c.sst[tk.off + u], sround) != sround)
+        if (ch == 0 && re > le && atomicExch(&c.sst[tk.s(u)], sround) != sround)
           scnt = nchunks(re - le);
       } else {
         beg = le + (int32_t)(ch << CHUNK_SHIFT);
@@ -224,24 +270,23 @@ __device__ __forceinline__ void expand_body(
       bool far = false;
       uint32_t v = 0;
       if (k < total) {
-        const STask<Tr> tk = c.tasks[tj];
-        const SGraph<typename Tr::W> G = c.graphs[tk.graph];
-        v = (uint32_t)G.col[e];
-        const D nd = Tr::add(duj, G.w[e], ovf);
+        const TaskView<Tr, SINGLE> tk(c, G0, tj);
+        v = (uint32_t)tk.G.col[e];
+        const D nd = Tr::add(duj, tk.G.w[e], ovf);
         relax++;
-        D *dv = tk.dist + v;
+        D *dv = tk.d(v);
         if (nd < *dv) {
           const D old = Tr::amin(dv, nd);
           if (nd < old) {
             // dedup stamps: a plain read first, the exchange only when unset
-            if (Tr::bucket(nd, tk.delta) <= bucket) {
-              uint32_t *ns = &c.nst[tk.off + v];
+            if (Tr::bucket(nd, tk.delta, tk.magic) <= bucket) {
+              uint32_t *ns = &c.nst[tk.s(v)];
               if (*ns != gen_next && atomicExch(ns, gen_next) != gen_next) {
-                ncnt = nchunks(G.le[v] - G.rp[v]);
+                ncnt = nchunks(tk.G.le[v] - tk.G.rp[v]);
                 if (ncnt == 0) ncnt = 1;
               }
             } else {
-              uint32_t *fs = &c.fst[tk.off + v];
+              uint32_t *fs = &c.fst[tk.s(v)];
               if (*fs != fepoch && atomicExch(fs, fepoch) != fepoch) far = true;
             }
           }
@@ -264,9 +309,10 @@ __device__ __forceinline__ void expand_body(
   if (__ballot(ovf) && lane == 0) atomicOr(&c.cnt[C_OVF], 1ull);
 }
 
-template <class Tr>
+template <class Tr, bool SINGLE>
 __device__ __forceinline__ void far_scan_body(
-    const Ctx<Tr> &c, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
+    const Ctx<Tr> &c, const SGraph<typename Tr::W> &G0, const uint64_t *__restrict__ in,
+    const unsigned long long *__restrict__ n_in,
     uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ far_out,
     unsigned long long *n_far, uint64_t bnext, uint32_t gen_next, uint32_t fepoch_next) {
   const int lane = gb_lane();
@@ -283,16 +329,16 @@ __device__ __forceinline__ void far_scan_body(
       const uint64_t ent = in[i];
       t = (uint32_t)(ent >> 48);
       v = (uint32_t)ent;
-      const STask<Tr> tk = c.tasks[t];
-      const uint64_t bk = Tr::bucket(tk.dist[v], tk.delta);
+      const TaskView<Tr, SINGLE> tk(c, G0, t);
+      const uint64_t bk = Tr::bucket(*tk.d(v), tk.delta, tk.magic);
       if (bk == bnext) {
-        if (atomicExch(&c.nst[tk.off + v], gen_next) != gen_next) {
-          const SGraph<typename Tr::W> G = c.graphs[tk.graph];
+        if (atomicExch(&c.nst[tk.s(v)], gen_next) != gen_next) {
+          const SGraph<typename Tr::W> &G = tk.G;
           ncnt = nchunks(G.le[v] - G.rp[v]);
           if (ncnt == 0) ncnt = 1;
         }
       } else if (bk > bnext) {
-        if (atomicExch(&c.fst[tk.off + v], fepoch_next) != fepoch_next) far = true;
+        if (atomicExch(&c.fst[tk.s(v)], fepoch_next) != fepoch_next) far = true;
         minb = bk < minb ? bk : minb;
       }
     }
@@ -334,24 +380,26 @@ __global__ void sssp_ctrl_kernel(unsigned long long *__restrict__ cnt, SsspCtl *
   *ctl = s;
 }
 
-template <class Tr>
+template <class Tr, bool SINGLE>
 __global__ __launch_bounds__(SS_BLOCK) void sssp_step_kernel(Ctx<Tr> c, Queues q,
                                                               const SsspCtl *__restrict__ ctl) {
   const int act = ctl->action;
   if (act == ACT_NONE) return;
   unsigned long long *cnt = c.cnt;
   const int in = ctl->a_in, nr = ctl->a_near, fr = ctl->a_far;
+  SGraph<typename Tr::W> G0{};
+  if constexpr (SINGLE) G0 = c.graphs[0];
   if (act == ACT_FAR) {
-    far_scan_body<Tr>(c, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[fr], cnt + fr,
-                      ctl->a_bucket, ctl->a_gen, ctl->a_fepoch);
+    far_scan_body<Tr, SINGLE>(c, G0, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[fr], cnt + fr,
+                              ctl->a_bucket, ctl->a_gen, ctl->a_fepoch);
   } else if (act == ACT_LIGHT) {
-    expand_body<Tr, false>(c, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[C_S], cnt + C_S,
-                           q.q[fr], cnt + fr, ctl->a_bucket, ctl->a_gen, ctl->a_sround,
-                           ctl->a_fepoch, ctl->a_epw);
+    expand_body<Tr, false, SINGLE>(c, G0, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[C_S],
+                                   cnt + C_S, q.q[fr], cnt + fr, ctl->a_bucket, ctl->a_gen,
+                                   ctl->a_sround, ctl->a_fepoch, ctl->a_epw);
   } else {
-    expand_body<Tr, true>(c, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[C_S], cnt + C_S,
-                          q.q[fr], cnt + fr, ctl->a_bucket, ctl->a_gen, ctl->a_sround,
-                          ctl->a_fepoch, ctl->a_epw);
+    expand_body<Tr, true, SINGLE>(c, G0, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[C_S],
+                                  cnt + C_S, q.q[fr], cnt + fr, ctl->a_bucket, ctl->a_gen,
+                                  ctl->a_sround, ctl->a_fepoch, ctl->a_epw);
   }
 }
 
@@ -417,6 +465,12 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   // generation, with ceil(deg / CHUNK) >= 1 entries, so per task at most
   // V + E / CHUNK + 1 entries.
   for (auto &t : ht) qcap += (int64_t)hg[t.graph].V + hg[t.graph].E / CHUNK + 1;
+  for (auto &t : ht) t.magic = Tr::host_magic(t.delta);
+  // one graph and delta, distances of task i at dist0 + i * V: the kernels
+  // then derive task data from the task id
+  bool single = hg.size() == 1;
+  for (auto &t : ht)
+    single = single && t.graph == 0 && t.delta == ht[0].delta && t.dist == ht[0].dist + t.off;
   SsspWork &w = g_ssw;
   GB_TRY(w.graphs.ensure(sizeof(G) * hg.size()));
   GB_TRY(w.tasks.ensure(sizeof(STask<Tr>) * ht.size()));
@@ -432,7 +486,7 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   unsigned long long *cnt = w.cnt.as<unsigned long long>();
   GB_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * C_N, st));
   Ctx<Tr> c{w.graphs.as<G>(), w.tasks.as<STask<Tr>>(), w.nst.as<uint32_t>(), w.sst.as<uint32_t>(),
-            w.fst.as<uint32_t>(), cnt};
+            w.fst.as<uint32_t>(), cnt, ht[0].dist, ht[0].delta, ht[0].magic};
 
   hipEvent_t e0, e1;
   GB_HIP(hipEventCreate(&e0));
@@ -441,11 +495,20 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   {
     int gx = (int)std::min<int64_t>((maxV + SS_BLOCK - 1) / SS_BLOCK, 1024);
     if (gx < 1) gx = 1;
-    hipLaunchKernelGGL(sssp_fill_kernel<Tr>, dim3(gx, ntasks), dim3(SS_BLOCK), 0, st, c, ntasks);
+    if (single)
+      hipLaunchKernelGGL((sssp_fill_kernel<Tr, true>), dim3(gx, ntasks), dim3(SS_BLOCK), 0, st, c,
+                         ntasks);
+    else
+      hipLaunchKernelGGL((sssp_fill_kernel<Tr, false>), dim3(gx, ntasks), dim3(SS_BLOCK), 0, st, c,
+                         ntasks);
   }
   const uint32_t gen = 1;
-  hipLaunchKernelGGL(sssp_seed_kernel<Tr>, dim3((ntasks + 255) / 256), dim3(256), 0, st, c, ntasks,
-                     w.q[0].as<uint64_t>(), cnt + C_NEAR0, gen);
+  if (single)
+    hipLaunchKernelGGL((sssp_seed_kernel<Tr, true>), dim3((ntasks + 255) / 256), dim3(256), 0, st,
+                       c, ntasks, w.q[0].as<uint64_t>(), cnt + C_NEAR0, gen);
+  else
+    hipLaunchKernelGGL((sssp_seed_kernel<Tr, false>), dim3((ntasks + 255) / 256), dim3(256), 0, st,
+                       c, ntasks, w.q[0].as<uint64_t>(), cnt + C_NEAR0, gen);
   GB_HIP(hipGetLastError());
   // the loop state starts at bucket 0 with the seeds in near queue 0
   SsspCtl hc{};
@@ -462,8 +525,12 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   const uint32_t cus = (uint32_t)gb_num_cus();
   auto one_phase = [&]() {
     hipLaunchKernelGGL(sssp_ctrl_kernel, dim3(1), dim3(64), 0, st, cnt, ctl, cus);
-    hipLaunchKernelGGL(sssp_step_kernel<Tr>, dim3(grid), dim3(SS_BLOCK), 0, st, c, qs,
-                       (const SsspCtl *)ctl);
+    if (single)
+      hipLaunchKernelGGL((sssp_step_kernel<Tr, true>), dim3(grid), dim3(SS_BLOCK), 0, st, c, qs,
+                         (const SsspCtl *)ctl);
+    else
+      hipLaunchKernelGGL((sssp_step_kernel<Tr, false>), dim3(grid), dim3(SS_BLOCK), 0, st, c, qs,
+                         (const SsspCtl *)ctl);
   };
   // Chunks of PH phases are captured once per call and replayed until the
   // state is done; a replay past the end costs two near-empty launches per
@@ -558,6 +625,17 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   return status;
 }
 
+// Queue entries carry a 16-bit chunk index: a vertex may have at most
+// 65535 chunks of CHUNK edges.
+gb_status check_degree(gb_graph g, hipStream_t st) {
+  if (g->E <= (int64_t)0xFFFF * CHUNK) return GB_OK;
+  int32_t md = 0;
+  GB_TRY(gb_graph_max_outdeg(g, &md, st));
+  if ((int64_t)md > (int64_t)0xFFFF * CHUNK)
+    return gb_fail(GB_ERR_UNSUPPORTED, "sssp: out-degree %d above %d", md, 0xFFFF * CHUNK);
+  return GB_OK;
+}
+
 uint32_t plain_delta(uint32_t max_w, int32_t V, int64_t E) {
   // max_weight / average degree, as an integer bucket width >= 1
   if (E <= 0 || V <= 0) return 1;
@@ -586,13 +664,14 @@ extern "C" gb_status gb_sssp(gb_graph g, const gb_sssp_params *p, const int32_t 
     delta = plain_delta(mw, g->V, g->E);
   }
   if (lmax == 0) lmax = delta;
+  GB_TRY(check_degree(g, st));
   GB_TRY(gb_graph_split(g, lmax, st));
   std::vector<SGraph<uint32_t>> hg(1);
   hg[0] = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
                            g->split_w.as<uint32_t>(), g->V, 0, g->E};
   std::vector<STask<TU32>> ht(nsrc);
   for (int32_t i = 0; i < nsrc; i++)
-    ht[i] = STask<TU32>{d_dist + (int64_t)i * g->V, 0, (uint64_t)delta, 0, h_sources[i]};
+    ht[i] = STask<TU32>{d_dist + (int64_t)i * g->V, 0, (uint64_t)delta, 0, 0, h_sources[i]};
   return sssp_engine<TU32>(hg, ht, p ? p->max_phases : 0, stats, st);
 }
 
@@ -607,13 +686,14 @@ extern "C" gb_status gb_sssp_f64(gb_graph g, double delta, const int32_t *h_sour
     if (h_sources[i] < 0 || h_sources[i] >= g->V)
       return gb_fail(GB_ERR_INVALID, "gb_sssp_f64: source %d out of range", h_sources[i]);
   hipStream_t st = gb_stream(stream);
+  GB_TRY(check_degree(g, st));
   GB_TRY(gb_graph_split64(g, delta, st));
   std::vector<SGraph<double>> hg(1);
   hg[0] = SGraph<double>{g->rp, g->split64_le.as<int32_t>(), g->split64_col.as<int32_t>(),
                          g->split64_w.as<double>(), g->V, 0, g->E};
   std::vector<STask<TF64>> ht(nsrc);
   for (int32_t i = 0; i < nsrc; i++)
-    ht[i] = STask<TF64>{d_dist + (int64_t)i * g->V, 0, delta, 0, h_sources[i]};
+    ht[i] = STask<TF64>{d_dist + (int64_t)i * g->V, 0, delta, 0, 0, h_sources[i]};
   return sssp_engine<TF64>(hg, ht, 0, stats, st);
 }
 
@@ -636,10 +716,11 @@ extern "C" gb_status gb_sssp_multi(int32_t ngraphs, const gb_graph *gs, const in
       delta = plain_delta(mw, g->V, g->E);
     }
     uint32_t lm = light_max && light_max[i] ? light_max[i] : delta;
+    GB_TRY(check_degree(g, st));
     GB_TRY(gb_graph_split(g, lm, st));
     hg[i] = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
                              g->split_w.as<uint32_t>(), g->V, 0, g->E};
-    ht[i] = STask<TU32>{d_dist[i], 0, (uint64_t)delta, i, sources[i]};
+    ht[i] = STask<TU32>{d_dist[i], 0, (uint64_t)delta, 0, i, sources[i]};
   }
   return sssp_engine<TU32>(hg, ht, 0, stats, st);
 }
