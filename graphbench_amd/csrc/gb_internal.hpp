@@ -126,6 +126,15 @@ gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib,
 // per-module workspace release (gb_release_workspaces)
 void gb_pr_release();
 void gb_sssp_release();
+void gb_msssp_release();
+// Batched delta-stepping for nsrc >= 2 sources of one graph (u32 weights),
+// distances to d_dist rows as gb_sssp; accumulates phases, buckets,
+// relaxations and the overflow flag.
+gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int32_t nsrc,
+                       uint32_t *d_dist, int max_phases, int64_t *phases, int64_t *buckets,
+                       unsigned long long *relax, bool *ovf, hipStream_t st);
+gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64_t *reached,
+                         int64_t *traversed, hipStream_t st);
 void gb_bc_release();
 void gb_msbc_release();
 void gb_mlp_release();

@@ -1,0 +1,521 @@
+// gb_msssp.hip -- delta-stepping for many sources of one graph at once.
+//
+// The per-source engine (gb_sssp.hip) keeps one queue entry per (source,
+// vertex) and reads a vertex's edges once per source.  Here up to 64 sources
+// form a group and share everything but their distances:
+//   * distances are vertex-major, dist[v][W] (W = group width, 8..64), so the
+//     W sources' values of one vertex are one or two cache lines;
+//   * a queue entry is a vertex (or a 64-edge chunk of one) together with the
+//     mask of sources for which it must be expanded (mask[v], u64);
+//   * a wave takes one entry; its lanes are (edge slot, source) pairs, 64 / W
+//     edges per step, so an edge's col/w are read once for all W sources and
+//     the W relaxations of one target are one coalesced atomicMin;
+//   * an improved (target, source) pair sets the source's bit in the target's
+//     next-phase mask (same bucket) or far mask (later bucket) with one
+//     atomicOr per edge slot, and the lane that turns a mask from empty to
+//     non-empty appends the target to the next queue (dedup without stamps).
+// All sources step through bucket indices in lockstep: light phases while the
+// near queue is non-empty, then a far scan that moves the next non-empty
+// bucket's (vertex, source) pairs to the near queue.  There is no light/heavy
+// split: an expansion relaxes all edges of the vertex.  Delta-stepping is
+// exact for any bucket width, so distances equal the per-source engine's.
+// The bucket loop is the state machine ms_ctl_advance (gb_sssp_ctl.hpp),
+// replayed from a captured hipGraph as in the per-source engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "gb_device.hpp"
+#include "gb_internal.hpp"
+#include "gb_sssp_ctl.hpp"
+
+namespace {
+
+constexpr int MS_BLOCK = 256;
+constexpr int MS_CHUNK_SHIFT = 6;  // 64 edges per queue entry at most
+constexpr int32_t MS_CHUNK = 1 << MS_CHUNK_SHIFT;
+constexpr int MS_NCAP = 256, MS_FCAP = 128;  // LDS append buffers per wave (entries)
+
+__device__ __forceinline__ uint32_t ms_nchunks(int32_t deg) {
+  const uint32_t k = (uint32_t)((deg + MS_CHUNK - 1) >> MS_CHUNK_SHIFT);
+  return k ? k : 1u;
+}
+
+struct MsCtx {
+  const int32_t *rp;
+  const int32_t *col;
+  const uint32_t *w;
+  int32_t V;
+  int32_t ntask;                   // sources in this group (<= W)
+  uint32_t *dist;                  // [V][W]
+  unsigned long long *mask[2];     // near masks, one per near queue
+  unsigned long long *far;         // far masks
+  uint64_t *q[4];                  // near 0/1, far 2/3
+  unsigned long long *cnt;         // MS_* counters
+  uint64_t delta, magic;           // bucket width and its division constant
+};
+
+__device__ __forceinline__ uint64_t ms_bucket(uint32_t d, uint64_t magic) {
+  return magic ? __umul64hi((uint64_t)d, magic) : (uint64_t)d;
+}
+
+// Wave-private LDS staging of queue appends (see gb_sssp.hip): one counter
+// atomic per buffer flush.  All 64 lanes call push/flush together.
+template <int CAP>
+struct MsQueue {
+  uint64_t *buf;
+  uint64_t *q;
+  unsigned long long *counter;
+  uint32_t n;
+  __device__ __forceinline__ MsQueue(uint64_t *b, uint64_t *queue, unsigned long long *c)
+      : buf(b), q(queue), counter(c), n(0) {}
+  __device__ __forceinline__ void flush() {
+    if (n == 0) return;
+    __builtin_amdgcn_wave_barrier();
+    uint64_t base = 0;
+    if (gb_lane() == 0) base = atomicAdd(counter, (unsigned long long)n);
+    base = gb_shfl_u64(base, 0);
+    for (uint32_t k = gb_lane(); k < n; k += GB_WAVE) q[base + k] = buf[k];
+    __builtin_amdgcn_wave_barrier();
+    n = 0;
+  }
+  template <class F>
+  __device__ __forceinline__ void push(uint32_t cnt, F gen) {
+    uint32_t total;
+    const uint32_t excl = gb_wave_excl_sum(cnt, &total);
+    if (total == 0) return;
+    if (n + total > CAP) flush();
+    if (total <= CAP) {
+      for (uint32_t k = 0; k < cnt; k++) buf[n + excl + k] = gen(k);
+      n += total;
+    } else {
+      uint64_t base = 0;
+      if (gb_lane() == 0) base = atomicAdd(counter, (unsigned long long)total);
+      base = gb_shfl_u64(base, 0);
+      for (uint32_t k = 0; k < cnt; k++) q[base + excl + k] = gen(k);
+    }
+  }
+};
+
+__global__ void ms_fill_kernel(int64_t n, uint32_t *__restrict__ dist) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dist[i] = 0xFFFFFFFFu;
+}
+
+template <int W>
+__global__ void ms_seed_kernel(MsCtx c, const int32_t *__restrict__ src) {
+  const int t = threadIdx.x;
+  if (t >= c.ntask) return;
+  const int32_t s = src[t];
+  c.dist[(int64_t)s * W + t] = 0;
+  const unsigned long long old = atomicOr(&c.mask[0][s], 1ull << t);
+  if (old == 0) {
+    const uint32_t k = ms_nchunks(c.rp[s + 1] - c.rp[s]);
+    const unsigned long long b = atomicAdd(&c.cnt[MS_NEAR0], (unsigned long long)k);
+    for (uint32_t i = 0; i < k; i++) c.q[0][b + i] = ((uint64_t)i << 32) | (uint32_t)s;
+  }
+}
+
+__global__ void ms_ctrl_kernel(unsigned long long *__restrict__ cnt, MsCtl *__restrict__ ctl) {
+  if (threadIdx.x != 0) return;
+  MsCtl s = *ctl;
+  ms_ctl_advance(s, cnt);
+  *ctl = s;
+}
+
+// Light phase: expand the entries of near queue a_in.
+template <int W>
+__device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_t *lds_near,
+                                         uint64_t *lds_far, uint32_t &relax, bool &ovf) {
+  constexpr int ES = GB_WAVE / W;  // edge slots per step
+  constexpr int UNROLL = W >= 32 ? 4 : 2;
+  const int lane = gb_lane();
+  const int es = lane / W, t = lane % W;
+  const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
+  const uint64_t *in = c.q[a.a_in];
+  const unsigned long long *min_ = c.mask[a.a_in - MS_NEAR0];
+  unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
+  MsQueue<MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
+  MsQueue<MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
+  const uint64_t n = c.cnt[a.a_in];
+  const uint64_t b = a.a_bucket;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t i = wave; i < n; i += nwaves) {
+    const uint64_t ent = in[i];
+    const uint32_t u = (uint32_t)ent, ch = (uint32_t)(ent >> 32);
+    const unsigned long long m = min_[u];
+    const bool act = t < c.ntask && ((m >> t) & 1ull);
+    const uint32_t du = act ? c.dist[(int64_t)u * W + t] : 0xFFFFFFFFu;
+    const int32_t beg = c.rp[u] + (int32_t)(ch << MS_CHUNK_SHIFT);
+    const int32_t end = min(c.rp[u + 1], beg + MS_CHUNK);
+    for (int32_t e0 = beg; e0 < end; e0 += ES * UNROLL) {
+      int32_t x[UNROLL];
+      uint32_t nd[UNROLL];
+      bool ok[UNROLL];
+#pragma unroll
+      for (int k = 0; k < UNROLL; k++) {
+        const int32_t e = e0 + k * ES + es;
+        // every lane of a slot loads the slot's target: the slot's first
+        // lane needs it for the mask update even when its source is idle
+        x[k] = e < end ? c.col[e] : 0;
+        ok[k] = act && e < end;
+        nd[k] = 0;
+        if (ok[k]) {
+          relax++;
+          uint64_t s = (uint64_t)du + c.w[e];
+          if (s >= 0xFFFFFFFFull) {
+            ovf = true;
+            s = 0xFFFFFFFEull;
+          }
+          nd[k] = (uint32_t)s;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < UNROLL; k++)
+        if (ok[k]) ok[k] = nd[k] < c.dist[(int64_t)x[k] * W + t];
+#pragma unroll
+      for (int k = 0; k < UNROLL; k++) {
+        bool near = false, far = false;
+        if (ok[k]) {
+          if (nd[k] < atomicMin(&c.dist[(int64_t)x[k] * W + t], nd[k])) {
+            if (ms_bucket(nd[k], c.magic) <= b)
+              near = true;
+            else
+              far = true;
+          }
+        }
+        // one atomicOr per edge slot: the slot's first lane carries its mask
+        const uint64_t nb = __ballot(near), fb = __ballot(far);
+        const uint64_t nm = (nb >> (es * W)) & wmask, fm = (fb >> (es * W)) & wmask;
+        uint32_t pn = 0, pf = 0;
+        if (t == 0) {
+          if (nm && atomicOr(&mout[x[k]], (unsigned long long)nm) == 0ull)
+            pn = ms_nchunks(c.rp[x[k] + 1] - c.rp[x[k]]);
+          if (fm && atomicOr(&c.far[x[k]], (unsigned long long)fm) == 0ull) pf = 1;
+        }
+        const uint32_t xv = (uint32_t)x[k];
+        nq.push(pn, [&](uint32_t q) { return ((uint64_t)q << 32) | xv; });
+        fq.push(pf, [&](uint32_t) { return (uint64_t)xv; });
+      }
+    }
+  }
+  nq.flush();
+  fq.flush();
+}
+
+// Far scan: far queue a_in -> bucket a_bucket's pairs to near queue a_near,
+// later buckets stay (far queue a_far), settled ones are dropped.
+template <int W>
+__device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, uint64_t *lds_near,
+                                       uint64_t *lds_far) {
+  constexpr int ES = GB_WAVE / W;
+  const int lane = gb_lane();
+  const int es = lane / W, t = lane % W;
+  const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
+  const uint64_t *in = c.q[a.a_in];
+  unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
+  MsQueue<MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
+  MsQueue<MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
+  const uint64_t n = c.cnt[a.a_in];
+  const uint64_t bnext = a.a_bucket;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  uint64_t minb = ~0ull;
+  for (uint64_t i0 = wave * ES; i0 < n; i0 += nwaves * ES) {
+    const uint64_t i = i0 + es;
+    bool near = false, keep = false;
+    uint32_t x = 0;
+    if (i < n) {
+      x = (uint32_t)in[i];
+      const unsigned long long m = c.far[x];
+      if ((m >> t) & 1ull) {
+        const uint64_t bk = ms_bucket(c.dist[(int64_t)x * W + t], c.magic);
+        if (bk == bnext) {
+          near = true;
+        } else if (bk > bnext) {
+          keep = true;
+          minb = bk < minb ? bk : minb;
+        }
+      }
+    }
+    const uint64_t nb = __ballot(near), kb = __ballot(keep);
+    const uint64_t nm = (nb >> (es * W)) & wmask, km = (kb >> (es * W)) & wmask;
+    uint32_t pn = 0, pf = 0;
+    if (t == 0 && i < n) {
+      c.far[x] = km;  // the entry is this vertex's only one in the scan
+      if (nm && atomicOr(&mout[x], (unsigned long long)nm) == 0ull)
+        pn = ms_nchunks(c.rp[x + 1] - c.rp[x]);
+      if (km) pf = 1;
+    }
+    nq.push(pn, [&](uint32_t q) { return ((uint64_t)q << 32) | x; });
+    fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
+  }
+  nq.flush();
+  fq.flush();
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint64_t o = gb_shfl_u64(minb, lane ^ d);
+    minb = o < minb ? o : minb;
+  }
+  if (lane == 0 && minb != ~0ull) atomicMin(&c.cnt[MS_MINB], (unsigned long long)minb);
+}
+
+template <int W>
+__global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
+                                                           const MsCtl *__restrict__ ctl) {
+  const int act = ctl->action;
+  if (act == ACT_NONE) return;
+  const MsCtl a = *ctl;
+  __shared__ uint64_t s_near[MS_BLOCK / GB_WAVE][MS_NCAP];
+  __shared__ uint64_t s_far[MS_BLOCK / GB_WAVE][MS_FCAP];
+  uint64_t *ln = s_near[threadIdx.x / GB_WAVE], *lf = s_far[threadIdx.x / GB_WAVE];
+  if (act == ACT_LIGHT) {
+    uint32_t relax = 0;
+    bool ovf = false;
+    ms_light<W>(c, a, ln, lf, relax, ovf);
+    uint32_t tot;
+    (void)gb_wave_excl_sum(relax, &tot);
+    if (gb_lane() == 0 && tot) atomicAdd(&c.cnt[MS_RELAX], (unsigned long long)tot);
+    if (__ballot(ovf) && gb_lane() == 0) atomicOr(&c.cnt[MS_OVF], 1ull);
+  } else {
+    ms_far<W>(c, a, ln, lf);
+  }
+}
+
+// After a light phase: clear the input queue's masks (the buffer is the
+// output of the phase after next).
+__global__ void ms_clear_kernel(MsCtx c, const MsCtl *__restrict__ ctl) {
+  if (ctl->action != ACT_LIGHT) return;
+  const int in = ctl->a_in;
+  const uint64_t n = c.cnt[in];
+  unsigned long long *m = c.mask[in - MS_NEAR0];
+  const uint64_t *q = c.q[in];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    m[(uint32_t)q[i]] = 0;
+}
+
+// group distances [V][W] -> caller rows out[t][V] for t < ntask
+__global__ __launch_bounds__(256) void ms_unpack_kernel(int64_t V, int W, int ntask,
+                                                         const uint32_t *__restrict__ dist,
+                                                         uint32_t *__restrict__ out) {
+  __shared__ uint32_t tile[64][65];
+  const int64_t v0 = (int64_t)blockIdx.x * 64;
+  const int t0 = blockIdx.y * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int vv = i / 64, tt = i % 64;
+    if (v0 + vv < V && t0 + tt < W) tile[tt][vv] = dist[(v0 + vv) * W + t0 + tt];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int tt = i / 64, vv = i % 64;
+    if (v0 + vv < V && t0 + tt < ntask) out[(int64_t)(t0 + tt) * V + v0 + vv] = tile[tt][vv];
+  }
+}
+
+// reached (source, vertex) pairs and their out-edges, over the caller's rows
+__global__ void ms_count_kernel(int64_t V, int32_t nsrc, const int32_t *__restrict__ rp,
+                                const uint32_t *__restrict__ rows,
+                                unsigned long long *__restrict__ out) {
+  unsigned long long reached = 0, edges = 0;
+  const int64_t n = V * nsrc;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (rows[i] != 0xFFFFFFFFu) {
+      const int64_t v = i % V;
+      reached++;
+      edges += (unsigned long long)(rp[v + 1] - rp[v]);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    reached += __shfl_xor(reached, d, 64);
+    edges += __shfl_xor(edges, d, 64);
+  }
+  if (gb_lane() == 0) {
+    atomicAdd(&out[0], reached);
+    atomicAdd(&out[1], edges);
+  }
+}
+
+struct MsWork {
+  DevBuf dist, m0, m1, far, q[4], cnt, ctl, src;
+  void release() {
+    for (DevBuf *b : {&dist, &m0, &m1, &far, &cnt, &ctl, &src}) b->release();
+    for (auto &b : q) b.release();
+  }
+};
+MsWork &g_msw_ref() {
+  static MsWork *w = new MsWork();
+  return *w;
+}
+#define g_msw (g_msw_ref())
+
+template <int W>
+gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st,
+                   uint32_t *out_rows, int64_t *phases, int64_t *buckets,
+                   unsigned long long *relax, bool *ovf) {
+  MsWork &w = g_msw;
+  const int64_t V = c.V;
+  hipLaunchKernelGGL(ms_fill_kernel, dim3((unsigned)std::min<int64_t>((V * W + 255) / 256, 65536)),
+                     dim3(256), 0, st, V * W, c.dist);
+  GB_HIP(hipMemsetAsync(c.mask[0], 0, sizeof(unsigned long long) * V, st));
+  GB_HIP(hipMemsetAsync(c.mask[1], 0, sizeof(unsigned long long) * V, st));
+  GB_HIP(hipMemsetAsync(c.far, 0, sizeof(unsigned long long) * V, st));
+  GB_HIP(hipMemsetAsync(c.cnt, 0, sizeof(unsigned long long) * MS_N, st));
+  hipLaunchKernelGGL(ms_seed_kernel<W>, dim3(1), dim3(64), 0, st, c, d_src);
+  GB_HIP(hipGetLastError());
+  MsCtl hc{};
+  hc.max_phases = max_phases > 0 ? max_phases : ((int64_t)1 << 40);
+  hc.state = MS_ST_LIGHT;
+  hc.buckets = 1;
+  GB_HIP(hipMemcpyAsync(w.ctl.p, &hc, sizeof(hc), hipMemcpyHostToDevice, st));
+  MsCtl *ctl = w.ctl.as<MsCtl>();
+  const int grid = gb_num_cus() * 8;
+  const int cgrid = gb_num_cus() * 2;
+  constexpr int PH = 16;
+  hipStream_t cs = st, own = nullptr;
+  if (!st) {  // the null stream cannot be captured
+    GB_HIP(hipStreamCreate(&own));
+    cs = own;
+    hipEvent_t ready;
+    GB_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    GB_HIP(hipEventRecord(ready, st));
+    GB_HIP(hipStreamWaitEvent(cs, ready, 0));
+    (void)hipEventDestroy(ready);
+  }
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  gb_status gst = GB_OK;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int k = 0; k < PH; k++) {
+      hipLaunchKernelGGL(ms_ctrl_kernel, dim3(1), dim3(64), 0, cs, c.cnt, ctl);
+      hipLaunchKernelGGL(ms_step_kernel<W>, dim3(grid), dim3(MS_BLOCK), 0, cs, c,
+                         (const MsCtl *)ctl);
+      hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, cs, c, (const MsCtl *)ctl);
+    }
+    e = hipStreamEndCapture(cs, &graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) gst = gb_fail(GB_ERR_HIP, "msssp: graph capture failed: %s", hipGetErrorString(e));
+  MsCtl hs{};
+  int64_t last = -1;
+  while (gst == GB_OK) {
+    if (hipGraphLaunch(exec, cs) != hipSuccess ||
+        hipMemcpyAsync(&hs, ctl, sizeof(hs), hipMemcpyDeviceToHost, cs) != hipSuccess ||
+        hipStreamSynchronize(cs) != hipSuccess) {
+      gst = gb_fail(GB_ERR_HIP, "msssp: phase replay failed");
+      break;
+    }
+    if (hs.done) break;
+    if (hs.phases == last) {
+      gst = gb_fail(GB_ERR_HIP, "msssp: bucket loop made no progress");
+      break;
+    }
+    last = hs.phases;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (own) {
+    (void)hipStreamSynchronize(own);
+    (void)hipStreamDestroy(own);
+  }
+  GB_TRY(gst);
+  if (hs.capped) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
+  hipLaunchKernelGGL(ms_unpack_kernel, dim3((unsigned)((V + 63) / 64), (unsigned)((W + 63) / 64)),
+                     dim3(256), 0, st, V, W, c.ntask, (const uint32_t *)c.dist, out_rows);
+  GB_HIP(hipGetLastError());
+  unsigned long long h[MS_N];
+  GB_HIP(hipMemcpyAsync(h, c.cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  *phases += hs.phases;
+  *buckets += hs.buckets;
+  *relax += h[MS_RELAX];
+  if (h[MS_OVF]) *ovf = true;
+  return GB_OK;
+}
+
+}  // namespace
+
+void gb_msssp_release() { g_msw.release(); }
+
+// Distances from nsrc sources in groups of up to 64 (u32 weights); rows of
+// d_dist as gb_sssp.  Called by gb_sssp for nsrc >= 2.
+gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64_t *reached,
+                         int64_t *traversed, hipStream_t st) {
+  MsWork &w = g_msw;
+  GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
+  GB_HIP(hipMemsetAsync(w.cnt.p, 0, 2 * sizeof(unsigned long long), st));
+  const int64_t n = (int64_t)g->V * nsrc;
+  hipLaunchKernelGGL(ms_count_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))),
+                     dim3(256), 0, st, (int64_t)g->V, nsrc, g->rp, d_dist,
+                     w.cnt.as<unsigned long long>());
+  GB_HIP(hipGetLastError());
+  unsigned long long h[2];
+  GB_HIP(hipMemcpyAsync(h, w.cnt.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  *reached = (int64_t)h[0];
+  *traversed = (int64_t)h[1];
+  return GB_OK;
+}
+
+gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int32_t nsrc,
+                       uint32_t *d_dist, int max_phases, int64_t *phases, int64_t *buckets,
+                       unsigned long long *relax, bool *ovf, hipStream_t st) {
+  MsWork &w = g_msw;
+  const int64_t V = g->V, E = g->E;
+  // queue capacity: a vertex enters a near queue at most once per phase with
+  // ceil(deg / 64) entries, a far queue at most once
+  const int64_t qn = V + E / MS_CHUNK + 1;
+  const int Wmax = nsrc >= 64 ? 64 : (nsrc > 32 ? 64 : nsrc > 16 ? 32 : nsrc > 8 ? 16 : 8);
+  GB_TRY(w.dist.ensure(sizeof(uint32_t) * (size_t)V * Wmax));
+  GB_TRY(w.m0.ensure(sizeof(unsigned long long) * V));
+  GB_TRY(w.m1.ensure(sizeof(unsigned long long) * V));
+  GB_TRY(w.far.ensure(sizeof(unsigned long long) * V));
+  GB_TRY(w.q[0].ensure(sizeof(uint64_t) * qn));
+  GB_TRY(w.q[1].ensure(sizeof(uint64_t) * qn));
+  GB_TRY(w.q[2].ensure(sizeof(uint64_t) * (V + 1)));
+  GB_TRY(w.q[3].ensure(sizeof(uint64_t) * (V + 1)));
+  GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
+  GB_TRY(w.ctl.ensure(sizeof(MsCtl)));
+  GB_TRY(w.src.ensure(sizeof(int32_t) * 64));
+  MsCtx c{};
+  c.rp = g->rp;
+  c.col = g->col;
+  c.w = g->w;
+  c.V = g->V;
+  c.dist = w.dist.as<uint32_t>();
+  c.mask[0] = w.m0.as<unsigned long long>();
+  c.mask[1] = w.m1.as<unsigned long long>();
+  c.far = w.far.as<unsigned long long>();
+  for (int k = 0; k < 4; k++) c.q[k] = w.q[k].as<uint64_t>();
+  c.cnt = w.cnt.as<unsigned long long>();
+  c.delta = delta;
+  c.magic = delta > 1 ? (~0ull / delta) + 1 : 0;
+  for (int32_t g0 = 0; g0 < nsrc; g0 += 64) {
+    const int n = std::min<int32_t>(64, nsrc - g0);
+    c.ntask = n;
+    GB_HIP(hipMemcpyAsync(w.src.p, h_sources + g0, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    uint32_t *rows = d_dist + (int64_t)g0 * V;
+    const int W = n > 32 ? 64 : n > 16 ? 32 : n > 8 ? 16 : 8;
+    const int32_t *ds = w.src.as<int32_t>();
+    switch (W) {
+      case 8:
+        GB_TRY(ms_group<8>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+        break;
+      case 16:
+        GB_TRY(ms_group<16>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+        break;
+      case 32:
+        GB_TRY(ms_group<32>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+        break;
+      default:
+        GB_TRY(ms_group<64>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+    }
+  }
+  return GB_OK;
+}

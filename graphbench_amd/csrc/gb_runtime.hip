@@ -91,6 +91,7 @@ const char *gb_last_error(void) { return g_err; }
 void gb_release_workspaces(void) {
   gb_pr_release();
   gb_sssp_release();
+  gb_msssp_release();
   gb_bc_release();
   gb_msbc_release();
   gb_mlp_release();

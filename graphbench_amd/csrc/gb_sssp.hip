@@ -679,6 +679,14 @@ gb_status check_degree(gb_graph g, hipStream_t st) {
   return GB_OK;
 }
 
+// Several sources of one graph go to the batched engine unless
+// GB_SSSP_PER_SOURCE is set (the per-source engine then runs them in lockstep
+// with one queue entry per source and vertex).
+bool ms_enabled() {
+  const char *e = getenv("GB_SSSP_PER_SOURCE");
+  return !(e && e[0] && e[0] != '0');
+}
+
 uint32_t plain_delta(uint32_t max_w, int32_t V, int64_t E) {
   // max_weight / average degree, as an integer bucket width >= 1
   if (E <= 0 || V <= 0) return 1;
@@ -707,6 +715,33 @@ extern "C" gb_status gb_sssp(gb_graph g, const gb_sssp_params *p, const int32_t 
     delta = plain_delta(mw, g->V, g->E);
   }
   if (lmax == 0) lmax = delta;
+  if (nsrc >= 2 && ms_enabled()) {
+    // several sources: the batched engine (one entry per vertex and source mask)
+    gb_sssp_stats s{};
+    bool ovf = false;
+    unsigned long long relax = 0;
+    hipEvent_t e0, e1;
+    GB_HIP(hipEventCreate(&e0));
+    GB_HIP(hipEventCreate(&e1));
+    GB_HIP(hipEventRecord(e0, st));
+    gb_status r = gb_msssp_run(g, delta, h_sources, nsrc, d_dist, p ? p->max_phases : 0,
+                               &s.phases, &s.buckets, &relax, &ovf, st);
+    if (r == GB_OK) r = hipEventRecord(e1, st) == hipSuccess ? GB_OK : gb_fail(GB_ERR_HIP, "event");
+    if (r == GB_OK && stats) {
+      float ms = 0.0f;
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      s.ms = ms;
+      s.relax_edges = (int64_t)relax;
+      r = gb_msssp_count(g, nsrc, d_dist, &s.reached, &s.traversed, st);
+      *stats = s;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    GB_TRY(r);
+    if (ovf) return gb_fail(GB_ERR_OVERFLOW, "sssp: uint32 distance overflow");
+    return GB_OK;
+  }
   GB_TRY(check_degree(g, st));
   GB_TRY(gb_graph_split(g, lmax, st));
   std::vector<SGraph<uint32_t>> hg(1);

@@ -135,4 +135,87 @@ __host__ __device__ inline void sssp_ctl_advance(SsspCtl &s, unsigned long long 
   }
 }
 
+// ---- batched form: one vertex entry serves a mask of tasks -----------------
+// Near queues 0/1 (with per-vertex task masks 0/1), far queues 2/3 (one far
+// mask per vertex), counters: MS_NEAR0..1, MS_FAR0..1, MS_MINB, MS_RELAX,
+// MS_OVF.  No light/heavy split: an expansion relaxes all edges of a vertex.
+enum { MS_NEAR0 = 0, MS_NEAR1 = 1, MS_FAR0 = 2, MS_FAR1 = 3, MS_MINB = 4, MS_RELAX = 5,
+       MS_OVF = 6, MS_N = 7 };
+enum { MS_ST_LIGHT = 0, MS_ST_AFTER_BUCKET, MS_ST_FAR, MS_ST_AFTER_FAR };
+
+struct MsCtl {
+  uint64_t b, bnext;
+  int64_t phases, buckets, max_phases;
+  int32_t cur, fcur, state, done, capped;
+  // next action: ACT_LIGHT (near queue a_in -> a_near, far pushes -> a_far)
+  // or ACT_FAR (far queue a_in -> near a_near and far a_far)
+  int32_t action, a_in, a_near, a_far;
+  uint64_t a_bucket;
+};
+
+__host__ __device__ inline void ms_ctl_advance(MsCtl &s, unsigned long long *cnt) {
+  s.action = ACT_NONE;
+  for (int guard = 0; guard < 8 && s.action == ACT_NONE && !s.done; ++guard) {
+    switch (s.state) {
+      case MS_ST_LIGHT:
+        if (cnt[MS_NEAR0 + s.cur] > 0) {
+          if (++s.phases > s.max_phases) {
+            s.capped = 1;
+            s.done = 1;
+            break;
+          }
+          s.action = ACT_LIGHT;
+          s.a_in = MS_NEAR0 + s.cur;
+          s.a_near = MS_NEAR0 + (s.cur ^ 1);
+          s.a_far = MS_FAR0 + s.fcur;
+          s.a_bucket = s.b;
+          cnt[s.a_near] = 0;
+          s.cur ^= 1;
+        } else {
+          s.state = MS_ST_AFTER_BUCKET;
+        }
+        break;
+      case MS_ST_AFTER_BUCKET:
+        if (cnt[MS_FAR0 + s.fcur] == 0) {
+          s.done = 1;
+        } else {
+          s.bnext = s.b + 1;
+          s.state = MS_ST_FAR;
+        }
+        break;
+      case MS_ST_FAR:
+        if (++s.phases > s.max_phases) {
+          s.capped = 1;
+          s.done = 1;
+          break;
+        }
+        s.action = ACT_FAR;
+        s.a_in = MS_FAR0 + s.fcur;
+        s.a_near = MS_NEAR0 + s.cur;
+        s.a_far = MS_FAR0 + (s.fcur ^ 1);
+        s.a_bucket = s.bnext;
+        cnt[s.a_near] = 0;
+        cnt[s.a_far] = 0;
+        cnt[MS_MINB] = ~0ull;
+        s.fcur ^= 1;
+        s.state = MS_ST_AFTER_FAR;
+        break;
+      case MS_ST_AFTER_FAR:
+        if (cnt[MS_NEAR0 + s.cur] > 0) {
+          s.b = s.bnext;
+          s.buckets++;
+          s.state = MS_ST_LIGHT;
+        } else if (cnt[MS_FAR0 + s.fcur] == 0) {
+          s.done = 1;
+        } else {
+          s.bnext = cnt[MS_MINB];
+          s.state = MS_ST_FAR;
+        }
+        break;
+      default:
+        s.done = 1;
+    }
+  }
+}
+
 #endif

@@ -47,6 +47,43 @@ def test_sssp_small_bit_exact(gpu, seed, delta, light):
         assert np.array_equal(d[i], ref), f"source {s}"
 
 
+@pytest.mark.parametrize("nsrc", [2, 9, 17, 33, 64, 100])
+def test_sssp_batched_engine_matches_per_source(gpu, nsrc, monkeypatch):
+    """Several sources run in the batched engine (groups of up to 64 sources,
+    widths 8..64); distances equal the per-source engine's and Dijkstra's,
+    with duplicate sources and sources without out-edges included."""
+    gb = gpu
+    h = gb.gen_rmat(12, 8, seed=50 + nsrc, weights="uniform", wlo=1, whi=300)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    rng = np.random.default_rng(nsrc)
+    srcs = rng.integers(0, h.V, nsrc).astype(np.int32)
+    srcs[-1] = srcs[0]  # a duplicate
+    zero_out = np.flatnonzero(np.diff(h.row_ptr) == 0)
+    if len(zero_out) and nsrc > 2:
+        srcs[1] = zero_out[0]
+    _say(f"sssp batched nsrc={nsrc}")
+    for delta in (0, 64, 1 << 12):
+        d, st = gb.sssp(g, srcs, delta=delta, stats=True)
+        d = gb.u32(d)
+        monkeypatch.setenv("GB_SSSP_PER_SOURCE", "1")
+        ref_engine = gb.u32(gb.sssp(g, srcs, delta=delta))
+        monkeypatch.delenv("GB_SSSP_PER_SOURCE")
+        assert np.array_equal(d, ref_engine)
+        for i in (0, 1, nsrc // 2, nsrc - 1):
+            ref = oracle.dijkstra(h.row_ptr, h.col, h.w, int(srcs[i]))
+            assert np.array_equal(d[i], ref), (delta, i)
+        assert st["reached"] == int((d != 0xFFFFFFFF).sum())
+
+
+def test_sssp_batched_overflow_reported(gpu):
+    gb = gpu
+    h = gb.HostCSR.from_edges(4, [0, 1, 2], [1, 2, 3], [0xFFFFFFF0, 0xFFFFFFF0, 5])
+    g = gb.Graph.from_host(h, need_reverse=False)
+    with pytest.raises(gb.GBError) as ei:
+        gb.sssp(g, [0, 1], delta=1 << 30)
+    assert ei.value.status == 4
+
+
 def test_sssp_config1_rmat16(gpu):
     """R-MAT scale 16, edge factor 16, w in [1,255], seed 42, 8 sources, plain delta."""
     gb = gpu
