@@ -24,6 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per pr_gather launch measured by rocprofv3 (FETCH_SIZE and
+# WRITE_SIZE, scaled by a copy of known size); written with the profile summary
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pr_gather_rmat22_traffic.json")
 
 
 def log(msg):
@@ -347,6 +350,10 @@ def main():
         extra["batched_c4"] = run_batched_c4(gb, oracle, torch, np)
         log("learned + batched vs plain, R-MAT 24")
         extra["learned_c5"] = run_learned_c5(gb, oracle, torch, np, args.cpu_budget)
+    traffic = {}
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as f:
+            traffic = json.load(f)
     line = {
         "metric": "GTEPS + % HBM peak (SSSP/PR/BC); learned-delta+batched vs plain, same GPU",
         "value": pr["gteps"],
@@ -364,7 +371,9 @@ def main():
                    "seed": 7, "damping": 0.85, "tol": 1e-7, "max_iter": 100,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": pr["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": pr["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": pr["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": traffic.get("traffic_bytes_per_launch"),
+                     "traffic_source": traffic.get("source"),
                      "kernel": "pr_gather", "bytes_per_launch": pr["bytes_per_launch"],
                      "mean_launch_ms": pr["gather_ms_avg"],
                      "shape": {"items_per_thread": pr["items_per_thread"], "hot": pr["hot"],

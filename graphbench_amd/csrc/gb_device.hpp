@@ -41,6 +41,50 @@ __device__ __forceinline__ uint64_t gb_wave_reserve(unsigned long long *counter,
   return base + excl;
 }
 
+// Appends of one wave to a device queue, staged in a wave-private LDS buffer
+// of CAP elements and flushed with one counter atomicAdd per flush.  When
+// every wave of a launch appends to the same queue, one atomic per wave and
+// step on the single counter address serialises in the L2; staging divides
+// the atomics by the buffer fill.  Order within the queue is not kept.  All 64
+// lanes must call push and flush together (the fill n is wave-uniform).
+template <class T, int CAP>
+struct gb_wave_queue {
+  T *buf;  // this wave's LDS slice, CAP elements
+  T *q;
+  unsigned long long *counter;
+  uint32_t n;
+  __device__ __forceinline__ gb_wave_queue(T *b, T *queue, unsigned long long *c)
+      : buf(b), q(queue), counter(c), n(0) {}
+  __device__ __forceinline__ uint64_t reserve_global(uint32_t total) {
+    uint64_t base = 0;
+    if (gb_lane() == 0) base = atomicAdd(counter, (unsigned long long)total);
+    return gb_shfl_u64(base, 0);
+  }
+  __device__ __forceinline__ void flush() {
+    if (n == 0) return;
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t base = reserve_global(n);
+    for (uint32_t k = gb_lane(); k < n; k += GB_WAVE) q[base + k] = buf[k];
+    __builtin_amdgcn_wave_barrier();
+    n = 0;
+  }
+  // this lane appends cnt elements gen(0), ..., gen(cnt - 1)
+  template <class F>
+  __device__ __forceinline__ void push(uint32_t cnt, F gen) {
+    uint32_t total;
+    const uint32_t excl = gb_wave_excl_sum(cnt, &total);
+    if (total == 0) return;
+    if (n + total > CAP) flush();
+    if (total <= CAP) {
+      for (uint32_t k = 0; k < cnt; k++) buf[n + excl + k] = gen(k);
+      n += total;
+    } else {  // larger than the buffer: straight to the queue
+      const uint64_t base = reserve_global(total);
+      for (uint32_t k = 0; k < cnt; k++) q[base + excl + k] = gen(k);
+    }
+  }
+};
+
 // Deterministic wave sum (fixed butterfly order).
 __device__ __forceinline__ double gb_wave_sum_f64(double v) {
 #pragma unroll

@@ -30,7 +30,8 @@
 
 namespace {
 
-constexpr int MB = 256;  // threads per block (4 waves)
+constexpr int MB = 256;   // threads per block (4 waves)
+constexpr int MBQ = 256;  // staged list appends per wave
 
 __global__ void mbc_cand_init_kernel(int32_t V, const int32_t *__restrict__ rrp, int32_t hub,
                                      int32_t *__restrict__ cand, unsigned long long *count) {
@@ -199,6 +200,12 @@ __global__ __launch_bounds__(MB) void mbc_forward_kernel(
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   const uint64_t act = active[g];
   const unsigned long long *fmd = fm_d + gV;
+  // the level list and the candidate list are appended to by every wave:
+  // stage the appends per wave in LDS
+  __shared__ int32_t s_list[MB / GB_WAVE][MBQ], s_cand[MB / GB_WAVE][MBQ];
+  gb_wave_queue<int32_t, MBQ> lq(s_list[threadIdx.x / GB_WAVE], list_d1 + gV, &list_d1_count[g]);
+  gb_wave_queue<int32_t, MBQ> cq(s_cand[threadIdx.x / GB_WAVE], cand_out + gV,
+                                 &cand_out_count[g]);
   for (uint64_t i = wave; i < n; i += nwaves) {
     const int32_t x = cand_in[gV + i];
     const uint64_t need = ~visited[gV + x] & act;
@@ -215,15 +222,14 @@ __global__ __launch_bounds__(MB) void mbc_forward_kernel(
       if (lane == 0) {
         fm_d1[gV + x] = found;
         visited[gV + x] |= found;
-        const unsigned long long p = atomicAdd(&list_d1_count[g], 1ull);
-        list_d1[gV + p] = x;
       }
     }
-    if ((need & ~found) != 0 && lane == 0) {  // still unreached by some source
-      const unsigned long long p = atomicAdd(&cand_out_count[g], 1ull);
-      cand_out[gV + p] = x;
-    }
+    lq.push(lane == 0 && found ? 1u : 0u, [&](uint32_t) { return x; });
+    // still unreached by some source: a candidate again at the next level
+    cq.push(lane == 0 && (need & ~found) ? 1u : 0u, [&](uint32_t) { return x; });
   }
+  lq.flush();
+  cq.flush();
 }
 
 __global__ __launch_bounds__(MB) void mbc_backward_kernel(

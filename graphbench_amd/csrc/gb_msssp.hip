@@ -60,44 +60,6 @@ __device__ __forceinline__ uint64_t ms_bucket(uint32_t d, uint64_t magic) {
   return magic ? __umul64hi((uint64_t)d, magic) : (uint64_t)d;
 }
 
-// Wave-private LDS staging of queue appends (see gb_sssp.hip): one counter
-// atomic per buffer flush.  All 64 lanes call push/flush together.
-template <int CAP>
-struct MsQueue {
-  uint64_t *buf;
-  uint64_t *q;
-  unsigned long long *counter;
-  uint32_t n;
-  __device__ __forceinline__ MsQueue(uint64_t *b, uint64_t *queue, unsigned long long *c)
-      : buf(b), q(queue), counter(c), n(0) {}
-  __device__ __forceinline__ void flush() {
-    if (n == 0) return;
-    __builtin_amdgcn_wave_barrier();
-    uint64_t base = 0;
-    if (gb_lane() == 0) base = atomicAdd(counter, (unsigned long long)n);
-    base = gb_shfl_u64(base, 0);
-    for (uint32_t k = gb_lane(); k < n; k += GB_WAVE) q[base + k] = buf[k];
-    __builtin_amdgcn_wave_barrier();
-    n = 0;
-  }
-  template <class F>
-  __device__ __forceinline__ void push(uint32_t cnt, F gen) {
-    uint32_t total;
-    const uint32_t excl = gb_wave_excl_sum(cnt, &total);
-    if (total == 0) return;
-    if (n + total > CAP) flush();
-    if (total <= CAP) {
-      for (uint32_t k = 0; k < cnt; k++) buf[n + excl + k] = gen(k);
-      n += total;
-    } else {
-      uint64_t base = 0;
-      if (gb_lane() == 0) base = atomicAdd(counter, (unsigned long long)total);
-      base = gb_shfl_u64(base, 0);
-      for (uint32_t k = 0; k < cnt; k++) q[base + excl + k] = gen(k);
-    }
-  }
-};
-
 __global__ void ms_fill_kernel(int64_t n, uint32_t *__restrict__ dist) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -137,8 +99,8 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_
   const uint64_t *in = c.q[a.a_in];
   const unsigned long long *min_ = c.mask[a.a_in - MS_NEAR0];
   unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
-  MsQueue<MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
-  MsQueue<MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
+  gb_wave_queue<uint64_t, MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
+  gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
   const uint64_t n = c.cnt[a.a_in];
   const uint64_t b = a.a_bucket;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -217,8 +179,8 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, uint64_t 
   const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
   const uint64_t *in = c.q[a.a_in];
   unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
-  MsQueue<MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
-  MsQueue<MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
+  gb_wave_queue<uint64_t, MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
+  gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
   const uint64_t n = c.cnt[a.a_in];
   const uint64_t bnext = a.a_bucket;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;

@@ -188,49 +188,8 @@ __global__ void sssp_seed_kernel(Ctx<Tr> c, int32_t ntasks, uint64_t *near_out,
   for (uint32_t i = 0; i < k; i++) near_out[b + i] = mk_entry((uint32_t)t, i, (uint32_t)s);
 }
 
-// Appends of one wave to a device queue, staged in a wave-private LDS buffer
-// and flushed with one counter atomicAdd per CAP entries.  Every wave of a
-// phase appends to the same few queues; one atomic per 64-edge round on a
-// single counter address serialises at the L2, buffering divides that by the
-// fill of the buffer.  Entry order within a queue is irrelevant to the
-// algorithm.  All 64 lanes must call push/flush together (wave-uniform n).
-template <int CAP>
-struct WaveQueue {
-  uint64_t *buf;  // this wave's LDS slice, CAP entries
-  uint64_t *q;
-  unsigned long long *counter;
-  uint32_t n;
-  __device__ __forceinline__ WaveQueue(uint64_t *b, uint64_t *queue, unsigned long long *cnt)
-      : buf(b), q(queue), counter(cnt), n(0) {}
-  __device__ __forceinline__ void flush() {
-    if (n == 0) return;
-    __builtin_amdgcn_wave_barrier();
-    uint64_t base = 0;
-    if (gb_lane() == 0) base = atomicAdd(counter, (unsigned long long)n);
-    base = gb_shfl_u64(base, 0);
-    for (uint32_t k = gb_lane(); k < n; k += GB_WAVE) q[base + k] = buf[k];
-    __builtin_amdgcn_wave_barrier();
-    n = 0;
-  }
-  // lane appends cnt entries gen(0..cnt-1)
-  template <class F>
-  __device__ __forceinline__ void push(uint32_t cnt, F gen) {
-    uint32_t total;
-    const uint32_t excl = gb_wave_excl_sum(cnt, &total);
-    if (total == 0) return;
-    if (n + total > CAP) flush();
-    if (total <= CAP) {
-      for (uint32_t k = 0; k < cnt; k++) buf[n + excl + k] = gen(k);
-      n += total;
-    } else {  // larger than the buffer: straight to the queue
-      uint64_t base = 0;
-      if (gb_lane() == 0) base = atomicAdd(counter, (unsigned long long)total);
-      base = gb_shfl_u64(base, 0);
-      for (uint32_t k = 0; k < cnt; k++) q[base + excl + k] = gen(k);
-    }
-  }
-};
-constexpr int NEAR_CAP = 384, FAR_CAP = 128;  // 4 KiB of LDS per wave
+// queue appends are staged per wave in LDS (gb_wave_queue, gb_device.hpp)
+constexpr int NEAR_CAP = 384, FAR_CAP = 128, S_CAP = 128;  // 5 KiB of LDS per wave
 
 // Light (HEAVY=false) or heavy (HEAVY=true) expansion of one queue.
 // Each wave takes 64 queue entries at a time (one per lane) and spreads their
@@ -252,14 +211,15 @@ __device__ __forceinline__ double gb_shfl_any<double>(double v, int src) {
 template <class Tr, bool HEAVY, bool SINGLE>
 __device__ __forceinline__ void expand_body(
     const Ctx<Tr> &c, const SGraph<typename Tr::W> &G0, uint64_t *lds_near, uint64_t *lds_far,
-    const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
+    uint64_t *lds_s, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
     uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ s_out,
     unsigned long long *n_s, uint64_t *__restrict__ far_out, unsigned long long *n_far,
     uint64_t bucket, uint32_t gen_next, uint32_t sround, uint32_t fepoch, uint32_t epw) {
   using D = typename Tr::D;
   const int lane = gb_lane();
-  WaveQueue<NEAR_CAP> nearq(lds_near, near_out, n_near);
-  WaveQueue<FAR_CAP> farq(lds_far, far_out, n_far);
+  gb_wave_queue<uint64_t, NEAR_CAP> nearq(lds_near, near_out, n_near);
+  gb_wave_queue<uint64_t, FAR_CAP> farq(lds_far, far_out, n_far);
+  gb_wave_queue<uint64_t, S_CAP> sq(lds_s, s_out, n_s);
   const uint64_t n = *n_in;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -292,10 +252,7 @@ __device__ __forceinline__ void expand_body(
         end = min(re, beg + CHUNK);
       }
     }
-    if (!HEAVY && __ballot(scnt != 0)) {
-      const uint64_t sb = gb_wave_reserve(n_s, scnt);
-      for (uint32_t k = 0; k < scnt; k++) s_out[sb + k] = mk_entry(t, k, u);
-    }
+    if (!HEAVY) sq.push(scnt, [&](uint32_t k) { return mk_entry(t, k, u); });
     // ---- spread the 64 entries' edges over the lanes
     uint32_t total;
     const uint32_t off = gb_wave_excl_sum((uint32_t)max(end - beg, 0), &total);
@@ -344,6 +301,7 @@ __device__ __forceinline__ void expand_body(
   }
   nearq.flush();
   farq.flush();
+  sq.flush();
   // per-wave statistics
   uint32_t tot;
   (void)gb_wave_excl_sum(relax, &tot);
@@ -358,8 +316,8 @@ __device__ __forceinline__ void far_scan_body(
     uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ far_out,
     unsigned long long *n_far, uint64_t bnext, uint32_t gen_next, uint32_t fepoch_next) {
   const int lane = gb_lane();
-  WaveQueue<NEAR_CAP> nearq(lds_near, near_out, n_near);
-  WaveQueue<FAR_CAP> farq(lds_far, far_out, n_far);
+  gb_wave_queue<uint64_t, NEAR_CAP> nearq(lds_near, near_out, n_near);
+  gb_wave_queue<uint64_t, FAR_CAP> farq(lds_far, far_out, n_far);
   const uint64_t n = *n_in;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -431,16 +389,18 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_step_kernel(Ctx<Tr> c, Queues q
   if constexpr (SINGLE) G0 = c.graphs[0];
   __shared__ uint64_t s_near[SS_BLOCK / GB_WAVE][NEAR_CAP];
   __shared__ uint64_t s_far[SS_BLOCK / GB_WAVE][FAR_CAP];
+  __shared__ uint64_t s_set[SS_BLOCK / GB_WAVE][S_CAP];
   uint64_t *ln = s_near[threadIdx.x / GB_WAVE], *lf = s_far[threadIdx.x / GB_WAVE];
+  uint64_t *ls = s_set[threadIdx.x / GB_WAVE];
   if (act == ACT_FAR) {
     far_scan_body<Tr, SINGLE>(c, G0, ln, lf, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[fr],
                               cnt + fr, ctl->a_bucket, ctl->a_gen, ctl->a_fepoch);
   } else if (act == ACT_LIGHT) {
-    expand_body<Tr, false, SINGLE>(c, G0, ln, lf, q.q[in], cnt + in, q.q[nr], cnt + nr,
+    expand_body<Tr, false, SINGLE>(c, G0, ln, lf, ls, q.q[in], cnt + in, q.q[nr], cnt + nr,
                                    q.q[C_S], cnt + C_S, q.q[fr], cnt + fr, ctl->a_bucket,
                                    ctl->a_gen, ctl->a_sround, ctl->a_fepoch, ctl->a_epw);
   } else {
-    expand_body<Tr, true, SINGLE>(c, G0, ln, lf, q.q[in], cnt + in, q.q[nr], cnt + nr,
+    expand_body<Tr, true, SINGLE>(c, G0, ln, lf, ls, q.q[in], cnt + in, q.q[nr], cnt + nr,
                                   q.q[C_S], cnt + C_S, q.q[fr], cnt + fr, ctl->a_bucket,
                                   ctl->a_gen, ctl->a_sround, ctl->a_fepoch, ctl->a_epw);
   }
