@@ -5,7 +5,7 @@
 // form a group and share everything but their distances:
 //   * distances are vertex-major, dist[v][W] (W = group width, 8..64), so the
 //     W sources' values of one vertex are one or two cache lines;
-//   * a queue entry is a vertex (or a 64-edge chunk of one) together with the
+//   * a queue entry is a vertex (or a 32/64-edge chunk of one) together with the
 //     mask of sources for which it must be expanded (mask[v], u64);
 //   * a wave takes one entry; its lanes are (edge slot, source) pairs, 64 / W
 //     edges per step, so an edge's col/w are read once for all W sources and
@@ -33,12 +33,21 @@
 namespace {
 
 constexpr int MS_BLOCK = 256;
-constexpr int MS_CHUNK_SHIFT = 6;  // 64 edges per queue entry at most
-constexpr int32_t MS_CHUNK = 1 << MS_CHUNK_SHIFT;
+// Edges per queue entry at most: 32 for groups of up to 16 sources (many
+// edge slots per step: a short chunk keeps the per-entry critical path short),
+// 64 for wider groups.
+template <int W>
+struct MsShape {
+  static constexpr int CHUNK_SHIFT = W <= 16 ? 5 : 6;
+  static constexpr int32_t CHUNK = 1 << CHUNK_SHIFT;
+  static constexpr int UNROLL = W == 64 ? 8 : 4;  // steps in flight per wave
+};
+constexpr int32_t MS_MIN_CHUNK = 32;  // for queue capacities
 constexpr int MS_NCAP = 256, MS_FCAP = 128;  // LDS append buffers per wave (entries)
 
+template <int W>
 __device__ __forceinline__ uint32_t ms_nchunks(int32_t deg) {
-  const uint32_t k = (uint32_t)((deg + MS_CHUNK - 1) >> MS_CHUNK_SHIFT);
+  const uint32_t k = (uint32_t)((deg + MsShape<W>::CHUNK - 1) >> MsShape<W>::CHUNK_SHIFT);
   return k ? k : 1u;
 }
 
@@ -74,7 +83,7 @@ __global__ void ms_seed_kernel(MsCtx c, const int32_t *__restrict__ src) {
   c.dist[(int64_t)s * W + t] = 0;
   const unsigned long long old = atomicOr(&c.mask[0][s], 1ull << t);
   if (old == 0) {
-    const uint32_t k = ms_nchunks(c.rp[s + 1] - c.rp[s]);
+    const uint32_t k = ms_nchunks<W>(c.rp[s + 1] - c.rp[s]);
     const unsigned long long b = atomicAdd(&c.cnt[MS_NEAR0], (unsigned long long)k);
     for (uint32_t i = 0; i < k; i++) c.q[0][b + i] = ((uint64_t)i << 32) | (uint32_t)s;
   }
@@ -92,7 +101,7 @@ template <int W>
 __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_t *lds_near,
                                          uint64_t *lds_far, uint32_t &relax, bool &ovf) {
   constexpr int ES = GB_WAVE / W;  // edge slots per step
-  constexpr int UNROLL = W >= 32 ? 4 : 2;
+  constexpr int UNROLL = MsShape<W>::UNROLL;
   const int lane = gb_lane();
   const int es = lane / W, t = lane % W;
   const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
@@ -111,8 +120,8 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_
     const unsigned long long m = min_[u];
     const bool act = t < c.ntask && ((m >> t) & 1ull);
     const uint32_t du = act ? c.dist[(int64_t)u * W + t] : 0xFFFFFFFFu;
-    const int32_t beg = c.rp[u] + (int32_t)(ch << MS_CHUNK_SHIFT);
-    const int32_t end = min(c.rp[u + 1], beg + MS_CHUNK);
+    const int32_t beg = c.rp[u] + (int32_t)(ch << MsShape<W>::CHUNK_SHIFT);
+    const int32_t end = min(c.rp[u + 1], beg + MsShape<W>::CHUNK);
     for (int32_t e0 = beg; e0 < end; e0 += ES * UNROLL) {
       int32_t x[UNROLL];
       uint32_t nd[UNROLL];
@@ -155,7 +164,7 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_
         uint32_t pn = 0, pf = 0;
         if (t == 0) {
           if (nm && atomicOr(&mout[x[k]], (unsigned long long)nm) == 0ull)
-            pn = ms_nchunks(c.rp[x[k] + 1] - c.rp[x[k]]);
+            pn = ms_nchunks<W>(c.rp[x[k] + 1] - c.rp[x[k]]);
           if (fm && atomicOr(&c.far[x[k]], (unsigned long long)fm) == 0ull) pf = 1;
         }
         const uint32_t xv = (uint32_t)x[k];
@@ -209,7 +218,7 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, uint64_t 
     if (t == 0 && i < n) {
       c.far[x] = km;  // the entry is this vertex's only one in the scan
       if (nm && atomicOr(&mout[x], (unsigned long long)nm) == 0ull)
-        pn = ms_nchunks(c.rp[x + 1] - c.rp[x]);
+        pn = ms_nchunks<W>(c.rp[x + 1] - c.rp[x]);
       if (km) pf = 1;
     }
     nq.push(pn, [&](uint32_t q) { return ((uint64_t)q << 32) | x; });
@@ -279,17 +288,20 @@ __global__ __launch_bounds__(256) void ms_unpack_kernel(int64_t V, int W, int nt
 }
 
 // reached (source, vertex) pairs and their out-edges, over the caller's rows
+// grid.y strides over source rows; one atomic pair per block (blockDim 256)
 __global__ void ms_count_kernel(int64_t V, int32_t nsrc, const int32_t *__restrict__ rp,
                                 const uint32_t *__restrict__ rows,
                                 unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long s_r[4], s_e[4];
   unsigned long long reached = 0, edges = 0;
-  const int64_t n = V * nsrc;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (rows[i] != 0xFFFFFFFFu) {
-      const int64_t v = i % V;
-      reached++;
-      edges += (unsigned long long)(rp[v + 1] - rp[v]);
+  for (int64_t t = blockIdx.y; t < nsrc; t += gridDim.y) {
+    const uint32_t *row = rows + t * V;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+      if (row[v] != 0xFFFFFFFFu) {
+        reached++;
+        edges += (unsigned long long)(rp[v + 1] - rp[v]);
+      }
     }
   }
 #pragma unroll
@@ -298,8 +310,13 @@ __global__ void ms_count_kernel(int64_t V, int32_t nsrc, const int32_t *__restri
     edges += __shfl_xor(edges, d, 64);
   }
   if (gb_lane() == 0) {
-    atomicAdd(&out[0], reached);
-    atomicAdd(&out[1], edges);
+    s_r[threadIdx.x >> 6] = reached;
+    s_e[threadIdx.x >> 6] = edges;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], s_r[0] + s_r[1] + s_r[2] + s_r[3]);
+    atomicAdd(&out[1], s_e[0] + s_e[1] + s_e[2] + s_e[3]);
   }
 }
 
@@ -412,10 +429,10 @@ gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64
   MsWork &w = g_msw;
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
   GB_HIP(hipMemsetAsync(w.cnt.p, 0, 2 * sizeof(unsigned long long), st));
-  const int64_t n = (int64_t)g->V * nsrc;
-  hipLaunchKernelGGL(ms_count_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192))),
-                     dim3(256), 0, st, (int64_t)g->V, nsrc, g->rp, d_dist,
-                     w.cnt.as<unsigned long long>());
+  const int64_t bx = std::max<int64_t>(
+      1, std::min<int64_t>(((int64_t)g->V + 255) / 256, std::max<int64_t>(1, gb_num_cus() * 4 / nsrc)));
+  hipLaunchKernelGGL(ms_count_kernel, dim3((unsigned)bx, (unsigned)std::min(nsrc, 65535)), dim3(256), 0, st,
+                     (int64_t)g->V, nsrc, g->rp, d_dist, w.cnt.as<unsigned long long>());
   GB_HIP(hipGetLastError());
   unsigned long long h[2];
   GB_HIP(hipMemcpyAsync(h, w.cnt.p, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -431,8 +448,8 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   MsWork &w = g_msw;
   const int64_t V = g->V, E = g->E;
   // queue capacity: a vertex enters a near queue at most once per phase with
-  // ceil(deg / 64) entries, a far queue at most once
-  const int64_t qn = V + E / MS_CHUNK + 1;
+  // ceil(deg / chunk) entries, a far queue at most once
+  const int64_t qn = V + E / MS_MIN_CHUNK + 1;
   const int Wmax = nsrc >= 64 ? 64 : (nsrc > 32 ? 64 : nsrc > 16 ? 32 : nsrc > 8 ? 16 : 8);
   GB_TRY(w.dist.ensure(sizeof(uint32_t) * (size_t)V * Wmax));
   GB_TRY(w.m0.ensure(sizeof(unsigned long long) * V));

@@ -347,7 +347,10 @@ gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib,
 // ---------------------------------------------------------------------------
 // max weight
 
+// max over a[0..n): wave, then block reduction, one atomicMax per block
+// (blockDim 256)
 __global__ void max_u32_kernel(const uint32_t *__restrict__ a, int64_t n, unsigned int *out) {
+  __shared__ uint32_t s_max[4];
   uint32_t m = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -357,7 +360,12 @@ __global__ void max_u32_kernel(const uint32_t *__restrict__ a, int64_t n, unsign
     uint32_t o = __shfl_xor(m, d, 64);
     m = o > m ? o : m;
   }
-  if (gb_lane() == 0) atomicMax(out, m);
+  if (gb_lane() == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+    if (m) atomicMax(out, m);
+  }
 }
 
 gb_status gb_graph_max_outdeg(gb_graph g, int32_t *out, hipStream_t st) {
@@ -374,7 +382,7 @@ gb_status gb_graph_max_outdeg(gb_graph g, int32_t *out, hipStream_t st) {
   GB_TRY(tmp.ensure(sizeof(unsigned int)));
   GB_TRY(gb_graph_outdeg(g, deg.as<int32_t>(), st));
   GB_HIP(hipMemsetAsync(tmp.p, 0, sizeof(unsigned int), st));
-  int blocks = (int)std::min<int64_t>((g->V + 255) / 256, 4096);
+  int blocks = (int)std::min<int64_t>((g->V + 255) / 256, (int64_t)gb_num_cus() * 4);
   hipLaunchKernelGGL(max_u32_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st,
                      (const uint32_t *)deg.as<int32_t>(), (int64_t)g->V, tmp.as<unsigned int>());
   GB_HIP(hipGetLastError());
@@ -399,7 +407,7 @@ gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st) {
   DevBuf tmp;
   GB_TRY(tmp.ensure(sizeof(unsigned int)));
   GB_HIP(hipMemsetAsync(tmp.p, 0, sizeof(unsigned int), st));
-  int blocks = (int)std::min<int64_t>((g->E + 255) / 256, 4096);
+  int blocks = (int)std::min<int64_t>((g->E + 255) / 256, (int64_t)gb_num_cus() * 4);
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(max_u32_kernel, dim3(blocks), dim3(256), 0, st, g->w, g->E,
                      tmp.as<unsigned int>());
@@ -415,7 +423,44 @@ gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st) {
 
 // ---------------------------------------------------------------------------
 // light/heavy separation (separate_csr, graph/csr.c:75-130) as a stable
-// in-row partition: one wave per row, ballot ranks keep the original order.
+// in-row partition with ballot ranks that keep the original order.  A wave
+// takes 4 consecutive rows: when none has more than 64 edges, each 16-lane
+// quarter of the wave partitions one row; otherwise the whole wave takes the
+// 4 rows one after the other.  (Most rows of sparse graphs are short, and a
+// wave per row would leave most lanes idle.)
+
+// Partition row [s, e) with the lanes of `grp` (bit mask of the participating
+// lanes, glane = rank of this lane in it, gl = its width): light first.
+template <class W, class Pred>
+__device__ __forceinline__ void separate_row(int32_t s, int32_t e, uint64_t grp, int gshift,
+                                             int glane, int gw, const int32_t *__restrict__ col,
+                                             const W *__restrict__ w, Pred light,
+                                             int32_t *__restrict__ col_out,
+                                             W *__restrict__ w_out, int32_t &nl_out) {
+  const uint64_t below = glane == 0 ? 0ull : ((1ull << glane) - 1);
+  int32_t nl = 0;
+  for (int32_t b = s; b < e; b += gw) {
+    const int32_t i = b + glane;
+    const bool f = i < e && light(w[i]);
+    nl += __popcll((__ballot(f) & grp) >> gshift);
+  }
+  int32_t lpos = s, hpos = s + nl;
+  for (int32_t b = s; b < e; b += gw) {
+    const int32_t i = b + glane;
+    const bool in = i < e;
+    const bool f = in && light(w[i]);
+    const uint64_t bl = (__ballot(f) & grp) >> gshift;
+    const uint64_t bh = (__ballot(in && !f) & grp) >> gshift;
+    if (in) {
+      const int32_t dst = f ? lpos + __popcll(bl & below) : hpos + __popcll(bh & below);
+      col_out[dst] = col[i];
+      w_out[dst] = w[i];
+    }
+    lpos += __popcll(bl);
+    hpos += __popcll(bh);
+  }
+  nl_out = nl;
+}
 
 template <class W, class Pred>
 __global__ void separate_rows_kernel(int32_t V, const int32_t *__restrict__ rp,
@@ -423,34 +468,30 @@ __global__ void separate_rows_kernel(int32_t V, const int32_t *__restrict__ rp,
                                      Pred light, int32_t *__restrict__ col_out,
                                      W *__restrict__ w_out, int32_t *__restrict__ le) {
   const int lane = gb_lane();
+  const int q = lane >> 4, ql = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t v = wave; v < V; v += nwaves) {
-    const int32_t s = rp[v], e = rp[v + 1];
-    // pass 1: count light
-    int32_t nl = 0;
-    for (int32_t b = s; b < e; b += 64) {
-      int32_t i = b + lane;
-      bool f = i < e && light(w[i]);
-      nl += __popcll(__ballot(f));
-    }
-    int32_t lpos = s, hpos = s + nl;
-    for (int32_t b = s; b < e; b += 64) {
-      int32_t i = b + lane;
-      bool in = i < e;
-      bool f = in && light(w[i]);
-      uint64_t bl = __ballot(f);
-      uint64_t bh = __ballot(in && !f);
-      uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-      if (in) {
-        int32_t dst = f ? lpos + __popcll(bl & below) : hpos + __popcll(bh & below);
-        col_out[dst] = col[i];
-        w_out[dst] = w[i];
+  for (int64_t v0 = wave * 4; v0 < V; v0 += nwaves * 4) {
+    const int64_t vq = v0 + q;
+    const int32_t dq = vq < V ? rp[vq + 1] - rp[vq] : 0;
+    int32_t mx = dq;
+#pragma unroll
+    for (int d = 16; d <= 32; d <<= 1) mx = max(mx, __shfl_xor(mx, d, 64));
+    if (mx <= 64) {  // a quarter wave per row
+      if (vq < V) {
+        int32_t nl;
+        separate_row(rp[vq], rp[vq + 1], 0xFFFFull << (16 * q), 16 * q, ql, 16, col, w, light,
+                     col_out, w_out, nl);
+        if (ql == 0) le[vq] = rp[vq] + nl;
       }
-      lpos += __popcll(bl);
-      hpos += __popcll(bh);
+    } else {  // the whole wave per row
+      for (int k = 0; k < 4 && v0 + k < V; k++) {
+        const int64_t v = v0 + k;
+        int32_t nl;
+        separate_row(rp[v], rp[v + 1], ~0ull, 0, lane, 64, col, w, light, col_out, w_out, nl);
+        if (lane == 0) le[v] = rp[v] + nl;
+      }
     }
-    if (lane == 0) le[v] = s + nl;
   }
 }
 

@@ -427,9 +427,20 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_count_kernel(Ctx<Tr> c, int32_t
     reached += __shfl_xor(reached, d, 64);
     edges += __shfl_xor(edges, d, 64);
   }
+  __shared__ unsigned long long s_r[SS_BLOCK / 64], s_e[SS_BLOCK / 64];
   if (gb_lane() == 0) {
-    atomicAdd(&out[0], reached);
-    atomicAdd(&out[1], edges);
+    s_r[threadIdx.x >> 6] = reached;
+    s_e[threadIdx.x >> 6] = edges;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // one atomic pair per block
+    unsigned long long r = 0, e = 0;
+    for (int k = 0; k < SS_BLOCK / 64; k++) {
+      r += s_r[k];
+      e += s_e[k];
+    }
+    atomicAdd(&out[0], r);
+    atomicAdd(&out[1], e);
   }
 }
 
@@ -615,7 +626,8 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
     stats->ms = ms;
     GB_TRY(w.counts.ensure(2 * sizeof(unsigned long long)));
     GB_HIP(hipMemsetAsync(w.counts.p, 0, 2 * sizeof(unsigned long long), st));
-    int gx = (int)std::min<int64_t>((maxV + SS_BLOCK - 1) / SS_BLOCK, 1024);
+    int gx = (int)std::min<int64_t>((maxV + SS_BLOCK - 1) / SS_BLOCK,
+                                    std::max<int64_t>(1, (int64_t)gb_num_cus() * 4 / ntasks));
     if (gx < 1) gx = 1;
     hipLaunchKernelGGL(sssp_count_kernel<Tr>, dim3(gx, ntasks), dim3(SS_BLOCK), 0, st, c, ntasks,
                        w.counts.as<unsigned long long>());
