@@ -61,7 +61,8 @@ class _SsspParams(C.Structure):
 
 class _SsspStats(C.Structure):
     _fields_ = [("buckets", C.c_int64), ("phases", C.c_int64), ("relax_edges", C.c_int64),
-                ("reached", C.c_int64), ("traversed", C.c_int64), ("ms", C.c_double)]
+                ("reached", C.c_int64), ("traversed", C.c_int64), ("ms", C.c_double),
+                ("improved_near", C.c_int64), ("improved_far", C.c_int64)]
 
 
 class _PrParams(C.Structure):
@@ -117,6 +118,8 @@ _SIGS = {
     "gb_mlp_loss_grad": (_i32, [_p, _i32, _i32, _p, _p, _i64, _p, C.POINTER(_f64), _p]),
     "gb_mlp_adam": (_i32, [_p, _p, _p, _p, _i32, C.c_float, C.c_float, C.c_float, C.c_float,
                            _i32, _p]),
+    "gb_csr_from_edges": (_i32, [_i32, _i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
+    "gb_csr_transpose": (_i32, [_i32, _i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
     "gb_graph_features": (_i32, [_p, _i32, _p, _p]),
     "gb_pr_warm_start": (_i32, [_p, _p, _p, _p, _p]),
 }
@@ -434,6 +437,34 @@ def separate(g: Graph, light_max: int):
     le = torch.empty(max(g.V, 1), dtype=torch.int32, device="cuda")
     _check(lib().gb_separate(g.handle, light_max, _ptr(col), _ptr(w), _ptr(le), _stream()))
     return col[:g.E], w[:g.E], le[:g.V]
+
+
+def csr_from_edges(V: int, src, dst, w=None):
+    """Device CSR (row_ptr, col, w) of device edge arrays; rows in source order,
+    a row's edges in input order.  w: None, uint32 (int32 storage) or float64."""
+    torch = _torch()
+    E = int(src.numel())
+    rp = torch.empty(V + 1, dtype=torch.int32, device="cuda")
+    col = torch.empty(max(E, 1), dtype=torch.int32, device="cuda")
+    wb = 0 if w is None else w.element_size()
+    wo = None if w is None else torch.empty(max(E, 1), dtype=w.dtype, device="cuda")
+    _check(lib().gb_csr_from_edges(V, E, _ptr(src), _ptr(dst), _ptr(w), wb, _ptr(rp), _ptr(col),
+                                   _ptr(wo), _stream()))
+    return rp, col[:E], (None if wo is None else wo[:E])
+
+
+def csr_transpose(row_ptr, col, w=None):
+    """Device reverse CSR of a device CSR (rows keep the order of their sources)."""
+    torch = _torch()
+    V = row_ptr.numel() - 1
+    E = int(col.numel())
+    rp = torch.empty(V + 1, dtype=torch.int32, device="cuda")
+    rc = torch.empty(max(E, 1), dtype=torch.int32, device="cuda")
+    wb = 0 if w is None else w.element_size()
+    wo = None if w is None else torch.empty(max(E, 1), dtype=w.dtype, device="cuda")
+    _check(lib().gb_csr_transpose(V, E, _ptr(row_ptr), _ptr(col), _ptr(w), wb, _ptr(rp), _ptr(rc),
+                                  _ptr(wo), _stream()))
+    return rp, rc[:E], (None if wo is None else wo[:E])
 
 
 def u32(t) -> np.ndarray:

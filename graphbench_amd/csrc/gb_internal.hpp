@@ -132,7 +132,8 @@ void gb_msssp_release();
 // relaxations and the overflow flag.
 gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int32_t nsrc,
                        uint32_t *d_dist, int max_phases, int64_t *phases, int64_t *buckets,
-                       unsigned long long *relax, bool *ovf, hipStream_t st);
+                       unsigned long long *relax, int64_t *imp_near, int64_t *imp_far, bool *ovf,
+                       hipStream_t st);
 gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64_t *reached,
                          int64_t *traversed, hipStream_t st);
 void gb_bc_release();

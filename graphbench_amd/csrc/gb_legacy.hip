@@ -107,7 +107,7 @@ void upload(DevBuf &b, const std::vector<T> &v) {
 }
 
 // t_csr -> row_ptr/col (int32), weights f64 (and u32 when all are integers in
-// [0, 2^32)); optional reverse CSR with weights (stable counting sort).
+// [0, 2^32)); optional reverse CSR with weights, transposed on the device.
 void to_device(const t_csr *gs, DevGraph &dg, bool reverse, bool weights) {
   if (!gs || gs->v_size <= 0 || gs->v_size >= (1ll << 31) || gs->e_size < 0 ||
       gs->e_size >= (1ll << 31))
@@ -150,21 +150,15 @@ void to_device(const t_csr *gs, DevGraph &dg, bool reverse, bool weights) {
   upload(dg.col, col);
   if (dg.integral) upload(dg.w, w32);
   if (weights) upload(dg.w64, w64);
-  if (reverse) {
-    std::vector<int32_t> rrp(V + 1, 0), rcol(E);
-    std::vector<double> rw(weights ? E : 0);
-    for (int64_t e = 0; e < E; e++) rrp[col[e] + 1]++;
-    for (int32_t i = 0; i < V; i++) rrp[i + 1] += rrp[i];
-    std::vector<int32_t> pos(rrp.begin(), rrp.end() - 1);
-    for (int32_t u = 0; u < V; u++)
-      for (int32_t e = rp[u]; e < rp[u + 1]; e++) {
-        int32_t p = pos[col[e]]++;
-        rcol[p] = u;
-        if (weights) rw[p] = w64[e];
-      }
-    upload(dg.rrp, rrp);
-    upload(dg.rcol, rcol);
-    if (weights) upload(dg.rw64, rw);
+  if (reverse) {  // built on the device (gb_csr_transpose)
+    check(dg.rrp.ensure(sizeof(int32_t) * (V + 1)), "device allocation");
+    check(dg.rcol.ensure(sizeof(int32_t) * (E + 1)), "device allocation");
+    if (weights) check(dg.rw64.ensure(sizeof(double) * (E + 1)), "device allocation");
+    check(gb_csr_transpose(V, E, dg.rp.as<int32_t>(), dg.col.as<int32_t>(),
+                           weights ? (const void *)dg.w64.as<double>() : nullptr, weights ? 8 : 0,
+                           dg.rrp.as<int32_t>(), dg.rcol.as<int32_t>(),
+                           weights ? (void *)dg.rw64.as<double>() : nullptr, nullptr),
+          "reverse CSR");
   }
   check(gb_graph_wrap(&dg.g, V, E, dg.rp.as<int32_t>(), dg.col.as<int32_t>(),
                       dg.integral ? dg.w.as<uint32_t>() : nullptr,

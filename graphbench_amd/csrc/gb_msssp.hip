@@ -99,7 +99,8 @@ __global__ void ms_ctrl_kernel(unsigned long long *__restrict__ cnt, MsCtl *__re
 // Light phase: expand the entries of near queue a_in.
 template <int W>
 __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_t *lds_near,
-                                         uint64_t *lds_far, uint32_t &relax, bool &ovf) {
+                                         uint64_t *lds_far, uint32_t &relax, uint32_t &imp_near,
+                                         uint32_t &imp_far, bool &ovf) {
   constexpr int ES = GB_WAVE / W;  // edge slots per step
   constexpr int UNROLL = MsShape<W>::UNROLL;
   const int lane = gb_lane();
@@ -156,6 +157,8 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_
               near = true;
             else
               far = true;
+            imp_near += near;
+            imp_far += far;
           }
         }
         // one atomicOr per edge slot: the slot's first lane carries its mask
@@ -244,12 +247,16 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
   __shared__ uint64_t s_far[MS_BLOCK / GB_WAVE][MS_FCAP];
   uint64_t *ln = s_near[threadIdx.x / GB_WAVE], *lf = s_far[threadIdx.x / GB_WAVE];
   if (act == ACT_LIGHT) {
-    uint32_t relax = 0;
+    uint32_t relax = 0, imp_near = 0, imp_far = 0;
     bool ovf = false;
-    ms_light<W>(c, a, ln, lf, relax, ovf);
-    uint32_t tot;
+    ms_light<W>(c, a, ln, lf, relax, imp_near, imp_far, ovf);
+    uint32_t tot, tn, tf;
     (void)gb_wave_excl_sum(relax, &tot);
+    (void)gb_wave_excl_sum(imp_near, &tn);
+    (void)gb_wave_excl_sum(imp_far, &tf);
     if (gb_lane() == 0 && tot) atomicAdd(&c.cnt[MS_RELAX], (unsigned long long)tot);
+    if (gb_lane() == 0 && tn) atomicAdd(&c.cnt[MS_IMP_NEAR], (unsigned long long)tn);
+    if (gb_lane() == 0 && tf) atomicAdd(&c.cnt[MS_IMP_FAR], (unsigned long long)tf);
     if (__ballot(ovf) && gb_lane() == 0) atomicOr(&c.cnt[MS_OVF], 1ull);
   } else {
     ms_far<W>(c, a, ln, lf);
@@ -336,7 +343,7 @@ MsWork &g_msw_ref() {
 template <int W>
 gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st,
                    uint32_t *out_rows, int64_t *phases, int64_t *buckets,
-                   unsigned long long *relax, bool *ovf) {
+                   unsigned long long *relax, int64_t *imp_near, int64_t *imp_far, bool *ovf) {
   MsWork &w = g_msw;
   const int64_t V = c.V;
   hipLaunchKernelGGL(ms_fill_kernel, dim3((unsigned)std::min<int64_t>((V * W + 255) / 256, 65536)),
@@ -414,6 +421,8 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   *phases += hs.phases;
   *buckets += hs.buckets;
   *relax += h[MS_RELAX];
+  *imp_near += (int64_t)h[MS_IMP_NEAR];
+  *imp_far += (int64_t)h[MS_IMP_FAR];
   if (h[MS_OVF]) *ovf = true;
   return GB_OK;
 }
@@ -444,7 +453,8 @@ gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64
 
 gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int32_t nsrc,
                        uint32_t *d_dist, int max_phases, int64_t *phases, int64_t *buckets,
-                       unsigned long long *relax, bool *ovf, hipStream_t st) {
+                       unsigned long long *relax, int64_t *imp_near, int64_t *imp_far, bool *ovf,
+                       hipStream_t st) {
   MsWork &w = g_msw;
   const int64_t V = g->V, E = g->E;
   // queue capacity: a vertex enters a near queue at most once per phase with
@@ -484,16 +494,16 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
     const int32_t *ds = w.src.as<int32_t>();
     switch (W) {
       case 8:
-        GB_TRY(ms_group<8>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+        GB_TRY(ms_group<8>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
         break;
       case 16:
-        GB_TRY(ms_group<16>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+        GB_TRY(ms_group<16>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
         break;
       case 32:
-        GB_TRY(ms_group<32>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+        GB_TRY(ms_group<32>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
         break;
       default:
-        GB_TRY(ms_group<64>(c, ds, max_phases, st, rows, phases, buckets, relax, ovf));
+        GB_TRY(ms_group<64>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
     }
   }
   return GB_OK;

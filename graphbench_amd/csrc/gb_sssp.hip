@@ -223,7 +223,7 @@ __device__ __forceinline__ void expand_body(
   const uint64_t n = *n_in;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  uint32_t relax = 0;
+  uint32_t relax = 0, imp_near = 0, imp_far = 0;
   bool ovf = false;
   // epw (1..64) entries per wave and round: small queues use fewer entries per
   // wave so that more waves run
@@ -283,12 +283,14 @@ __device__ __forceinline__ void expand_body(
           if (nd < old) {
             // dedup stamps: a plain read first, the exchange only when unset
             if (Tr::bucket(nd, tk.delta, tk.magic) <= bucket) {
+              imp_near++;
               uint32_t *ns = &c.nst[tk.s(v)];
               if (*ns != gen_next && atomicExch(ns, gen_next) != gen_next) {
                 ncnt = nchunks(tk.G.le[v] - tk.G.rp[v]);
                 if (ncnt == 0) ncnt = 1;
               }
             } else {
+              imp_far++;
               uint32_t *fs = &c.fst[tk.s(v)];
               if (*fs != fepoch && atomicExch(fs, fepoch) != fepoch) far = true;
             }
@@ -303,9 +305,13 @@ __device__ __forceinline__ void expand_body(
   farq.flush();
   sq.flush();
   // per-wave statistics
-  uint32_t tot;
+  uint32_t tot, tn, tf;
   (void)gb_wave_excl_sum(relax, &tot);
+  (void)gb_wave_excl_sum(imp_near, &tn);
+  (void)gb_wave_excl_sum(imp_far, &tf);
   if (lane == 0 && tot) atomicAdd(&c.cnt[C_RELAX], (unsigned long long)tot);
+  if (lane == 0 && tn) atomicAdd(&c.cnt[C_IMP_NEAR], (unsigned long long)tn);
+  if (lane == 0 && tf) atomicAdd(&c.cnt[C_IMP_FAR], (unsigned long long)tf);
   if (__ballot(ovf) && lane == 0) atomicOr(&c.cnt[C_OVF], 1ull);
 }
 
@@ -623,6 +629,8 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
     stats->buckets = buckets;
     stats->phases = phases;
     stats->relax_edges = (int64_t)h[C_RELAX];
+    stats->improved_near = (int64_t)h[C_IMP_NEAR];
+    stats->improved_far = (int64_t)h[C_IMP_FAR];
     stats->ms = ms;
     GB_TRY(w.counts.ensure(2 * sizeof(unsigned long long)));
     GB_HIP(hipMemsetAsync(w.counts.p, 0, 2 * sizeof(unsigned long long), st));
@@ -697,7 +705,8 @@ extern "C" gb_status gb_sssp(gb_graph g, const gb_sssp_params *p, const int32_t 
     GB_HIP(hipEventCreate(&e1));
     GB_HIP(hipEventRecord(e0, st));
     gb_status r = gb_msssp_run(g, delta, h_sources, nsrc, d_dist, p ? p->max_phases : 0,
-                               &s.phases, &s.buckets, &relax, &ovf, st);
+                               &s.phases, &s.buckets, &relax, &s.improved_near,
+                               &s.improved_far, &ovf, st);
     if (r == GB_OK) r = hipEventRecord(e1, st) == hipSuccess ? GB_OK : gb_fail(GB_ERR_HIP, "event");
     if (r == GB_OK && stats) {
       float ms = 0.0f;

@@ -13,8 +13,10 @@
 
 // counters (u64): near A/B, S, far A/B, min bucket, relaxations, overflow.
 // Queue k (0/1 near, 2 settled set S, 3/4 far) has length counter k.
+// C_IMP_NEAR / C_IMP_FAR count relaxations that improved a distance, with
+// the new distance in the current bucket or in a later one.
 enum { C_NEAR0 = 0, C_NEAR1 = 1, C_S = 2, C_FAR0 = 3, C_FAR1 = 4, C_MINB = 5, C_RELAX = 6,
-       C_OVF = 7, C_N = 8 };
+       C_OVF = 7, C_IMP_NEAR = 8, C_IMP_FAR = 9, C_N = 10 };
 constexpr uint32_t SSSP_MAX_EPW = 64;  // queue entries per wave at most
 
 enum { ACT_NONE = 0, ACT_LIGHT = 1, ACT_HEAVY = 2, ACT_FAR = 3 };
@@ -140,7 +142,7 @@ __host__ __device__ inline void sssp_ctl_advance(SsspCtl &s, unsigned long long 
 // mask per vertex), counters: MS_NEAR0..1, MS_FAR0..1, MS_MINB, MS_RELAX,
 // MS_OVF.  No light/heavy split: an expansion relaxes all edges of a vertex.
 enum { MS_NEAR0 = 0, MS_NEAR1 = 1, MS_FAR0 = 2, MS_FAR1 = 3, MS_MINB = 4, MS_RELAX = 5,
-       MS_OVF = 6, MS_N = 7 };
+       MS_OVF = 6, MS_IMP_NEAR = 7, MS_IMP_FAR = 8, MS_N = 9 };
 enum { MS_ST_LIGHT = 0, MS_ST_AFTER_BUCKET, MS_ST_FAR, MS_ST_AFTER_FAR };
 
 struct MsCtl {

@@ -88,6 +88,25 @@ gb_status gb_merge_path_split(int32_t V, const int32_t *d_row_ptr, int64_t items
 gb_status gb_edge_balanced_parts(int32_t V, const int32_t *d_row_ptr,
                                  int32_t p, int64_t *d_distrib, void *stream);
 
+/* ---- CSR construction on the device -----------------------------------------
+ * Replace the host CSR builds of the reference's readers (scan_csr_idx
+ * graph/csr.c:133-202 counts, read_graph_csr graph/csr.c:4-66 and
+ * read_csr_bin graph/csr.c:211-299 scatter in file order).
+ * gb_csr_from_edges: CSR of E device edges (d_src[e] -> d_dst[e]), rows in
+ * source order, the edges of a row in input order (stable), optional weights
+ * of w_bytes 4 (uint32) or 8 (double) carried along (w_bytes 0: none).
+ * d_row_ptr has V+1 entries, d_col (and d_w_out) E.  Endpoints outside
+ * [0, V) are rejected.  gb_csr_transpose: the reverse CSR of a CSR (rows of
+ * the transpose keep the order of their source rows). */
+gb_status gb_csr_from_edges(int32_t V, int64_t E, const int32_t *d_src,
+                            const int32_t *d_dst, const void *d_w, int32_t w_bytes,
+                            int32_t *d_row_ptr, int32_t *d_col, void *d_w_out,
+                            void *stream);
+gb_status gb_csr_transpose(int32_t V, int64_t E, const int32_t *d_row_ptr,
+                           const int32_t *d_col, const void *d_w, int32_t w_bytes,
+                           int32_t *d_trow_ptr, int32_t *d_tcol, void *d_tw,
+                           void *stream);
+
 /* ---- light/heavy split ----------------------------------------------------
  * Stable per-row partition: light edges (w <= light_max) first.  Writes
  * d_col_out/d_w_out (E each) and d_light_end[V] (absolute offset where the
@@ -110,6 +129,11 @@ typedef struct gb_sssp_stats {
   int64_t reached;      /* (source, vertex) pairs with finite distance */
   int64_t traversed;    /* out-edges of reached vertices, summed over sources */
   double ms;            /* device time of the solve */
+  /* relaxation outcomes (the reference's REV_STATS, csr_sspath_array.c:738-759,
+   * as totals): relaxations that lowered a distance, with the new distance in
+   * the current bucket (re-expanded there) or in a later bucket */
+  int64_t improved_near;
+  int64_t improved_far;
 } gb_sssp_stats;
 
 /* Distances from nsrc sources, all processed together in lockstep buckets.

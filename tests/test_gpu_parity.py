@@ -75,6 +75,27 @@ def test_sssp_batched_engine_matches_per_source(gpu, nsrc, monkeypatch):
         assert st["reached"] == int((d != 0xFFFFFFFF).sum())
 
 
+@pytest.mark.parametrize("nsrc", [1, 6])
+def test_sssp_relaxation_outcome_counters(gpu, nsrc, monkeypatch):
+    """Relaxation outcomes (the reference's REV_STATS totals): every reached
+    vertex other than a source was improved at least once, improvements are a
+    subset of relaxations, and small buckets send improvements to later
+    buckets while one bucket covering every distance sends none there."""
+    gb = gpu
+    h = gb.gen_rmat(12, 8, seed=77, weights="uniform", wlo=1, whi=200)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    srcs = gb.pick_sources(h, nsrc, 78)
+    for per_source in (False, True):
+        if per_source:
+            monkeypatch.setenv("GB_SSSP_PER_SOURCE", "1")
+        for delta, expect_far in ((8, True), (1 << 30, False)):
+            _, st = gb.sssp(g, srcs, delta=delta, stats=True)
+            imp = st["improved_near"] + st["improved_far"]
+            assert st["reached"] - nsrc <= imp <= st["relax_edges"], st
+            assert (st["improved_far"] > 0) == expect_far, st
+        monkeypatch.delenv("GB_SSSP_PER_SOURCE", raising=False)
+
+
 def test_sssp_batched_overflow_reported(gpu):
     gb = gpu
     h = gb.HostCSR.from_edges(4, [0, 1, 2], [1, 2, 3], [0xFFFFFFF0, 0xFFFFFFF0, 5])
@@ -434,6 +455,41 @@ def test_edge_balanced_parts(gpu):
         d = gb.edge_balanced_parts(rp, p).cpu().numpy()
         ref = oracle.comp_distrib(h.outdeg(), h.E, p)
         assert np.array_equal(d, ref), p
+
+
+def test_csr_from_edges_and_transpose(gpu):
+    """Device CSR construction: rows in source order, a row's edges in input
+    order (a stable sort, like the reference's scatter in file order); the
+    transpose equals the host transpose; weights ride along; bad ids fail."""
+    gb = gpu
+    import torch
+    rng = np.random.default_rng(3)
+    V, E = 5000, 60000
+    src = rng.integers(0, V, E).astype(np.int32)
+    dst = rng.integers(0, V, E).astype(np.int32)
+    w = rng.integers(1, 1000, E).astype(np.uint32)
+    w64 = rng.random(E)
+    d = gb.to_device
+    rp, col, wo = gb.csr_from_edges(V, d(src), d(dst), d(w))
+    order = np.argsort(src, kind="stable")
+    ref_rp = np.concatenate([[0], np.cumsum(np.bincount(src, minlength=V))]).astype(np.int32)
+    assert np.array_equal(rp.cpu().numpy(), ref_rp)
+    assert np.array_equal(col.cpu().numpy(), dst[order])
+    assert np.array_equal(gb.u32(wo), w[order])
+    _, _, wf = gb.csr_from_edges(V, d(src), d(dst), torch.from_numpy(w64).cuda())
+    assert np.array_equal(wf.cpu().numpy(), w64[order])
+    h = gb.gen_rmat(13, 8, seed=4, weights="uniform", wlo=1, whi=50)
+    r = h.transpose()
+    trp, tcol, tw = gb.csr_transpose(d(h.row_ptr), d(h.col), d(h.w))
+    assert np.array_equal(trp.cpu().numpy(), r.row_ptr)
+    assert np.array_equal(tcol.cpu().numpy(), r.col)
+    assert np.array_equal(gb.u32(tw), r.w)
+    erp, ecol, _ = gb.csr_from_edges(7, d(np.zeros(0, np.int32)), d(np.zeros(0, np.int32)))
+    assert np.array_equal(erp.cpu().numpy(), np.zeros(8, np.int32)) and ecol.numel() == 0
+    bad = src.copy()
+    bad[10] = V
+    with pytest.raises(gb.GBError):
+        gb.csr_from_edges(V, d(bad), d(dst))
 
 
 def test_separate(gpu):
