@@ -3,6 +3,10 @@
  * (libgraphbench.so).  Plain C types only: device pointers are passed as
  * pointers, streams as void* (a hipStream_t, NULL = the default stream).
  *
+ * Calls share per-process device workspaces (queues, scratch arrays) and
+ * cache derived structures on the graph handle: issue them from one host
+ * thread at a time.  Results are ready when a call returns.
+ *
  * Graph layout on the device (one graph):
  *   row_ptr[V+1]  int32   CSR offsets, row_ptr[0] = 0, E = row_ptr[V] < 2^31
  *   col[E]        int32   destination ids (source ids for the reverse CSR)
