@@ -5,15 +5,22 @@
 // form a group and share everything but their distances:
 //   * distances are vertex-major, dist[v][W] (W = group width, 8..64), so the
 //     W sources' values of one vertex are one or two cache lines;
-//   * a queue entry is a vertex (or a 32/64-edge chunk of one) together with the
-//     mask of sources for which it must be expanded (mask[v], u64);
-//   * a wave takes one entry; its lanes are (edge slot, source) pairs, 64 / W
-//     edges per step, so an edge's col/w are read once for all W sources and
-//     the W relaxations of one target are one coalesced atomicMin;
+//   * a queue entry is a 32/64-edge chunk of a vertex (vertex and edge range,
+//     16 bytes) together with the mask of sources for which it must be
+//     expanded (mask[v], u64);
+//   * a wave takes one entry and loads its col/w with one coalesced load; its
+//     lanes are then (edge slot, source) pairs, 64 / W edges per step, the
+//     slot's target and weight taken from the loading lane by a cross-lane
+//     read, so the W relaxations of one target are one coalesced line;
+//   * a group of steps is issued together: all target reads, then the
+//     atomicMin of every improving pair (no return value: the decision uses
+//     the value read), then one atomicOr per (step, slot) issued by one lane
+//     each in a single instruction, so a group costs three memory round trips
+//     whatever its size; the next entry's loads are in flight meanwhile;
 //   * an improved (target, source) pair sets the source's bit in the target's
-//     next-phase mask (same bucket) or far mask (later bucket) with one
-//     atomicOr per edge slot, and the lane that turns a mask from empty to
-//     non-empty appends the target to the next queue (dedup without stamps).
+//     next-phase mask (same bucket) or far mask (later bucket), and the lane
+//     that turns a mask from empty to non-empty appends the target to the next
+//     queue (dedup without stamps).
 // All sources step through bucket indices in lockstep: light phases while the
 // near queue is non-empty, then a far scan that moves the next non-empty
 // bucket's (vertex, source) pairs to the near queue.  There is no light/heavy
@@ -40,15 +47,32 @@ template <int W>
 struct MsShape {
   static constexpr int CHUNK_SHIFT = W <= 16 ? 5 : 6;
   static constexpr int32_t CHUNK = 1 << CHUNK_SHIFT;
-  static constexpr int UNROLL = W == 64 ? 8 : 4;  // steps in flight per wave
+  static constexpr int ES = GB_WAVE / W;                // edge slots per step
+  static constexpr int STEPS = CHUNK / ES;              // steps per full chunk
+  static constexpr int UNROLL = STEPS < 8 ? STEPS : 8;  // steps per group
+  static constexpr int G = ES * UNROLL;                 // edges per group (<= 64)
 };
 constexpr int32_t MS_MIN_CHUNK = 32;  // for queue capacities
-constexpr int MS_NCAP = 256, MS_FCAP = 128;  // LDS append buffers per wave (entries)
+constexpr int MS_NCAP = 128, MS_FCAP = 128;  // LDS append buffers per wave (entries)
+
+// near-queue entry: one chunk [beg, end) of vertex v's edges
+struct __align__(16) MsEnt {
+  uint32_t v;
+  int32_t beg, end;
+  uint32_t pad;
+};
 
 template <int W>
 __device__ __forceinline__ uint32_t ms_nchunks(int32_t deg) {
   const uint32_t k = (uint32_t)((deg + MsShape<W>::CHUNK - 1) >> MsShape<W>::CHUNK_SHIFT);
   return k ? k : 1u;
+}
+
+// chunk q of the vertex whose edges are [r0, r1)
+template <int W>
+__device__ __forceinline__ MsEnt ms_chunk(uint32_t v, int32_t r0, int32_t r1, uint32_t q) {
+  const int32_t b = r0 + (int32_t)(q << MsShape<W>::CHUNK_SHIFT);
+  return MsEnt{v, b, min(r1, b + MsShape<W>::CHUNK), 0u};
 }
 
 struct MsCtx {
@@ -60,7 +84,8 @@ struct MsCtx {
   uint32_t *dist;                  // [V][W]
   unsigned long long *mask[2];     // near masks, one per near queue
   unsigned long long *far;         // far masks
-  uint64_t *q[4];                  // near 0/1, far 2/3
+  MsEnt *qn[2];                    // near queues 0/1 (chunks)
+  uint64_t *qf[2];                 // far queues 0/1 (vertices)
   unsigned long long *cnt;         // MS_* counters
   uint64_t delta, magic;           // bucket width and its division constant
 };
@@ -83,9 +108,10 @@ __global__ void ms_seed_kernel(MsCtx c, const int32_t *__restrict__ src) {
   c.dist[(int64_t)s * W + t] = 0;
   const unsigned long long old = atomicOr(&c.mask[0][s], 1ull << t);
   if (old == 0) {
-    const uint32_t k = ms_nchunks<W>(c.rp[s + 1] - c.rp[s]);
+    const int32_t r0 = c.rp[s], r1 = c.rp[s + 1];
+    const uint32_t k = ms_nchunks<W>(r1 - r0);
     const unsigned long long b = atomicAdd(&c.cnt[MS_NEAR0], (unsigned long long)k);
-    for (uint32_t i = 0; i < k; i++) c.q[0][b + i] = ((uint64_t)i << 32) | (uint32_t)s;
+    for (uint32_t i = 0; i < k; i++) c.qn[0][b + i] = ms_chunk<W>((uint32_t)s, r0, r1, i);
   }
 }
 
@@ -96,85 +122,124 @@ __global__ void ms_ctrl_kernel(unsigned long long *__restrict__ cnt, MsCtl *__re
   *ctl = s;
 }
 
-// Light phase: expand the entries of near queue a_in.
+// What a wave loads for one near entry: the entry's source mask, this lane's
+// source distance at the vertex, and the chunk's edges (lane r holds edge
+// beg + r).
+struct MsIn {
+  unsigned long long m;
+  uint32_t du;
+  int32_t x;
+  uint32_t w;
+};
+
 template <int W>
-__device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_t *lds_near,
+__device__ __forceinline__ MsIn ms_load(const MsCtx &c, const unsigned long long *min_,
+                                        const MsEnt &e, int lane, int t) {
+  MsIn r;
+  r.m = min_[e.v];
+  r.du = t < c.ntask ? c.dist[(int64_t)e.v * W + t] : 0xFFFFFFFFu;
+  const bool in = lane < e.end - e.beg;
+  r.x = in ? c.col[e.beg + lane] : 0;
+  r.w = in ? c.w[e.beg + lane] : 0u;
+  return r;
+}
+
+// Light phase: expand the entries of near queue a_in.  Entries are software
+// pipelined: entry i is expanded while entry i + nwaves's loads and entry
+// i + 2 nwaves's descriptor are in flight.
+template <int W>
+__device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *lds_near,
                                          uint64_t *lds_far, uint32_t &relax, uint32_t &imp_near,
                                          uint32_t &imp_far, bool &ovf) {
-  constexpr int ES = GB_WAVE / W;  // edge slots per step
-  constexpr int UNROLL = MsShape<W>::UNROLL;
+  using S = MsShape<W>;
+  constexpr int ES = S::ES, UNROLL = S::UNROLL, G = S::G;
   const int lane = gb_lane();
   const int es = lane / W, t = lane % W;
+  // the lane that issues the mask update of (step lane / ES, slot lane % ES)
+  const int kj = lane / ES, ej = lane % ES;
   const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
-  const uint64_t *in = c.q[a.a_in];
+  const MsEnt *in = c.qn[a.a_in - MS_NEAR0];
   const unsigned long long *min_ = c.mask[a.a_in - MS_NEAR0];
   unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
-  gb_wave_queue<uint64_t, MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
-  gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
+  gb_wave_queue<MsEnt, MS_NCAP> nq(lds_near, c.qn[a.a_near - MS_NEAR0], &c.cnt[a.a_near]);
+  gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.qf[a.a_far - MS_FAR0], &c.cnt[a.a_far]);
   const uint64_t n = c.cnt[a.a_in];
   const uint64_t b = a.a_bucket;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = wave; i < n; i += nwaves) {
-    const uint64_t ent = in[i];
-    const uint32_t u = (uint32_t)ent, ch = (uint32_t)(ent >> 32);
-    const unsigned long long m = min_[u];
-    const bool act = t < c.ntask && ((m >> t) & 1ull);
-    const uint32_t du = act ? c.dist[(int64_t)u * W + t] : 0xFFFFFFFFu;
-    const int32_t beg = c.rp[u] + (int32_t)(ch << MsShape<W>::CHUNK_SHIFT);
-    const int32_t end = min(c.rp[u + 1], beg + MsShape<W>::CHUNK);
-    for (int32_t e0 = beg; e0 < end; e0 += ES * UNROLL) {
+  uint64_t i = wave, i1 = wave + nwaves;
+  MsEnt ecur{}, enxt{};
+  MsIn cur{};
+  if (i < n) {
+    ecur = in[i];
+    cur = ms_load<W>(c, min_, ecur, lane, t);
+  }
+  if (i1 < n) enxt = in[i1];
+  while (i < n) {
+    MsIn nxt{};
+    MsEnt enn{};
+    const uint64_t i2 = i1 + nwaves;
+    if (i1 < n) nxt = ms_load<W>(c, min_, enxt, lane, t);
+    if (i2 < n) enn = in[i2];
+    const bool act = t < c.ntask && ((cur.m >> t) & 1ull);
+    const int cnt = ecur.end - ecur.beg;
+    for (int g0 = 0; g0 < cnt; g0 += G) {
       int32_t x[UNROLL];
-      uint32_t nd[UNROLL];
+      uint32_t nd[UNROLL], old[UNROLL];
       bool ok[UNROLL];
 #pragma unroll
       for (int k = 0; k < UNROLL; k++) {
-        const int32_t e = e0 + k * ES + es;
-        // every lane of a slot loads the slot's target: the slot's first
-        // lane needs it for the mask update even when its source is idle
-        x[k] = e < end ? c.col[e] : 0;
-        ok[k] = act && e < end;
-        nd[k] = 0;
-        if (ok[k]) {
-          relax++;
-          uint64_t s = (uint64_t)du + c.w[e];
-          if (s >= 0xFFFFFFFFull) {
-            ovf = true;
-            s = 0xFFFFFFFEull;
-          }
-          nd[k] = (uint32_t)s;
+        const int r = g0 + k * ES + es;  // < CHUNK <= 64
+        x[k] = __shfl(cur.x, r, GB_WAVE);
+        const uint32_t wk = __shfl(cur.w, r, GB_WAVE);
+        ok[k] = act && r < cnt;
+        uint64_t s = (uint64_t)cur.du + wk;
+        if (ok[k] && s >= 0xFFFFFFFFull) {
+          ovf = true;
+          s = 0xFFFFFFFEull;
         }
+        nd[k] = (uint32_t)s;
+        relax += ok[k];
       }
 #pragma unroll
-      for (int k = 0; k < UNROLL; k++)
-        if (ok[k]) ok[k] = nd[k] < c.dist[(int64_t)x[k] * W + t];
+      for (int k = 0; k < UNROLL; k++) old[k] = ok[k] ? c.dist[(int64_t)x[k] * W + t] : 0u;
+      // lane j < G updates the masks of edge g0 + j = (step j / ES, slot j % ES)
+      uint64_t nm = 0, fm = 0;
 #pragma unroll
       for (int k = 0; k < UNROLL; k++) {
         bool near = false, far = false;
-        if (ok[k]) {
-          if (nd[k] < atomicMin(&c.dist[(int64_t)x[k] * W + t], nd[k])) {
-            if (ms_bucket(nd[k], c.magic) <= b)
-              near = true;
-            else
-              far = true;
-            imp_near += near;
-            imp_far += far;
-          }
+        if (ok[k] && nd[k] < old[k]) {
+          atomicMin(&c.dist[(int64_t)x[k] * W + t], nd[k]);
+          near = ms_bucket(nd[k], c.magic) <= b;
+          far = !near;
+          imp_near += near;
+          imp_far += far;
         }
-        // one atomicOr per edge slot: the slot's first lane carries its mask
-        const uint64_t nb = __ballot(near), fb = __ballot(far);
-        const uint64_t nm = (nb >> (es * W)) & wmask, fm = (fb >> (es * W)) & wmask;
-        uint32_t pn = 0, pf = 0;
-        if (t == 0) {
-          if (nm && atomicOr(&mout[x[k]], (unsigned long long)nm) == 0ull)
-            pn = ms_nchunks<W>(c.rp[x[k] + 1] - c.rp[x[k]]);
-          if (fm && atomicOr(&c.far[x[k]], (unsigned long long)fm) == 0ull) pf = 1;
-        }
-        const uint32_t xv = (uint32_t)x[k];
-        nq.push(pn, [&](uint32_t q) { return ((uint64_t)q << 32) | xv; });
-        fq.push(pf, [&](uint32_t) { return (uint64_t)xv; });
+        const uint64_t nbk = __ballot(near), fbk = __ballot(far);
+        nm = kj == k ? nbk : nm;
+        fm = kj == k ? fbk : fm;
       }
+      nm = (nm >> (ej * W)) & wmask;
+      fm = (fm >> (ej * W)) & wmask;
+      const uint32_t xj = (uint32_t)__shfl(cur.x, (g0 + lane) & (GB_WAVE - 1), GB_WAVE);
+      unsigned long long on = 1ull, of = 1ull;
+      if (lane < G && nm) on = atomicOr(&mout[xj], (unsigned long long)nm);
+      if (lane < G && fm) of = atomicOr(&c.far[xj], (unsigned long long)fm);
+      int32_t r0 = 0, r1 = 0;
+      const bool newn = on == 0ull;
+      if (newn) {
+        r0 = c.rp[xj];
+        r1 = c.rp[xj + 1];
+      }
+      const uint32_t pn = newn ? ms_nchunks<W>(r1 - r0) : 0u;
+      nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(xj, r0, r1, q); });
+      fq.push(of == 0ull ? 1u : 0u, [&](uint32_t) { return (uint64_t)xj; });
     }
+    ecur = enxt;
+    cur = nxt;
+    enxt = enn;
+    i = i1;
+    i1 = i2;
   }
   nq.flush();
   fq.flush();
@@ -183,16 +248,16 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, uint64_
 // Far scan: far queue a_in -> bucket a_bucket's pairs to near queue a_near,
 // later buckets stay (far queue a_far), settled ones are dropped.
 template <int W>
-__device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, uint64_t *lds_near,
+__device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *lds_near,
                                        uint64_t *lds_far) {
   constexpr int ES = GB_WAVE / W;
   const int lane = gb_lane();
   const int es = lane / W, t = lane % W;
   const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
-  const uint64_t *in = c.q[a.a_in];
+  const uint64_t *in = c.qf[a.a_in - MS_FAR0];
   unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
-  gb_wave_queue<uint64_t, MS_NCAP> nq(lds_near, c.q[a.a_near], &c.cnt[a.a_near]);
-  gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.q[a.a_far], &c.cnt[a.a_far]);
+  gb_wave_queue<MsEnt, MS_NCAP> nq(lds_near, c.qn[a.a_near - MS_NEAR0], &c.cnt[a.a_near]);
+  gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.qf[a.a_far - MS_FAR0], &c.cnt[a.a_far]);
   const uint64_t n = c.cnt[a.a_in];
   const uint64_t bnext = a.a_bucket;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -218,13 +283,17 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, uint64_t 
     const uint64_t nb = __ballot(near), kb = __ballot(keep);
     const uint64_t nm = (nb >> (es * W)) & wmask, km = (kb >> (es * W)) & wmask;
     uint32_t pn = 0, pf = 0;
+    int32_t r0 = 0, r1 = 0;
     if (t == 0 && i < n) {
       c.far[x] = km;  // the entry is this vertex's only one in the scan
-      if (nm && atomicOr(&mout[x], (unsigned long long)nm) == 0ull)
-        pn = ms_nchunks<W>(c.rp[x + 1] - c.rp[x]);
+      if (nm && atomicOr(&mout[x], (unsigned long long)nm) == 0ull) {
+        r0 = c.rp[x];
+        r1 = c.rp[x + 1];
+        pn = ms_nchunks<W>(r1 - r0);
+      }
       if (km) pf = 1;
     }
-    nq.push(pn, [&](uint32_t q) { return ((uint64_t)q << 32) | x; });
+    nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(x, r0, r1, q); });
     fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
   }
   nq.flush();
@@ -243,9 +312,10 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
   const int act = ctl->action;
   if (act == ACT_NONE) return;
   const MsCtl a = *ctl;
-  __shared__ uint64_t s_near[MS_BLOCK / GB_WAVE][MS_NCAP];
+  __shared__ MsEnt s_near[MS_BLOCK / GB_WAVE][MS_NCAP];
   __shared__ uint64_t s_far[MS_BLOCK / GB_WAVE][MS_FCAP];
-  uint64_t *ln = s_near[threadIdx.x / GB_WAVE], *lf = s_far[threadIdx.x / GB_WAVE];
+  MsEnt *ln = s_near[threadIdx.x / GB_WAVE];
+  uint64_t *lf = s_far[threadIdx.x / GB_WAVE];
   if (act == ACT_LIGHT) {
     uint32_t relax = 0, imp_near = 0, imp_far = 0;
     bool ovf = false;
@@ -270,10 +340,10 @@ __global__ void ms_clear_kernel(MsCtx c, const MsCtl *__restrict__ ctl) {
   const int in = ctl->a_in;
   const uint64_t n = c.cnt[in];
   unsigned long long *m = c.mask[in - MS_NEAR0];
-  const uint64_t *q = c.q[in];
+  const MsEnt *q = c.qn[in - MS_NEAR0];
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
-    m[(uint32_t)q[i]] = 0;
+    m[q[i].v] = 0;
 }
 
 // group distances [V][W] -> caller rows out[t][V] for t < ntask
@@ -465,8 +535,8 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   GB_TRY(w.m0.ensure(sizeof(unsigned long long) * V));
   GB_TRY(w.m1.ensure(sizeof(unsigned long long) * V));
   GB_TRY(w.far.ensure(sizeof(unsigned long long) * V));
-  GB_TRY(w.q[0].ensure(sizeof(uint64_t) * qn));
-  GB_TRY(w.q[1].ensure(sizeof(uint64_t) * qn));
+  GB_TRY(w.q[0].ensure(sizeof(MsEnt) * qn));
+  GB_TRY(w.q[1].ensure(sizeof(MsEnt) * qn));
   GB_TRY(w.q[2].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.q[3].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
@@ -481,7 +551,10 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   c.mask[0] = w.m0.as<unsigned long long>();
   c.mask[1] = w.m1.as<unsigned long long>();
   c.far = w.far.as<unsigned long long>();
-  for (int k = 0; k < 4; k++) c.q[k] = w.q[k].as<uint64_t>();
+  c.qn[0] = w.q[0].as<MsEnt>();
+  c.qn[1] = w.q[1].as<MsEnt>();
+  c.qf[0] = w.q[2].as<uint64_t>();
+  c.qf[1] = w.q[3].as<uint64_t>();
   c.cnt = w.cnt.as<unsigned long long>();
   c.delta = delta;
   c.magic = delta > 1 ? (~0ull / delta) + 1 : 0;
