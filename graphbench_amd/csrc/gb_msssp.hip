@@ -192,48 +192,55 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
         const int r = g0 + k * ES + es;  // < CHUNK <= 64
         x[k] = __shfl(cur.x, r, GB_WAVE);
         const uint32_t wk = __shfl(cur.w, r, GB_WAVE);
-        ok[k] = act && r < cnt;
-        uint64_t s = (uint64_t)cur.du + wk;
-        if (ok[k] && s >= 0xFFFFFFFFull) {
-          ovf = true;
-          s = 0xFFFFFFFEull;
-        }
-        nd[k] = (uint32_t)s;
+        // branch-free, so that the reads below stay one batch
+        ok[k] = act & (r < cnt);
+        const uint64_t s = (uint64_t)cur.du + wk;
+        const bool big = s >= 0xFFFFFFFFull;
+        ovf |= ok[k] & big;
+        nd[k] = big ? 0xFFFFFFFEu : (uint32_t)s;
         relax += ok[k];
       }
+      // unconditional: x[k] is a vertex for every lane (0 past the chunk's
+      // end) and t < W, so the read is in bounds; a conditional read would be
+      // compiled to a branch per step that waits for its own load
 #pragma unroll
-      for (int k = 0; k < UNROLL; k++) old[k] = ok[k] ? c.dist[(int64_t)x[k] * W + t] : 0u;
+      for (int k = 0; k < UNROLL; k++) old[k] = c.dist[(int64_t)x[k] * W + t];
       // lane j < G updates the masks of edge g0 + j = (step j / ES, slot j % ES)
       uint64_t nm = 0, fm = 0;
 #pragma unroll
       for (int k = 0; k < UNROLL; k++) {
-        bool near = false, far = false;
-        if (ok[k] && nd[k] < old[k]) {
-          atomicMin(&c.dist[(int64_t)x[k] * W + t], nd[k]);
-          near = ms_bucket(nd[k], c.magic) <= b;
-          far = !near;
-          imp_near += near;
-          imp_far += far;
-        }
+        ok[k] = ok[k] & (nd[k] < old[k]);
+        const bool isnear = ms_bucket(nd[k], c.magic) <= b;
+        const bool near = ok[k] & isnear;
+        const bool far = ok[k] & !isnear;
+        imp_near += near;
+        imp_far += far;
         const uint64_t nbk = __ballot(near), fbk = __ballot(far);
         nm = kj == k ? nbk : nm;
         fm = kj == k ? fbk : fm;
       }
+      // the atomics go after every read of the group: a wait for a later read
+      // would otherwise also wait for the atomics issued before it
+#pragma unroll
+      for (int k = 0; k < UNROLL; k++)
+        if (ok[k]) atomicMin(&c.dist[(int64_t)x[k] * W + t], nd[k]);
       nm = (nm >> (ej * W)) & wmask;
       fm = (fm >> (ej * W)) & wmask;
       const uint32_t xj = (uint32_t)__shfl(cur.x, (g0 + lane) & (GB_WAVE - 1), GB_WAVE);
       unsigned long long on = 1ull, of = 1ull;
       if (lane < G && nm) on = atomicOr(&mout[xj], (unsigned long long)nm);
       if (lane < G && fm) of = atomicOr(&c.far[xj], (unsigned long long)fm);
+      // a target without out-edges is not queued: its mask bits stay set,
+      // which only suppresses later appends that would expand nothing
       int32_t r0 = 0, r1 = 0;
-      const bool newn = on == 0ull;
-      if (newn) {
+      const bool newn = on == 0ull, newf = of == 0ull;
+      if (newn || newf) {
         r0 = c.rp[xj];
         r1 = c.rp[xj + 1];
       }
-      const uint32_t pn = newn ? ms_nchunks<W>(r1 - r0) : 0u;
+      const uint32_t pn = newn && r1 > r0 ? ms_nchunks<W>(r1 - r0) : 0u;
       nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(xj, r0, r1, q); });
-      fq.push(of == 0ull ? 1u : 0u, [&](uint32_t) { return (uint64_t)xj; });
+      fq.push(newf && r1 > r0 ? 1u : 0u, [&](uint32_t) { return (uint64_t)xj; });
     }
     ecur = enxt;
     cur = nxt;

@@ -284,15 +284,20 @@ __device__ __forceinline__ void expand_body(
             // dedup stamps: a plain read first, the exchange only when unset
             if (Tr::bucket(nd, tk.delta, tk.magic) <= bucket) {
               imp_near++;
+              // a vertex without out-edges is stamped but not queued; one
+              // with only heavy edges gets one (empty) chunk, which records
+              // it in S
               uint32_t *ns = &c.nst[tk.s(v)];
               if (*ns != gen_next && atomicExch(ns, gen_next) != gen_next) {
-                ncnt = nchunks(tk.G.le[v] - tk.G.rp[v]);
-                if (ncnt == 0) ncnt = 1;
+                const int32_t rs = tk.G.rp[v];
+                ncnt = nchunks(tk.G.le[v] - rs);
+                if (ncnt == 0 && tk.G.rp[v + 1] > rs) ncnt = 1;
               }
             } else {
               imp_far++;
               uint32_t *fs = &c.fst[tk.s(v)];
-              if (*fs != fepoch && atomicExch(fs, fepoch) != fepoch) far = true;
+              if (*fs != fepoch && atomicExch(fs, fepoch) != fepoch)
+                far = tk.G.rp[v + 1] > tk.G.rp[v];
             }
           }
         }
