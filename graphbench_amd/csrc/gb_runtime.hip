@@ -2,7 +2,9 @@
 // graph-structure kernels (merge-path split, edge-balanced parts, light/heavy
 // separation, max weight).
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_scan.hpp>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -423,73 +425,73 @@ gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st) {
 
 // ---------------------------------------------------------------------------
 // light/heavy separation (separate_csr, graph/csr.c:75-130) as a stable
-// in-row partition with ballot ranks that keep the original order.  A wave
-// takes 4 consecutive rows: when none has more than 64 edges, each 16-lane
-// quarter of the wave partitions one row; otherwise the whole wave takes the
-// 4 rows one after the other.  (Most rows of sparse graphs are short, and a
-// wave per row would leave most lanes idle.)
+// in-row partition: light edges first, each part in row order.  It is
+// edge-parallel, so a hub row is spread over the whole grid:
+//   * lrank[i + 1] = light(w[i]), and an inclusive scan makes lrank[i] the
+//     number of light edges before edge i;
+//   * edge i of row v goes to rp[v] + (lrank[i] - lrank[rp[v]]) when light,
+//     and after the row's lrank[rp[v+1]] - lrank[rp[v]] light edges, in order,
+//     when heavy;
+//   * le[v] = rp[v] + the row's light count.
 
-// Partition row [s, e) with the lanes of `grp` (bit mask of the participating
-// lanes, glane = rank of this lane in it, gl = its width): light first.
 template <class W, class Pred>
-__device__ __forceinline__ void separate_row(int32_t s, int32_t e, uint64_t grp, int gshift,
-                                             int glane, int gw, const int32_t *__restrict__ col,
-                                             const W *__restrict__ w, Pred light,
-                                             int32_t *__restrict__ col_out,
-                                             W *__restrict__ w_out, int32_t &nl_out) {
-  const uint64_t below = glane == 0 ? 0ull : ((1ull << glane) - 1);
-  int32_t nl = 0;
-  for (int32_t b = s; b < e; b += gw) {
-    const int32_t i = b + glane;
-    const bool f = i < e && light(w[i]);
-    nl += __popcll((__ballot(f) & grp) >> gshift);
-  }
-  int32_t lpos = s, hpos = s + nl;
-  for (int32_t b = s; b < e; b += gw) {
-    const int32_t i = b + glane;
-    const bool in = i < e;
-    const bool f = in && light(w[i]);
-    const uint64_t bl = (__ballot(f) & grp) >> gshift;
-    const uint64_t bh = (__ballot(in && !f) & grp) >> gshift;
-    if (in) {
-      const int32_t dst = f ? lpos + __popcll(bl & below) : hpos + __popcll(bh & below);
-      col_out[dst] = col[i];
-      w_out[dst] = w[i];
-    }
-    lpos += __popcll(bl);
-    hpos += __popcll(bh);
-  }
-  nl_out = nl;
+__global__ void light_flags_kernel(int64_t E, const W *__restrict__ w, Pred light,
+                                   int32_t *__restrict__ f) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (int64_t)gridDim.x * blockDim.x)
+    f[i] = light(w[i]) ? 1 : 0;
 }
 
+__global__ void light_end_kernel(int32_t V, const int32_t *__restrict__ rp,
+                                 const int32_t *__restrict__ lrank, int32_t *__restrict__ le) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x)
+    le[v] = rp[v] + (lrank[rp[v + 1]] - lrank[rp[v]]);
+}
+
+// largest row r in [lo, hi] with rp[r] <= i (the row holding edge i when
+// rp[lo] <= i < rp[hi + 1])
+__device__ __forceinline__ int32_t row_of_edge(const int32_t *__restrict__ rp, int32_t lo,
+                                               int32_t hi, int64_t i) {
+  while (lo < hi) {
+    const int32_t mid = (int32_t)(((int64_t)lo + hi + 1) >> 1);
+    if (rp[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+constexpr int SPLIT_IPT = 8;  // edges per lane and wave tile
+
+// A wave takes tiles of 64 * SPLIT_IPT consecutive edges (lane-consecutive
+// edges, so loads and most stores coalesce); the rows spanned by the tile are
+// found once per tile, each edge's row by a short search among them.
 template <class W, class Pred>
-__global__ void separate_rows_kernel(int32_t V, const int32_t *__restrict__ rp,
+__global__ void split_scatter_kernel(int32_t V, int64_t E, const int32_t *__restrict__ rp,
                                      const int32_t *__restrict__ col, const W *__restrict__ w,
-                                     Pred light, int32_t *__restrict__ col_out,
-                                     W *__restrict__ w_out, int32_t *__restrict__ le) {
+                                     Pred light, const int32_t *__restrict__ lrank,
+                                     int32_t *__restrict__ col_out, W *__restrict__ w_out) {
+  constexpr int64_t TILE = (int64_t)GB_WAVE * SPLIT_IPT;
   const int lane = gb_lane();
-  const int q = lane >> 4, ql = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t v0 = wave * 4; v0 < V; v0 += nwaves * 4) {
-    const int64_t vq = v0 + q;
-    const int32_t dq = vq < V ? rp[vq + 1] - rp[vq] : 0;
-    int32_t mx = dq;
+  for (int64_t t0 = wave * TILE; t0 < E; t0 += nwaves * TILE) {
+    const int64_t t1 = t0 + TILE < E ? t0 + TILE : E;
+    const int32_t r_lo = row_of_edge(rp, 0, V - 1, t0);
+    const int32_t r_hi = row_of_edge(rp, r_lo, V - 1, t1 - 1);
 #pragma unroll
-    for (int d = 16; d <= 32; d <<= 1) mx = max(mx, __shfl_xor(mx, d, 64));
-    if (mx <= 64) {  // a quarter wave per row
-      if (vq < V) {
-        int32_t nl;
-        separate_row(rp[vq], rp[vq + 1], 0xFFFFull << (16 * q), 16 * q, ql, 16, col, w, light,
-                     col_out, w_out, nl);
-        if (ql == 0) le[vq] = rp[vq] + nl;
-      }
-    } else {  // the whole wave per row
-      for (int k = 0; k < 4 && v0 + k < V; k++) {
-        const int64_t v = v0 + k;
-        int32_t nl;
-        separate_row(rp[v], rp[v + 1], ~0ull, 0, lane, 64, col, w, light, col_out, w_out, nl);
-        if (lane == 0) le[v] = rp[v] + nl;
+    for (int k = 0; k < SPLIT_IPT; k++) {
+      const int64_t i = t0 + (int64_t)k * GB_WAVE + lane;
+      if (i < t1) {
+        const int32_t v = row_of_edge(rp, r_lo, r_hi, i);
+        const int32_t rs = rp[v];
+        const int32_t lrs = lrank[rs], nl = lrank[rp[v + 1]] - lrs, lr = lrank[i] - lrs;
+        const W wi = w[i];
+        const int64_t dst = light(wi) ? rs + lr : rs + nl + (i - rs) - lr;
+        col_out[dst] = col[i];
+        w_out[dst] = wi;
       }
     }
   }
@@ -504,12 +506,43 @@ struct LightF64 {
   __device__ bool operator()(double x) const { return x <= t; }
 };
 
-static int rows_grid(int64_t V) {
-  int64_t waves = V;
-  int64_t blocks = (waves + 3) / 4;
-  int64_t cap = (int64_t)gb_num_cus() * 32;
-  if (blocks > cap) blocks = cap;
-  return blocks < 1 ? 1 : (int)blocks;
+static int split_grid(int64_t n, int64_t per_thread) {
+  const int64_t blocks = (n + 256 * per_thread - 1) / (256 * per_thread);
+  const int64_t cap = (int64_t)gb_num_cus() * 16;
+  return (int)std::max<int64_t>(1, std::min(blocks, cap));
+}
+
+// light/heavy split of a CSR into col_out / w_out / le (see above); returns
+// after the work is enqueued and its scratch released (stream-ordered)
+template <class W, class Pred>
+static gb_status split_rows(int32_t V, int64_t E, const int32_t *rp, const int32_t *col,
+                            const W *w, Pred light, int32_t *col_out, W *w_out, int32_t *le,
+                            hipStream_t st) {
+  if (V <= 0) return GB_OK;
+  DevBuf flags, lrank, tmp;
+  GB_TRY(lrank.ensure(sizeof(int32_t) * ((size_t)E + 1)));
+  GB_HIP(hipMemsetAsync(lrank.p, 0, sizeof(int32_t), st));
+  if (E > 0) {
+    GB_TRY(flags.ensure(sizeof(int32_t) * (size_t)E));
+    hipLaunchKernelGGL((light_flags_kernel<W, Pred>), dim3(split_grid(E, 4)), dim3(256), 0, st, E,
+                       w, light, flags.as<int32_t>());
+    GB_HIP(hipGetLastError());
+    size_t tb = 0;
+    GB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, flags.as<int32_t>(),
+                                            lrank.as<int32_t>() + 1, (int)E, st));
+    GB_TRY(tmp.ensure(tb + 16));
+    GB_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, flags.as<int32_t>(),
+                                            lrank.as<int32_t>() + 1, (int)E, st));
+    hipLaunchKernelGGL((split_scatter_kernel<W, Pred>), dim3(split_grid(E, 4 * SPLIT_IPT)),
+                       dim3(256), 0, st, V, E, rp, col, w, light, lrank.as<int32_t>(), col_out,
+                       w_out);
+    GB_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(light_end_kernel, dim3(split_grid(V, 4)), dim3(256), 0, st, V, rp,
+                     lrank.as<int32_t>(), le);
+  GB_HIP(hipGetLastError());
+  GB_HIP(hipStreamSynchronize(st));  // the scratch buffers are freed on return
+  return GB_OK;
 }
 
 gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st) {
@@ -518,13 +551,9 @@ gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st) {
   GB_TRY(g->split_col.ensure(sizeof(int32_t) * (g->E + 1)));
   GB_TRY(g->split_w.ensure(sizeof(uint32_t) * (g->E + 1)));
   GB_TRY(g->split_le.ensure(sizeof(int32_t) * (g->V + 1)));
-  if (g->V > 0) {
-    hipLaunchKernelGGL((separate_rows_kernel<uint32_t, LightU32>), dim3(rows_grid(g->V)),
-                       dim3(256), 0, st, g->V, g->rp, g->col, g->w, LightU32{light_max},
-                       g->split_col.as<int32_t>(), g->split_w.as<uint32_t>(),
-                       g->split_le.as<int32_t>());
-    GB_HIP(hipGetLastError());
-  }
+  GB_TRY(split_rows<uint32_t>(g->V, g->E, g->rp, g->col, g->w, LightU32{light_max},
+                              g->split_col.as<int32_t>(), g->split_w.as<uint32_t>(),
+                              g->split_le.as<int32_t>(), st));
   g->split_valid = true;
   g->split_light_max = light_max;
   return GB_OK;
@@ -536,13 +565,9 @@ gb_status gb_graph_split64(gb_graph g, double light_max, hipStream_t st) {
   GB_TRY(g->split64_col.ensure(sizeof(int32_t) * (g->E + 1)));
   GB_TRY(g->split64_w.ensure(sizeof(double) * (g->E + 1)));
   GB_TRY(g->split64_le.ensure(sizeof(int32_t) * (g->V + 1)));
-  if (g->V > 0) {
-    hipLaunchKernelGGL((separate_rows_kernel<double, LightF64>), dim3(rows_grid(g->V)), dim3(256),
-                       0, st, g->V, g->rp, g->col, g->w64, LightF64{light_max},
-                       g->split64_col.as<int32_t>(), g->split64_w.as<double>(),
-                       g->split64_le.as<int32_t>());
-    GB_HIP(hipGetLastError());
-  }
+  GB_TRY(split_rows<double>(g->V, g->E, g->rp, g->col, g->w64, LightF64{light_max},
+                            g->split64_col.as<int32_t>(), g->split64_w.as<double>(),
+                            g->split64_le.as<int32_t>(), st));
   g->split64_valid = true;
   g->split64_delta = light_max;
   return GB_OK;
@@ -553,11 +578,6 @@ extern "C" gb_status gb_separate(gb_graph g, uint32_t light_max, int32_t *d_col_
   if (!g || !g->w || !d_col_out || !d_w_out || !d_light_end)
     return gb_fail(GB_ERR_INVALID, "gb_separate: bad arguments");
   hipStream_t st = gb_stream(stream);
-  if (g->V > 0) {
-    hipLaunchKernelGGL((separate_rows_kernel<uint32_t, LightU32>), dim3(rows_grid(g->V)),
-                       dim3(256), 0, st, g->V, g->rp, g->col, g->w, LightU32{light_max}, d_col_out,
-                       d_w_out, d_light_end);
-    GB_HIP(hipGetLastError());
-  }
-  return GB_OK;
+  return split_rows<uint32_t>(g->V, g->E, g->rp, g->col, g->w, LightU32{light_max}, d_col_out,
+                              d_w_out, d_light_end, st);
 }

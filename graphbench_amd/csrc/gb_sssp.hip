@@ -680,6 +680,22 @@ uint32_t plain_delta(uint32_t max_w, int32_t V, int64_t E) {
   return r < 1 ? 1 : r;
 }
 
+// The graph's rows as light and heavy parts for threshold lm.  When no weight
+// exceeds lm every edge is light: the graph's own arrays serve, with the row
+// end as the light end, and nothing is copied.
+gb_status light_rows(gb_graph g, uint32_t lm, hipStream_t st, SGraph<uint32_t> &out) {
+  uint32_t mw = 0;
+  GB_TRY(gb_graph_max_w(g, &mw, st));
+  if (lm >= mw) {
+    out = SGraph<uint32_t>{g->rp, g->rp + 1, g->col, g->w, g->V, 0, g->E};
+    return GB_OK;
+  }
+  GB_TRY(gb_graph_split(g, lm, st));
+  out = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
+                         g->split_w.as<uint32_t>(), g->V, 0, g->E};
+  return GB_OK;
+}
+
 }  // namespace
 
 void gb_sssp_release() { g_ssw.release(); }
@@ -729,10 +745,8 @@ extern "C" gb_status gb_sssp(gb_graph g, const gb_sssp_params *p, const int32_t 
     return GB_OK;
   }
   GB_TRY(check_degree(g, st));
-  GB_TRY(gb_graph_split(g, lmax, st));
   std::vector<SGraph<uint32_t>> hg(1);
-  hg[0] = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
-                           g->split_w.as<uint32_t>(), g->V, 0, g->E};
+  GB_TRY(light_rows(g, lmax, st, hg[0]));
   std::vector<STask<TU32>> ht(nsrc);
   for (int32_t i = 0; i < nsrc; i++)
     ht[i] = STask<TU32>{d_dist + (int64_t)i * g->V, 0, (uint64_t)delta, 0, 0, h_sources[i]};
@@ -781,9 +795,7 @@ extern "C" gb_status gb_sssp_multi(int32_t ngraphs, const gb_graph *gs, const in
     }
     uint32_t lm = light_max && light_max[i] ? light_max[i] : delta;
     GB_TRY(check_degree(g, st));
-    GB_TRY(gb_graph_split(g, lm, st));
-    hg[i] = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
-                             g->split_w.as<uint32_t>(), g->V, 0, g->E};
+    GB_TRY(light_rows(g, lm, st, hg[i]));
     ht[i] = STask<TU32>{d_dist[i], 0, (uint64_t)delta, 0, i, sources[i]};
   }
   return sssp_engine<TU32>(hg, ht, 0, stats, st);

@@ -492,16 +492,34 @@ def test_csr_from_edges_and_transpose(gpu):
         gb.csr_from_edges(V, d(bad), d(dst))
 
 
-def test_separate(gpu):
+def _hub_graph(gb):
+    """Empty rows, a 5000-edge hub spanning many edge tiles, short rows."""
+    rng = np.random.default_rng(5)
+    V = 3000
+    src = np.concatenate([np.full(5000, 7), rng.integers(100, V, 4000)])
+    dst = rng.integers(0, V, src.size)
+    w = rng.integers(1, 300, src.size)
+    return gb.HostCSR.from_edges(V, src, dst, w)
+
+
+@pytest.mark.parametrize("case,light", [("small", 100), ("rmat", 0), ("rmat", 60),
+                                        ("rmat", 255), ("hub", 150)])
+def test_separate(gpu, case, light):
     gb = gpu
-    h = small_graph(41)
+    if case == "small":
+        h = small_graph(41)
+    elif case == "rmat":
+        h = gb.gen_rmat(14, 16, seed=3, weights="uniform", wlo=1, whi=255)
+    else:
+        h = _hub_graph(gb)
     g = gb.Graph.from_host(h, need_reverse=False)
-    _say("separate")
-    col, w, le = gb.separate(g, 100)
-    (lrp, lcol, lw), (hrp, hcol, hw) = oracle.separate_csr(h.row_ptr, h.col, h.w, 100)
+    _say(f"separate {case} light<={light}")
+    col, w, le = gb.separate(g, light)
+    (lrp, lcol, lw), (hrp, hcol, hw) = oracle.separate_csr(h.row_ptr, h.col, h.w, light)
     col, w, le = col.cpu().numpy(), w.cpu().numpy().view(np.uint32), le.cpu().numpy()
     for v in range(h.V):
         a, b, c = h.row_ptr[v], le[v], h.row_ptr[v + 1]
         assert np.array_equal(col[a:b], lcol[lrp[v]:lrp[v + 1]])
         assert np.array_equal(w[a:b], lw[lrp[v]:lrp[v + 1]])
         assert np.array_equal(col[b:c], hcol[hrp[v]:hrp[v + 1]])
+        assert np.array_equal(w[b:c], hw[hrp[v]:hrp[v + 1]])
