@@ -59,6 +59,7 @@ struct TU32 {
     return (D)nd;
   }
   static __device__ __forceinline__ D amin(D *p, D v) { return atomicMin(p, v); }
+  static __device__ __forceinline__ void amin_noret(D *p, D v) { (void)atomicMin(p, v); }
   // floor(d / delta) as a multiply-high by magic = floor((2^64 - 1) / delta) + 1,
   // exact for d, delta < 2^32 (delta = 1: magic = 0, the quotient is d)
   static __device__ __forceinline__ uint64_t bucket(D d, DT delta, uint64_t magic) {
@@ -77,6 +78,9 @@ struct TF64 {
     unsigned long long old =
         atomicMin((unsigned long long *)p, (unsigned long long)__double_as_longlong(v));
     return __longlong_as_double((long long)old);
+  }
+  static __device__ __forceinline__ void amin_noret(D *p, D v) {
+    (void)atomicMin((unsigned long long *)p, (unsigned long long)__double_as_longlong(v));
   }
   static __device__ __forceinline__ uint64_t bucket(D d, DT delta, uint64_t) {
     return (uint64_t)(long long)(d / delta);
@@ -190,13 +194,21 @@ __global__ void sssp_seed_kernel(Ctx<Tr> c, int32_t ntasks, uint64_t *near_out,
 
 // queue appends are staged per wave in LDS (gb_wave_queue, gb_device.hpp)
 constexpr int NEAR_CAP = 384, FAR_CAP = 128, S_CAP = 128;  // 5 KiB of LDS per wave
+// edges relaxed per wave and round: EXP_U slots of 64; their improved targets
+// are listed in LDS (IMP_CAP per wave) before the stamps are taken
+constexpr int EXP_U = 4, IMP_CAP = GB_WAVE * EXP_U;
 
 // Light (HEAVY=false) or heavy (HEAVY=true) expansion of one queue.
 // Each wave takes 64 queue entries at a time (one per lane) and spreads their
 // edges over its lanes: a wave prefix sum of the entries' edge counts gives
 // every edge slot k an owner lane, found by a binary search over the lanes'
 // offsets with cross-lane reads.  Lanes therefore stay busy whether entries
-// hold 3 edges or 256.
+// hold 3 edges or 256.  A round covers EXP_U * 64 edge slots in phases, so
+// that each phase's memory accesses are in flight together: the slots' col/w
+// reads, then their target distance reads, then the atomicMin of every
+// improving slot (no return value: improvement is judged on the value read);
+// the improved targets are listed in LDS and then stamped and queued 64 at a
+// time.
 template <class T>
 __device__ __forceinline__ T gb_shfl_any(T v, int src);
 template <>
@@ -211,10 +223,11 @@ __device__ __forceinline__ double gb_shfl_any<double>(double v, int src) {
 template <class Tr, bool HEAVY, bool SINGLE>
 __device__ __forceinline__ void expand_body(
     const Ctx<Tr> &c, const SGraph<typename Tr::W> &G0, uint64_t *lds_near, uint64_t *lds_far,
-    uint64_t *lds_s, const uint64_t *__restrict__ in, const unsigned long long *__restrict__ n_in,
-    uint64_t *__restrict__ near_out, unsigned long long *n_near, uint64_t *__restrict__ s_out,
-    unsigned long long *n_s, uint64_t *__restrict__ far_out, unsigned long long *n_far,
-    uint64_t bucket, uint32_t gen_next, uint32_t sround, uint32_t fepoch, uint32_t epw) {
+    uint64_t *lds_s, uint64_t *lds_imp, const uint64_t *__restrict__ in,
+    const unsigned long long *__restrict__ n_in, uint64_t *__restrict__ near_out,
+    unsigned long long *n_near, uint64_t *__restrict__ s_out, unsigned long long *n_s,
+    uint64_t *__restrict__ far_out, unsigned long long *n_far, uint64_t bucket,
+    uint32_t gen_next, uint32_t sround, uint32_t fepoch, uint32_t epw) {
   using D = typename Tr::D;
   const int lane = gb_lane();
   gb_wave_queue<uint64_t, NEAR_CAP> nearq(lds_near, near_out, n_near);
@@ -256,54 +269,104 @@ __device__ __forceinline__ void expand_body(
     // ---- spread the 64 entries' edges over the lanes
     uint32_t total;
     const uint32_t off = gb_wave_excl_sum((uint32_t)max(end - beg, 0), &total);
-    for (uint32_t k0 = 0; k0 < total; k0 += GB_WAVE) {
-      const uint32_t k = k0 + lane;
-      // owner: the last lane j with off_j <= k (offsets are non-decreasing)
-      int j = 0;
+    for (uint32_t k0 = 0; k0 < total; k0 += GB_WAVE * EXP_U) {
+      uint32_t tj[EXP_U], v[EXP_U];
+      int32_t e[EXP_U];
+      D duj[EXP_U], nd[EXP_U], old[EXP_U];
+      bool ok[EXP_U];
+      // slots past the end take slot k0's edge (k0 < total), so that every
+      // read below is in bounds and none needs a branch
 #pragma unroll
-      for (int step = 32; step >= 1; step >>= 1) {
-        const int cand = j + step;
-        const uint32_t oc = __shfl(off, cand < GB_WAVE ? cand : GB_WAVE - 1, GB_WAVE);
-        if (cand < GB_WAVE && oc <= k) j = cand;
+      for (int s = 0; s < EXP_U; s++) {
+        const uint32_t k = k0 + (uint32_t)(s * GB_WAVE + lane);
+        // owner: the last lane j with off_j <= k (offsets are non-decreasing)
+        int j = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+          const int cand = j + step;
+          const uint32_t oc = __shfl(off, cand < GB_WAVE ? cand : GB_WAVE - 1, GB_WAVE);
+          if (cand < GB_WAVE && oc <= k) j = cand;
+        }
+        tj[s] = __shfl(t, j, GB_WAVE);
+        e[s] = __shfl(beg, j, GB_WAVE) + (int32_t)(k - __shfl(off, j, GB_WAVE));
+        duj[s] = gb_shfl_any<D>(du, j);
+        ok[s] = k < total;
       }
-      const uint32_t tj = __shfl(t, j, GB_WAVE);
-      const int32_t e = __shfl(beg, j, GB_WAVE) + (int32_t)(k - __shfl(off, j, GB_WAVE));
-      const D duj = gb_shfl_any<D>(du, j);
-      uint32_t ncnt = 0;
-      bool far = false;
-      uint32_t v = 0;
-      if (k < total) {
-        const TaskView<Tr, SINGLE> tk(c, G0, tj);
-        v = (uint32_t)tk.G.col[e];
-        const D nd = Tr::add(duj, tk.G.w[e], ovf);
-        relax++;
-        D *dv = tk.d(v);
-        if (nd < *dv) {
-          const D old = Tr::amin(dv, nd);
-          if (nd < old) {
-            // dedup stamps: a plain read first, the exchange only when unset
-            if (Tr::bucket(nd, tk.delta, tk.magic) <= bucket) {
-              imp_near++;
-              // a vertex without out-edges is stamped but not queued; one
-              // with only heavy edges gets one (empty) chunk, which records
-              // it in S
-              uint32_t *ns = &c.nst[tk.s(v)];
-              if (*ns != gen_next && atomicExch(ns, gen_next) != gen_next) {
-                const int32_t rs = tk.G.rp[v];
-                ncnt = nchunks(tk.G.le[v] - rs);
-                if (ncnt == 0 && tk.G.rp[v + 1] > rs) ncnt = 1;
-              }
-            } else {
-              imp_far++;
-              uint32_t *fs = &c.fst[tk.s(v)];
-              if (*fs != fepoch && atomicExch(fs, fepoch) != fepoch)
-                far = tk.G.rp[v + 1] > tk.G.rp[v];
-            }
+#pragma unroll
+      for (int s = 0; s < EXP_U; s++) {
+        const uint32_t t0 = __shfl(tj[0], 0, GB_WAVE);
+        const int32_t e0 = __shfl(e[0], 0, GB_WAVE);
+        tj[s] = ok[s] ? tj[s] : t0;
+        e[s] = ok[s] ? e[s] : e0;
+      }
+      typename Tr::W wv[EXP_U];
+#pragma unroll
+      for (int s = 0; s < EXP_U; s++) {
+        const TaskView<Tr, SINGLE> tk(c, G0, tj[s]);
+        v[s] = (uint32_t)tk.G.col[e[s]];
+        wv[s] = tk.G.w[e[s]];
+      }
+      D *dp[EXP_U];
+      bool isfar[EXP_U];
+#pragma unroll
+      for (int s = 0; s < EXP_U; s++) {
+        const TaskView<Tr, SINGLE> tk(c, G0, tj[s]);
+        bool o = false;
+        nd[s] = Tr::add(duj[s], wv[s], o);
+        ovf |= o & ok[s];
+        relax += ok[s];
+        isfar[s] = Tr::bucket(nd[s], tk.delta, tk.magic) > bucket;
+        dp[s] = tk.d(v[s]);
+        old[s] = *dp[s];
+      }
+#pragma unroll
+      for (int s = 0; s < EXP_U; s++) ok[s] = ok[s] & (nd[s] < old[s]);
+#pragma unroll
+      for (int s = 0; s < EXP_U; s++)
+        if (ok[s]) Tr::amin_noret(dp[s], nd[s]);
+      // list the improved targets: bit 63 = later bucket, task, vertex
+      uint32_t nimp = 0;
+#pragma unroll
+      for (int s = 0; s < EXP_U; s++) {
+        imp_near += ok[s] & !isfar[s];
+        imp_far += ok[s] & isfar[s];
+        uint32_t cnt_s;
+        const uint32_t pos = gb_wave_excl_sum(ok[s] ? 1u : 0u, &cnt_s);
+        if (ok[s])
+          lds_imp[nimp + pos] =
+              ((uint64_t)isfar[s] << 63) | ((uint64_t)tj[s] << 32) | (uint64_t)v[s];
+        nimp += cnt_s;
+      }
+      __builtin_amdgcn_wave_barrier();
+      // stamp and queue them: a plain read of the stamp first, the exchange
+      // only when unset.  A vertex without out-edges is stamped but not
+      // queued; one with only heavy edges gets one (empty) chunk, which
+      // records it in S.
+      for (uint32_t q0 = 0; q0 < nimp; q0 += GB_WAVE) {
+        const bool has = q0 + lane < nimp;
+        const uint64_t item = has ? lds_imp[q0 + lane] : 0ull;
+        const uint32_t ti = (uint32_t)(item >> 32) & 0xFFFFu, vi = (uint32_t)item;
+        const bool isfar = (item >> 63) != 0;
+        const TaskView<Tr, SINGLE> tk(c, G0, ti);
+        uint32_t *sp = isfar ? &c.fst[tk.s(vi)] : &c.nst[tk.s(vi)];
+        const uint32_t g = isfar ? fepoch : gen_next;
+        bool fresh = false;
+        if (has && *sp != g) fresh = atomicExch(sp, g) != g;
+        uint32_t ncnt = 0;
+        bool far = false;
+        if (fresh) {
+          const int32_t rs = tk.G.rp[vi], re = tk.G.rp[vi + 1];
+          if (isfar) {
+            far = re > rs;
+          } else {
+            ncnt = nchunks(tk.G.le[vi] - rs);
+            if (ncnt == 0 && re > rs) ncnt = 1;
           }
         }
+        nearq.push(ncnt, [&](uint32_t q) { return mk_entry(ti, q, vi); });
+        farq.push(far ? 1u : 0u, [&](uint32_t) { return mk_entry(ti, 0, vi); });
       }
-      nearq.push(ncnt, [&](uint32_t q) { return mk_entry(tj, q, v); });
-      farq.push(far ? 1u : 0u, [&](uint32_t) { return mk_entry(tj, 0, v); });
+      __builtin_amdgcn_wave_barrier();  // the list is rewritten next round
     }
   }
   nearq.flush();
@@ -401,17 +464,18 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_step_kernel(Ctx<Tr> c, Queues q
   __shared__ uint64_t s_near[SS_BLOCK / GB_WAVE][NEAR_CAP];
   __shared__ uint64_t s_far[SS_BLOCK / GB_WAVE][FAR_CAP];
   __shared__ uint64_t s_set[SS_BLOCK / GB_WAVE][S_CAP];
+  __shared__ uint64_t s_imp[SS_BLOCK / GB_WAVE][IMP_CAP];
   uint64_t *ln = s_near[threadIdx.x / GB_WAVE], *lf = s_far[threadIdx.x / GB_WAVE];
-  uint64_t *ls = s_set[threadIdx.x / GB_WAVE];
+  uint64_t *ls = s_set[threadIdx.x / GB_WAVE], *li = s_imp[threadIdx.x / GB_WAVE];
   if (act == ACT_FAR) {
     far_scan_body<Tr, SINGLE>(c, G0, ln, lf, q.q[in], cnt + in, q.q[nr], cnt + nr, q.q[fr],
                               cnt + fr, ctl->a_bucket, ctl->a_gen, ctl->a_fepoch);
   } else if (act == ACT_LIGHT) {
-    expand_body<Tr, false, SINGLE>(c, G0, ln, lf, ls, q.q[in], cnt + in, q.q[nr], cnt + nr,
+    expand_body<Tr, false, SINGLE>(c, G0, ln, lf, ls, li, q.q[in], cnt + in, q.q[nr], cnt + nr,
                                    q.q[C_S], cnt + C_S, q.q[fr], cnt + fr, ctl->a_bucket,
                                    ctl->a_gen, ctl->a_sround, ctl->a_fepoch, ctl->a_epw);
   } else {
-    expand_body<Tr, true, SINGLE>(c, G0, ln, lf, ls, q.q[in], cnt + in, q.q[nr], cnt + nr,
+    expand_body<Tr, true, SINGLE>(c, G0, ln, lf, ls, li, q.q[in], cnt + in, q.q[nr], cnt + nr,
                                   q.q[C_S], cnt + C_S, q.q[fr], cnt + fr, ctl->a_bucket,
                                   ctl->a_gen, ctl->a_sround, ctl->a_fepoch, ctl->a_epw);
   }
