@@ -24,7 +24,6 @@ __device__ __forceinline__ uint64_t gb_shfl_u64(uint64_t v, int src) {
   uint32_t hi = __shfl((uint32_t)(v >> 32), src, GB_WAVE);
   return ((uint64_t)hi << 32) | lo;
 }
-
 // Reserve `cnt` consecutive slots of a device queue for this lane.  One
 // atomicAdd per wave.  Must be reached by every active lane of the wave.
 __device__ __forceinline__ uint64_t gb_wave_reserve(unsigned long long *counter, uint32_t cnt) {

@@ -126,15 +126,17 @@ struct RowOut {
   const float *__restrict__ inv_od;
   float *__restrict__ x_new;
   float *__restrict__ contrib_new;
-  __device__ __forceinline__ void operator()(int32_t v, F2 tot, double &res, double &dang) const {
-    const float s = f2_value(tot);
+  // row v's new rank from its gathered sum s, given x_old[v] and inv_od[v]
+  __device__ __forceinline__ void put(int32_t v, float s, float xo, float io, double &res,
+                                      double &dang) const {
     const float xn = base + damping * s;
-    const float xo = x_old[v];
     res += fabs((double)xn - (double)xo);
-    const float io = inv_od[v];
     x_new[v] = xn;
     contrib_new[v] = xn * io;
     if (io == 0.0f) dang += (double)xn;
+  }
+  __device__ __forceinline__ void operator()(int32_t v, F2 tot, double &res, double &dang) const {
+    put(v, f2_value(tot), x_old[v], inv_od[v], res, dang);
   }
 };
 
@@ -169,26 +171,51 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
   const int32_t r0 = tile_row[t], e0 = tile_edge[t];
   const int32_t r1 = tile_row[t + 1], e1 = tile_edge[t + 1];
   const int nrows = r1 - r0, nedges = e1 - e0;
+  const bool cont = irp[r0] < e0;  // row r0 began in an earlier tile
   // Stage the tile: IPT independent column loads per thread (streamed past the
   // caches), then IPT independent gathers, so many requests are in flight.
-  int32_t cidx[IPT];
+  // The loads are unconditional, on indices clamped into the tile (items past
+  // its end re-read its last edge or row): a load under a per-item condition
+  // is compiled to a branch that waits for it, one round trip per item.
+  // nedges and nrows are uniform over the block.
+  if (nedges > 0) {
+    const int32_t *icol_t = icol + e0;
+    int32_t cidx[IPT];
+    float val[IPT];
 #pragma unroll
-  for (int m = 0; m < IPT; ++m) {
-    const int k = tid + m * PR_BLOCK;
-    cidx[m] = k < nedges ? __builtin_nontemporal_load(icol + e0 + k) : 0;
-  }
+    for (int m = 0; m < IPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      cidx[m] = __builtin_nontemporal_load(icol_t + (k < nedges ? k : nedges - 1));
+    }
 #pragma unroll
-  for (int m = 0; m < IPT; ++m) {
-    const int k = tid + m * PR_BLOCK;
-    if (k < nedges) {
-      const int32_t c = cidx[m];
-      s_val[k] = (HOT > 0 && c < HOT) ? s_hot[c] : contrib[c];
+    for (int m = 0; m < IPT; ++m) {
+      const uint32_t c = (uint32_t)cidx[m];
+      if (HOT > 0) {
+        const float g = contrib[c < (uint32_t)HOT ? 0u : c];  // hot lanes: a dummy read
+        val[m] = c < (uint32_t)HOT ? s_hot[c] : g;
+      } else {
+        val[m] = contrib[c];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < IPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      if (k < nedges) s_val[k] = val[m];
     }
   }
+  if (nrows > 0) {
+    const int32_t *irp_t = irp + r0 + 1;
+    int32_t ends[IPT];
 #pragma unroll
-  for (int m = 0; m < IPT; ++m) {
-    const int k = tid + m * PR_BLOCK;
-    if (k < nrows) s_end[k] = irp[r0 + k + 1] - e0;
+    for (int m = 0; m < IPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      ends[m] = irp_t[k < nrows ? k : nrows - 1];
+    }
+#pragma unroll
+    for (int m = 0; m < IPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      if (k < nrows) s_end[k] = ends[m] - e0;
+    }
   }
   __syncthreads();
 
@@ -209,14 +236,21 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
   F2 acc = {0.0f, 0.0f}, first = {0.0f, 0.0f};
   int first_row = -1;
   double res = 0.0, dang = 0.0;
+  // rows closed after the first one, at the steps set in done_mask: they are
+  // the rows first_row + 1, first_row + 2, ... in step order.  Their sums go
+  // to LDS after the scan and are written out by a row-parallel pass.
+  uint32_t done_mask = 0;
+  float done_sum[IPT];
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
+    done_sum[k] = 0.0f;
     if (i < nrows && s_end[i] <= j) {
       if (first_row < 0) {
         first_row = i;
         first = acc;
       } else {
-        out(r0 + i, acc, res, dang);
+        done_mask |= 1u << k;
+        done_sum[k] = f2_value(acc);
       }
       acc.s = 0.0f;
       acc.c = 0.0f;
@@ -247,16 +281,25 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
     }
     __syncthreads();
   }
+  // every row of the tile closes here; its sum goes to s_val[row] (the edge
+  // values are no longer read: the scan's barriers came after the merge)
   if (first_row >= 0) {
     F2 cin = tid > 0 ? s_scan[tid - 1] : F2{0.0f, 0.0f};
     F2 tot = f2_merge(cin, first);
-    if (first_row == 0 && irp[r0] < e0) {
+    if (first_row == 0 && cont)
       head[t] = tot;  // row r0 began in an earlier tile: pr_fixup finishes it
-    } else {
-      out(r0 + first_row, tot, res, dang);
-    }
+    else
+      s_val[first_row] = f2_value(tot);
+    int ir = first_row + 1;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k)
+      if ((done_mask >> k) & 1u) s_val[ir++] = done_sum[k];
   }
   if (tid == PR_BLOCK - 1) tail[t] = s_scan[PR_BLOCK - 1];
+  __syncthreads();
+  // row-parallel output: coalesced reads of x_old / inv_od, coalesced writes
+  for (int k = (cont ? 1 : 0) + tid; k < nrows; k += PR_BLOCK)
+    out.put(r0 + k, s_val[k], out.x_old[r0 + k], out.inv_od[r0 + k], res, dang);
   double rs = block_sum_f64(res, s_red);
   double ds = block_sum_f64(dang, s_red);
   if (tid == 0) {
