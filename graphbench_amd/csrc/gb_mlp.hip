@@ -1,9 +1,11 @@
 // gb_mlp.hip -- the small MLP of the learned delta / PageRank warm start.
 //
 // y = W3 relu(W2 relu(W1 x + b1) + b2) + b3, hidden width 64, FP32, trained
-// with mean squared error and Adam.  Nothing here is large enough for MFMA
-// tiles to pay: one thread per row, the parameters staged in LDS and read by
-// all lanes at once (LDS broadcast).
+// with mean squared error and Adam.  Training batches are small: one thread
+// per row, the parameters staged in LDS and read by all lanes at once (LDS
+// broadcast).  Inference over one row per vertex (the PageRank warm start,
+// millions of rows) runs the 64 x 64 layer on bf16 MFMA instead
+// (mlp_infer_mfma_kernel below).
 // Training step kernels:
 //   mlp_forward      per row: pre-activations A1, A2 (kept) and output Y
 //   mlp_backward     per row: output error G3, GA2 = (W3^T G3)*[A2>0],
@@ -215,7 +217,124 @@ MlpWork &mlpw() {
 
 bool dims_ok(int din, int dout) { return din >= 1 && din <= MAXIN && dout >= 1 && dout <= MAXOUT; }
 
+// ---- inference with the hidden 64 x 64 layer on MFMA (bf16 in, FP32 sums) --
+// For the warm start's one row per vertex (millions of rows) the 64 x 64
+// layer is a GEMM: a wave takes 32 rows, computes layer 1 in FP32 straight
+// into the A-operand layout, runs 2 x 8 v_mfma_f32_32x32x8_bf16 against W2
+// held in registers as B operands, adds b2 and applies ReLU in FP32, passes
+// the 32 x 64 activations through LDS and finishes layer 3 in FP32.
+// Operand layouts of v_mfma_f32_32x32x8_bf16 (lane l, element e):
+//   A (32 x 8):  row l % 32, k = 4 (l / 32) + e, e < 4
+//   B (8 x 32):  col l % 32, k = 4 (l / 32) + e
+//   D (32 x 32): col l % 32, row 8 (r / 4) + 4 (l / 32) + r % 4 in register r
+typedef short mf_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mf_f32x16 __attribute__((ext_vector_type(16)));
+constexpr int MF_ROWS = 32;         // rows per wave tile
+constexpr int MF_LD = H + 1;        // LDS row stride of the activations (no bank conflicts)
+constexpr int MF_BLOCK = 256;
+
+__device__ __forceinline__ short mf_bf16(float f) {  // round to nearest even (finite f)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+__global__ __launch_bounds__(MF_BLOCK) void mlp_infer_mfma_kernel(
+    const float *__restrict__ P, int din, int dout, const float *__restrict__ X, int64_t n,
+    float *__restrict__ Y) {
+  __shared__ float s_p[H * MAXIN + H + H + MAXOUT * H + MAXOUT];  // W1 b1 b2 W3 b3
+  __shared__ float s_h[MF_BLOCK / GB_WAVE][MF_ROWS * MF_LD];
+  const Lay L = layout(din, dout);
+  float *sW1 = s_p, *sb1 = sW1 + H * din, *sb2 = sb1 + H, *sW3 = sb2 + H, *sb3 = sW3 + dout * H;
+  for (int i = threadIdx.x; i < H * din; i += blockDim.x) sW1[i] = P[L.oW1 + i];
+  for (int i = threadIdx.x; i < H; i += blockDim.x) {
+    sb1[i] = P[L.ob1 + i];
+    sb2[i] = P[L.ob2 + i];
+  }
+  for (int i = threadIdx.x; i < dout * H; i += blockDim.x) sW3[i] = P[L.oW3 + i];
+  for (int i = threadIdx.x; i < dout; i += blockDim.x) sb3[i] = P[L.ob3 + i];
+  __syncthreads();
+  const int lane = gb_lane(), half = lane >> 5, c32 = lane & 31;
+  float *sh = s_h[threadIdx.x / GB_WAVE];
+  // W2 as B operands: tile nt (hidden outputs 32 nt ..), k-step ks (inputs 8 ks ..)
+  mf_bf16x4 bw[2][8];
+#pragma unroll
+  for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        bw[nt][ks][e] = mf_bf16(P[L.oW2 + (32 * nt + c32) * H + 8 * ks + 4 * half + e]);
+  const int64_t tiles = (n + MF_ROWS - 1) / MF_ROWS;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t tl = wave; tl < tiles; tl += nwaves) {
+    const int64_t r0 = tl * MF_ROWS;
+    int64_t row = r0 + c32;
+    if (row > n - 1) row = n - 1;  // rows past the end recompute the last row
+    float x[MAXIN];
+#pragma unroll
+    for (int d = 0; d < MAXIN; d++) x[d] = d < din ? X[row * din + d] : 0.0f;
+    // layer 1 into A operands: hidden unit k = 8 ks + 4 half + e of this row
+    mf_bf16x4 a[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int k = 8 * ks + 4 * half + e;
+        float z = sb1[k];
+        for (int d = 0; d < din; d++) z += sW1[k * din + d] * x[d];
+        a[ks][e] = mf_bf16(z > 0.0f ? z : 0.0f);
+      }
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) {
+      mf_f32x16 acc = {};
+#pragma unroll
+      for (int ks = 0; ks < 8; ks++)
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[ks], bw[nt][ks], acc, 0, 0, 0);
+      const int col = 32 * nt + c32;
+      const float b = sb2[col];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int i = 8 * (r / 4) + 4 * half + (r % 4);
+        const float z = acc[r] + b;
+        sh[i * MF_LD + col] = z > 0.0f ? z : 0.0f;
+      }
+    }
+    // a wave's LDS accesses execute in order: its lanes' writes above are seen
+    // by the reads below
+    __builtin_amdgcn_wave_barrier();
+    // layer 3: lane (half, c32) sums hidden units [32 half, 32 half + 32) of row c32
+    float y[MAXOUT];
+#pragma unroll
+    for (int o = 0; o < MAXOUT; o++) {
+      float s = 0.0f;
+      if (o < dout)
+        for (int j = 0; j < 32; j++) s += sW3[o * H + 32 * half + j] * sh[c32 * MF_LD + 32 * half + j];
+      y[o] = s + __shfl_xor(s, 32, GB_WAVE);
+    }
+    __builtin_amdgcn_wave_barrier();  // the tile is rewritten next round
+    const int64_t out_row = r0 + c32;
+    if (half == 0 && out_row < n)
+      for (int o = 0; o < dout; o++) Y[out_row * dout + o] = y[o] + sb3[o];
+  }
+}
+
 }  // namespace
+
+extern "C" gb_status gb_mlp_forward_bf16(const float *d_params, int32_t din, int32_t dout,
+                                         const float *d_X, int64_t n, float *d_Y, void *stream) {
+  if (!dims_ok(din, dout) || !d_params || (n > 0 && (!d_X || !d_Y)))
+    return gb_fail(GB_ERR_INVALID, "gb_mlp_forward_bf16: bad arguments");
+  if (n == 0) return GB_OK;
+  const int64_t tiles = (n + MF_ROWS - 1) / MF_ROWS;
+  const int64_t blocks = std::max<int64_t>(
+      1, std::min<int64_t>((tiles + 3) / 4, (int64_t)gb_num_cus() * 8));
+  hipLaunchKernelGGL(mlp_infer_mfma_kernel, dim3((unsigned)blocks), dim3(MF_BLOCK), 0,
+                     gb_stream(stream), d_params, din, dout, d_X, n, d_Y);
+  GB_HIP(hipGetLastError());
+  return GB_OK;
+}
 
 void gb_mlp_release() { mlpw().release(); }
 

@@ -92,11 +92,13 @@ class MLP:
         _check(lib().gb_mlp_adam(_ptr(self.params), _ptr(self.grad), _ptr(self.m), _ptr(self.v),
                                  self.total, lr, b1, b2, eps, self.step, _stream()))
 
-    def forward(self, X):
+    def forward(self, X, bf16: bool = False):
+        """FP32 forward, or with bf16=True the MFMA inference (hidden 64 x 64
+        layer in bf16 with FP32 sums)."""
         torch = _torch()
         Y = torch.empty((X.shape[0], self.dout), dtype=torch.float32, device="cuda")
-        _check(lib().gb_mlp_forward(_ptr(self.params), self.din, self.dout, _ptr(X), X.shape[0],
-                                    _ptr(Y), _stream()))
+        fn = lib().gb_mlp_forward_bf16 if bf16 else lib().gb_mlp_forward
+        _check(fn(_ptr(self.params), self.din, self.dout, _ptr(X), X.shape[0], _ptr(Y), _stream()))
         return Y
 
     def fit(self, X, T, steps=500, lr=3e-3, log_every=0):

@@ -85,6 +85,34 @@ def test_mlp_forward_vs_torch(gpu):
     np.testing.assert_allclose(Y, ref.numpy(), rtol=1e-4, atol=1e-5)
 
 
+def _bf16(a):
+    """Round float32 to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("din,dout,n", [(3, 1, 10007), (6, 2, 4096), (16, 4, 33)])
+def test_mlp_forward_bf16_mfma(gpu, din, dout, n):
+    """The MFMA inference against (a) an FP64 forward over the same bf16
+    roundings of the layer-1 activations and of W2 -- any slip in the MFMA
+    operand or result layouts mixes hidden units and fails it -- and (b) the
+    FP32 forward, within bf16 rounding."""
+    torch = _torch()
+    m = learned.MLP(din, dout, seed=11)
+    X, _ = _data(n, din, dout, seed=12)
+    Y = m.forward(torch.from_numpy(X).cuda(), bf16=True).cpu().numpy()
+    Y32 = m.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    W1, b1, W2, b2, W3, b3 = _np_params(m)
+    h1 = np.maximum(X @ W1.T + b1, 0).astype(np.float32)
+    z2 = _bf16(h1).astype(np.float64) @ _bf16(W2).astype(np.float64).T + b2
+    ref = np.maximum(z2, 0) @ W3.T.astype(np.float64) + b3
+    scale = max(1.0, float(np.abs(ref).max()))
+    np.testing.assert_allclose(Y, ref, rtol=1e-4, atol=2e-5 * scale)
+    np.testing.assert_allclose(Y, Y32, rtol=2e-2, atol=2e-2 * scale)
+
+
 @pytest.mark.gpu
 def test_mlp_adam_vs_torch_optim(gpu):
     torch = _torch()
