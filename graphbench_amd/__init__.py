@@ -121,6 +121,7 @@ _SIGS = {
                            _i32, _p]),
     "gb_csr_from_edges": (_i32, [_i32, _i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
     "gb_csr_transpose": (_i32, [_i32, _i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
+    "gb_edges_unique": (_i32, [_i64, _p, _p, _p, _p, C.POINTER(_i64), _p]),
     "gb_graph_features": (_i32, [_p, _i32, _p, _p]),
     "gb_pr_warm_start": (_i32, [_p, _p, _p, _p, _p]),
 }
@@ -466,6 +467,19 @@ def csr_transpose(row_ptr, col, w=None):
     _check(lib().gb_csr_transpose(V, E, _ptr(row_ptr), _ptr(col), _ptr(w), wb, _ptr(rp), _ptr(rc),
                                   _ptr(wo), _stream()))
     return rp, rc[:E], (None if wo is None else wo[:E])
+
+
+def edges_unique(src, dst):
+    """First occurrences of the device edge pairs (src[e], dst[e]), in input
+    order (the dedup of remove_duplicate): (src_out, dst_out) device tensors."""
+    torch = _torch()
+    E = int(src.numel())
+    so = torch.empty(max(E, 1), dtype=torch.int32, device="cuda")
+    do = torch.empty(max(E, 1), dtype=torch.int32, device="cuda")
+    k = _i64(0)
+    _check(lib().gb_edges_unique(E, _ptr(src), _ptr(dst), _ptr(so), _ptr(do), C.byref(k),
+                                 _stream()))
+    return so[:k.value], do[:k.value]
 
 
 def u32(t) -> np.ndarray:

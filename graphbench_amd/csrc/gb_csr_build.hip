@@ -64,7 +64,98 @@ int grid_for(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)gb_num_cus() * 16));
 }
 
+// ---- duplicate edges ------------------------------------------------------
+__global__ void pair_keys_kernel(int64_t E, const int32_t *__restrict__ src,
+                                 const int32_t *__restrict__ dst, uint64_t *__restrict__ key,
+                                 int32_t *__restrict__ ids, int *__restrict__ bad) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t s = src[e], d = dst[e];
+    if (s < 0 || d < 0) *bad = 1;
+    key[e] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)d;
+    ids[e] = (int32_t)e;
+  }
+}
+
+// keep[e] = 1 for the first index of each run of equal keys (the sort is
+// stable, so that is the pair's first occurrence in input order)
+__global__ void first_of_run_kernel(int64_t E, const uint64_t *__restrict__ skey,
+                                    const int32_t *__restrict__ sid, int32_t *__restrict__ keep) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (int64_t)gridDim.x * blockDim.x)
+    keep[sid[i]] = (i == 0 || skey[i] != skey[i - 1]) ? 1 : 0;
+}
+
+__global__ void compact_pairs_kernel(int64_t E, const int32_t *__restrict__ keep,
+                                     const int32_t *__restrict__ pos,
+                                     const int32_t *__restrict__ src,
+                                     const int32_t *__restrict__ dst, int32_t *__restrict__ src_out,
+                                     int32_t *__restrict__ dst_out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x)
+    if (keep[e]) {
+      src_out[pos[e]] = src[e];
+      dst_out[pos[e]] = dst[e];
+    }
+}
+
 }  // namespace
+
+// Keeps the first occurrence of every (src, dst) pair, in input order, as
+// remove_duplicate (partition/csr_partition_bin_mpi.c:31-106) does with a hash
+// set over the file order: a stable radix sort of the 64-bit pair keys with
+// their indices marks the first index of each run; a scan compacts.
+extern "C" gb_status gb_edges_unique(int64_t E, const int32_t *d_src, const int32_t *d_dst,
+                                     int32_t *d_src_out, int32_t *d_dst_out, int64_t *h_kept,
+                                     void *stream) {
+  if (E < 0 || E >= ((int64_t)1 << 31) || !h_kept ||
+      (E > 0 && (!d_src || !d_dst || !d_src_out || !d_dst_out)))
+    return gb_fail(GB_ERR_INVALID, "gb_edges_unique: bad arguments");
+  *h_kept = 0;
+  if (E == 0) return GB_OK;
+  hipStream_t st = gb_stream(stream);
+  DevBuf key, skey, ids, sids, keep, pos, flag, tmp;
+  GB_TRY(key.ensure(sizeof(uint64_t) * E));
+  GB_TRY(skey.ensure(sizeof(uint64_t) * E));
+  GB_TRY(ids.ensure(sizeof(int32_t) * E));
+  GB_TRY(sids.ensure(sizeof(int32_t) * E));
+  GB_TRY(keep.ensure(sizeof(int32_t) * E));
+  GB_TRY(pos.ensure(sizeof(int32_t) * (E + 1)));
+  GB_TRY(flag.ensure(sizeof(int)));
+  GB_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+  hipLaunchKernelGGL(pair_keys_kernel, dim3(grid_for(E)), dim3(256), 0, st, E, d_src, d_dst,
+                     key.as<uint64_t>(), ids.as<int32_t>(), flag.as<int>());
+  GB_HIP(hipGetLastError());
+  int bad = 0;
+  GB_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  if (bad) return gb_fail(GB_ERR_INVALID, "gb_edges_unique: negative vertex id");
+  size_t tb_sort = 0, tb_scan = 0;
+  GB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb_sort, key.as<uint64_t>(),
+                                            skey.as<uint64_t>(), ids.as<int32_t>(),
+                                            sids.as<int32_t>(), (int)E, 0, 64, st));
+  GB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb_scan, keep.as<int32_t>(),
+                                          pos.as<int32_t>(), (int)E, st));
+  GB_TRY(tmp.ensure(std::max(tb_sort, tb_scan) + 16));
+  GB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb_sort, key.as<uint64_t>(),
+                                            skey.as<uint64_t>(), ids.as<int32_t>(),
+                                            sids.as<int32_t>(), (int)E, 0, 64, st));
+  hipLaunchKernelGGL(first_of_run_kernel, dim3(grid_for(E)), dim3(256), 0, st, E,
+                     skey.as<uint64_t>(), sids.as<int32_t>(), keep.as<int32_t>());
+  GB_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb_scan, keep.as<int32_t>(), pos.as<int32_t>(),
+                                          (int)E, st));
+  hipLaunchKernelGGL(compact_pairs_kernel, dim3(grid_for(E)), dim3(256), 0, st, E,
+                     keep.as<int32_t>(), pos.as<int32_t>(), d_src, d_dst, d_src_out, d_dst_out);
+  GB_HIP(hipGetLastError());
+  int32_t last_pos = 0, last_keep = 0;
+  GB_HIP(hipMemcpyAsync(&last_pos, pos.as<int32_t>() + (E - 1), sizeof(int32_t),
+                        hipMemcpyDeviceToHost, st));
+  GB_HIP(hipMemcpyAsync(&last_keep, keep.as<int32_t>() + (E - 1), sizeof(int32_t),
+                        hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));  // scratch buffers are freed on return
+  *h_kept = (int64_t)last_pos + last_keep;
+  return GB_OK;
+}
 
 extern "C" gb_status gb_csr_from_edges(int32_t V, int64_t E, const int32_t *d_src,
                                        const int32_t *d_dst, const void *d_w, int32_t w_bytes,

@@ -652,6 +652,75 @@ void init_node_id(t_csr *graph, lint *distrib, sint nparts) {
   for (int64_t e = 0; e < E; e++) graph->edge_info[e].node_id = out[e];
 }
 
+void remove_duplicate(char *file, char *dir, sint myrank) {
+  // partition/csr_partition_bin_mpi.c:31-106: <dir><file>_<rank> holds the
+  // edge count (lint) and (from, to) lint pairs; the pairs without repeats,
+  // first occurrences in file order, go to
+  // <dir><data_extend><file>_phy_<rank>_<num_nodes>, headed by the edge count
+  // (and, for one node, g_v_num before it and OFFSET 0, g_v_num after it)
+  char path[4096];
+  snprintf(path, sizeof(path), "%s%s_%d", dir, file, myrank);
+  FILE *reader = fopen(path, "rb");
+  if (!reader) {
+    printf("THE FILE %s does not exist!\n", path);
+    fatal("remove_duplicate: cannot open the edge file");
+  }
+  snprintf(path, sizeof(path), "%s%s%s_phy_%d_%d", dir, data_extend, file, myrank, num_nodes);
+  FILE *writer = fopen(path, "wb");
+  if (!writer) {
+    printf("THE FILE %s does not exist!\n", path);
+    fatal("remove_duplicate: cannot open the output file");
+  }
+  lint edge_num = 0;
+  if (fread(&edge_num, sizeof(lint), 1, reader) != 1 || edge_num < 0)
+    fatal("remove_duplicate: short edge file");
+  std::vector<lint> buf((size_t)edge_num * 2);
+  if (edge_num > 0 && fread(buf.data(), sizeof(lint), buf.size(), reader) != buf.size())
+    fatal("remove_duplicate: short edge file");
+  fclose(reader);
+  if (edge_num >= ((lint)1 << 31)) fatal("remove_duplicate: more than 2^31 - 1 edges");
+  std::vector<int32_t> src(edge_num), dst(edge_num);
+  for (lint i = 0; i < edge_num; i++) {
+    const lint from = buf[2 * i], to = buf[2 * i + 1];
+    if (from == -1 || to == -1) {
+      printf("Node: %d ::: i: %lld, graph500 generated graph has -1 in it", myrank, i);
+      fatal("remove_duplicate: vertex id -1");
+    }
+    if (from < 0 || to < 0 || from >= ((lint)1 << 31) || to >= ((lint)1 << 31))
+      fatal("remove_duplicate: vertex id outside [0, 2^31)");
+    src[i] = (int32_t)from;
+    dst[i] = (int32_t)to;
+  }
+  DevBuf ds, dd, os, od;
+  upload(ds, src);
+  upload(dd, dst);
+  check(os.ensure(sizeof(int32_t) * ((size_t)edge_num + 1)), "device allocation");
+  check(od.ensure(sizeof(int32_t) * ((size_t)edge_num + 1)), "device allocation");
+  int64_t kept = 0;
+  check(gb_edges_unique(edge_num, ds.as<int32_t>(), dd.as<int32_t>(), os.as<int32_t>(),
+                        od.as<int32_t>(), &kept, nullptr),
+        "remove_duplicate");
+  std::vector<int32_t> ks(kept), kd(kept);
+  if (kept > 0) {
+    hipcheck(hipMemcpy(ks.data(), os.p, sizeof(int32_t) * kept, hipMemcpyDeviceToHost), "download");
+    hipcheck(hipMemcpy(kd.data(), od.p, sizeof(int32_t) * kept, hipMemcpyDeviceToHost), "download");
+  }
+  const lint nondup = kept, off = 0;
+  if (num_nodes == 1) fwrite(&g_v_num, sizeof(lint), 1, writer);
+  fwrite(&nondup, sizeof(lint), 1, writer);
+  if (num_nodes == 1) {
+    fwrite(&off, sizeof(lint), 1, writer);
+    fwrite(&g_v_num, sizeof(lint), 1, writer);
+  }
+  std::vector<lint> out((size_t)kept * 2);
+  for (int64_t i = 0; i < kept; i++) {
+    out[2 * i] = ks[i];
+    out[2 * i + 1] = kd[i];
+  }
+  if (kept > 0) fwrite(out.data(), sizeof(lint), out.size(), writer);
+  fclose(writer);
+}
+
 lint bin_search(lint *distrib, lint val, lint start, lint end) {
   // utils/utils.c:222-239
   for (;;) {

@@ -59,6 +59,7 @@ def lib():
             "separate_csr": [P(TCsr), P(TCsr), P(TCsr), C.c_double],
             "comp_distrib_csr": [P(TCsr), P(TCsr), P(lint), P(lint), P(lint), sint],
             "init_node_id": [P(TCsr), P(lint), sint],
+            "remove_duplicate": [C.c_char_p, C.c_char_p, sint],
         }.items():
             f = getattr(L, name)
             f.restype = None

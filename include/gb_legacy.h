@@ -115,6 +115,14 @@ void comp_distrib_csr(t_csr *graph_send, t_csr *graph_recv, lint *distrib,
 /* init_node_id, partition/csr_partition.c:127-131: owner part of every edge
  * target (bin_search of utils.c:222-239), on the GPU. */
 void init_node_id(t_csr *graph, lint *distrib, sint num_nodes);
+/* Replaces remove_duplicate, partition/csr_partition_bin_mpi.c:31-106
+ * (csr_partition_bin_mpi.h): reads <dir><file>_<myrank> (lint edge count,
+ * then (from, to) lint pairs), drops repeated pairs on the GPU (first
+ * occurrence kept, file order kept) and writes
+ * <dir><data_extend><file>_phy_<myrank>_<num_nodes> in the reference's
+ * format (for num_nodes == 1: g_v_num, count, 0, g_v_num, pairs; otherwise
+ * count, pairs).  Ids must lie in [0, 2^31); -1 stops as the reference does. */
+void remove_duplicate(char *file, char *dir, sint myrank);
 /* bin_search, utils/utils.c:222-239 (single lookup, host). */
 lint bin_search(lint *distrib, lint val, lint start, lint end);
 

@@ -110,6 +110,14 @@ gb_status gb_csr_transpose(int32_t V, int64_t E, const int32_t *d_row_ptr,
                            const int32_t *d_col, const void *d_w, int32_t w_bytes,
                            int32_t *d_trow_ptr, int32_t *d_tcol, void *d_tw,
                            void *stream);
+/* Replaces the dedup of remove_duplicate (partition/csr_partition_bin_mpi.c:
+ * 31-106): keeps the first occurrence of every (d_src[e], d_dst[e]) pair, in
+ * input order, writing the kept pairs to d_src_out / d_dst_out (E entries of
+ * room) and their number to *h_kept.  Negative ids are rejected (the
+ * reference stops on -1).  Synchronises the stream. */
+gb_status gb_edges_unique(int64_t E, const int32_t *d_src, const int32_t *d_dst,
+                          int32_t *d_src_out, int32_t *d_dst_out, int64_t *h_kept,
+                          void *stream);
 
 /* ---- light/heavy split ----------------------------------------------------
  * Stable per-row partition: light edges (w <= light_max) first.  Writes

@@ -62,6 +62,7 @@ def _declare(L):
         "orc_merge_path_split": (None, [_i32, _p, _i64, _i64, _p, _p]),
         "orc_separate_csr": (None, [_i32, _p, _p, _p, _u32, _p, _p, _p, _p, _p, _p]),
         "orc_separate_csr_f64": (None, [_i32, _p, _p, _p, _f64, _p, _p, _p, _p, _p, _p]),
+        "orc_remove_duplicate": (_i64, [_i64, _p, _p]),
         "orc_mlp_loss_grad": (_f64, [_i32, _i32, _i32, _i32] + [_p] * 14),
         "orc_set_threads": (None, [_i32]),
         "orc_get_threads": (_i32, []),
@@ -189,6 +190,14 @@ def merge_path_split(rp, items, ntiles):
     te = np.empty(ntiles + 1, np.int32)
     lib().orc_merge_path_split(V, _ptr(rp), int(items), int(ntiles), _ptr(tr), _ptr(te))
     return tr, te
+
+
+def remove_duplicate(pairs):
+    """First occurrences of (from, to) pairs, in order: (k, 2) int64."""
+    pairs = _c(np.asarray(pairs).reshape(-1, 2), np.int64)
+    out = np.empty_like(pairs)
+    k = lib().orc_remove_duplicate(len(pairs), _ptr(pairs), _ptr(out))
+    return out[:k]
 
 
 def separate_csr(rp, col, w, light_max):

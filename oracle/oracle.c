@@ -678,6 +678,37 @@ void orc_separate_csr(int32_t V, const int32_t *rp, const int32_t *col,
   }
 }
 
+int64_t orc_remove_duplicate(int64_t E, const int64_t *pairs, int64_t *out) {
+  /* open-addressing set of seen (from, to) pairs; slot holds index + 1 */
+  int64_t cap = 16;
+  while (cap < 2 * E + 2) cap <<= 1;
+  int64_t *slot = (int64_t *)calloc((size_t)cap, sizeof(int64_t));
+  int64_t kept = 0;
+  for (int64_t i = 0; i < E; i++) {
+    const int64_t a = pairs[2 * i], b = pairs[2 * i + 1];
+    uint64_t h = (uint64_t)a * 0x9E3779B97F4A7C15ull ^ ((uint64_t)b + 0x632BE59BD9B4E019ull);
+    h ^= h >> 29;
+    int64_t s = (int64_t)(h & (uint64_t)(cap - 1));
+    int found = 0;
+    while (slot[s]) {
+      const int64_t j = slot[s] - 1;
+      if (pairs[2 * j] == a && pairs[2 * j + 1] == b) {
+        found = 1;
+        break;
+      }
+      s = (s + 1) & (cap - 1);
+    }
+    if (!found) {
+      slot[s] = i + 1;
+      out[2 * kept] = a;
+      out[2 * kept + 1] = b;
+      kept++;
+    }
+  }
+  free(slot);
+  return kept;
+}
+
 void orc_separate_csr_f64(int32_t V, const int32_t *rp, const int32_t *col,
                           const double *w, double delta, int32_t *lrp, int32_t *lcol,
                           double *lw, int32_t *hrp, int32_t *hcol, double *hw) {

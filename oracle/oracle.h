@@ -112,6 +112,12 @@ void orc_separate_csr(int32_t V, const int32_t *rp, const int32_t *col,
                       int32_t *lcol, uint32_t *lw, int32_t *hrp, int32_t *hcol,
                       uint32_t *hw);
 
+/* remove_duplicate (partition/csr_partition_bin_mpi.c:31-106): walks the
+ * (from, to) pairs in file order and keeps a pair the first time it is seen
+ * (a hash set of seen pairs, as the reference's search_hash/insert_hash).
+ * pairs and out hold 2*E values; returns the number of pairs kept. */
+int64_t orc_remove_duplicate(int64_t E, const int64_t *pairs, int64_t *out);
+
 /* Same on f64 weights (the reference's edge_weight type). */
 void orc_separate_csr_f64(int32_t V, const int32_t *rp, const int32_t *col,
                           const double *w, double delta, int32_t *lrp, int32_t *lcol,

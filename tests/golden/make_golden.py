@@ -10,6 +10,11 @@ outputs, as JSON under tests/golden/.  Cases:
                    chunked rc and recip files, every ITER line, final values,
                    iteration count
   bc.json          the reference CLI -e (run_bc, csr_bc.c:16-68) on text graphs
+  remove_duplicate.json
+                   the reference CLI -y (remove_duplicate,
+                   csr_partition_bin_mpi.c:31-106): binary edge files in, the
+                   deduplicated file's bytes out
+Arguments: fixture names to (re)write; none = all.
 """
 from __future__ import annotations
 
@@ -207,16 +212,49 @@ def bc_cases():
     return out
 
 
+def remove_duplicate_cases():
+    out = []
+    rng = np.random.default_rng(31)
+    inputs = [
+        ("dups", rng.integers(0, 40, (600, 2))),            # many repeats
+        ("few", rng.integers(0, 1 << 20, (500, 2))),        # almost none
+        ("one_pair", np.tile([[7, 3]], (50, 1))),            # a single pair repeated
+        ("reversed", np.array([[1, 2], [2, 1], [1, 2], [2, 1], [3, 3], [3, 3]])),
+    ]
+    for name, pairs in inputs:
+        pairs = np.asarray(pairs, np.int64)
+        for nodes, scale in ((1, 10), (2, 6)):
+            with tempfile.TemporaryDirectory() as td:
+                g = os.path.join(td, "g") + "/"
+                os.makedirs(g + "datex")
+                with open(g + "edges_0", "wb") as f:
+                    f.write(np.int64(len(pairs)).tobytes())
+                    f.write(pairs.tobytes())
+                # with more than one node main() goes on to the partitioning
+                # steps, which stop on files this case does not make; the
+                # deduplicated file is complete by then
+                run(["cli", "-y", "-f", "edges", "-b", g, "-n", str(nodes), "-s", str(scale)],
+                    cwd=td, check=nodes == 1)
+                raw = open(g + f"datex/edges_phy_0_{nodes}", "rb").read()
+            out.append(dict(name=name, num_nodes=nodes, g_v_num=1 << scale,
+                            pairs=pairs.ravel().tolist(),
+                            output=np.frombuffer(raw, np.int64).tolist()))
+    return out
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference first: make -C oracle/ref")
-    fixtures = {
-        "sssp_list.json": sssp_cases() + [sssp_rmat_case()],
-        "distrib.json": distrib_cases(),
-        "separate.json": separate_cases(),
-        "pagerank.json": pagerank_cases(),
-        "bc.json": bc_cases(),
+    builders = {
+        "sssp_list.json": lambda: sssp_cases() + [sssp_rmat_case()],
+        "distrib.json": distrib_cases,
+        "separate.json": separate_cases,
+        "pagerank.json": pagerank_cases,
+        "bc.json": bc_cases,
+        "remove_duplicate.json": remove_duplicate_cases,
     }
+    wanted = sys.argv[1:] or list(builders)
+    fixtures = {name: builders[name]() for name in wanted}
     for name, data in fixtures.items():
         with open(os.path.join(HERE, name), "w") as f:
             json.dump({"generator": "tests/golden/make_golden.py (reference build oracle/_ref)",

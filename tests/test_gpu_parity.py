@@ -492,6 +492,28 @@ def test_csr_from_edges_and_transpose(gpu):
         gb.csr_from_edges(V, d(bad), d(dst))
 
 
+@pytest.mark.parametrize("E,V", [(0, 1), (1, 1), (300000, 700), (300000, 1 << 30)])
+def test_edges_unique_vs_oracle(gpu, E, V):
+    """Device dedup (remove_duplicate's first-occurrence rule) against the
+    oracle, whose own rule is pinned on the reference's output files."""
+    gb = gpu
+    rng = np.random.default_rng(E + 17)
+    src = rng.integers(0, V, E).astype(np.int32)
+    dst = rng.integers(0, V, E).astype(np.int32)
+    _say(f"edges_unique E={E} V={V}")
+    so, do = gb.edges_unique(gb.to_device(src), gb.to_device(dst))
+    ref = oracle.remove_duplicate(np.stack([src, dst], axis=1).astype(np.int64))
+    assert np.array_equal(so.cpu().numpy(), ref[:, 0]) and np.array_equal(do.cpu().numpy(), ref[:, 1])
+
+
+def test_edges_unique_rejects_negative_ids(gpu):
+    gb = gpu
+    src = np.array([0, 1, -1], np.int32)
+    dst = np.array([1, 2, 3], np.int32)
+    with pytest.raises(gb.GBError):
+        gb.edges_unique(gb.to_device(src), gb.to_device(dst))
+
+
 def _hub_graph(gb):
     """Empty rows, a 5000-edge hub spanning many edge tiles, short rows."""
     rng = np.random.default_rng(5)

@@ -236,6 +236,30 @@ def test_separate_csr_vs_reference(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", load("remove_duplicate.json"),
+                         ids=lambda c: f"{c['name']}-n{c['num_nodes']}")
+def test_remove_duplicate_vs_reference(gpu, case, tmp_path):
+    """The output file is byte-identical to the reference's."""
+    L = legacy()
+    lib = L.lib()
+    d = str(tmp_path) + "/"
+    os.makedirs(d + "datex")
+    pairs = np.array(case["pairs"], np.int64)
+    with open(d + "edges_0", "wb") as f:
+        f.write(np.int64(len(pairs) // 2).tobytes() + pairs.tobytes())
+    C.c_int.in_dll(lib, "num_nodes").value = case["num_nodes"]
+    C.c_longlong.in_dll(lib, "g_v_num").value = case["g_v_num"]
+    try:
+        lib.remove_duplicate(b"edges", d.encode(), 0)
+    finally:
+        C.c_int.in_dll(lib, "num_nodes").value = 1
+        C.c_longlong.in_dll(lib, "g_v_num").value = 1
+    with open(d + f"datex/edges_phy_0_{case['num_nodes']}", "rb") as f:
+        got = np.frombuffer(f.read(), np.int64)
+    assert got.tolist() == case["output"]
+
+
+@pytest.mark.gpu
 def test_comp_distrib_and_init_node_id_vs_reference(gpu):
     L = legacy()
     lib = L.lib()

@@ -112,6 +112,33 @@ def test_separate_vs_reference(case):
     assert lw.tolist() == case["light_w"] and hw.tolist() == case["heavy_w"]
 
 
+@pytest.mark.parametrize("case", load("remove_duplicate.json"),
+                         ids=lambda c: f"{c['name']}-n{c['num_nodes']}")
+def test_remove_duplicate_vs_reference(case):
+    """The oracle's kept pairs equal the pairs in the reference's output file
+    (after its header: g_v_num, count, 0, g_v_num for one node; count otherwise)."""
+    out = np.array(case["output"], np.int64)
+    hdr = 4 if case["num_nodes"] == 1 else 1
+    kept = oracle.remove_duplicate(case["pairs"])
+    assert (out[1] if hdr == 4 else out[0]) == len(kept)
+    assert np.array_equal(out[hdr:].reshape(-1, 2), kept)
+    if hdr == 4:
+        assert out[0] == out[3] == case["g_v_num"] and out[2] == 0
+
+
+def test_remove_duplicate_brute_force():
+    """Against a dictionary of first occurrences on random small inputs."""
+    rng = np.random.default_rng(9)
+    for E in (0, 1, 2, 50, 777):
+        p = rng.integers(0, 12, (E, 2))
+        seen, ref = set(), []
+        for a, b in p.tolist():
+            if (a, b) not in seen:
+                seen.add((a, b))
+                ref.append((a, b))
+        assert oracle.remove_duplicate(p).tolist() == [list(x) for x in ref]
+
+
 def parse_rc(text):
     """The reference's chunked RECV file (p sp V E off gV / a u v w) -> in-CSR
     in file order, as scan_csr_idx + read_graph_csr (RECV) build it."""
