@@ -75,14 +75,21 @@ struct gb_graph_s {
   bool max_w_valid = false;
   DevBuf inv_outdeg;  // float[V]
   bool inv_outdeg_valid = false;
-  // PageRank gather shape {items per thread, hot-set size} picked by the
-  // autotuner, per (fixed items per thread or none, relabelled or not);
-  // items per thread 0 = not tuned yet
-  int32_t pr_tuned[10][2] = {};
+  // PageRank gather shape {items per thread, hot-set size, row-block edges}
+  // picked by the autotuner, per (fixed items per thread or none, relabelled
+  // or not, row blocks among the candidates or not); merge-path form when
+  // row-block edges is 0; all 0 = not tuned yet
+  int32_t pr_tuned[20][3] = {};
   int32_t pr_tile_items = 0;
   bool pr_tile_relabeled = false;
   int64_t pr_ntiles = 0;
   DevBuf pr_tile_row, pr_tile_edge;
+  // PageRank row blocks (row-block form), per edges-per-block and layout
+  int32_t pr_rb_nb = 0;
+  bool pr_rb_relabeled = false;
+  int64_t pr_rb_nblocks = 0;
+  int32_t pr_rb_nlong = 0;
+  DevBuf pr_rb, pr_rb_long;
   // PageRank relabelling: vertices renumbered by out-degree (descending), the
   // in-CSR and 1/outdeg in the new numbering (gb_graph_pr_relabel)
   bool rl_valid = false;

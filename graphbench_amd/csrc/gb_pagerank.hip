@@ -413,6 +413,272 @@ __global__ void pr_ref_update_kernel(int32_t V, const double *__restrict__ recip
   if (__ballot(big) != 0ull && gb_lane() == 0) atomicOr(not_done, 1);
 }
 
+// ---- row-block form (no merge path) -----------------------------------------
+// The rows are cut, once per graph layout, into blocks of consecutive rows
+// holding at most NB edges and RB_MAXR rows; a row with more than NB edges is
+// cut into chunks of NB edges, one block each.  A row block stages its edges'
+// contributions in LDS (coalesced col reads, one batch of gathers) and sums
+// every row there: one thread per row, or a power-of-two group of lanes per
+// row when the block has few rows (lanes stride the row, then a fixed
+// butterfly merges their partial sums).  No merge-path search, no segmented
+// scan, no cross-tile fix-up for rows that fit a block.  Chunks of long rows
+// leave one partial each; pr_rb_fixup_kernel joins them in chunk order.
+// Residual and dangling partials are per wave (no block barrier).
+constexpr int RB_MAXR = 1024;  // rows per block at most
+constexpr int RB_RPT = RB_MAXR / PR_BLOCK;
+
+struct RowBlock {
+  int32_t r0, nrows, e0, e1;  // nrows == 0: one chunk [e0, e1) of long row r0
+};
+struct LongRow {
+  int32_t row, b0, nb;  // chunks are blocks b0 .. b0 + nb - 1
+};
+
+// Row block bi.  HOT > 0: contributions of sources [0, HOT) come from s_hot.
+template <int NB, int HOT>
+__device__ __forceinline__ void rowblock_body(
+    const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
+    const RowBlock *__restrict__ blocks, const float *__restrict__ contrib, const RowOut &out,
+    F2 *__restrict__ chunk_sum, double *__restrict__ part_res, double *__restrict__ part_dang,
+    int64_t bi, const float *s_hot) {
+  constexpr int EPT = NB / PR_BLOCK;  // staged edges per thread
+  __shared__ float s_val[NB];
+  __shared__ int32_t s_rp[RB_MAXR + 1];
+  __shared__ F2 s_red[PR_BLOCK / GB_WAVE];
+  const int tid = threadIdx.x, lane = gb_lane();
+  const RowBlock b = blocks[bi];
+  const int nnz = b.e1 - b.e0;
+  // edge values: unconditional reads on clamped indices, one batch each
+  // (hot lanes read a dummy address)
+  float val[EPT];
+  if (nnz > 0) {
+    const int32_t *col_b = icol + b.e0;
+    int32_t c[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      c[m] = __builtin_nontemporal_load(col_b + (k < nnz ? k : nnz - 1));
+    }
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+      const uint32_t cu = (uint32_t)c[m];
+      if (HOT > 0) {
+        const float g = contrib[cu < (uint32_t)HOT ? 0u : cu];
+        val[m] = cu < (uint32_t)HOT ? s_hot[cu] : g;
+      } else {
+        val[m] = contrib[cu];
+      }
+    }
+  }
+  if (b.nrows == 0) {  // a chunk of a long row: one compensated sum for the block
+    F2 acc = {0.0f, 0.0f};
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) f2_add(acc, tid + m * PR_BLOCK < nnz ? val[m] : 0.0f);
+#pragma unroll
+    for (int d = 1; d < GB_WAVE; d <<= 1) {
+      F2 o;
+      o.s = __shfl_xor(acc.s, d, GB_WAVE);
+      o.c = __shfl_xor(acc.c, d, GB_WAVE);
+      acc = (lane & d) ? f2_merge(o, acc) : f2_merge(acc, o);
+    }
+    if (lane == 0) s_red[tid / GB_WAVE] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      F2 tot = s_red[0];
+      for (int w = 1; w < PR_BLOCK / GB_WAVE; w++) tot = f2_merge(tot, s_red[w]);
+      chunk_sum[bi] = tot;
+    }
+    if (lane == 0) {
+      part_res[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = 0.0;
+      part_dang[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = 0.0;
+    }
+    return;
+  }
+  const int nrows = b.nrows;
+#pragma unroll
+  for (int m = 0; m < EPT; ++m) {
+    const int k = tid + m * PR_BLOCK;
+    if (k < nnz) s_val[k] = val[m];
+  }
+  {
+    int32_t rp_v[RB_RPT + 1];
+#pragma unroll
+    for (int m = 0; m <= RB_RPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      rp_v[m] = irp[b.r0 + (k <= nrows ? k : nrows)];
+    }
+#pragma unroll
+    for (int m = 0; m <= RB_RPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      if (k <= nrows) s_rp[k] = rp_v[m] - b.e0;
+    }
+  }
+  __syncthreads();
+  double res = 0.0, dang = 0.0;
+  // lanes per row: the largest power of two with nrows * tpr <= 256, at most 64
+  int tpr = 1;
+  while (tpr < GB_WAVE && nrows * tpr * 2 <= PR_BLOCK) tpr <<= 1;
+  if (tpr == 1) {
+    float sum[RB_RPT];
+#pragma unroll
+    for (int m = 0; m < RB_RPT; ++m) {
+      const int r = tid + m * PR_BLOCK;
+      F2 acc = {0.0f, 0.0f};
+      if (r < nrows)
+        for (int k = s_rp[r]; k < s_rp[r + 1]; ++k) f2_add(acc, s_val[k]);
+      sum[m] = f2_value(acc);
+    }
+    float xo[RB_RPT], io[RB_RPT];
+#pragma unroll
+    for (int m = 0; m < RB_RPT; ++m) {
+      const int r = tid + m * PR_BLOCK;
+      const int32_t v = b.r0 + (r < nrows ? r : nrows - 1);
+      xo[m] = out.x_old[v];
+      io[m] = out.inv_od[v];
+    }
+#pragma unroll
+    for (int m = 0; m < RB_RPT; ++m) {
+      const int r = tid + m * PR_BLOCK;
+      if (r < nrows) out.put(b.r0 + r, sum[m], xo[m], io[m], res, dang);
+    }
+  } else {
+    const int r = tid / tpr, q = tid % tpr;
+    F2 acc = {0.0f, 0.0f};
+    if (r < nrows)
+      for (int k = s_rp[r] + q; k < s_rp[r + 1]; k += tpr) f2_add(acc, s_val[k]);
+    // fixed butterfly over the row's lanes (aligned groups inside one wave)
+    for (int d = 1; d < tpr; d <<= 1) {
+      F2 o;
+      o.s = __shfl_xor(acc.s, d, GB_WAVE);
+      o.c = __shfl_xor(acc.c, d, GB_WAVE);
+      acc = (q & d) ? f2_merge(o, acc) : f2_merge(acc, o);
+    }
+    const int32_t v = b.r0 + (r < nrows ? r : nrows - 1);
+    const float xo = out.x_old[v], io = out.inv_od[v];
+    if (r < nrows && q == 0) out.put(v, f2_value(acc), xo, io, res, dang);
+  }
+  res = gb_wave_sum_f64(res);
+  dang = gb_wave_sum_f64(dang);
+  if (lane == 0) {
+    part_res[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = res;
+    part_dang[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = dang;
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(PR_BLOCK) void pr_rowblock_kernel(
+    const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
+    const RowBlock *__restrict__ blocks, const float *__restrict__ contrib, RowOut out,
+    F2 *__restrict__ chunk_sum, double *__restrict__ part_res, double *__restrict__ part_dang,
+    const PrState *__restrict__ st) {
+  if (st->done) return;
+  out.base = st->base;
+  rowblock_body<NB, 0>(irp, icol, blocks, contrib, out, chunk_sum, part_res, part_dang,
+                       (int64_t)blockIdx.x, nullptr);
+}
+
+// Persistent variant with an LDS copy of the hottest HOT contributions (the
+// highest out-degrees once relabelled): blocks walk row blocks blockIdx.x,
+// blockIdx.x + gridDim.x, ...
+template <int NB, int HOT>
+__global__ __launch_bounds__(PR_BLOCK) void pr_rowblock_hot_kernel(
+    const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
+    const RowBlock *__restrict__ blocks, const float *__restrict__ contrib, RowOut out,
+    F2 *__restrict__ chunk_sum, double *__restrict__ part_res, double *__restrict__ part_dang,
+    const PrState *__restrict__ st, int64_t nblocks) {
+  __shared__ float s_hot[HOT];
+  if (st->done) return;
+  out.base = st->base;
+  const int nh = (int)min((int64_t)HOT, (int64_t)st->V);
+  for (int k = threadIdx.x; k < nh; k += PR_BLOCK) s_hot[k] = contrib[k];
+  __syncthreads();
+  for (int64_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+    rowblock_body<NB, HOT>(irp, icol, blocks, contrib, out, chunk_sum, part_res, part_dang, bi,
+                           s_hot);
+    __syncthreads();  // the next row block reuses the LDS staging
+  }
+}
+
+// Long rows from their chunk sums (chunk order), the gather's per-wave
+// partials summed per block; the last block reduces the block sums in a fixed
+// order and sets the next base and the stop flag (as pr_fixup_kernel).
+__global__ __launch_bounds__(PR_BLOCK) void pr_rb_fixup_kernel(
+    int64_t nparts, int32_t nlong, const LongRow *__restrict__ longs,
+    const F2 *__restrict__ chunk_sum, RowOut out, const double *__restrict__ p_res,
+    const double *__restrict__ p_dang, double *__restrict__ fix_res,
+    double *__restrict__ fix_dang, unsigned int *__restrict__ done_blocks,
+    PrState *__restrict__ st) {
+  __shared__ double s_red[PR_BLOCK / 64];
+  __shared__ int s_last;
+  if (st->done) return;
+  out.base = st->base;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double res = 0.0, dang = 0.0;
+  if (t < nparts) {
+    res = p_res[t];
+    dang = p_dang[t];
+  }
+  if (t < nlong) {
+    const LongRow L = longs[t];
+    F2 acc = chunk_sum[L.b0];
+    for (int32_t k = 1; k < L.nb; k++) acc = f2_merge(acc, chunk_sum[L.b0 + k]);
+    out(L.row, acc, res, dang);
+  }
+  double rsum = block_sum_f64(res, s_red);
+  double dsum = block_sum_f64(dang, s_red);
+  if (threadIdx.x == 0) {
+    fix_res[blockIdx.x] = rsum;
+    fix_dang[blockIdx.x] = dsum;
+    __threadfence();
+    s_last = atomicAdd(done_blocks, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int64_t n2 = gridDim.x;
+  double a = 0.0, c = 0.0;
+  for (int64_t i = threadIdx.x; i < n2; i += blockDim.x) {
+    a += __builtin_nontemporal_load(fix_res + i);
+    c += __builtin_nontemporal_load(fix_dang + i);
+  }
+  const double rtot = block_sum_f64(a, s_red);
+  const double dtot = block_sum_f64(c, s_red);
+  if (threadIdx.x == 0) {
+    const double N = (double)st->V;
+    const double d = (double)st->damping;
+    st->iter += 1;
+    st->residual = rtot;
+    st->dangling = dtot;
+    st->base = (float)((1.0 - d) / N + d * dtot / N);
+    st->done = (rtot < st->tol || st->iter >= st->max_iter) ? 1 : 0;
+    *done_blocks = 0;  // ready for the next iteration
+  }
+}
+
+// Row blocks of an in-CSR (host side; irp copied from the device).
+void build_row_blocks(const std::vector<int32_t> &rp, int nb, std::vector<RowBlock> &blocks,
+                      std::vector<LongRow> &longs) {
+  blocks.clear();
+  longs.clear();
+  const int32_t V = (int32_t)rp.size() - 1;
+  int32_t r = 0;
+  while (r < V) {
+    const int32_t deg = rp[r + 1] - rp[r];
+    if (deg > nb) {
+      const int32_t n = (deg + nb - 1) / nb;
+      longs.push_back(LongRow{r, (int32_t)blocks.size(), n});
+      for (int32_t k = 0; k < n; k++)
+        blocks.push_back(RowBlock{r, 0, rp[r] + k * nb, std::min(rp[r + 1], rp[r] + (k + 1) * nb)});
+      r++;
+      continue;
+    }
+    int32_t r1 = r + 1;
+    while (r1 < V && r1 - r < RB_MAXR && rp[r1 + 1] - rp[r] <= nb) r1++;
+    blocks.push_back(RowBlock{r, r1 - r, rp[r], rp[r1]});
+    r = r1;
+  }
+}
+
 struct PrWork {
   DevBuf xa, xb, ca, cb, tail, head, pres, pdang, fres, fdang, state, counter;
   void release() {
@@ -478,6 +744,57 @@ void launch_gather_hot(int hot, int64_t ntiles, hipStream_t st, const int32_t *i
   }
 }
 
+// row-block launches: one block per row block, or the persistent hot-set
+// variant on as many blocks as are resident at once
+struct RbArgs {
+  const int32_t *irp, *icol;
+  const RowBlock *blocks;
+  const float *contrib;
+  RowOut out;
+  F2 *chunk_sum;
+  double *pres, *pdang;
+  const PrState *state;
+  int64_t nblocks;
+};
+
+template <int NB, int HOT>
+void launch_rb(const RbArgs &a, hipStream_t st) {
+  if constexpr (HOT == 0) {
+    hipLaunchKernelGGL((pr_rowblock_kernel<NB>), dim3((unsigned)std::max<int64_t>(a.nblocks, 1)),
+                       dim3(PR_BLOCK), 0, st, a.irp, a.icol, a.blocks, a.contrib, a.out,
+                       a.chunk_sum, a.pres, a.pdang, a.state);
+  } else {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pr_rowblock_hot_kernel<NB, HOT>,
+                                                       PR_BLOCK, 0) != hipSuccess ||
+          nb < 1)
+        nb = 1;
+      per_cu = nb;
+    }
+    const int64_t grid =
+        std::max<int64_t>(1, std::min<int64_t>(a.nblocks, (int64_t)gb_num_cus() * per_cu));
+    hipLaunchKernelGGL((pr_rowblock_hot_kernel<NB, HOT>), dim3((unsigned)grid), dim3(PR_BLOCK), 0,
+                       st, a.irp, a.icol, a.blocks, a.contrib, a.out, a.chunk_sum, a.pres, a.pdang,
+                       a.state, a.nblocks);
+  }
+}
+
+template <int NB>
+void launch_rb_hot(int hot, const RbArgs &a, hipStream_t st) {
+  switch (hot) {
+    case 4096:
+      launch_rb<NB, 4096>(a, st);
+      break;
+    case 8192:
+      launch_rb<NB, 8192>(a, st);
+      break;
+    default:
+      launch_rb<NB, 0>(a, st);
+  }
+}
+
 }  // namespace
 
 void gb_pr_release() { g_prw.release(); }
@@ -507,25 +824,44 @@ static gb_status ensure_inv_outdeg(gb_graph g, hipStream_t st) {
   return GB_OK;
 }
 
+// The gather's form and shape: merge path with ipt items per thread (tile =
+// 256 x ipt items) and an LDS hot set of `hot` contributions, or row blocks of
+// rb edges (rb > 0; ipt and hot unused).
+struct PrShape {
+  int ipt, hot, rb;
+};
+static bool valid_rb(int rb) { return rb == 1024 || rb == 2048 || rb == 4096; }
+
 static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_pr_result *res,
-                              void *stream, int ipt, int hot);
+                              void *stream, PrShape sh);
 
 static bool use_relabel(gb_graph g, const gb_pr_params *p) {
   return p->relabel > 0 || (p->relabel == 0 && g->E >= ((int64_t)1 << 20));
 }
 
-// Gather shape for a graph: items per thread (tile = 256 x ipt items) and
-// the LDS hot-set size.  GB_PR_IPT / GB_PR_HOT override; otherwise the
-// autotuner runs once per graph handle and layout: three timed iterations
-// for each candidate, keeping the fastest mean gather.  Small tiles cost
-// staging and scan work per item, large ones occupancy and rows that span
-// threads; a hot set (relabelled graphs only) takes the most-read sources off
-// the L2 but its LDS lowers occupancy.
+// Gather form and shape for a graph.  gb_pr_params.row_block > 0 or GB_PR_RB
+// fix the row-block form; items_per_thread (or GB_PR_IPT) and GB_PR_HOT fix
+// merge-path shapes.  Otherwise the autotuner runs once per graph handle and
+// layout: three timed iterations for each candidate, keeping the fastest mean
+// gather.  Merge-path candidates: small tiles cost staging and scan work per
+// item, large ones occupancy and rows that span threads; a hot set (relabelled
+// graphs only) takes the most-read sources off the L2 but its LDS lowers
+// occupancy.  Row-block candidates (unless row_block is -1) skip the merge-path
+// search and scan but leave lanes idle in blocks of uneven rows.
 static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void *stream,
-                            int *ipt_out, int *hot_out) {
+                            PrShape *out) {
   const bool rl = use_relabel(g, p);
   const char *ei = getenv("GB_PR_IPT");
   const char *eh = getenv("GB_PR_HOT");
+  const char *er = getenv("GB_PR_RB");
+  int fixed_rb = p->row_block > 0 ? p->row_block : 0;
+  if (!fixed_rb && p->row_block == 0 && !p->items_per_thread && er && valid_rb(atoi(er)))
+    fixed_rb = atoi(er);
+  if (fixed_rb) {
+    const int h = eh ? atoi(eh) : 0;
+    *out = PrShape{0, rl && (h == 4096 || h == 8192) ? h : 0, fixed_rb};
+    return GB_OK;
+  }
   // fixed parts of the shape (0 / -1 = left to the tuner)
   int fixed_ipt = p->items_per_thread;
   if (!fixed_ipt && ei && valid_ipt(atoi(ei))) fixed_ipt = atoi(ei);
@@ -533,43 +869,45 @@ static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void 
   if (eh && (atoi(eh) == 0 || atoi(eh) == 4096 || atoi(eh) == 8192)) fixed_hot = atoi(eh);
   if (!rl) fixed_hot = 0;
   if (fixed_ipt && fixed_hot >= 0) {
-    *ipt_out = fixed_ipt;
-    *hot_out = fixed_hot;
+    *out = PrShape{fixed_ipt, fixed_hot, 0};
     return GB_OK;
   }
-  // cached per (fixed ipt or none, layout)
-  const int key = (fixed_ipt ? fixed_ipt / 4 : 0) * 2 + (rl ? 1 : 0);
-  if (g->pr_tuned[key][0]) {
-    *ipt_out = g->pr_tuned[key][0];
-    *hot_out = g->pr_tuned[key][1];
+  const bool try_rb = !fixed_ipt && fixed_hot <= 0 && p->row_block == 0;
+  // cached per (fixed ipt or none, row blocks allowed, layout)
+  const int key = (fixed_ipt ? fixed_ipt / 4 : 0) * 2 + (rl ? 1 : 0) + (try_rb ? 0 : 10);
+  if (g->pr_tuned[key][0] || g->pr_tuned[key][2]) {
+    *out = PrShape{g->pr_tuned[key][0], g->pr_tuned[key][1], g->pr_tuned[key][2]};
     return GB_OK;
   }
-  std::vector<int> ipts = fixed_ipt ? std::vector<int>{fixed_ipt} : std::vector<int>{4, 8, 12, 16};
-  std::vector<int> hots =
-      fixed_hot >= 0 ? std::vector<int>{fixed_hot} : std::vector<int>{0, 4096, 8192};
-  int best_i = ipts[0], best_h = hots[0];
+  std::vector<PrShape> cands;
+  for (int ci : fixed_ipt ? std::vector<int>{fixed_ipt} : std::vector<int>{4, 8, 12, 16})
+    for (int ch : fixed_hot >= 0 ? std::vector<int>{fixed_hot} : std::vector<int>{0, 4096, 8192})
+      cands.push_back(PrShape{ci, ch, 0});
+  if (try_rb)
+    for (int rb : {1024, 2048, 4096})
+      for (int ch : rl ? std::vector<int>{0, 4096, 8192} : std::vector<int>{0})
+        cands.push_back(PrShape{0, ch, rb});
+  PrShape best = cands[0];
   double best_ms = 1e300;
-  for (int ci : ipts)
-    for (int ch : hots) {
-      gb_pr_params q = *p;
-      q.max_iter = 3;
-      q.tol = 0.0;
-      q.use_graph = 0;
-      q.time_kernels = 1;
-      q.d_x0 = nullptr;
-      gb_pr_result r{};
-      GB_TRY(pagerank_run(g, &q, d_x, &r, stream, ci, ch));
-      const double ms = r.gather_launches ? r.gather_ms / r.gather_launches : 1e300;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best_i = ci;
-        best_h = ch;
-      }
+  for (const PrShape &c : cands) {
+    gb_pr_params q = *p;
+    q.max_iter = 3;
+    q.tol = 0.0;
+    q.use_graph = 0;
+    q.time_kernels = 1;
+    q.d_x0 = nullptr;
+    gb_pr_result r{};
+    GB_TRY(pagerank_run(g, &q, d_x, &r, stream, c));
+    const double ms = r.gather_launches ? r.gather_ms / r.gather_launches : 1e300;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
     }
-  g->pr_tuned[key][0] = best_i;
-  g->pr_tuned[key][1] = best_h;
-  *ipt_out = best_i;
-  *hot_out = best_h;
+  }
+  g->pr_tuned[key][0] = best.ipt;
+  g->pr_tuned[key][1] = best.hot;
+  g->pr_tuned[key][2] = best.rb;
+  *out = best;
   return GB_OK;
 }
 
@@ -583,14 +921,20 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
   if (g->V <= 0) return gb_fail(GB_ERR_INVALID, "gb_pagerank: empty graph");
   if (!(p->damping >= 0.0 && p->damping <= 1.0) || p->max_iter < 0)
     return gb_fail(GB_ERR_INVALID, "gb_pagerank: bad damping/max_iter");
-  int ipt = 8, hot = 0;
+  if (p->row_block < -1 || (p->row_block > 0 && !valid_rb(p->row_block)))
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: row_block must be -1, 0, 1024, 2048 or 4096");
+  if (p->row_block > 0 && p->items_per_thread != 0)
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: row_block and items_per_thread both set");
+  PrShape sh{8, 0, 0};
   GB_TRY(ensure_inv_outdeg(g, gb_stream(stream)));
-  GB_TRY(pick_shape(g, p, d_x, stream, &ipt, &hot));
-  return pagerank_run(g, p, d_x, res, stream, ipt, hot);
+  GB_TRY(pick_shape(g, p, d_x, stream, &sh));
+  return pagerank_run(g, p, d_x, res, stream, sh);
 }
 
 static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_pr_result *res,
-                              void *stream, int ipt, int hot) {
+                              void *stream, PrShape sh) {
+  const int ipt = sh.rb ? 8 : sh.ipt, rb = sh.rb;
+  int hot = sh.hot;
   hipStream_t st = gb_stream(stream);
   hipStream_t own = nullptr;
   if (!st && p->use_graph && !p->time_kernels) {
@@ -618,7 +962,7 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   const int32_t *irp = relabel ? g->rl_rrp.as<int32_t>() : g->rrp;
   const int32_t *icol = relabel ? g->rl_rcol.as<int32_t>() : g->rcol;
   const float *inv = relabel ? g->rl_inv.as<float>() : g->inv_outdeg.as<float>();
-  if (g->pr_tile_items != tile || g->pr_tile_relabeled != relabel) {
+  if (!rb && (g->pr_tile_items != tile || g->pr_tile_relabeled != relabel)) {
     const int64_t ntiles = ((int64_t)V + g->E + tile - 1) / tile;
     GB_TRY(g->pr_tile_row.ensure(sizeof(int32_t) * (ntiles + 1)));
     GB_TRY(g->pr_tile_edge.ensure(sizeof(int32_t) * (ntiles + 1)));
@@ -628,8 +972,30 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     g->pr_tile_relabeled = relabel;
     g->pr_ntiles = ntiles;
   }
-  const int64_t ntiles = g->pr_ntiles;
-  const int64_t nfix = (ntiles + PR_BLOCK - 1) / PR_BLOCK;
+  if (rb && (g->pr_rb_nb != rb || g->pr_rb_relabeled != relabel)) {
+    std::vector<int32_t> hrp((size_t)V + 1);
+    GB_HIP(hipMemcpyAsync(hrp.data(), irp, sizeof(int32_t) * (V + 1), hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    std::vector<RowBlock> blocks;
+    std::vector<LongRow> longs;
+    build_row_blocks(hrp, rb, blocks, longs);
+    GB_TRY(g->pr_rb.ensure(sizeof(RowBlock) * blocks.size()));
+    GB_TRY(g->pr_rb_long.ensure(sizeof(LongRow) * (longs.size() + 1)));
+    GB_HIP(hipMemcpyAsync(g->pr_rb.p, blocks.data(), sizeof(RowBlock) * blocks.size(),
+                          hipMemcpyHostToDevice, st));
+    if (!longs.empty())
+      GB_HIP(hipMemcpyAsync(g->pr_rb_long.p, longs.data(), sizeof(LongRow) * longs.size(),
+                            hipMemcpyHostToDevice, st));
+    GB_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
+    g->pr_rb_nb = rb;
+    g->pr_rb_relabeled = relabel;
+    g->pr_rb_nblocks = (int64_t)blocks.size();
+    g->pr_rb_nlong = (int32_t)longs.size();
+  }
+  // merge path: tiles and their partials; row blocks: blocks, 4 partials each
+  const int64_t ntiles = rb ? g->pr_rb_nblocks : g->pr_ntiles;
+  const int64_t nparts = rb ? ntiles * (PR_BLOCK / GB_WAVE) : ntiles;
+  const int64_t nfix = (std::max<int64_t>(nparts, rb ? g->pr_rb_nlong : 0) + PR_BLOCK - 1) / PR_BLOCK;
   const int64_t vblocks = ((int64_t)V + PR_BLOCK - 1) / PR_BLOCK;
   PrWork &w = g_prw;
   GB_TRY(w.xa.ensure(sizeof(float) * V));
@@ -638,8 +1004,8 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   GB_TRY(w.cb.ensure(sizeof(float) * V));
   GB_TRY(w.tail.ensure(sizeof(F2) * ntiles));
   GB_TRY(w.head.ensure(sizeof(F2) * ntiles));
-  GB_TRY(w.pres.ensure(sizeof(double) * std::max(ntiles, vblocks)));
-  GB_TRY(w.pdang.ensure(sizeof(double) * std::max(ntiles, vblocks)));
+  GB_TRY(w.pres.ensure(sizeof(double) * std::max(nparts, vblocks)));
+  GB_TRY(w.pdang.ensure(sizeof(double) * std::max(nparts, vblocks)));
   GB_TRY(w.fres.ensure(sizeof(double) * nfix));
   GB_TRY(w.fdang.ensure(sizeof(double) * nfix));
   GB_TRY(w.state.ensure(sizeof(PrState)));
@@ -668,6 +1034,22 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
                       hipEvent_t ev0, hipEvent_t ev1) {
     RowOut out{0.0f, (float)p->damping, x_old, inv, x_new, c_new};
     if (ev0) (void)hipEventRecord(ev0, st);
+    if (rb) {
+      const RbArgs a{irp, icol, g->pr_rb.as<RowBlock>(), c_old, out, w.tail.as<F2>(),
+                     w.pres.as<double>(), w.pdang.as<double>(), state, ntiles};
+      if (rb == 1024)
+        launch_rb_hot<1024>(hot, a, st);
+      else if (rb == 2048)
+        launch_rb_hot<2048>(hot, a, st);
+      else
+        launch_rb_hot<4096>(hot, a, st);
+      if (ev1) (void)hipEventRecord(ev1, st);
+      hipLaunchKernelGGL(pr_rb_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, nparts,
+                         g->pr_rb_nlong, g->pr_rb_long.as<LongRow>(), (const F2 *)w.tail.as<F2>(),
+                         out, w.pres.as<double>(), w.pdang.as<double>(), w.fres.as<double>(),
+                         w.fdang.as<double>(), w.counter.as<unsigned int>(), state);
+      return;
+    }
     switch (ipt) {
       case 4:
         launch_gather_hot<4>(hot, ntiles, st, irp, icol, trow, tedge, c_old, out, w.tail.as<F2>(),
@@ -771,10 +1153,10 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     res->gather_ms = gather_ms;
     res->gather_launches = gather_launches;
     res->tiles = (int32_t)ntiles;
-    res->items_per_thread = ipt;
+    res->items_per_thread = rb ? 0 : ipt;
     res->hot = hot;
     res->relabeled = relabel ? 1 : 0;
-    res->reserved = 0;
+    res->row_block = rb;
   }
   return GB_OK;
 }

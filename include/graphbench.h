@@ -180,7 +180,9 @@ typedef struct gb_pr_params {
   const float *d_x0; /* optional warm start (device, V; NULL = 1/N) */
   int32_t relabel;   /* gather in out-degree order: 0 auto (E >= 2^20), 1 on, -1 off;
                         built once per graph handle, results in the caller's ids */
-  int32_t reserved;
+  int32_t row_block; /* gather form: 0 autotuned (merge path or row blocks),
+                        1024 / 2048 / 4096 row blocks of that many edges,
+                        -1 merge path only */
 } gb_pr_params;
 
 typedef struct gb_pr_result {
@@ -191,10 +193,10 @@ typedef struct gb_pr_result {
   double gather_ms;      /* summed gather-kernel time (time_kernels=1) */
   int32_t gather_launches;
   int32_t tiles;
-  int32_t items_per_thread; /* gather shape used: tile = 256 x items_per_thread */
+  int32_t items_per_thread; /* merge-path shape used: tile = 256 x items_per_thread */
   int32_t hot;              /* sources served from LDS (0 = none) */
   int32_t relabeled;        /* 1 if the gather ran in out-degree order */
-  int32_t reserved;
+  int32_t row_block;        /* row-block form used: edges per block (0 = merge path) */
 } gb_pr_result;
 
 /* Configured PageRank: in-edge pull gather over the reverse CSR, FP32 ranks
