@@ -140,6 +140,24 @@ struct RowOut {
   }
 };
 
+// s_hot[k] = contrib[k] for k < nh (nh >= 1) by the BT threads of a block:
+// every read issued before the first store, so the copy costs one round trip.
+template <int HOT, int BT>
+__device__ __forceinline__ void fill_hot(float *s_hot, const float *__restrict__ contrib, int nh) {
+  constexpr int PER = (HOT + BT - 1) / BT;
+  float v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = (int)threadIdx.x + j * BT;
+    v[j] = contrib[k < nh ? k : nh - 1];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = (int)threadIdx.x + j * BT;
+    if (k < nh) s_hot[k] = v[j];
+  }
+}
+
 // HOT > 0: the contributions of vertices [0, HOT) -- the highest out-degrees
 // once the graph is relabelled -- are copied into LDS once per block, and the
 // block walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...; gathers of hot
@@ -161,9 +179,8 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_gather_kernel(
   if (st->done) return;
   out.base = st->base;
   const int tid = threadIdx.x;
-  if (HOT > 0) {
-    const int nh = min((int64_t)HOT, (int64_t)st->V);
-    for (int k = tid; k < nh; k += PR_BLOCK) s_hot[k] = contrib[k];
+  if constexpr (HOT > 0) {
+    fill_hot<HOT, PR_BLOCK>(s_hot, contrib, (int)min((int64_t)HOT, (int64_t)st->V));
     __syncthreads();
   }
   for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
@@ -423,7 +440,7 @@ __global__ void pr_ref_update_kernel(int32_t V, const double *__restrict__ recip
 // butterfly merges their partial sums).  No merge-path search, no segmented
 // scan, no cross-tile fix-up for rows that fit a block.  Chunks of long rows
 // leave one partial each; pr_rb_fixup_kernel joins them in chunk order.
-// Residual and dangling partials are per wave (no block barrier).
+// Residual and dangling partials are per wave.
 constexpr int RB_MAXR = 1024;  // rows per block at most
 constexpr int RB_RPT = RB_MAXR / PR_BLOCK;
 
@@ -434,20 +451,26 @@ struct LongRow {
   int32_t row, b0, nb;  // chunks are blocks b0 .. b0 + nb - 1
 };
 
-// Row block bi.  HOT > 0: contributions of sources [0, HOT) come from s_hot.
+// Row block bi, summed by PR_BLOCK threads (tid = thread among them) with the
+// NB floats at s_val.  All reads (column indices, gathers, the rows' pointers,
+// old ranks and inverse out-degrees) are issued before the one barrier, so
+// they are in flight together; every thread of the launch block reaches that
+// barrier exactly once per call, also when the row block is inactive (active
+// false: past the last row block).  HOT > 0: contributions of sources
+// [0, HOT) come from s_hot.
 template <int NB, int HOT>
 __device__ __forceinline__ void rowblock_body(
     const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
     const RowBlock *__restrict__ blocks, const float *__restrict__ contrib, const RowOut &out,
     F2 *__restrict__ chunk_sum, double *__restrict__ part_res, double *__restrict__ part_dang,
-    int64_t bi, const float *s_hot) {
+    int64_t bi, bool active, int tid, float *s_val, const float *s_hot) {
   constexpr int EPT = NB / PR_BLOCK;  // staged edges per thread
-  __shared__ float s_val[NB];
-  __shared__ int32_t s_rp[RB_MAXR + 1];
-  __shared__ F2 s_red[PR_BLOCK / GB_WAVE];
-  const int tid = threadIdx.x, lane = gb_lane();
-  const RowBlock b = blocks[bi];
+  const int lane = gb_lane(), wave = tid / GB_WAVE;
+  RowBlock b = {0, 0, 0, 0};
+  if (active) b = blocks[bi];
   const int nnz = b.e1 - b.e0;
+  const int nrows = b.nrows;
+  const bool chunk = active && nrows == 0;
   // edge values: unconditional reads on clamped indices, one batch each
   // (hot lanes read a dummy address)
   float val[EPT];
@@ -470,7 +493,25 @@ __device__ __forceinline__ void rowblock_body(
       }
     }
   }
-  if (b.nrows == 0) {  // a chunk of a long row: one compensated sum for the block
+  // lanes per row: the largest power of two with nrows * tpr <= 256, at most 64
+  int tpr = 1;
+  while (tpr < GB_WAVE && nrows * tpr * 2 <= PR_BLOCK) tpr <<= 1;
+  // this thread's rows: tid + m * PR_BLOCK (tpr == 1) or tid / tpr; their
+  // edge ranges [lo, hi) and old values, read on clamped rows
+  int32_t lo[RB_RPT], hi[RB_RPT];
+  float xo[RB_RPT], io[RB_RPT];
+  if (nrows > 0) {
+#pragma unroll
+    for (int m = 0; m < RB_RPT; ++m) {
+      const int r = tpr == 1 ? tid + m * PR_BLOCK : tid / tpr;
+      const int rc = r < nrows ? r : nrows - 1;
+      lo[m] = irp[b.r0 + rc];
+      hi[m] = irp[b.r0 + rc + 1];
+      xo[m] = out.x_old[b.r0 + rc];
+      io[m] = out.inv_od[b.r0 + rc];
+    }
+  }
+  if (chunk) {  // a chunk of a long row: one compensated sum for the block
     F2 acc = {0.0f, 0.0f};
 #pragma unroll
     for (int m = 0; m < EPT; ++m) f2_add(acc, tid + m * PR_BLOCK < nnz ? val[m] : 0.0f);
@@ -481,71 +522,44 @@ __device__ __forceinline__ void rowblock_body(
       o.c = __shfl_xor(acc.c, d, GB_WAVE);
       acc = (lane & d) ? f2_merge(o, acc) : f2_merge(acc, o);
     }
-    if (lane == 0) s_red[tid / GB_WAVE] = acc;
-    __syncthreads();
+    if (lane == 0) reinterpret_cast<F2 *>(s_val)[wave] = acc;  // s_val is free here
+  } else if (nrows > 0) {
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+      const int k = tid + m * PR_BLOCK;
+      if (k < nnz) s_val[k] = val[m];
+    }
+  }
+  __syncthreads();
+  if (chunk) {
     if (tid == 0) {
+      const F2 *s_red = reinterpret_cast<const F2 *>(s_val);
       F2 tot = s_red[0];
       for (int w = 1; w < PR_BLOCK / GB_WAVE; w++) tot = f2_merge(tot, s_red[w]);
       chunk_sum[bi] = tot;
     }
     if (lane == 0) {
-      part_res[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = 0.0;
-      part_dang[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = 0.0;
+      part_res[bi * (PR_BLOCK / GB_WAVE) + wave] = 0.0;
+      part_dang[bi * (PR_BLOCK / GB_WAVE) + wave] = 0.0;
     }
     return;
   }
-  const int nrows = b.nrows;
-#pragma unroll
-  for (int m = 0; m < EPT; ++m) {
-    const int k = tid + m * PR_BLOCK;
-    if (k < nnz) s_val[k] = val[m];
-  }
-  {
-    int32_t rp_v[RB_RPT + 1];
-#pragma unroll
-    for (int m = 0; m <= RB_RPT; ++m) {
-      const int k = tid + m * PR_BLOCK;
-      rp_v[m] = irp[b.r0 + (k <= nrows ? k : nrows)];
-    }
-#pragma unroll
-    for (int m = 0; m <= RB_RPT; ++m) {
-      const int k = tid + m * PR_BLOCK;
-      if (k <= nrows) s_rp[k] = rp_v[m] - b.e0;
-    }
-  }
-  __syncthreads();
+  if (nrows == 0) return;  // inactive
   double res = 0.0, dang = 0.0;
-  // lanes per row: the largest power of two with nrows * tpr <= 256, at most 64
-  int tpr = 1;
-  while (tpr < GB_WAVE && nrows * tpr * 2 <= PR_BLOCK) tpr <<= 1;
   if (tpr == 1) {
-    float sum[RB_RPT];
 #pragma unroll
     for (int m = 0; m < RB_RPT; ++m) {
       const int r = tid + m * PR_BLOCK;
       F2 acc = {0.0f, 0.0f};
       if (r < nrows)
-        for (int k = s_rp[r]; k < s_rp[r + 1]; ++k) f2_add(acc, s_val[k]);
-      sum[m] = f2_value(acc);
-    }
-    float xo[RB_RPT], io[RB_RPT];
-#pragma unroll
-    for (int m = 0; m < RB_RPT; ++m) {
-      const int r = tid + m * PR_BLOCK;
-      const int32_t v = b.r0 + (r < nrows ? r : nrows - 1);
-      xo[m] = out.x_old[v];
-      io[m] = out.inv_od[v];
-    }
-#pragma unroll
-    for (int m = 0; m < RB_RPT; ++m) {
-      const int r = tid + m * PR_BLOCK;
-      if (r < nrows) out.put(b.r0 + r, sum[m], xo[m], io[m], res, dang);
+        for (int k = lo[m] - b.e0; k < hi[m] - b.e0; ++k) f2_add(acc, s_val[k]);
+      if (r < nrows) out.put(b.r0 + r, f2_value(acc), xo[m], io[m], res, dang);
     }
   } else {
     const int r = tid / tpr, q = tid % tpr;
     F2 acc = {0.0f, 0.0f};
     if (r < nrows)
-      for (int k = s_rp[r] + q; k < s_rp[r + 1]; k += tpr) f2_add(acc, s_val[k]);
+      for (int k = lo[0] - b.e0 + q; k < hi[0] - b.e0; k += tpr) f2_add(acc, s_val[k]);
     // fixed butterfly over the row's lanes (aligned groups inside one wave)
     for (int d = 1; d < tpr; d <<= 1) {
       F2 o;
@@ -553,15 +567,13 @@ __device__ __forceinline__ void rowblock_body(
       o.c = __shfl_xor(acc.c, d, GB_WAVE);
       acc = (q & d) ? f2_merge(o, acc) : f2_merge(acc, o);
     }
-    const int32_t v = b.r0 + (r < nrows ? r : nrows - 1);
-    const float xo = out.x_old[v], io = out.inv_od[v];
-    if (r < nrows && q == 0) out.put(v, f2_value(acc), xo, io, res, dang);
+    if (r < nrows && q == 0) out.put(b.r0 + r, f2_value(acc), xo[0], io[0], res, dang);
   }
   res = gb_wave_sum_f64(res);
   dang = gb_wave_sum_f64(dang);
   if (lane == 0) {
-    part_res[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = res;
-    part_dang[bi * (PR_BLOCK / GB_WAVE) + tid / GB_WAVE] = dang;
+    part_res[bi * (PR_BLOCK / GB_WAVE) + wave] = res;
+    part_dang[bi * (PR_BLOCK / GB_WAVE) + wave] = dang;
   }
 }
 
@@ -571,31 +583,55 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_rowblock_kernel(
     const RowBlock *__restrict__ blocks, const float *__restrict__ contrib, RowOut out,
     F2 *__restrict__ chunk_sum, double *__restrict__ part_res, double *__restrict__ part_dang,
     const PrState *__restrict__ st) {
+  __shared__ float s_val[NB];
   if (st->done) return;
   out.base = st->base;
   rowblock_body<NB, 0>(irp, icol, blocks, contrib, out, chunk_sum, part_res, part_dang,
-                       (int64_t)blockIdx.x, nullptr);
+                       (int64_t)blockIdx.x, true, threadIdx.x, s_val, nullptr);
+}
+
+// Row blocks per launch block of the hot-set variant: the count that keeps the
+// most waves resident per CU (160 KiB of LDS per CU shared by blocks of
+// HOT + SUBS * NB floats; 24 waves per CU counted at most), the fewest on ties.
+constexpr int rb_subs(int nb, int hot) {
+  int best = 1, best_waves = 0;
+  for (int s = 1; s <= 4; ++s) {
+    const int per_cu = 160 * 1024 / (4 * (hot + s * nb));
+    int waves = per_cu * s * (PR_BLOCK / GB_WAVE);
+    if (waves > 24) waves = 24;
+    if (waves > best_waves) {
+      best_waves = waves;
+      best = s;
+    }
+  }
+  return best;
 }
 
 // Persistent variant with an LDS copy of the hottest HOT contributions (the
-// highest out-degrees once relabelled): blocks walk row blocks blockIdx.x,
-// blockIdx.x + gridDim.x, ...
+// highest out-degrees once relabelled), shared by SUBS row blocks summed side
+// by side (PR_BLOCK threads each): launch block j walks row blocks
+// SUBS * (j + k * gridDim.x) + sub, k = 0, 1, ...
 template <int NB, int HOT>
-__global__ __launch_bounds__(PR_BLOCK) void pr_rowblock_hot_kernel(
+__global__ __launch_bounds__(PR_BLOCK * rb_subs(NB, HOT)) void pr_rowblock_hot_kernel(
     const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
     const RowBlock *__restrict__ blocks, const float *__restrict__ contrib, RowOut out,
     F2 *__restrict__ chunk_sum, double *__restrict__ part_res, double *__restrict__ part_dang,
     const PrState *__restrict__ st, int64_t nblocks) {
+  constexpr int SUBS = rb_subs(NB, HOT);
   __shared__ float s_hot[HOT];
+  __shared__ float s_val[SUBS][NB];
   if (st->done) return;
   out.base = st->base;
-  const int nh = (int)min((int64_t)HOT, (int64_t)st->V);
-  for (int k = threadIdx.x; k < nh; k += PR_BLOCK) s_hot[k] = contrib[k];
+  fill_hot<HOT, PR_BLOCK * SUBS>(s_hot, contrib, (int)min((int64_t)HOT, (int64_t)st->V));
   __syncthreads();
-  for (int64_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+  // wave-uniform sub-block index (every lane of a wave has the same one)
+  const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x / PR_BLOCK);
+  const int tid = (int)threadIdx.x - sub * PR_BLOCK;
+  for (int64_t b0 = (int64_t)blockIdx.x * SUBS; b0 < nblocks; b0 += (int64_t)gridDim.x * SUBS) {
+    const int64_t bi = b0 + sub;
     rowblock_body<NB, HOT>(irp, icol, blocks, contrib, out, chunk_sum, part_res, part_dang, bi,
-                           s_hot);
-    __syncthreads();  // the next row block reuses the LDS staging
+                           bi < nblocks, tid, s_val[sub], s_hot);
+    __syncthreads();  // the next row blocks reuse the LDS staging
   }
 }
 
@@ -764,18 +800,21 @@ void launch_rb(const RbArgs &a, hipStream_t st) {
                        dim3(PR_BLOCK), 0, st, a.irp, a.icol, a.blocks, a.contrib, a.out,
                        a.chunk_sum, a.pres, a.pdang, a.state);
   } else {
+    constexpr int SUBS = rb_subs(NB, HOT);
     static int per_cu = 0;
     if (per_cu == 0) {
       int nb = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pr_rowblock_hot_kernel<NB, HOT>,
-                                                       PR_BLOCK, 0) != hipSuccess ||
+                                                       PR_BLOCK * SUBS, 0) != hipSuccess ||
           nb < 1)
         nb = 1;
       per_cu = nb;
     }
+    const int64_t need = (a.nblocks + SUBS - 1) / SUBS;
     const int64_t grid =
-        std::max<int64_t>(1, std::min<int64_t>(a.nblocks, (int64_t)gb_num_cus() * per_cu));
-    hipLaunchKernelGGL((pr_rowblock_hot_kernel<NB, HOT>), dim3((unsigned)grid), dim3(PR_BLOCK), 0,
+        std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)gb_num_cus() * per_cu));
+    hipLaunchKernelGGL((pr_rowblock_hot_kernel<NB, HOT>), dim3((unsigned)grid),
+                       dim3(PR_BLOCK * SUBS), 0,
                        st, a.irp, a.icol, a.blocks, a.contrib, a.out, a.chunk_sum, a.pres, a.pdang,
                        a.state, a.nblocks);
   }

@@ -373,13 +373,18 @@ def test_pagerank_hubs_span_tiles(gpu):
         assert np.abs(x - ref).sum() <= PR_L1
 
 
+@pytest.mark.parametrize("hot", [0, 4096, 8192])
 @pytest.mark.parametrize("rb", [1024, 2048, 4096])
-def test_pagerank_row_blocks_match_oracle(gpu, rb):
+def test_pagerank_row_blocks_match_oracle(gpu, rb, hot, monkeypatch):
     """The row-block form: rows summed in LDS blocks, rows longer than a block
     split into chunks joined by the fix-up.  Cases: skewed R-MAT (relabelled
     and not), an in-degree hub of 40000 edges, runs of rows without in-edges,
-    an edgeless graph, a cycle (closed form: uniform), and graph replay."""
+    an edgeless graph, a cycle (closed form: uniform), and graph replay.  With
+    a hot set (relabelled graphs), several row blocks share one launch block
+    and the persistent grid walks them in rounds: R-MAT 18 takes several
+    rounds and ends with launch blocks holding fewer row blocks."""
     gb = gpu
+    monkeypatch.setenv("GB_PR_HOT", str(hot))
     rng = np.random.default_rng(rb)
     V = 3000
     src = np.concatenate([rng.integers(0, V, 40000), rng.integers(0, V, 5000)])
@@ -390,10 +395,13 @@ def test_pagerank_row_blocks_match_oracle(gpu, rb):
     graphs = [(gb.gen_rmat(14, 16, seed=21, weights="none"), 1, True),
               (gb.gen_rmat(13, 8, seed=23, weights="none"), -1, True),
               (gb.HostCSR.from_edges(V, src[m], dst[m]), -1, False)]
+    if hot:
+        graphs.append((gb.gen_rmat(18, 16, seed=5, weights="none"), 1, True))
     for h, rl, check_it in graphs:
-        _say(f"pagerank row blocks rb={rb} V={h.V} E={h.E} relabel={rl}")
+        _say(f"pagerank row blocks rb={rb} hot={hot} V={h.V} E={h.E} relabel={rl}")
         x, res, ref, it = _pr_case(gb, h, row_block=rb, relabel=rl)
         assert res["row_block"] == rb and res["items_per_thread"] == 0
+        assert res["hot"] == (hot if rl == 1 else 0)
         assert np.abs(x - ref).sum() <= PR_L1
         assert not check_it or abs(res["iterations"] - it) <= 1
     h = gb.HostCSR(np.zeros(6, np.int32), np.zeros(0, np.int32))
