@@ -122,6 +122,7 @@ _SIGS = {
     "gb_csr_from_edges": (_i32, [_i32, _i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
     "gb_csr_transpose": (_i32, [_i32, _i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
     "gb_edges_unique": (_i32, [_i64, _p, _p, _p, _p, C.POINTER(_i64), _p]),
+    "gb_edge_parts": (_i32, [_i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
     "gb_graph_features": (_i32, [_p, _i32, _p, _p]),
     "gb_pr_warm_start": (_i32, [_p, _p, _p, _p, _p]),
 }
@@ -482,6 +483,23 @@ def edges_unique(src, dst):
     _check(lib().gb_edges_unique(E, _ptr(src), _ptr(dst), _ptr(so), _ptr(do), C.byref(k),
                                  _stream()))
     return so[:k.value], do[:k.value]
+
+
+def edge_parts(src, dst, distrib):
+    """Owners of both endpoints of every device edge under the 1D partition
+    distrib (each part's last vertex; part = the reference's bin_search), and
+    the p x p cross-part edge counts, as chunk_files computes them:
+    (send_part, recv_part, send_ub) device tensors (int32, int32, int64)."""
+    torch = _torch()
+    E = int(src.numel())
+    distrib = torch.as_tensor(distrib, dtype=torch.int64).to("cuda")
+    p = int(distrib.numel())
+    sp = torch.empty(max(E, 1), dtype=torch.int32, device="cuda")
+    rp = torch.empty(max(E, 1), dtype=torch.int32, device="cuda")
+    ub = torch.empty(max(p * p, 1), dtype=torch.int64, device="cuda")
+    _check(lib().gb_edge_parts(E, _ptr(src), _ptr(dst), _ptr(distrib), p, _ptr(sp), _ptr(rp),
+                               _ptr(ub), _stream()))
+    return sp[:E], rp[:E], ub[:p * p].view(p, p)
 
 
 def u32(t) -> np.ndarray:

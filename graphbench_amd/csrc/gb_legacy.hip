@@ -652,6 +652,162 @@ void init_node_id(t_csr *graph, lint *distrib, sint nparts) {
   for (int64_t e = 0; e < E; e++) graph->edge_info[e].node_id = out[e];
 }
 
+void write_recip(char *file, char *dir, lint *distrib, t_csr *graph_send) {
+  // partition/csr_partition.c:133-164: part j's file
+  // <dir><par_extend><file>_recip_<j>_<nodes_extension> holds "%f " of the
+  // recip of its vertices (the walk moves to the next part after vertex
+  // distrib[j]), then "\n", every distrib entry as "%lld ", "\n", and a
+  // closing "\n"
+  const int p = num_nodes;
+  if (p <= 0) fatal("write_recip: num_nodes <= 0");
+  if (!nodes_extension) fatal("write_recip: nodes_extension is not set");
+  std::vector<FILE *> w(p, nullptr);
+  char buf[4096];
+  for (int i = 0; i < p; i++) {
+    snprintf(buf, sizeof(buf), "%s%s%s_recip_%d_%s", dir, par_extend, file, i, nodes_extension);
+    if ((w[i] = fopen(buf, "w")) == nullptr) {
+      printf("the file does not exists!\n");
+      fatal("write_recip: cannot open an output file");
+    }
+  }
+  lint j = 0;
+  for (lint i = 0; i < graph_send->v_size; i++) {
+    if (j >= p) fatal("write_recip: vertex past the last part (distrib[p-1] < v_size-1)");
+    fprintf(w[j], "%f ", graph_send->vet_info[i].recip);
+    if (i == distrib[j]) j++;
+  }
+  for (int i = 0; i < p; i++) {
+    fprintf(w[i], "\n");
+    for (int k = 0; k < p; k++) fprintf(w[i], "%lld ", (long long)distrib[k]);
+    fprintf(w[i], "\n");
+  }
+  for (int i = 0; i < p; i++) {
+    fprintf(w[i], "\n");
+    fclose(w[i]);
+  }
+}
+
+void chunk_files(char *file, char *dir, lint *distrib, lint *send_size, lint *recv_size) {
+  // partition/csr_partition.c:32-125: every edge line "a u v w" of
+  // <dir><file> goes, in file order, to the send file of its source's part
+  // (<dir><data_extend><file>_sd_<i>_<nodes_extension>) and the recv file of
+  // its target's part (_rc_), each headed by "p sp size edges off V" at every
+  // 'p' line; the counts of edges whose two parts differ go to
+  // <dir><par_extend><file>_send_<nodes_extension> as a p x p matrix.  The
+  // owners and counts are computed on the device (gb_edge_parts), batch by
+  // batch between 'p' lines; the text is written here.
+  const int p = num_nodes;
+  if (p <= 0 || p > GB_MAX_PARTS) fatal("chunk_files: num_nodes outside [1, 4096]");
+  if (!nodes_extension) fatal("chunk_files: nodes_extension is not set");
+  char buf[4096];
+  snprintf(buf, sizeof(buf), "%s%s%s_send_%s", dir, par_extend, file, nodes_extension);
+  FILE *send_f = fopen(buf, "w");
+  if (!send_f) {
+    printf("the file %s does not exists!\n", buf);
+    fatal("chunk_files: cannot open the send file");
+  }
+  std::vector<FILE *> ws(p, nullptr), wr(p, nullptr);
+  for (int i = 0; i < p; i++) {
+    snprintf(buf, sizeof(buf), "%s%s%s_sd_%d_%s", dir, data_extend, file, i, nodes_extension);
+    if ((ws[i] = fopen(buf, "w")) == nullptr) {
+      printf("the file you input does not exist!");
+      fatal("chunk_files: cannot open a part file");
+    }
+    snprintf(buf, sizeof(buf), "%s%s%s_rc_%d_%s", dir, data_extend, file, i, nodes_extension);
+    if ((wr[i] = fopen(buf, "w")) == nullptr) {
+      printf("the file you input does not exist!");
+      fatal("chunk_files: cannot open a part file");
+    }
+  }
+  snprintf(buf, sizeof(buf), "%s%s", dir, file);
+  FILE *in = fopen(buf, "r");
+  if (!in) {
+    printf("the file you input does not exist!");
+    fatal("chunk_files: cannot open the graph file");
+  }
+  std::vector<int64_t> dist(distrib, distrib + p);
+  DevBuf d_dist, d_u, d_v, d_sp, d_rp, d_ub;
+  upload(d_dist, dist);
+  check(d_ub.ensure(sizeof(unsigned long long) * (size_t)p * p), "device allocation");
+  std::vector<unsigned long long> send_ub((size_t)p * p, 0ull), ub((size_t)p * p);
+  // edges read since the last 'p' line: as written (u, v, w) and the
+  // endpoints the parts are found for (u - 1, v - offset)
+  std::vector<long long> eu, ev, ew;
+  std::vector<int32_t> su, sv;
+  auto flush = [&]() {
+    const int64_t n = (int64_t)eu.size();
+    if (n == 0) return;
+    upload(d_u, su);
+    upload(d_v, sv);
+    check(d_sp.ensure(sizeof(int32_t) * n), "device allocation");
+    check(d_rp.ensure(sizeof(int32_t) * n), "device allocation");
+    check(gb_edge_parts(n, d_u.as<int32_t>(), d_v.as<int32_t>(), d_dist.as<int64_t>(), p,
+                        d_sp.as<int32_t>(), d_rp.as<int32_t>(), d_ub.as<unsigned long long>(),
+                        nullptr),
+          "chunk_files");
+    std::vector<int32_t> sp(n), rp(n);
+    hipcheck(hipMemcpy(sp.data(), d_sp.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost), "download");
+    hipcheck(hipMemcpy(rp.data(), d_rp.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost), "download");
+    hipcheck(hipMemcpy(ub.data(), d_ub.p, sizeof(unsigned long long) * ub.size(),
+                       hipMemcpyDeviceToHost),
+             "download");
+    for (size_t k = 0; k < ub.size(); k++) send_ub[k] += ub[k];
+    for (int64_t e = 0; e < n; e++) {
+      fprintf(ws[sp[e]], "a %lld %lld %lld\n", eu[e], ev[e], ew[e]);
+      fprintf(wr[rp[e]], "a %lld %lld %lld\n", eu[e], ev[e], ew[e]);
+    }
+    eu.clear();
+    ev.clear();
+    ew.clear();
+    su.clear();
+    sv.clear();
+  };
+  Tok t{in};
+  char str[128];
+  long long v_num = 0, e_num = 0, offset = 0, read = 0;
+  bool header = false;
+  while (t.next(str, sizeof(str))) {
+    if (str[0] == 'p' && str[1] == '\0') {
+      char kind[128];
+      // "%s %lld %lld %lld": the fifth header field is read as a skipped token
+      if (!t.next(kind, sizeof(kind)) || !t.num(&v_num) || !t.num(&e_num) || !t.num(&offset))
+        fatal("chunk_files: bad 'p' line");
+      flush();
+      for (int i = 0; i < p; i++) {
+        const long long size = i == 0 ? distrib[0] + 1 : distrib[i] - distrib[i - 1];
+        const long long off = (i == 0 ? 0 : distrib[i - 1] + 1) + 1;
+        fprintf(ws[i], "p sp %lld %lld %lld %lld\n", size, (long long)send_size[i], off, v_num);
+        fprintf(wr[i], "p sp %lld %lld %lld %lld\n", size, (long long)recv_size[i], off, v_num);
+      }
+      header = true;
+    } else if (str[0] == 'a' && str[1] == '\0') {
+      long long u, v, w;
+      if (!header) fatal("chunk_files: 'a' line before the 'p' line");
+      if (!t.num(&u) || !t.num(&v) || !t.num(&w)) fatal("chunk_files: bad 'a' line");
+      const long long x = u - 1, y = v - offset;
+      if (x < INT32_MIN || x > INT32_MAX || y < INT32_MIN || y > INT32_MAX)
+        fatal("chunk_files: vertex id outside the int32 range");
+      eu.push_back(u);
+      ev.push_back(v);
+      ew.push_back(w);
+      su.push_back((int32_t)x);
+      sv.push_back((int32_t)y);
+      if (++read == e_num) break;
+    }
+  }
+  flush();
+  for (int i = 0; i < p; i++) {
+    for (int k = 0; k < p; k++) fprintf(send_f, "%lld ", (long long)send_ub[(size_t)i * p + k]);
+    fprintf(send_f, "\n");
+  }
+  for (int i = 0; i < p; i++) {
+    fclose(ws[i]);
+    fclose(wr[i]);
+  }
+  fclose(in);
+  fclose(send_f);
+}
+
 void remove_duplicate(char *file, char *dir, sint myrank) {
   // partition/csr_partition_bin_mpi.c:31-106: <dir><file>_<rank> holds the
   // edge count (lint) and (from, to) lint pairs; the pairs without repeats,

@@ -347,6 +347,80 @@ gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib,
 }
 
 // ---------------------------------------------------------------------------
+// edge owners and cross-part counts (the per-edge work of chunk_files,
+// partition/csr_partition.c:99-108)
+
+constexpr int EP_LDS_CELLS = 8192;  // p * p counters held in LDS per block at most
+
+__global__ void fill_u64_kernel(unsigned long long *a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0ull;
+}
+
+// One thread per edge (grid-stride).  Parts come from the reference's
+// bin_search, which stays in [0, p-1] for any distrib.  With
+// p * p <= EP_LDS_CELLS the block counts in LDS and adds its counts to the
+// global ones once.
+__global__ __launch_bounds__(256) void edge_parts_kernel(
+    int64_t E, const int32_t *__restrict__ src, const int32_t *__restrict__ dst,
+    const int64_t *__restrict__ distrib, int32_t p, int32_t *__restrict__ send_part,
+    int32_t *__restrict__ recv_part, unsigned long long *__restrict__ ub) {
+  __shared__ unsigned int s_cnt[EP_LDS_CELLS];
+  const int64_t cells = (int64_t)p * p;
+  const bool in_lds = ub && cells <= EP_LDS_CELLS;
+  if (in_lds) {
+    for (int64_t k = threadIdx.x; k < cells; k += blockDim.x) s_cnt[k] = 0u;
+    __syncthreads();
+  }
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = gb_bin_search(distrib, src[e], 0, p - 1);
+    const int64_t r = gb_bin_search(distrib, dst[e], 0, p - 1);
+    if (send_part) send_part[e] = (int32_t)s;
+    if (recv_part) recv_part[e] = (int32_t)r;
+    if (ub && s != r) {
+      if (in_lds)
+        atomicAdd(&s_cnt[s * p + r], 1u);
+      else
+        atomicAdd(&ub[s * p + r], 1ull);
+    }
+  }
+  if (in_lds) {
+    __syncthreads();
+    for (int64_t k = threadIdx.x; k < cells; k += blockDim.x)
+      if (s_cnt[k]) atomicAdd(&ub[k], (unsigned long long)s_cnt[k]);
+  }
+}
+
+extern "C" gb_status gb_edge_parts(int64_t E, const int32_t *d_src, const int32_t *d_dst,
+                                   const int64_t *d_distrib, int32_t p, int32_t *d_send_part,
+                                   int32_t *d_recv_part, unsigned long long *d_send_ub,
+                                   void *stream) {
+  if (E < 0 || p < 1 || p > GB_MAX_PARTS)
+    return gb_fail(GB_ERR_INVALID, "gb_edge_parts: need E >= 0 and 1 <= p <= %d", GB_MAX_PARTS);
+  if (E > 0 && (!d_src || !d_dst || !d_distrib))
+    return gb_fail(GB_ERR_INVALID, "gb_edge_parts: null edge or distrib array");
+  hipStream_t st = gb_stream(stream);
+  const int64_t cells = (int64_t)p * p;
+  if (d_send_ub) {
+    const int64_t fb = std::min<int64_t>((cells + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)fb), dim3(256), 0, st, d_send_ub, cells);
+    GB_HIP(hipGetLastError());
+  }
+  if (E == 0) return GB_OK;
+  // enough blocks to fill the chip; with LDS counts, few enough that each
+  // block's flush of p * p counters stays small against its edges
+  int64_t blocks = (E + 255) / 256;
+  const int64_t cap = cells <= EP_LDS_CELLS ? (int64_t)gb_num_cus() * 8 : 65536;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(edge_parts_kernel, dim3((unsigned)blocks), dim3(256), 0, st, E, d_src, d_dst,
+                     d_distrib, p, d_send_part, d_recv_part, d_send_ub);
+  GB_HIP(hipGetLastError());
+  return GB_OK;
+}
+
+// ---------------------------------------------------------------------------
 // max weight
 
 // max over a[0..n): wave, then block reduction, one atomicMax per block

@@ -60,6 +60,8 @@ def lib():
             "comp_distrib_csr": [P(TCsr), P(TCsr), P(lint), P(lint), P(lint), sint],
             "init_node_id": [P(TCsr), P(lint), sint],
             "remove_duplicate": [C.c_char_p, C.c_char_p, sint],
+            "write_recip": [C.c_char_p, C.c_char_p, P(lint), P(TCsr)],
+            "chunk_files": [C.c_char_p, C.c_char_p, P(lint), P(lint), P(lint)],
         }.items():
             f = getattr(L, name)
             f.restype = None

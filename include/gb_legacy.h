@@ -115,6 +115,20 @@ void comp_distrib_csr(t_csr *graph_send, t_csr *graph_recv, lint *distrib,
 /* init_node_id, partition/csr_partition.c:127-131: owner part of every edge
  * target (bin_search of utils.c:222-239), on the GPU. */
 void init_node_id(t_csr *graph, lint *distrib, sint num_nodes);
+/* Replace write_recip and chunk_files, partition/csr_partition.c:133-164 and
+ * 32-125 (csr_partition.h), the file writers of the serial chunking step
+ * (main.c:116-136: scan_csr_idx SEND and RECV, comp_distrib_csr, write_recip,
+ * chunk_files).  write_recip writes
+ * <dir><par_extend><file>_recip_<j>_<nodes_extension> per part j: "%f " of
+ * each of its vertices' recip, then the distrib line.  chunk_files splits the
+ * text graph <dir><file> into <dir><data_extend><file>_sd_<j>_<nodes_extension>
+ * (edges by source part) and _rc_ (by target part) in file order, and writes
+ * the cross-part edge counts to <dir><par_extend><file>_send_<nodes_extension>.
+ * The edge owners and counts are computed on the GPU (gb_edge_parts).  Both
+ * need num_nodes and nodes_extension set as main.c does for -n; the files are
+ * byte-identical to the reference's.  Endpoint ids must fit int32. */
+void write_recip(char *file, char *dir, lint *distrib, t_csr *graph_send);
+void chunk_files(char *file, char *dir, lint *distrib, lint *send_size, lint *recv_size);
 /* Replaces remove_duplicate, partition/csr_partition_bin_mpi.c:31-106
  * (csr_partition_bin_mpi.h): reads <dir><file>_<myrank> (lint edge count,
  * then (from, to) lint pairs), drops repeated pairs on the GPU (first

@@ -91,6 +91,21 @@ gb_status gb_merge_path_split(int32_t V, const int32_t *d_row_ptr, int64_t items
  * (same cut rule as the reference).  d_distrib has p entries (int64). */
 gb_status gb_edge_balanced_parts(int32_t V, const int32_t *d_row_ptr,
                                  int32_t p, int64_t *d_distrib, void *stream);
+/* The per-edge work of chunk_files (partition/csr_partition.c:99-108):
+ * owners of both endpoints of every edge under a 1D vertex partition, and the
+ * cross-part edge counts written to the <file>_send_<nodes> file.  d_distrib
+ * holds each part's last vertex (comp_distrib_csr's output); an endpoint x
+ * (0-based) belongs to part bin_search(distrib, x, 0, p-1) (utils/utils.c:
+ * 222-239).  Writes d_send_part[e] (owner of d_src[e]) and d_recv_part[e]
+ * (owner of d_dst[e]) when non-NULL, and, when d_send_ub is non-NULL,
+ * overwrites its p*p counters with the number of edges whose source part s
+ * differs from their target part r, at d_send_ub[s*p + r].  Parts lie in
+ * [0, p-1] for any distrib, ascending or not.  1 <= p <= GB_MAX_PARTS. */
+#define GB_MAX_PARTS 4096
+gb_status gb_edge_parts(int64_t E, const int32_t *d_src, const int32_t *d_dst,
+                        const int64_t *d_distrib, int32_t p, int32_t *d_send_part,
+                        int32_t *d_recv_part, unsigned long long *d_send_ub,
+                        void *stream);
 
 /* ---- CSR construction on the device -----------------------------------------
  * Replace the host CSR builds of the reference's readers (scan_csr_idx

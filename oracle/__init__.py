@@ -63,6 +63,7 @@ def _declare(L):
         "orc_separate_csr": (None, [_i32, _p, _p, _p, _u32, _p, _p, _p, _p, _p, _p]),
         "orc_separate_csr_f64": (None, [_i32, _p, _p, _p, _f64, _p, _p, _p, _p, _p, _p]),
         "orc_remove_duplicate": (_i64, [_i64, _p, _p]),
+        "orc_edge_parts": (None, [_i64, _p, _p, _p, _i32, _p, _p, _p]),
         "orc_mlp_loss_grad": (_f64, [_i32, _i32, _i32, _i32] + [_p] * 14),
         "orc_set_threads": (None, [_i32]),
         "orc_get_threads": (_i32, []),
@@ -190,6 +191,106 @@ def merge_path_split(rp, items, ntiles):
     te = np.empty(ntiles + 1, np.int32)
     lib().orc_merge_path_split(V, _ptr(rp), int(items), int(ntiles), _ptr(tr), _ptr(te))
     return tr, te
+
+
+def edge_parts(src, dst, distrib):
+    """Per-edge (send part, recv part) and the p x p cross-part counts of
+    chunk_files: (sp int32[E], rp int32[E], ub int64[p, p])."""
+    src = _c(src, np.int32)
+    dst = _c(dst, np.int32)
+    distrib = _c(distrib, np.int64)
+    p = len(distrib)
+    sp = np.empty(len(src), np.int32)
+    rp = np.empty(len(src), np.int32)
+    ub = np.empty(p * p, np.int64)
+    lib().orc_edge_parts(len(src), _ptr(src), _ptr(dst), _ptr(distrib), p, _ptr(sp), _ptr(rp),
+                         _ptr(ub))
+    return sp, rp, ub.reshape(p, p)
+
+
+def chunk_files(text, file, p):
+    """The serial chunking step of main.c:116-136 on a text graph held in
+    `text` (the file <dir><file>), with num_nodes = p: scan_csr_idx SEND and
+    RECV (graph/csr.c:133-202), comp_distrib_csr SEND
+    (partition/csr_partition.c:5-30), write_recip (133-164) and chunk_files
+    (32-125).  Returns {path relative to dir: file text}.  Pure Python over
+    the tokens, for small graphs."""
+    tok = text.split()
+
+    def scan(kind):  # degrees and header, as scan_csr_idx reads them
+        deg, hdr, i, n = None, None, 0, 0
+        while i < len(tok):
+            t = tok[i]
+            i += 1
+            if t == "p":
+                v_num, e_num, off = int(tok[i + 1]), int(tok[i + 2]), int(tok[i + 3])
+                i += 5
+                hdr = (v_num, e_num, off)
+                deg = [0] * v_num
+            elif t == "a":
+                u, v = int(tok[i]), int(tok[i + 1])
+                i += 3
+                deg[u - hdr[2] if kind == "send" else v - hdr[2]] += 1
+                n += 1
+                if n == hdr[1]:
+                    break
+        return deg, hdr
+
+    sdeg, (V, E, _) = scan("send")
+    rdeg, _ = scan("recv")
+    # comp_distrib_csr (SEND): cut on the send degrees at E // p
+    distrib, ss, rs, j = [0] * p, [0] * p, [0] * p, 0
+    for i in range(V):
+        ss[j] += sdeg[i]
+        rs[j] += rdeg[i]
+        if ss[j] > E // p:
+            distrib[j] = i
+            j += 1
+    distrib[p - 1] = V - 1
+    assert distrib == comp_distrib(sdeg, E, p).tolist()
+    ext = str(p)
+    out = {}
+    # write_recip
+    rec = [""] * p
+    j = 0
+    for i in range(V):
+        rec[j] += "%f " % (1.0 / sdeg[i] if sdeg[i] else float("inf"))
+        if i == distrib[j]:
+            j += 1
+    tail = "\n" + "".join("%d " % d for d in distrib) + "\n\n"
+    for k in range(p):
+        out[f"parex/{file}_recip_{k}_{ext}"] = rec[k] + tail
+    # chunk_files
+    sd, rc = [""] * p, [""] * p
+    ub = np.zeros((p, p), np.int64)
+    i, n, off, v_num, e_num = 0, 0, 0, 0, 0
+    while i < len(tok):
+        t = tok[i]
+        i += 1
+        if t == "p":
+            v_num, e_num, off = int(tok[i + 1]), int(tok[i + 2]), int(tok[i + 3])
+            i += 4
+            for k in range(p):
+                size = distrib[0] + 1 if k == 0 else distrib[k] - distrib[k - 1]
+                o = (0 if k == 0 else distrib[k - 1] + 1) + 1
+                sd[k] += f"p sp {size} {ss[k]} {o} {v_num}\n"
+                rc[k] += f"p sp {size} {rs[k]} {o} {v_num}\n"
+        elif t == "a":
+            u, v, w = int(tok[i]), int(tok[i + 1]), int(tok[i + 2])
+            i += 3
+            s_, r_, c = edge_parts([u - 1], [v - off], distrib)
+            ub += c
+            sd[int(s_[0])] += f"a {u} {v} {w}\n"
+            rc[int(r_[0])] += f"a {u} {v} {w}\n"
+            n += 1
+            if n == e_num:
+                break
+    for k in range(p):
+        out[f"datex/{file}_sd_{k}_{ext}"] = sd[k]
+        out[f"datex/{file}_rc_{k}_{ext}"] = rc[k]
+    out[f"parex/{file}_send_{ext}"] = "".join(
+        "".join("%d " % x for x in row) + "\n" for row in ub.tolist())
+    return out
 
 
 def remove_duplicate(pairs):

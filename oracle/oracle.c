@@ -678,6 +678,22 @@ void orc_separate_csr(int32_t V, const int32_t *rp, const int32_t *col,
   }
 }
 
+/* chunk_files (partition/csr_partition.c:99-108): send_id and recv_id by
+ * bin_search(distrib, x, 0, num_nodes-1), send_ub[send_id][recv_id]++ when
+ * they differ. */
+void orc_edge_parts(int64_t E, const int32_t *src, const int32_t *dst,
+                    const int64_t *distrib, int32_t p, int32_t *sp, int32_t *rp,
+                    int64_t *ub) {
+  for (int64_t k = 0; k < (int64_t)p * p; k++) ub[k] = 0;
+  for (int64_t e = 0; e < E; e++) {
+    const int64_t s = orc_bin_search(distrib, src[e], 0, p - 1);
+    const int64_t r = orc_bin_search(distrib, dst[e], 0, p - 1);
+    sp[e] = (int32_t)s;
+    rp[e] = (int32_t)r;
+    if (s != r) ub[s * p + r]++;
+  }
+}
+
 int64_t orc_remove_duplicate(int64_t E, const int64_t *pairs, int64_t *out) {
   /* open-addressing set of seen (from, to) pairs; slot holds index + 1 */
   int64_t cap = 16;

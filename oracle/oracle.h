@@ -118,6 +118,14 @@ void orc_separate_csr(int32_t V, const int32_t *rp, const int32_t *col,
  * pairs and out hold 2*E values; returns the number of pairs kept. */
 int64_t orc_remove_duplicate(int64_t E, const int64_t *pairs, int64_t *out);
 
+/* The per-edge work of chunk_files (partition/csr_partition.c:99-108): the
+ * part of each edge's source (sp) and target (rp) by bin_search over the p
+ * entries of distrib, and ub[s*p + r] = number of edges whose parts s != r
+ * (ub is zeroed first). */
+void orc_edge_parts(int64_t E, const int32_t *src, const int32_t *dst,
+                    const int64_t *distrib, int32_t p, int32_t *sp, int32_t *rp,
+                    int64_t *ub);
+
 /* Same on f64 weights (the reference's edge_weight type). */
 void orc_separate_csr_f64(int32_t V, const int32_t *rp, const int32_t *col,
                           const double *w, double delta, int32_t *lrp, int32_t *lcol,

@@ -14,6 +14,9 @@ outputs, as JSON under tests/golden/.  Cases:
                    the reference CLI -y (remove_duplicate,
                    csr_partition_bin_mpi.c:31-106): binary edge files in, the
                    deduplicated file's bytes out
+  chunk.json       the reference CLI -k (write_recip + chunk_files,
+                   csr_partition.c:32-164) on text graphs: every recip, part
+                   and send file it writes
 Arguments: fixture names to (re)write; none = all.
 """
 from __future__ import annotations
@@ -212,6 +215,46 @@ def bc_cases():
     return out
 
 
+def chunk_cases():
+    """The reference CLI -k (main.c:116-136: scan_csr_idx SEND/RECV,
+    comp_distrib_csr, write_recip, chunk_files) on text graphs, num_nodes p:
+    every file it writes, as text."""
+    out = []
+    rng = np.random.default_rng(41)
+    graphs = []
+    # hand-made: weights kept as written, a vertex without out-edges (inf recip)
+    graphs.append(("hand", "p sp 6 8 1 6\na 1 2 5\na 1 3 1\na 2 3 2\na 3 1 4\n"
+                           "a 4 5 1\na 5 6 3\na 6 1 1\na 6 4 2\n", (1, 2, 3)))
+    # a comment line first, and edge lines past the header's count (not read)
+    graphs.append(("extra", "c made for the chunking test 10 20\np sp 5 4 1 5\na 1 2 7\n"
+                            "a 2 3 8\na 3 4 9\na 4 5 10\na 5 1 11\na 1 5 12\n", (1, 2)))
+    for V, E in [(20, 60), (40, 200)]:
+        u = rng.integers(0, V, E)
+        v = rng.integers(0, V, E)
+        w = rng.integers(1, 100, E)
+        lines = [f"p sp {V} {E} 1 {V}"] + [f"a {a + 1} {b + 1} {c}" for a, b, c in zip(u, v, w)]
+        graphs.append((f"rand{V}", "\n".join(lines) + "\n", (1, 2, 3, 4)))
+    ur, vr = rmat_edges(8, 8, 3)
+    lines = [f"p sp 256 {len(ur)} 1 256"] + [f"a {a + 1} {b + 1} 1" for a, b in zip(ur, vr)]
+    graphs.append(("rmat8", "\n".join(lines) + "\n", (1, 4, 7)))
+    for name, text, parts in graphs:
+        for p in parts:
+            with tempfile.TemporaryDirectory() as td:
+                g = os.path.join(td, "g") + "/"
+                for sub in ("datex", "parex", "log"):
+                    os.makedirs(g + sub)
+                with open(g + "g.txt", "w") as f:
+                    f.write(text)
+                run(["cli", "-k", "-f", "g.txt", "-b", g, "-n", str(p)], cwd=td)
+                files = {}
+                for sub in ("datex", "parex"):
+                    for fn in sorted(os.listdir(g + sub)):
+                        with open(g + sub + "/" + fn) as f:
+                            files[sub + "/" + fn] = f.read()
+            out.append(dict(name=name, num_nodes=p, text=text, files=files))
+    return out
+
+
 def remove_duplicate_cases():
     out = []
     rng = np.random.default_rng(31)
@@ -252,6 +295,7 @@ def main():
         "pagerank.json": pagerank_cases,
         "bc.json": bc_cases,
         "remove_duplicate.json": remove_duplicate_cases,
+        "chunk.json": chunk_cases,
     }
     wanted = sys.argv[1:] or list(builders)
     fixtures = {name: builders[name]() for name in wanted}

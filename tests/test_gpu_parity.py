@@ -567,6 +567,44 @@ def test_edges_unique_rejects_negative_ids(gpu):
         gb.edges_unique(gb.to_device(src), gb.to_device(dst))
 
 
+@pytest.mark.parametrize("E,V,p", [(0, 10, 3), (1, 10, 1), (3_000_000, 1 << 20, 2),
+                                   (3_000_000, 1 << 20, 64), (1_000_000, 1 << 16, 91),
+                                   (200_000, 1 << 16, 4096)])
+def test_edge_parts_vs_oracle(gpu, E, V, p):
+    """Owners of both endpoints and the cross-part counts of chunk_files.
+    p = 91 and 4096 exceed the per-block LDS counts (global atomics)."""
+    gb = gpu
+    rng = np.random.default_rng(E + p)
+    _say(f"edge_parts E={E} V={V} p={p}")
+    cuts = np.sort(rng.choice(np.arange(V - 1), p - 1, replace=False)) if p > 1 else []
+    distrib = np.append(np.asarray(cuts, np.int64), V - 1)
+    src = rng.integers(0, V, E).astype(np.int32)
+    dst = rng.integers(0, V, E).astype(np.int32)
+    sp, rp, ub = gb.edge_parts(gb.to_device(src), gb.to_device(dst), distrib)
+    osp, orp, oub = oracle.edge_parts(src, dst, distrib)
+    assert np.array_equal(sp.cpu().numpy(), osp)
+    assert np.array_equal(rp.cpu().numpy(), orp)
+    assert np.array_equal(ub.cpu().numpy(), oub)
+
+
+def test_edge_parts_unordered_distrib_and_bad_p(gpu):
+    """A distrib with parts left at 0 (comp_distrib_csr cut fewer parts than
+    num_nodes): bin_search's answers as the oracle gives them; p outside
+    [1, 4096] is rejected."""
+    gb = gpu
+    rng = np.random.default_rng(3)
+    distrib = np.array([40, 0, 0, 99], np.int64)
+    src = rng.integers(0, 100, 5000).astype(np.int32)
+    dst = rng.integers(0, 100, 5000).astype(np.int32)
+    sp, rp, ub = gb.edge_parts(gb.to_device(src), gb.to_device(dst), distrib)
+    osp, orp, oub = oracle.edge_parts(src, dst, distrib)
+    assert np.array_equal(sp.cpu().numpy(), osp) and np.array_equal(rp.cpu().numpy(), orp)
+    assert np.array_equal(ub.cpu().numpy(), oub)
+    for bad in (np.zeros(0, np.int64), np.arange(4097, dtype=np.int64)):
+        with pytest.raises(gb.GBError):
+            gb.edge_parts(gb.to_device(src), gb.to_device(dst), bad)
+
+
 def _hub_graph(gb):
     """Empty rows, a 5000-edge hub spanning many edge tiles, short rows."""
     rng = np.random.default_rng(5)

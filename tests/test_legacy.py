@@ -2,7 +2,8 @@
 
 CPU part: the readers (host I/O, no GPU).  GPU part (marked gpu): run_sspath_arr,
 run_sspath, run_pagerank_csr_mpi, run_bc, separate_csr, comp_distrib_csr,
-init_node_id against the oracle and the reference's golden vectors.
+init_node_id, remove_duplicate, write_recip and chunk_files against the oracle
+and the reference's golden vectors and files.
 """
 import ctypes as C
 import json
@@ -257,6 +258,47 @@ def test_remove_duplicate_vs_reference(gpu, case, tmp_path):
     with open(d + f"datex/edges_phy_0_{case['num_nodes']}", "rb") as f:
         got = np.frombuffer(f.read(), np.int64)
     assert got.tolist() == case["output"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", load("chunk.json"), ids=lambda c: f"{c['name']}-n{c['num_nodes']}")
+def test_chunk_pipeline_vs_reference(gpu, case, tmp_path):
+    """The serial chunking step as main.c -k runs it (scan_csr_idx SEND and
+    RECV, comp_distrib_csr, write_recip, chunk_files) writes files
+    byte-identical to the reference's."""
+    L = legacy()
+    lib = L.lib()
+    d = str(tmp_path) + "/"
+    for sub in ("datex", "parex"):
+        os.makedirs(d + sub)
+    with open(d + "g.txt", "w") as f:
+        f.write(case["text"])
+    p = case["num_nodes"]
+    ext = C.create_string_buffer(str(p).encode())
+    C.c_int.in_dll(lib, "num_nodes").value = p
+    C.c_void_p.in_dll(lib, "nodes_extension").value = C.addressof(ext)
+    print(f"[case] chunk {case['name']} p={p}", flush=True)
+    try:
+        gs, gr = L.TCsr(), L.TCsr()
+        lib.scan_csr_idx(C.byref(gs), b"g.txt", d.encode(), L.SEND)
+        lib.scan_csr_idx(C.byref(gr), b"g.txt", d.encode(), L.RECV)
+        dist = (C.c_longlong * p)()
+        ss = (C.c_longlong * p)()
+        rs = (C.c_longlong * p)()
+        lib.comp_distrib_csr(C.byref(gs), C.byref(gr), dist, ss, rs, L.SEND)
+        lib.write_recip(b"g.txt", d.encode(), dist, C.byref(gs))
+        lib.chunk_files(b"g.txt", d.encode(), dist, ss, rs)
+    finally:
+        C.c_int.in_dll(lib, "num_nodes").value = 1
+        C.c_void_p.in_dll(lib, "nodes_extension").value = None
+    got = {}
+    for sub in ("datex", "parex"):
+        for fn in os.listdir(d + sub):
+            with open(d + sub + "/" + fn) as f:
+                got[sub + "/" + fn] = f.read()
+    assert sorted(got) == sorted(case["files"])
+    for name, text in case["files"].items():
+        assert got[name] == text, name
 
 
 @pytest.mark.gpu

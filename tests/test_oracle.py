@@ -139,6 +139,37 @@ def test_remove_duplicate_brute_force():
         assert oracle.remove_duplicate(p).tolist() == [list(x) for x in ref]
 
 
+@pytest.mark.parametrize("case", load("chunk.json"), ids=lambda c: f"{c['name']}-n{c['num_nodes']}")
+def test_chunk_files_vs_reference(case):
+    """The oracle's chunking step (degrees, distrib, write_recip, chunk_files)
+    writes exactly the files the reference CLI -k wrote."""
+    got = oracle.chunk_files(case["text"], "g.txt", case["num_nodes"])
+    assert sorted(got) == sorted(case["files"])
+    for name, text in case["files"].items():
+        assert got[name] == text, name
+
+
+def test_edge_parts_lower_bound_and_counts():
+    """bin_search over an ascending distrib is the first part whose last
+    vertex is >= x (lower bound); the cross-part counts are a 2-D histogram
+    that skips the diagonal and sums to the number of cut edges."""
+    rng = np.random.default_rng(17)
+    for V, E, p in ((10, 0, 3), (1000, 5000, 1), (1000, 5000, 2), (5000, 20000, 7),
+                    (300, 4000, 64)):
+        cuts = np.sort(rng.choice(np.arange(V - 1), p - 1, replace=False)) if p > 1 else []
+        distrib = np.append(np.asarray(cuts, np.int64), V - 1)
+        src = rng.integers(0, V, E).astype(np.int32)
+        dst = rng.integers(0, V, E).astype(np.int32)
+        sp, rp, ub = oracle.edge_parts(src, dst, distrib)
+        assert np.array_equal(sp, np.searchsorted(distrib, src, side="left"))
+        assert np.array_equal(rp, np.searchsorted(distrib, dst, side="left"))
+        ref = np.zeros((p, p), np.int64)
+        np.add.at(ref, (sp, rp), 1)
+        np.fill_diagonal(ref, 0)
+        assert np.array_equal(ub, ref)
+        assert ub.sum() == int((sp != rp).sum())
+
+
 def parse_rc(text):
     """The reference's chunked RECV file (p sp V E off gV / a u v w) -> in-CSR
     in file order, as scan_csr_idx + read_graph_csr (RECV) build it."""
