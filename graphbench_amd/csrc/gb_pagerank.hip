@@ -457,13 +457,15 @@ struct LongRow {
 // they are in flight together; every thread of the launch block reaches that
 // barrier exactly once per call, also when the row block is inactive (active
 // false: past the last row block).  HOT > 0: contributions of sources
-// [0, HOT) come from s_hot.
+// [0, HOT) come from s_hot.  w_res / w_dang: the wave's sums of |x_new - x_old|
+// and of the dangling mass over its rows (every lane; 0 for a chunk of a long
+// row, whose row the fix-up finishes, and for an inactive row block).
 template <int NB, int HOT>
 __device__ __forceinline__ void rowblock_body(
     const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
     const RowBlock *__restrict__ blocks, const float *__restrict__ contrib, const RowOut &out,
-    F2 *__restrict__ chunk_sum, double *__restrict__ part_res, double *__restrict__ part_dang,
-    int64_t bi, bool active, int tid, float *s_val, const float *s_hot) {
+    F2 *__restrict__ chunk_sum, int64_t bi, bool active, int tid, float *s_val,
+    const float *s_hot, double &w_res, double &w_dang) {
   constexpr int EPT = NB / PR_BLOCK;  // staged edges per thread
   const int lane = gb_lane(), wave = tid / GB_WAVE;
   RowBlock b = {0, 0, 0, 0};
@@ -531,16 +533,14 @@ __device__ __forceinline__ void rowblock_body(
     }
   }
   __syncthreads();
+  w_res = 0.0;
+  w_dang = 0.0;
   if (chunk) {
     if (tid == 0) {
       const F2 *s_red = reinterpret_cast<const F2 *>(s_val);
       F2 tot = s_red[0];
       for (int w = 1; w < PR_BLOCK / GB_WAVE; w++) tot = f2_merge(tot, s_red[w]);
       chunk_sum[bi] = tot;
-    }
-    if (lane == 0) {
-      part_res[bi * (PR_BLOCK / GB_WAVE) + wave] = 0.0;
-      part_dang[bi * (PR_BLOCK / GB_WAVE) + wave] = 0.0;
     }
     return;
   }
@@ -569,12 +569,8 @@ __device__ __forceinline__ void rowblock_body(
     }
     if (r < nrows && q == 0) out.put(b.r0 + r, f2_value(acc), xo[0], io[0], res, dang);
   }
-  res = gb_wave_sum_f64(res);
-  dang = gb_wave_sum_f64(dang);
-  if (lane == 0) {
-    part_res[bi * (PR_BLOCK / GB_WAVE) + wave] = res;
-    part_dang[bi * (PR_BLOCK / GB_WAVE) + wave] = dang;
-  }
+  w_res = gb_wave_sum_f64(res);
+  w_dang = gb_wave_sum_f64(dang);
 }
 
 template <int NB>
@@ -586,8 +582,14 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_rowblock_kernel(
   __shared__ float s_val[NB];
   if (st->done) return;
   out.base = st->base;
-  rowblock_body<NB, 0>(irp, icol, blocks, contrib, out, chunk_sum, part_res, part_dang,
-                       (int64_t)blockIdx.x, true, threadIdx.x, s_val, nullptr);
+  const int64_t bi = blockIdx.x;
+  double r, d;
+  rowblock_body<NB, 0>(irp, icol, blocks, contrib, out, chunk_sum, bi, true, threadIdx.x, s_val,
+                       nullptr, r, d);
+  if (gb_lane() == 0) {
+    part_res[bi * (PR_BLOCK / GB_WAVE) + threadIdx.x / GB_WAVE] = r;
+    part_dang[bi * (PR_BLOCK / GB_WAVE) + threadIdx.x / GB_WAVE] = d;
+  }
 }
 
 // Row blocks per launch block of the hot-set variant: the count that keeps the
@@ -610,7 +612,9 @@ constexpr int rb_subs(int nb, int hot) {
 // Persistent variant with an LDS copy of the hottest HOT contributions (the
 // highest out-degrees once relabelled), shared by SUBS row blocks summed side
 // by side (PR_BLOCK threads each): launch block j walks row blocks
-// SUBS * (j + k * gridDim.x) + sub, k = 0, 1, ...
+// SUBS * (j + k * gridDim.x) + sub, k = 0, 1, ...  Each wave sums its
+// residual and dangling partials over its walk (in walk order) and writes
+// them once: gridDim.x * SUBS * 4 partials.
 template <int NB, int HOT>
 __global__ __launch_bounds__(PR_BLOCK * rb_subs(NB, HOT)) void pr_rowblock_hot_kernel(
     const int32_t *__restrict__ irp, const int32_t *__restrict__ icol,
@@ -627,12 +631,88 @@ __global__ __launch_bounds__(PR_BLOCK * rb_subs(NB, HOT)) void pr_rowblock_hot_k
   // wave-uniform sub-block index (every lane of a wave has the same one)
   const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x / PR_BLOCK);
   const int tid = (int)threadIdx.x - sub * PR_BLOCK;
+  double acc_res = 0.0, acc_dang = 0.0;
   for (int64_t b0 = (int64_t)blockIdx.x * SUBS; b0 < nblocks; b0 += (int64_t)gridDim.x * SUBS) {
     const int64_t bi = b0 + sub;
-    rowblock_body<NB, HOT>(irp, icol, blocks, contrib, out, chunk_sum, part_res, part_dang, bi,
-                           bi < nblocks, tid, s_val[sub], s_hot);
+    double r, d;
+    rowblock_body<NB, HOT>(irp, icol, blocks, contrib, out, chunk_sum, bi, bi < nblocks, tid,
+                           s_val[sub], s_hot, r, d);
+    acc_res += r;
+    acc_dang += d;
     __syncthreads();  // the next row blocks reuse the LDS staging
   }
+  if (gb_lane() == 0) {
+    const int64_t gw = (int64_t)blockIdx.x * (PR_BLOCK * SUBS / GB_WAVE) + threadIdx.x / GB_WAVE;
+    part_res[gw] = acc_res;
+    part_dang[gw] = acc_dang;
+  }
+}
+
+// A long row's sum from its chunk partials, merged in chunk order; the loads
+// go out four at a time.
+__device__ __forceinline__ F2 join_chunks(const F2 *__restrict__ chunk_sum, const LongRow &L) {
+  F2 acc = chunk_sum[L.b0];
+  int32_t k = 1;
+  for (; k + 4 <= L.nb; k += 4) {
+    const F2 c0 = chunk_sum[L.b0 + k], c1 = chunk_sum[L.b0 + k + 1];
+    const F2 c2 = chunk_sum[L.b0 + k + 2], c3 = chunk_sum[L.b0 + k + 3];
+    acc = f2_merge(f2_merge(f2_merge(f2_merge(acc, c0), c1), c2), c3);
+  }
+  for (; k < L.nb; k++) acc = f2_merge(acc, chunk_sum[L.b0 + k]);
+  return acc;
+}
+
+// The end of an iteration from its residual and dangling totals: the next
+// base and the stop flag.
+__device__ __forceinline__ void pr_finish_iteration(PrState *st, double rtot, double dtot) {
+  const double N = (double)st->V;
+  const double d = (double)st->damping;
+  st->iter += 1;
+  st->residual = rtot;
+  st->dangling = dtot;
+  st->base = (float)((1.0 - d) / N + d * dtot / N);
+  st->done = (rtot < st->tol || st->iter >= st->max_iter) ? 1 : 0;
+}
+
+// The fix-up in one block, when the long rows and the partials are few (no
+// cross-block reduction, so no fence): long rows from their chunk sums (chunk
+// order), then the partials, each thread's share read in batches of 8 on
+// clamped indices, summed in a fixed order.
+constexpr int RB_FIX1_THREADS = 1024;
+constexpr int64_t RB_FIX1_MAX_PARTS = 16384;
+constexpr int32_t RB_FIX1_MAX_LONG = 4 * RB_FIX1_THREADS;
+__global__ __launch_bounds__(RB_FIX1_THREADS) void pr_rb_fixup_one_kernel(
+    int64_t nparts, int32_t nlong, const LongRow *__restrict__ longs,
+    const F2 *__restrict__ chunk_sum, RowOut out, const double *__restrict__ p_res,
+    const double *__restrict__ p_dang, PrState *__restrict__ st) {
+  __shared__ double s_red[RB_FIX1_THREADS / 64];
+  if (st->done) return;
+  out.base = st->base;
+  double res = 0.0, dang = 0.0;
+  for (int32_t t = threadIdx.x; t < nlong; t += RB_FIX1_THREADS) {
+    const LongRow L = longs[t];
+    out(L.row, join_chunks(chunk_sum, L), res, dang);
+  }
+  constexpr int U = 8;
+  for (int64_t b = threadIdx.x; b < nparts; b += (int64_t)RB_FIX1_THREADS * U) {
+    double a[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + (int64_t)u * RB_FIX1_THREADS;
+      const int64_t j = i < nparts ? i : nparts - 1;
+      a[u] = p_res[j];
+      c[u] = p_dang[j];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = b + (int64_t)u * RB_FIX1_THREADS < nparts;
+      res += in ? a[u] : 0.0;
+      dang += in ? c[u] : 0.0;
+    }
+  }
+  const double rtot = block_sum_f64(res, s_red);
+  const double dtot = block_sum_f64(dang, s_red);
+  if (threadIdx.x == 0) pr_finish_iteration(st, rtot, dtot);
 }
 
 // Long rows from their chunk sums (chunk order), the gather's per-wave
@@ -656,9 +736,7 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_rb_fixup_kernel(
   }
   if (t < nlong) {
     const LongRow L = longs[t];
-    F2 acc = chunk_sum[L.b0];
-    for (int32_t k = 1; k < L.nb; k++) acc = f2_merge(acc, chunk_sum[L.b0 + k]);
-    out(L.row, acc, res, dang);
+    out(L.row, join_chunks(chunk_sum, L), res, dang);
   }
   double rsum = block_sum_f64(res, s_red);
   double dsum = block_sum_f64(dang, s_red);
@@ -680,13 +758,7 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_rb_fixup_kernel(
   const double rtot = block_sum_f64(a, s_red);
   const double dtot = block_sum_f64(c, s_red);
   if (threadIdx.x == 0) {
-    const double N = (double)st->V;
-    const double d = (double)st->damping;
-    st->iter += 1;
-    st->residual = rtot;
-    st->dangling = dtot;
-    st->base = (float)((1.0 - d) / N + d * dtot / N);
-    st->done = (rtot < st->tol || st->iter >= st->max_iter) ? 1 : 0;
+    pr_finish_iteration(st, rtot, dtot);
     *done_blocks = 0;  // ready for the next iteration
   }
 }
@@ -793,12 +865,14 @@ struct RbArgs {
   int64_t nblocks;
 };
 
+// Returns the number of residual/dangling partials the launch writes.
 template <int NB, int HOT>
-void launch_rb(const RbArgs &a, hipStream_t st) {
+int64_t launch_rb(const RbArgs &a, hipStream_t st) {
   if constexpr (HOT == 0) {
     hipLaunchKernelGGL((pr_rowblock_kernel<NB>), dim3((unsigned)std::max<int64_t>(a.nblocks, 1)),
                        dim3(PR_BLOCK), 0, st, a.irp, a.icol, a.blocks, a.contrib, a.out,
                        a.chunk_sum, a.pres, a.pdang, a.state);
+    return std::max<int64_t>(a.nblocks, 1) * (PR_BLOCK / GB_WAVE);
   } else {
     constexpr int SUBS = rb_subs(NB, HOT);
     static int per_cu = 0;
@@ -813,24 +887,24 @@ void launch_rb(const RbArgs &a, hipStream_t st) {
     const int64_t need = (a.nblocks + SUBS - 1) / SUBS;
     const int64_t grid =
         std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)gb_num_cus() * per_cu));
+    // grid * SUBS <= nblocks + SUBS - 1: at most 4 * nblocks + 12 partials
     hipLaunchKernelGGL((pr_rowblock_hot_kernel<NB, HOT>), dim3((unsigned)grid),
                        dim3(PR_BLOCK * SUBS), 0,
                        st, a.irp, a.icol, a.blocks, a.contrib, a.out, a.chunk_sum, a.pres, a.pdang,
                        a.state, a.nblocks);
+    return grid * SUBS * (PR_BLOCK / GB_WAVE);
   }
 }
 
 template <int NB>
-void launch_rb_hot(int hot, const RbArgs &a, hipStream_t st) {
+int64_t launch_rb_hot(int hot, const RbArgs &a, hipStream_t st) {
   switch (hot) {
     case 4096:
-      launch_rb<NB, 4096>(a, st);
-      break;
+      return launch_rb<NB, 4096>(a, st);
     case 8192:
-      launch_rb<NB, 8192>(a, st);
-      break;
+      return launch_rb<NB, 8192>(a, st);
     default:
-      launch_rb<NB, 0>(a, st);
+      return launch_rb<NB, 0>(a, st);
   }
 }
 
@@ -1043,10 +1117,11 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   GB_TRY(w.cb.ensure(sizeof(float) * V));
   GB_TRY(w.tail.ensure(sizeof(F2) * ntiles));
   GB_TRY(w.head.ensure(sizeof(F2) * ntiles));
-  GB_TRY(w.pres.ensure(sizeof(double) * std::max(nparts, vblocks)));
-  GB_TRY(w.pdang.ensure(sizeof(double) * std::max(nparts, vblocks)));
-  GB_TRY(w.fres.ensure(sizeof(double) * nfix));
-  GB_TRY(w.fdang.ensure(sizeof(double) * nfix));
+  // + 16: the persistent row-block form writes up to 4 * nblocks + 12 partials
+  GB_TRY(w.pres.ensure(sizeof(double) * std::max(nparts + 16, vblocks)));
+  GB_TRY(w.pdang.ensure(sizeof(double) * std::max(nparts + 16, vblocks)));
+  GB_TRY(w.fres.ensure(sizeof(double) * (nfix + 1)));
+  GB_TRY(w.fdang.ensure(sizeof(double) * (nfix + 1)));
   GB_TRY(w.state.ensure(sizeof(PrState)));
   GB_TRY(w.counter.ensure(sizeof(unsigned int)));
   GB_HIP(hipMemsetAsync(w.counter.p, 0, sizeof(unsigned int), st));
@@ -1076,14 +1151,19 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     if (rb) {
       const RbArgs a{irp, icol, g->pr_rb.as<RowBlock>(), c_old, out, w.tail.as<F2>(),
                      w.pres.as<double>(), w.pdang.as<double>(), state, ntiles};
-      if (rb == 1024)
-        launch_rb_hot<1024>(hot, a, st);
-      else if (rb == 2048)
-        launch_rb_hot<2048>(hot, a, st);
-      else
-        launch_rb_hot<4096>(hot, a, st);
+      const int64_t np = rb == 1024   ? launch_rb_hot<1024>(hot, a, st)
+                         : rb == 2048 ? launch_rb_hot<2048>(hot, a, st)
+                                      : launch_rb_hot<4096>(hot, a, st);
       if (ev1) (void)hipEventRecord(ev1, st);
-      hipLaunchKernelGGL(pr_rb_fixup_kernel, dim3((unsigned)nfix), dim3(PR_BLOCK), 0, st, nparts,
+      if (np <= RB_FIX1_MAX_PARTS && g->pr_rb_nlong <= RB_FIX1_MAX_LONG) {
+        hipLaunchKernelGGL(pr_rb_fixup_one_kernel, dim3(1), dim3(RB_FIX1_THREADS), 0, st, np,
+                           g->pr_rb_nlong, g->pr_rb_long.as<LongRow>(),
+                           (const F2 *)w.tail.as<F2>(), out, w.pres.as<double>(),
+                           w.pdang.as<double>(), state);
+        return;
+      }
+      const int64_t nf = (std::max<int64_t>(np, g->pr_rb_nlong) + PR_BLOCK - 1) / PR_BLOCK;
+      hipLaunchKernelGGL(pr_rb_fixup_kernel, dim3((unsigned)nf), dim3(PR_BLOCK), 0, st, np,
                          g->pr_rb_nlong, g->pr_rb_long.as<LongRow>(), (const F2 *)w.tail.as<F2>(),
                          out, w.pres.as<double>(), w.pdang.as<double>(), w.fres.as<double>(),
                          w.fdang.as<double>(), w.counter.as<unsigned int>(), state);

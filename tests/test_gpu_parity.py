@@ -382,7 +382,10 @@ def test_pagerank_row_blocks_match_oracle(gpu, rb, hot, monkeypatch):
     an edgeless graph, a cycle (closed form: uniform), and graph replay.  With
     a hot set (relabelled graphs), several row blocks share one launch block
     and the persistent grid walks them in rounds: R-MAT 18 takes several
-    rounds and ends with launch blocks holding fewer row blocks."""
+    rounds and ends with launch blocks holding fewer row blocks.  The fix-up
+    runs in one block when partials and long rows are few, else in many:
+    R-MAT 18 at NB 1024 without a hot set, and 4200 long rows at NB 1024,
+    take the multi-block form."""
     gb = gpu
     monkeypatch.setenv("GB_PR_HOT", str(hot))
     rng = np.random.default_rng(rb)
@@ -395,8 +398,15 @@ def test_pagerank_row_blocks_match_oracle(gpu, rb, hot, monkeypatch):
     graphs = [(gb.gen_rmat(14, 16, seed=21, weights="none"), 1, True),
               (gb.gen_rmat(13, 8, seed=23, weights="none"), -1, True),
               (gb.HostCSR.from_edges(V, src[m], dst[m]), -1, False)]
-    if hot:
+    if hot or rb == 1024:
+        # (rb 1024, no hot set: more than 16384 partials, the multi-block fix-up)
         graphs.append((gb.gen_rmat(18, 16, seed=5, weights="none"), 1, True))
+    if rb == 1024:
+        # 4200 long rows (more than the one-block fix-up takes)
+        n = 4200
+        hs = rng.integers(0, 3 * n, n * 1030)
+        hd = np.repeat(np.arange(n), 1030)
+        graphs.append((gb.HostCSR.from_edges(3 * n, hs[hs != hd], hd[hs != hd]), -1, True))
     for h, rl, check_it in graphs:
         _say(f"pagerank row blocks rb={rb} hot={hot} V={h.V} E={h.E} relabel={rl}")
         x, res, ref, it = _pr_case(gb, h, row_block=rb, relabel=rl)
