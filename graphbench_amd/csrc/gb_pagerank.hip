@@ -789,10 +789,24 @@ void build_row_blocks(const std::vector<int32_t> &rp, int nb, std::vector<RowBlo
 
 struct PrWork {
   DevBuf xa, xb, ca, cb, tail, head, pres, pdang, fres, fdang, state, counter;
+  // gather timing events (time_kernels), created once and reused: creating
+  // two per iteration on every call held up the first launches
+  std::vector<hipEvent_t> events;
+  hipError_t ensure_events(size_t n) {
+    while (events.size() < n) {
+      hipEvent_t e = nullptr;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      events.push_back(e);
+    }
+    return hipSuccess;
+  }
   void release() {
     for (DevBuf *b :
          {&xa, &xb, &ca, &cb, &tail, &head, &pres, &pdang, &fres, &fdang, &state, &counter})
       b->release();
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    events.clear();
   }
 };
 // Heap singleton, never destroyed: device memory must not be freed from a
@@ -1222,11 +1236,8 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   } else {
     // eager launches; with time_kernels every gather is bracketed by its own
     // event pair and the stop flag is read every CH iterations only
-    std::vector<hipEvent_t> evs;
-    if (p->time_kernels) {
-      evs.resize(2 * (size_t)p->max_iter, nullptr);
-      for (auto &e : evs) GB_HIP(hipEventCreate(&e));
-    }
+    if (p->time_kernels) GB_HIP(w.ensure_events(2 * (size_t)p->max_iter));
+    const std::vector<hipEvent_t> &evs = w.events;
     for (int k = 0; k < p->max_iter; k++) {
       const float *xo = (k % 2 == 0) ? xa : xb;
       const float *co = (k % 2 == 0) ? ca : cb;
@@ -1249,7 +1260,6 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
         gather_ms += ms;
         gather_launches++;
       }
-      for (auto &e : evs) (void)hipEventDestroy(e);
     }
   }
   GB_HIP(hipGetLastError());
