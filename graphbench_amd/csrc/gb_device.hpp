@@ -84,6 +84,19 @@ struct gb_wave_queue {
   }
 };
 
+// Per-wave statistics without a shared address: wave w of a launch adds its
+// counts to slots[w * GB_WSTAT + k] with no-return atomics (each slot is one
+// wave's), and the slots are folded into the counters once, after the solve
+// (gb_wstat_fold).  Thousands of waves adding to one counter per phase would
+// serialise on its cache line.
+constexpr int GB_WSTAT = 4;
+__device__ __forceinline__ void gb_wstat_add(unsigned long long *slots, int k, uint32_t v) {
+  if (v) {
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    atomicAdd(&slots[wave * GB_WSTAT + k], (unsigned long long)v);
+  }
+}
+
 // Deterministic wave sum (fixed butterfly order).
 __device__ __forceinline__ double gb_wave_sum_f64(double v) {
 #pragma unroll

@@ -143,6 +143,10 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
                        hipStream_t st);
 gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64_t *reached,
                          int64_t *traversed, hipStream_t st);
+// Adds the per-wave statistics slots (nwaves * GB_WSTAT, gb_device.hpp) to
+// the counters: cnt[idx[k]] += sum over waves of slot k, k < 3.
+gb_status gb_wstat_fold(int64_t nwaves, const unsigned long long *slots, unsigned long long *cnt,
+                        int idx0, int idx1, int idx2, hipStream_t st);
 void gb_bc_release();
 void gb_msbc_release();
 void gb_mlp_release();

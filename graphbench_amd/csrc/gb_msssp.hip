@@ -87,6 +87,7 @@ struct MsCtx {
   MsEnt *qn[2];                    // near queues 0/1 (chunks)
   uint64_t *qf[2];                 // far queues 0/1 (vertices)
   unsigned long long *cnt;         // MS_* counters
+  unsigned long long *wstat;       // per-wave statistics slots (GB_WSTAT per wave)
   uint64_t delta, magic;           // bucket width and its division constant
 };
 
@@ -331,9 +332,11 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
     (void)gb_wave_excl_sum(relax, &tot);
     (void)gb_wave_excl_sum(imp_near, &tn);
     (void)gb_wave_excl_sum(imp_far, &tf);
-    if (gb_lane() == 0 && tot) atomicAdd(&c.cnt[MS_RELAX], (unsigned long long)tot);
-    if (gb_lane() == 0 && tn) atomicAdd(&c.cnt[MS_IMP_NEAR], (unsigned long long)tn);
-    if (gb_lane() == 0 && tf) atomicAdd(&c.cnt[MS_IMP_FAR], (unsigned long long)tf);
+    if (gb_lane() == 0) {
+      gb_wstat_add(c.wstat, 0, tot);
+      gb_wstat_add(c.wstat, 1, tn);
+      gb_wstat_add(c.wstat, 2, tf);
+    }
     if (__ballot(ovf) && gb_lane() == 0) atomicOr(&c.cnt[MS_OVF], 1ull);
   } else {
     ms_far<W>(c, a, ln, lf);
@@ -405,9 +408,9 @@ __global__ void ms_count_kernel(int64_t V, int32_t nsrc, const int32_t *__restri
 }
 
 struct MsWork {
-  DevBuf dist, m0, m1, far, q[4], cnt, ctl, src;
+  DevBuf dist, m0, m1, far, q[4], cnt, ctl, src, wstat;
   void release() {
-    for (DevBuf *b : {&dist, &m0, &m1, &far, &cnt, &ctl, &src}) b->release();
+    for (DevBuf *b : {&dist, &m0, &m1, &far, &cnt, &ctl, &src, &wstat}) b->release();
     for (auto &b : q) b.release();
   }
 };
@@ -429,6 +432,9 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   GB_HIP(hipMemsetAsync(c.mask[1], 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.far, 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.cnt, 0, sizeof(unsigned long long) * MS_N, st));
+  const int grid = gb_num_cus() * 8;
+  const int64_t nwaves = (int64_t)grid * (MS_BLOCK / GB_WAVE);
+  GB_HIP(hipMemsetAsync(c.wstat, 0, sizeof(unsigned long long) * GB_WSTAT * nwaves, st));
   hipLaunchKernelGGL(ms_seed_kernel<W>, dim3(1), dim3(64), 0, st, c, d_src);
   GB_HIP(hipGetLastError());
   MsCtl hc{};
@@ -437,7 +443,6 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   hc.buckets = 1;
   GB_HIP(hipMemcpyAsync(w.ctl.p, &hc, sizeof(hc), hipMemcpyHostToDevice, st));
   MsCtl *ctl = w.ctl.as<MsCtl>();
-  const int grid = gb_num_cus() * 8;
   const int cgrid = gb_num_cus() * 2;
   constexpr int PH = 16;
   hipStream_t cs = st, own = nullptr;
@@ -492,6 +497,7 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   hipLaunchKernelGGL(ms_unpack_kernel, dim3((unsigned)((V + 63) / 64), (unsigned)((W + 63) / 64)),
                      dim3(256), 0, st, V, W, c.ntask, (const uint32_t *)c.dist, out_rows);
   GB_HIP(hipGetLastError());
+  GB_TRY(gb_wstat_fold(nwaves, c.wstat, c.cnt, MS_RELAX, MS_IMP_NEAR, MS_IMP_FAR, st));
   unsigned long long h[MS_N];
   GB_HIP(hipMemcpyAsync(h, c.cnt, sizeof(h), hipMemcpyDeviceToHost, st));
   GB_HIP(hipStreamSynchronize(st));
@@ -547,6 +553,8 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   GB_TRY(w.q[2].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.q[3].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
+  GB_TRY(w.wstat.ensure(sizeof(unsigned long long) * GB_WSTAT * gb_num_cus() * 8 *
+                        (MS_BLOCK / GB_WAVE)));
   GB_TRY(w.ctl.ensure(sizeof(MsCtl)));
   GB_TRY(w.src.ensure(sizeof(int32_t) * 64));
   MsCtx c{};
@@ -563,6 +571,7 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   c.qf[0] = w.q[2].as<uint64_t>();
   c.qf[1] = w.q[3].as<uint64_t>();
   c.cnt = w.cnt.as<unsigned long long>();
+  c.wstat = w.wstat.as<unsigned long long>();
   c.delta = delta;
   c.magic = delta > 1 ? (~0ull / delta) + 1 : 0;
   for (int32_t g0 = 0; g0 < nsrc; g0 += 64) {

@@ -347,6 +347,38 @@ gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib,
 }
 
 // ---------------------------------------------------------------------------
+// per-wave statistics folded into counters (one block, fixed order)
+
+__global__ __launch_bounds__(256) void wstat_fold_kernel(int64_t nwaves,
+                                                         const unsigned long long *__restrict__ s,
+                                                         unsigned long long *__restrict__ cnt,
+                                                         int i0, int i1, int i2) {
+  __shared__ unsigned long long s_sum[3][4];
+  unsigned long long a[3] = {0ull, 0ull, 0ull};
+  for (int64_t w = threadIdx.x; w < nwaves; w += blockDim.x)
+    for (int k = 0; k < 3; k++) a[k] += s[w * GB_WSTAT + k];
+  for (int k = 0; k < 3; k++) {
+    unsigned long long v = a[k];
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (gb_lane() == 0) s_sum[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    const unsigned long long t = s_sum[k][0] + s_sum[k][1] + s_sum[k][2] + s_sum[k][3];
+    cnt[k == 0 ? i0 : k == 1 ? i1 : i2] += t;
+  }
+}
+
+gb_status gb_wstat_fold(int64_t nwaves, const unsigned long long *slots, unsigned long long *cnt,
+                        int idx0, int idx1, int idx2, hipStream_t st) {
+  hipLaunchKernelGGL(wstat_fold_kernel, dim3(1), dim3(256), 0, st, nwaves, slots, cnt, idx0, idx1,
+                     idx2);
+  GB_HIP(hipGetLastError());
+  return GB_OK;
+}
+
+// ---------------------------------------------------------------------------
 // edge owners and cross-part counts (the per-edge work of chunk_files,
 // partition/csr_partition.c:99-108)
 

@@ -118,6 +118,7 @@ struct Ctx {
   uint32_t *sst;  // settled-set round stamp
   uint32_t *fst;  // far epoch stamp
   unsigned long long *cnt;
+  unsigned long long *wstat;  // per-wave statistics slots (GB_WSTAT per wave)
   // one graph and one delta, task t's distances at dist0 + t * V (sources of a
   // single graph): task data then follow from t without per-edge lookups
   typename Tr::D *dist0;
@@ -377,9 +378,11 @@ __device__ __forceinline__ void expand_body(
   (void)gb_wave_excl_sum(relax, &tot);
   (void)gb_wave_excl_sum(imp_near, &tn);
   (void)gb_wave_excl_sum(imp_far, &tf);
-  if (lane == 0 && tot) atomicAdd(&c.cnt[C_RELAX], (unsigned long long)tot);
-  if (lane == 0 && tn) atomicAdd(&c.cnt[C_IMP_NEAR], (unsigned long long)tn);
-  if (lane == 0 && tf) atomicAdd(&c.cnt[C_IMP_FAR], (unsigned long long)tf);
+  if (lane == 0) {
+    gb_wstat_add(c.wstat, 0, tot);
+    gb_wstat_add(c.wstat, 1, tn);
+    gb_wstat_add(c.wstat, 2, tf);
+  }
   if (__ballot(ovf) && lane == 0) atomicOr(&c.cnt[C_OVF], 1ull);
 }
 
@@ -520,9 +523,9 @@ __global__ __launch_bounds__(SS_BLOCK) void sssp_count_kernel(Ctx<Tr> c, int32_t
 }
 
 struct SsspWork {
-  DevBuf graphs, tasks, nst, sst, fst, q[5], cnt, counts, ctl;
+  DevBuf graphs, tasks, nst, sst, fst, q[5], cnt, counts, ctl, wstat;
   void release() {
-    for (DevBuf *b : {&graphs, &tasks, &nst, &sst, &fst, &cnt, &counts, &ctl}) b->release();
+    for (DevBuf *b : {&graphs, &tasks, &nst, &sst, &fst, &cnt, &counts, &ctl, &wstat}) b->release();
     for (auto &b : q) b.release();
   }
 };
@@ -568,14 +571,19 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   GB_TRY(w.fst.ensure(sizeof(uint32_t) * total));
   for (int k = 0; k < 5; k++) GB_TRY(w.q[k].ensure(sizeof(uint64_t) * qcap));
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * C_N));
+  const int grid = gb_num_cus() * 16;
+  const int64_t nwaves = (int64_t)grid * (SS_BLOCK / GB_WAVE);
+  GB_TRY(w.wstat.ensure(sizeof(unsigned long long) * GB_WSTAT * nwaves));
   GB_TRY(w.ctl.ensure(sizeof(SsspCtl)));
   GB_HIP(hipMemcpyAsync(w.graphs.p, hg.data(), sizeof(G) * hg.size(), hipMemcpyHostToDevice, st));
   GB_HIP(hipMemcpyAsync(w.tasks.p, ht.data(), sizeof(STask<Tr>) * ht.size(),
                         hipMemcpyHostToDevice, st));
   unsigned long long *cnt = w.cnt.as<unsigned long long>();
   GB_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * C_N, st));
+  unsigned long long *wstat = w.wstat.as<unsigned long long>();
+  GB_HIP(hipMemsetAsync(wstat, 0, sizeof(unsigned long long) * GB_WSTAT * nwaves, st));
   Ctx<Tr> c{w.graphs.as<G>(), w.tasks.as<STask<Tr>>(), w.nst.as<uint32_t>(), w.sst.as<uint32_t>(),
-            w.fst.as<uint32_t>(), cnt, ht[0].dist, ht[0].delta, ht[0].magic};
+            w.fst.as<uint32_t>(), cnt, wstat, ht[0].dist, ht[0].delta, ht[0].magic};
 
   hipEvent_t e0, e1;
   GB_HIP(hipEventCreate(&e0));
@@ -610,7 +618,6 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   SsspCtl *ctl = w.ctl.as<SsspCtl>();
   Queues qs{{w.q[0].as<uint64_t>(), w.q[1].as<uint64_t>(), w.q[2].as<uint64_t>(),
              w.q[3].as<uint64_t>(), w.q[4].as<uint64_t>()}};
-  const int grid = gb_num_cus() * 16;
   const uint32_t cus = (uint32_t)gb_num_cus();
   auto one_phase = [&]() {
     hipLaunchKernelGGL(sssp_ctrl_kernel, dim3(1), dim3(64), 0, st, cnt, ctl, cus);
@@ -684,6 +691,7 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
   GB_HIP(hipGetLastError());
   gb_status status = GB_OK;
   const int64_t phases = hs.phases, buckets = hs.buckets;
+  GB_TRY(gb_wstat_fold(nwaves, wstat, cnt, C_RELAX, C_IMP_NEAR, C_IMP_FAR, st));
   GB_HIP(hipEventRecord(e1, st));
   unsigned long long h[C_N];
   GB_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
