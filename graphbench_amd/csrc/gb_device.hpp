@@ -84,6 +84,44 @@ struct gb_wave_queue {
   }
 };
 
+// Final flush of the wave queues of a block, by every thread of the block at
+// the same point (a block barrier; no wave may have left early): one global
+// reservation per queue and block instead of one per wave.  At the end of a
+// phase nearly every wave has entries, and thousands of reservations on one
+// counter serialise on its cache line.  Entries keep their order within a
+// wave; waves are laid out in wave order.
+template <class... Q>
+__device__ __forceinline__ void gb_block_flush(Q &...qs) {
+  constexpr int NQ = sizeof...(Q);
+  constexpr int MAXW = 16;  // waves per block (1024 threads)
+  __shared__ uint32_t s_n[NQ][MAXW];
+  __shared__ unsigned long long s_base[NQ];
+  const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6), lane = gb_lane();
+  const uint32_t ns[NQ] = {qs.n...};
+  unsigned long long *const ctr[NQ] = {qs.counter...};
+  if (lane == 0)
+    for (int i = 0; i < NQ; i++) s_n[i][w] = ns[i];
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const int i = (int)threadIdx.x;
+    uint32_t tot = 0;
+    for (int k = 0; k < nw; k++) tot += s_n[i][k];
+    s_base[i] = tot ? atomicAdd(ctr[i], (unsigned long long)tot) : 0ull;
+  }
+  __syncthreads();
+  int i = 0;
+  (
+      [&] {
+        uint64_t off = s_base[i];
+        for (int k = 0; k < w; k++) off += s_n[i][k];
+        for (uint32_t k = lane; k < qs.n; k += GB_WAVE) qs.q[off + k] = qs.buf[k];
+        qs.n = 0;
+        ++i;
+      }(),
+      ...);
+  __syncthreads();  // the LDS counts may be reused by a later flush
+}
+
 // Per-wave statistics without a shared address: wave w of a launch adds its
 // counts to slots[w * GB_WSTAT + k] with no-return atomics (each slot is one
 // wave's), and the slots are folded into the counters once, after the solve

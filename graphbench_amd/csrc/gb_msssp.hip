@@ -249,8 +249,7 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
     i = i1;
     i1 = i2;
   }
-  nq.flush();
-  fq.flush();
+  gb_block_flush(nq, fq);
 }
 
 // Far scan: far queue a_in -> bucket a_bucket's pairs to near queue a_near,
@@ -304,8 +303,7 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *ld
     nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(x, r0, r1, q); });
     fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
   }
-  nq.flush();
-  fq.flush();
+  gb_block_flush(nq, fq);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     const uint64_t o = gb_shfl_u64(minb, lane ^ d);

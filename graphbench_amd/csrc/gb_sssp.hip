@@ -370,9 +370,7 @@ __device__ __forceinline__ void expand_body(
       __builtin_amdgcn_wave_barrier();  // the list is rewritten next round
     }
   }
-  nearq.flush();
-  farq.flush();
-  sq.flush();
+  gb_block_flush(nearq, farq, sq);
   // per-wave statistics
   uint32_t tot, tn, tf;
   (void)gb_wave_excl_sum(relax, &tot);
@@ -424,8 +422,7 @@ __device__ __forceinline__ void far_scan_body(
     nearq.push(ncnt, [&](uint32_t k) { return mk_entry(t, k, v); });
     farq.push(far ? 1u : 0u, [&](uint32_t) { return mk_entry(t, 0, v); });
   }
-  nearq.flush();
-  farq.flush();
+  gb_block_flush(nearq, farq);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     uint64_t o = gb_shfl_u64(minb, lane ^ d);
