@@ -122,6 +122,24 @@ __device__ __forceinline__ void gb_block_flush(Q &...qs) {
   __syncthreads();  // the LDS counts may be reused by a later flush
 }
 
+// Block minimum of v, then one atomicMin on *dst by thread 0 when it is below
+// `none` (every thread of the block calls it at the same point).
+__device__ __forceinline__ void gb_block_atomic_min(uint64_t v, uint64_t none,
+                                                    unsigned long long *dst) {
+  __shared__ uint64_t s_min[16];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint64_t o = gb_shfl_u64(v, gb_lane() ^ d);
+    v = o < v ? o : v;
+  }
+  if (gb_lane() == 0) s_min[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) v = s_min[k] < v ? s_min[k] : v;
+    if (v != none) atomicMin(dst, (unsigned long long)v);
+  }
+}
+
 // Per-wave statistics without a shared address: wave w of a launch adds its
 // counts to slots[w * GB_WSTAT + k] with no-return atomics (each slot is one
 // wave's), and the slots are folded into the counters once, after the solve

@@ -304,12 +304,7 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *ld
     fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
   }
   gb_block_flush(nq, fq);
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    const uint64_t o = gb_shfl_u64(minb, lane ^ d);
-    minb = o < minb ? o : minb;
-  }
-  if (lane == 0 && minb != ~0ull) atomicMin(&c.cnt[MS_MINB], (unsigned long long)minb);
+  gb_block_atomic_min(minb, ~0ull, &c.cnt[MS_MINB]);
 }
 
 template <int W>

@@ -423,12 +423,7 @@ __device__ __forceinline__ void far_scan_body(
     farq.push(far ? 1u : 0u, [&](uint32_t) { return mk_entry(t, 0, v); });
   }
   gb_block_flush(nearq, farq);
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    uint64_t o = gb_shfl_u64(minb, lane ^ d);
-    minb = o < minb ? o : minb;
-  }
-  if (lane == 0 && minb != ~0ull) atomicMin(&c.cnt[C_MINB], (unsigned long long)minb);
+  gb_block_atomic_min(minb, ~0ull, &c.cnt[C_MINB]);
 }
 
 // ---- device-driven phase loop ----------------------------------------------
