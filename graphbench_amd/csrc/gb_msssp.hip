@@ -27,7 +27,9 @@
 // split: an expansion relaxes all edges of the vertex.  Delta-stepping is
 // exact for any bucket width, so distances equal the per-source engine's.
 // The bucket loop is the state machine ms_ctl_advance (gb_sssp_ctl.hpp),
-// replayed from a captured hipGraph as in the per-source engine.
+// replayed from a captured hipGraph as in the per-source engine; a phase is
+// two launches, the state advance fused with the previous phase's mask clear,
+// then the step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -114,13 +116,6 @@ __global__ void ms_seed_kernel(MsCtx c, const int32_t *__restrict__ src) {
     const unsigned long long b = atomicAdd(&c.cnt[MS_NEAR0], (unsigned long long)k);
     for (uint32_t i = 0; i < k; i++) c.qn[0][b + i] = ms_chunk<W>((uint32_t)s, r0, r1, i);
   }
-}
-
-__global__ void ms_ctrl_kernel(unsigned long long *__restrict__ cnt, MsCtl *__restrict__ ctl) {
-  if (threadIdx.x != 0) return;
-  MsCtl s = *ctl;
-  ms_ctl_advance(s, cnt);
-  *ctl = s;
 }
 
 // What a wave loads for one near entry: the entry's source mask, this lane's
@@ -336,12 +331,23 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
   }
 }
 
-// After a light phase: clear the input queue's masks (the buffer is the
-// output of the phase after next).
-__global__ void ms_clear_kernel(MsCtx c, const MsCtl *__restrict__ ctl) {
-  if (ctl->action != ACT_LIGHT) return;
-  const int in = ctl->a_in;
-  const uint64_t n = c.cnt[in];
+// Between two phases, one launch: thread 0 computes the next action from
+// the previous state `prev` into `next` (the state machine ms_ctl_advance),
+// and, when the previous action was a light phase, every thread clears that
+// phase's input masks (the buffer is the output of the phase after next).
+// The two states are distinct buffers, alternating per phase, and the clear
+// takes its entry count from `prev` rather than the counter, which the
+// advance may reset, so the two halves never touch the same data.
+__global__ void ms_clear_ctrl_kernel(MsCtx c, const MsCtl *__restrict__ prev,
+                                     MsCtl *__restrict__ next) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    MsCtl s = *prev;
+    ms_ctl_advance(s, c.cnt);
+    *next = s;
+  }
+  if (prev->action != ACT_LIGHT) return;
+  const int in = prev->a_in;
+  const uint64_t n = prev->a_in_n;
   unsigned long long *m = c.mask[in - MS_NEAR0];
   const MsEnt *q = c.qn[in - MS_NEAR0];
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -430,14 +436,18 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   GB_HIP(hipMemsetAsync(c.wstat, 0, sizeof(unsigned long long) * GB_WSTAT * nwaves, st));
   hipLaunchKernelGGL(ms_seed_kernel<W>, dim3(1), dim3(64), 0, st, c, d_src);
   GB_HIP(hipGetLastError());
+  // two loop states, alternating per phase: phase k acts on ctl[k % 2], which
+  // the launch before it computed from ctl[(k + 1) % 2]; the first launch
+  // starts from the initial state in ctl[1] (no action taken yet)
   MsCtl hc{};
   hc.max_phases = max_phases > 0 ? max_phases : ((int64_t)1 << 40);
   hc.state = MS_ST_LIGHT;
   hc.buckets = 1;
-  GB_HIP(hipMemcpyAsync(w.ctl.p, &hc, sizeof(hc), hipMemcpyHostToDevice, st));
+  hc.action = ACT_NONE;
   MsCtl *ctl = w.ctl.as<MsCtl>();
+  GB_HIP(hipMemcpyAsync(ctl + 1, &hc, sizeof(hc), hipMemcpyHostToDevice, st));
   const int cgrid = gb_num_cus() * 2;
-  constexpr int PH = 16;
+  constexpr int PH = 16;  // even: the replay ends on ctl[1], where the next one starts
   hipStream_t cs = st, own = nullptr;
   if (!st) {  // the null stream cannot be captured
     GB_HIP(hipStreamCreate(&own));
@@ -454,10 +464,11 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     for (int k = 0; k < PH; k++) {
-      hipLaunchKernelGGL(ms_ctrl_kernel, dim3(1), dim3(64), 0, cs, c.cnt, ctl);
+      MsCtl *cur = ctl + (k % 2), *prev = ctl + ((k + 1) % 2);
+      hipLaunchKernelGGL(ms_clear_ctrl_kernel, dim3(cgrid), dim3(256), 0, cs, c,
+                         (const MsCtl *)prev, cur);
       hipLaunchKernelGGL(ms_step_kernel<W>, dim3(grid), dim3(MS_BLOCK), 0, cs, c,
-                         (const MsCtl *)ctl);
-      hipLaunchKernelGGL(ms_clear_kernel, dim3(cgrid), dim3(256), 0, cs, c, (const MsCtl *)ctl);
+                         (const MsCtl *)cur);
     }
     e = hipStreamEndCapture(cs, &graph);
   }
@@ -467,7 +478,7 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   int64_t last = -1;
   while (gst == GB_OK) {
     if (hipGraphLaunch(exec, cs) != hipSuccess ||
-        hipMemcpyAsync(&hs, ctl, sizeof(hs), hipMemcpyDeviceToHost, cs) != hipSuccess ||
+        hipMemcpyAsync(&hs, ctl + 1, sizeof(hs), hipMemcpyDeviceToHost, cs) != hipSuccess ||
         hipStreamSynchronize(cs) != hipSuccess) {
       gst = gb_fail(GB_ERR_HIP, "msssp: phase replay failed");
       break;
@@ -548,7 +559,7 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
   GB_TRY(w.wstat.ensure(sizeof(unsigned long long) * GB_WSTAT * gb_num_cus() * 8 *
                         (MS_BLOCK / GB_WAVE)));
-  GB_TRY(w.ctl.ensure(sizeof(MsCtl)));
+  GB_TRY(w.ctl.ensure(2 * sizeof(MsCtl)));
   GB_TRY(w.src.ensure(sizeof(int32_t) * 64));
   MsCtx c{};
   c.rp = g->rp;

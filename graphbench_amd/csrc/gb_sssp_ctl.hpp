@@ -153,6 +153,7 @@ struct MsCtl {
   // or ACT_FAR (far queue a_in -> near a_near and far a_far)
   int32_t action, a_in, a_near, a_far;
   uint64_t a_bucket;
+  uint64_t a_in_n;  // light action: entries in queue a_in (its counter is reused later)
 };
 
 __host__ __device__ inline void ms_ctl_advance(MsCtl &s, unsigned long long *cnt) {
@@ -171,6 +172,7 @@ __host__ __device__ inline void ms_ctl_advance(MsCtl &s, unsigned long long *cnt
           s.a_near = MS_NEAR0 + (s.cur ^ 1);
           s.a_far = MS_FAR0 + s.fcur;
           s.a_bucket = s.b;
+          s.a_in_n = cnt[s.a_in];
           cnt[s.a_near] = 0;
           s.cur ^= 1;
         } else {
