@@ -730,8 +730,12 @@ void chunk_files(char *file, char *dir, lint *distrib, lint *send_size, lint *re
   upload(d_dist, dist);
   check(d_ub.ensure(sizeof(unsigned long long) * (size_t)p * p), "device allocation");
   std::vector<unsigned long long> send_ub((size_t)p * p, 0ull), ub((size_t)p * p);
-  // edges read since the last 'p' line: as written (u, v, w) and the
-  // endpoints the parts are found for (u - 1, v - offset)
+  // edges read since the last flush: as written (u, v, w) and the endpoints
+  // the parts are found for (u - 1, v - offset); flushed at every 'p' line,
+  // every CHUNK_EDGES edges and at the end
+  size_t CHUNK_EDGES = (size_t)1 << 24;
+  if (const char *e = getenv("GB_CHUNK_EDGES"))  // smaller batches (tests)
+    if (atoll(e) > 0) CHUNK_EDGES = (size_t)atoll(e);
   std::vector<long long> eu, ev, ew;
   std::vector<int32_t> su, sv;
   auto flush = [&]() {
@@ -793,6 +797,7 @@ void chunk_files(char *file, char *dir, lint *distrib, lint *send_size, lint *re
       su.push_back((int32_t)x);
       sv.push_back((int32_t)y);
       if (++read == e_num) break;
+      if (eu.size() >= CHUNK_EDGES) flush();  // bounded host memory; order is kept
     }
   }
   flush();

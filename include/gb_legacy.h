@@ -126,7 +126,9 @@ void init_node_id(t_csr *graph, lint *distrib, sint num_nodes);
  * the cross-part edge counts to <dir><par_extend><file>_send_<nodes_extension>.
  * The edge owners and counts are computed on the GPU (gb_edge_parts).  Both
  * need num_nodes and nodes_extension set as main.c does for -n; the files are
- * byte-identical to the reference's.  Endpoint ids must fit int32. */
+ * byte-identical to the reference's.  Endpoint ids must fit int32.  The
+ * edges go to the GPU in batches of 2^24 (GB_CHUNK_EDGES in the environment
+ * sets another size), so host memory stays bounded. */
 void write_recip(char *file, char *dir, lint *distrib, t_csr *graph_send);
 void chunk_files(char *file, char *dir, lint *distrib, lint *send_size, lint *recv_size);
 /* Replaces remove_duplicate, partition/csr_partition_bin_mpi.c:31-106

@@ -261,11 +261,15 @@ def test_remove_duplicate_vs_reference(gpu, case, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("batch", [0, 7])
 @pytest.mark.parametrize("case", load("chunk.json"), ids=lambda c: f"{c['name']}-n{c['num_nodes']}")
-def test_chunk_pipeline_vs_reference(gpu, case, tmp_path):
+def test_chunk_pipeline_vs_reference(gpu, case, batch, tmp_path, monkeypatch):
     """The serial chunking step as main.c -k runs it (scan_csr_idx SEND and
     RECV, comp_distrib_csr, write_recip, chunk_files) writes files
-    byte-identical to the reference's."""
+    byte-identical to the reference's; also when chunk_files sends the edges
+    to the GPU in batches of 7 (GB_CHUNK_EDGES) instead of 2^24."""
+    if batch:
+        monkeypatch.setenv("GB_CHUNK_EDGES", str(batch))
     L = legacy()
     lib = L.lib()
     d = str(tmp_path) + "/"
