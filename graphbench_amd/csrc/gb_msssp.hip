@@ -82,6 +82,7 @@ struct MsCtx {
   const int32_t *col;
   const uint32_t *w;
   int32_t V;
+  int64_t E;
   int32_t ntask;                   // sources in this group (<= W)
   uint32_t *dist;                  // [V][W]
   unsigned long long *mask[2];     // near masks, one per near queue
@@ -406,6 +407,15 @@ __global__ void ms_count_kernel(int64_t V, int32_t nsrc, const int32_t *__restri
   }
 }
 
+// Step-kernel blocks: per CU, 4 below 2^23 edges (small phases: fewer idle
+// blocks to launch and flush), 8 up to 2^25, 16 beyond (more waves in flight
+// in the large phases).  Measured: R-MAT 16 with 8 sources 3.3 / 3.7 / 4.7 ms
+// and R-MAT 22 with 32 sources 27.9 / 27.1 / 26.5 ms at 4 / 8 / 16.
+int ms_grid(int64_t E) {
+  const int per_cu = E < ((int64_t)1 << 23) ? 4 : E < ((int64_t)1 << 25) ? 8 : 16;
+  return gb_num_cus() * per_cu;
+}
+
 struct MsWork {
   DevBuf dist, m0, m1, far, q[4], cnt, ctl, src, wstat;
   void release() {
@@ -431,7 +441,7 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   GB_HIP(hipMemsetAsync(c.mask[1], 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.far, 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.cnt, 0, sizeof(unsigned long long) * MS_N, st));
-  const int grid = gb_num_cus() * 8;
+  const int grid = ms_grid(c.E);
   const int64_t nwaves = (int64_t)grid * (MS_BLOCK / GB_WAVE);
   GB_HIP(hipMemsetAsync(c.wstat, 0, sizeof(unsigned long long) * GB_WSTAT * nwaves, st));
   hipLaunchKernelGGL(ms_seed_kernel<W>, dim3(1), dim3(64), 0, st, c, d_src);
@@ -557,7 +567,7 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   GB_TRY(w.q[2].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.q[3].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
-  GB_TRY(w.wstat.ensure(sizeof(unsigned long long) * GB_WSTAT * gb_num_cus() * 8 *
+  GB_TRY(w.wstat.ensure(sizeof(unsigned long long) * GB_WSTAT * ms_grid(E) *
                         (MS_BLOCK / GB_WAVE)));
   GB_TRY(w.ctl.ensure(2 * sizeof(MsCtl)));
   GB_TRY(w.src.ensure(sizeof(int32_t) * 64));
@@ -566,6 +576,7 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   c.col = g->col;
   c.w = g->w;
   c.V = g->V;
+  c.E = g->E;
   c.dist = w.dist.as<uint32_t>();
   c.mask[0] = w.m0.as<unsigned long long>();
   c.mask[1] = w.m1.as<unsigned long long>();
