@@ -409,8 +409,9 @@ __global__ void ms_count_kernel(int64_t V, int32_t nsrc, const int32_t *__restri
 
 // Step-kernel blocks: per CU, 4 below 2^23 edges (small phases: fewer idle
 // blocks to launch and flush), 8 up to 2^25, 16 beyond (more waves in flight
-// in the large phases).  Measured: R-MAT 16 with 8 sources 3.3 / 3.7 / 4.7 ms
-// and R-MAT 22 with 32 sources 27.9 / 27.1 / 26.5 ms at 4 / 8 / 16.
+// in the large phases).  Measured at 4 / 8 / 16 blocks per CU: R-MAT 16 with
+// 8 sources 3.3 / 3.7-3.9 / 4.7 ms; R-MAT 22 with 32 sources 27.9 /
+// 26.3-27.2 / 26.3-26.5 ms.
 int ms_grid(int64_t E) {
   const int per_cu = E < ((int64_t)1 << 23) ? 4 : E < ((int64_t)1 << 25) ? 8 : 16;
   return gb_num_cus() * per_cu;
