@@ -124,6 +124,7 @@ _SIGS = {
     "gb_edges_unique": (_i32, [_i64, _p, _p, _p, _p, C.POINTER(_i64), _p]),
     "gb_edge_parts": (_i32, [_i64, _p, _p, _p, _i32, _p, _p, _p, _p]),
     "gb_graph_features": (_i32, [_p, _i32, _p, _p]),
+    "gb_graph_features_multi": (_i32, [_i32, _p, _i32, _p, _p]),
     "gb_pr_warm_start": (_i32, [_p, _p, _p, _p, _p]),
 }
 

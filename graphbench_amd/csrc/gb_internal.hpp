@@ -150,5 +150,6 @@ gb_status gb_wstat_fold(int64_t nwaves, const unsigned long long *slots, unsigne
 void gb_bc_release();
 void gb_msbc_release();
 void gb_mlp_release();
+void gb_learned_release();
 
 #endif

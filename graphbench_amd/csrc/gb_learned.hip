@@ -17,33 +17,6 @@
 
 namespace {
 
-__global__ void weight_stats_kernel(const uint32_t *__restrict__ w, int64_t E,
-                                    unsigned long long *__restrict__ sum) {
-  unsigned long long s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
-       i += (int64_t)gridDim.x * blockDim.x)
-    s += w[i];
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-  if (gb_lane() == 0) atomicAdd(sum, s);
-}
-
-__global__ void max_degree_kernel(int32_t V, const int32_t *__restrict__ rp,
-                                  unsigned int *__restrict__ out) {
-  unsigned int m = 0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    const unsigned int d = (unsigned int)(rp[v + 1] - rp[v]);
-    m = d > m ? d : m;
-  }
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    const unsigned int o = __shfl_xor(m, d, 64);
-    m = o > m ? o : m;
-  }
-  if (gb_lane() == 0) atomicMax(out, m);
-}
-
 __global__ void vertex_features_kernel(int32_t V, const int32_t *__restrict__ rp,
                                        const int32_t *__restrict__ rrp, float m0, float s0,
                                        float m1, float s1, float m2, float s2,
@@ -94,39 +67,129 @@ __global__ void scale_kernel(int32_t V, float *__restrict__ x, const double *__r
   if (v < V) x[v] = (float)((double)x[v] / *total);
 }
 
+// Per-graph inputs of the statistics kernel.
+struct GStat {
+  const uint32_t *w;
+  const int32_t *rp;
+  int64_t E;
+  int32_t V;
+};
+
+// Weight sum, largest weight and largest out-degree of n graphs in one
+// launch: blockIdx.y is the graph, the x blocks of a row stride over its
+// edges and vertices; out[3 * g + k] (zeroed first) takes the block results.
+__global__ __launch_bounds__(256) void graph_stats_multi_kernel(const GStat *__restrict__ gs,
+                                                                unsigned long long *__restrict__ out) {
+  const GStat G = gs[blockIdx.y];
+  unsigned long long s = 0, mw = 0, md = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G.E;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = G.w[i];
+    s += x;
+    mw = x > mw ? x : mw;
+  }
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < G.V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long d = (unsigned long long)(G.rp[v + 1] - G.rp[v]);
+    md = d > md ? d : md;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    s += __shfl_xor(s, d, 64);
+    const unsigned long long a = __shfl_xor(mw, d, 64), b = __shfl_xor(md, d, 64);
+    mw = a > mw ? a : mw;
+    md = b > md ? b : md;
+  }
+  if (gb_lane() == 0) {
+    unsigned long long *o = out + 3 * (int64_t)blockIdx.y;
+    if (s) atomicAdd(&o[0], s);
+    if (mw) atomicMax(&o[1], mw);
+    if (md) atomicMax(&o[2], md);
+  }
+}
+
+__global__ void fill_zero_u64_kernel(unsigned long long *a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0ull;
+}
+
+struct FeatWork {
+  DevBuf in, out;
+};
+FeatWork &g_featw() {  // heap singleton, never destroyed (as the other workspaces)
+  static FeatWork *w = new FeatWork();
+  return *w;
+}
+
+// Features from the reduced statistics (the layout of gb_graph_features).
+void features_from(const gb_graph g, int32_t nsrc, double sum_w, double max_w, double max_deg,
+                   float *f) {
+  const double V = g->V, E = (double)g->E, avg = E / V;
+  f[0] = (float)std::log2(V);
+  f[1] = (float)std::log2(avg);
+  f[2] = (float)std::log2(std::max(1.0, max_w));
+  f[3] = (float)std::log2(std::max(1.0, sum_w / E));
+  f[4] = (float)std::log2(std::max(1.0, max_deg / avg));
+  f[5] = (float)std::log2((double)nsrc);
+}
+
 }  // namespace
+
+void gb_learned_release() {
+  g_featw().in.release();
+  g_featw().out.release();
+}
+
+extern "C" gb_status gb_graph_features_multi(int32_t n, const gb_graph *graphs, int32_t nsrc,
+                                             float *h_feat, void *stream) {
+  if (n < 0 || (n > 0 && (!graphs || !h_feat)) || nsrc < 1 || n > 65535)
+    return gb_fail(GB_ERR_INVALID, "gb_graph_features_multi: bad arguments");
+  if (n == 0) return GB_OK;
+  std::vector<GStat> hs(n);
+  int64_t maxE = 1;
+  for (int32_t i = 0; i < n; i++) {
+    const gb_graph g = graphs[i];
+    if (!g || !g->rp || g->V <= 0 || g->E <= 0)
+      return gb_fail(GB_ERR_INVALID, "gb_graph_features_multi: graph %d: bad graph", i);
+    if (!g->w)
+      return gb_fail(GB_ERR_INVALID, "gb_graph_features_multi: graph %d has no uint32 weights", i);
+    hs[i] = GStat{g->w, g->rp, g->E, g->V};
+    maxE = std::max<int64_t>(maxE, std::max<int64_t>(g->E, g->V));
+  }
+  hipStream_t st = gb_stream(stream);
+  FeatWork &w = g_featw();
+  GB_TRY(w.in.ensure(sizeof(GStat) * n));
+  GB_TRY(w.out.ensure(sizeof(unsigned long long) * 3 * n));
+  GB_HIP(hipMemcpyAsync(w.in.p, hs.data(), sizeof(GStat) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(fill_zero_u64_kernel, dim3((unsigned)std::min<int64_t>((3 * n + 255) / 256, 1024)),
+                     dim3(256), 0, st, w.out.as<unsigned long long>(), (int64_t)3 * n);
+  // x blocks per graph: enough for the largest graph, the chip filled over all graphs
+  const int64_t bx = std::max<int64_t>(1, std::min<int64_t>((maxE + 255) / 256,
+                                                            std::max<int64_t>(1, gb_num_cus() * 8 / n)));
+  hipLaunchKernelGGL(graph_stats_multi_kernel, dim3((unsigned)bx, (unsigned)n), dim3(256), 0, st,
+                     w.in.as<GStat>(), w.out.as<unsigned long long>());
+  GB_HIP(hipGetLastError());
+  std::vector<unsigned long long> h((size_t)3 * n);
+  GB_HIP(hipMemcpyAsync(h.data(), w.out.p, sizeof(unsigned long long) * 3 * n,
+                        hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  for (int32_t i = 0; i < n; i++) {
+    const gb_graph g = graphs[i];
+    if (!g->max_w_valid) {  // the graph's cached largest weight (gb_graph_max_w)
+      g->max_w = (uint32_t)h[3 * i + 1];
+      g->max_w_valid = true;
+    }
+    features_from(g, nsrc, (double)h[3 * i], (double)h[3 * i + 1], (double)h[3 * i + 2],
+                  h_feat + 6 * (int64_t)i);
+  }
+  return GB_OK;
+}
 
 extern "C" gb_status gb_graph_features(gb_graph g, int32_t nsrc, float *h_feat, void *stream) {
   if (!g || !h_feat || !g->rp || g->V <= 0 || g->E <= 0 || nsrc < 1)
     return gb_fail(GB_ERR_INVALID, "gb_graph_features: bad arguments");
-  if (!g->w) return gb_fail(GB_ERR_INVALID, "gb_graph_features: graph has no uint32 weights");
-  hipStream_t st = gb_stream(stream);
-  uint32_t mw = 0;
-  GB_TRY(gb_graph_max_w(g, &mw, st));
-  DevBuf buf;
-  GB_TRY(buf.ensure(16));
-  GB_HIP(hipMemsetAsync(buf.p, 0, 16, st));
-  unsigned long long *sum = buf.as<unsigned long long>();
-  unsigned int *maxd = (unsigned int *)(sum + 1);
-  const int cap = gb_num_cus() * 8;
-  hipLaunchKernelGGL(weight_stats_kernel, dim3(std::max(1, (int)std::min<int64_t>((g->E + 255) / 256, cap))),
-                     dim3(256), 0, st, g->w, g->E, sum);
-  hipLaunchKernelGGL(max_degree_kernel, dim3(std::max(1, (int)std::min<int64_t>((g->V + 255) / 256, cap))),
-                     dim3(256), 0, st, g->V, g->rp, maxd);
-  GB_HIP(hipGetLastError());
-  unsigned long long h[2] = {0, 0};
-  GB_HIP(hipMemcpyAsync(h, buf.p, 16, hipMemcpyDeviceToHost, st));
-  GB_HIP(hipStreamSynchronize(st));
-  const double V = g->V, E = (double)g->E, avg = E / V;
-  const double mean_w = (double)h[0] / E;
-  const double maxdeg = (double)(uint32_t)h[1];
-  h_feat[0] = (float)std::log2(V);
-  h_feat[1] = (float)std::log2(avg);
-  h_feat[2] = (float)std::log2(std::max(1.0, (double)mw));
-  h_feat[3] = (float)std::log2(std::max(1.0, mean_w));
-  h_feat[4] = (float)std::log2(std::max(1.0, maxdeg / avg));
-  h_feat[5] = (float)std::log2((double)nsrc);
-  return GB_OK;
+  return gb_graph_features_multi(1, &g, nsrc, h_feat, stream);
 }
 
 extern "C" gb_status gb_pr_warm_start(gb_graph g, const float *d_params, const float *h_norm,

@@ -97,6 +97,7 @@ void gb_release_workspaces(void) {
   gb_bc_release();
   gb_msbc_release();
   gb_mlp_release();
+  gb_learned_release();
 }
 
 void gb_set_mem_cap(uint64_t bytes) {

@@ -120,6 +120,17 @@ def graph_features(g: Graph, nsrc: int) -> np.ndarray:
     return f
 
 
+def graph_features_multi(graphs, nsrc: int) -> np.ndarray:
+    """graph_features of many graphs with one launch: (n, 6) float32."""
+    import ctypes as C
+    n = len(graphs)
+    f = np.zeros((n, 6), np.float32)
+    if n:
+        hs = (C.c_void_p * n)(*[g.handle.value for g in graphs])
+        _check(lib().gb_graph_features_multi(n, hs, nsrc, f.ctypes.data, _stream()))
+    return f
+
+
 def plain_delta(h: HostCSR) -> int:
     """max_weight / average degree (the reference's rule), as an integer width."""
     return max(1, int(math.floor(float(h.w.max()) * h.V / max(h.E, 1))))
@@ -223,7 +234,7 @@ class LearnedModels:
         """(delta, light_max) per graph, with one batched forward pass."""
         torch = _torch()
         d = self.data["delta"]
-        F = np.stack([graph_features(g, nsrc) for g in graphs])
+        F = graph_features_multi(graphs, nsrc)
         xs = (F - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
         Y = self.delta_mlp.forward(torch.from_numpy(xs.astype(np.float32)).cuda()).cpu().numpy()
         Y = Y * np.array(d["y_std"]) + np.array(d["y_mean"])

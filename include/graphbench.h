@@ -275,6 +275,11 @@ gb_status gb_mlp_adam(float *d_params, const float *d_grad, float *d_m, float *d
  * log2 max weight, log2 mean weight, log2(max out-degree / avg degree),
  * log2 nsrc. */
 gb_status gb_graph_features(gb_graph g, int32_t nsrc, float *h_feat, void *stream);
+/* The same for n graphs at once (h_feat: n rows of 6), one launch and one
+ * copy back; also caches each graph's largest weight.  1 <= n <= 65535 (n = 0
+ * is a no-op). */
+gb_status gb_graph_features_multi(int32_t n, const gb_graph *graphs, int32_t nsrc, float *h_feat,
+                                  void *stream);
 /* PageRank warm start from the warm-start model (din 3, dout 1): h_norm holds
  * the feature means[3], stds[3] and the target mean, std.  d_x0: V floats,
  * normalised to sum 1 (pass it as gb_pr_params.d_x0). */

@@ -74,6 +74,28 @@ def test_mlp_loss_grad_vs_oracle(gpu, n, din, dout):
 
 
 @pytest.mark.gpu
+def test_graph_features_multi_vs_host(gpu):
+    """The delta model's inputs for several graphs in one launch equal the
+    same statistics computed on the host from the CSR (log2 V, log2 average
+    degree, log2 max and mean weight, log2 max out-degree / average degree,
+    log2 sources), and the single-graph call gives the same row."""
+    hs = [gb.gen_rmat(12, 8, seed=3, weights="uniform", wlo=1, whi=255),
+          gb.gen_er(3000, 5, seed=4, weights="uniform", wlo=7, whi=7),
+          gb.gen_rmat(14, 16, seed=5, weights="uniform", wlo=1, whi=65535)]
+    gs = [gb.Graph.from_host(h, need_reverse=False) for h in hs]
+    F = learned.graph_features_multi(gs, 8)
+    for h, g, f in zip(hs, gs, F):
+        deg = np.diff(h.row_ptr.astype(np.int64))
+        avg = h.E / h.V
+        ref = [np.log2(h.V), np.log2(avg), np.log2(max(1, int(h.w.max()))),
+               np.log2(max(1.0, float(h.w.astype(np.float64).sum()) / h.E)),
+               np.log2(max(1.0, deg.max() / avg)), np.log2(8)]
+        np.testing.assert_allclose(f, np.array(ref, np.float32), rtol=1e-6, atol=1e-6)
+        assert np.array_equal(learned.graph_features(g, 8), f)
+    assert learned.graph_features_multi([], 8).shape == (0, 6)
+
+
+@pytest.mark.gpu
 def test_mlp_forward_vs_torch(gpu):
     torch = _torch()
     m = learned.MLP(6, 2, seed=3)
