@@ -90,6 +90,8 @@ const char *gb_status_string(gb_status s) {
 
 const char *gb_last_error(void) { return g_err; }
 
+static void split_release();  // the light/heavy split's scratch (below)
+
 void gb_release_workspaces(void) {
   gb_pr_release();
   gb_sssp_release();
@@ -98,6 +100,7 @@ void gb_release_workspaces(void) {
   gb_msbc_release();
   gb_mlp_release();
   gb_learned_release();
+  split_release();
 }
 
 void gb_set_mem_cap(uint64_t bytes) {
@@ -619,14 +622,29 @@ static int split_grid(int64_t n, int64_t per_thread) {
   return (int)std::max<int64_t>(1, std::min(blocks, cap));
 }
 
+// Scratch of the light/heavy split, grown on demand and reused: a batch of
+// graphs would otherwise allocate and free three E-sized buffers per graph.
+struct SplitWork {
+  DevBuf flags, lrank, tmp;
+};
+static SplitWork &g_splitw() {  // heap singleton, never destroyed
+  static SplitWork *w = new SplitWork();
+  return *w;
+}
+static void split_release() {
+  g_splitw().flags.release();
+  g_splitw().lrank.release();
+  g_splitw().tmp.release();
+}
+
 // light/heavy split of a CSR into col_out / w_out / le (see above); returns
-// after the work is enqueued and its scratch released (stream-ordered)
+// after the work has run (the scratch is reused by the next split)
 template <class W, class Pred>
 static gb_status split_rows(int32_t V, int64_t E, const int32_t *rp, const int32_t *col,
                             const W *w, Pred light, int32_t *col_out, W *w_out, int32_t *le,
                             hipStream_t st) {
   if (V <= 0) return GB_OK;
-  DevBuf flags, lrank, tmp;
+  DevBuf &flags = g_splitw().flags, &lrank = g_splitw().lrank, &tmp = g_splitw().tmp;
   GB_TRY(lrank.ensure(sizeof(int32_t) * ((size_t)E + 1)));
   GB_HIP(hipMemsetAsync(lrank.p, 0, sizeof(int32_t), st));
   if (E > 0) {
@@ -648,7 +666,7 @@ static gb_status split_rows(int32_t V, int64_t E, const int32_t *rp, const int32
   hipLaunchKernelGGL(light_end_kernel, dim3(split_grid(V, 4)), dim3(256), 0, st, V, rp,
                      lrank.as<int32_t>(), le);
   GB_HIP(hipGetLastError());
-  GB_HIP(hipStreamSynchronize(st));  // the scratch buffers are freed on return
+  GB_HIP(hipStreamSynchronize(st));  // the scratch is reused by the next split
   return GB_OK;
 }
 
