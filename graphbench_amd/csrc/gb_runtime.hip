@@ -480,6 +480,22 @@ __global__ void max_u32_kernel(const uint32_t *__restrict__ a, int64_t n, unsign
   }
 }
 
+// Scratch of the light/heavy split and of the cached graph statistics below,
+// grown on demand and reused: a batch of graphs would otherwise allocate and
+// free device buffers per graph.  Every user synchronises before returning.
+struct SplitWork {
+  DevBuf flags, lrank, tmp;
+};
+static SplitWork &g_splitw() {  // heap singleton, never destroyed
+  static SplitWork *w = new SplitWork();
+  return *w;
+}
+static void split_release() {
+  g_splitw().flags.release();
+  g_splitw().lrank.release();
+  g_splitw().tmp.release();
+}
+
 gb_status gb_graph_max_outdeg(gb_graph g, int32_t *out, hipStream_t st) {
   if (g->max_outdeg >= 0) {
     *out = g->max_outdeg;
@@ -489,7 +505,7 @@ gb_status gb_graph_max_outdeg(gb_graph g, int32_t *out, hipStream_t st) {
     *out = g->max_outdeg = 0;
     return GB_OK;
   }
-  DevBuf deg, tmp;
+  DevBuf &deg = g_splitw().flags, &tmp = g_splitw().tmp;  // scratch, reused (synced below)
   GB_TRY(deg.ensure(sizeof(int32_t) * g->V));
   GB_TRY(tmp.ensure(sizeof(unsigned int)));
   GB_TRY(gb_graph_outdeg(g, deg.as<int32_t>(), st));
@@ -516,7 +532,7 @@ gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st) {
     return GB_OK;
   }
   if (!g->w) return gb_fail(GB_ERR_INVALID, "graph has no uint32 weights");
-  DevBuf tmp;
+  DevBuf &tmp = g_splitw().tmp;  // scratch, reused (synced below)
   GB_TRY(tmp.ensure(sizeof(unsigned int)));
   GB_HIP(hipMemsetAsync(tmp.p, 0, sizeof(unsigned int), st));
   int blocks = (int)std::min<int64_t>((g->E + 255) / 256, (int64_t)gb_num_cus() * 4);
@@ -620,21 +636,6 @@ static int split_grid(int64_t n, int64_t per_thread) {
   const int64_t blocks = (n + 256 * per_thread - 1) / (256 * per_thread);
   const int64_t cap = (int64_t)gb_num_cus() * 16;
   return (int)std::max<int64_t>(1, std::min(blocks, cap));
-}
-
-// Scratch of the light/heavy split, grown on demand and reused: a batch of
-// graphs would otherwise allocate and free three E-sized buffers per graph.
-struct SplitWork {
-  DevBuf flags, lrank, tmp;
-};
-static SplitWork &g_splitw() {  // heap singleton, never destroyed
-  static SplitWork *w = new SplitWork();
-  return *w;
-}
-static void split_release() {
-  g_splitw().flags.release();
-  g_splitw().lrank.release();
-  g_splitw().tmp.release();
 }
 
 // light/heavy split of a CSR into col_out / w_out / le (see above); returns
