@@ -410,6 +410,14 @@ int32_t hub_edges() {
   return v >= 64 ? v : 1024;
 }
 
+// Edges per backward hub chunk (one wave each): GB_BC_BCHUNK, else the hub
+// threshold.
+int32_t bc_bchunk(int32_t hub) {
+  const char *e = getenv("GB_BC_BCHUNK");
+  const int v = e ? atoi(e) : 0;
+  return v >= 64 ? v : hub;
+}
+
 int wgrid(uint64_t n) {
   uint64_t b = (n + 3) / 4;
   uint64_t cap = (uint64_t)gb_num_cus() * 8;
@@ -481,7 +489,8 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
   GB_HIP(hipStreamSynchronize(st));
   const int32_t nhub_in = (int32_t)hmh[0], nhub_out = (int32_t)hmh[2];
   const int32_t maxch_in = ((int32_t)(uint32_t)hmh[1] + hub - 1) / hub;
-  const int32_t maxch_out = ((int32_t)(uint32_t)hmh[3] + hub - 1) / hub;
+  const int32_t bch = bc_bchunk(hub);
+  const int32_t maxch_out = ((int32_t)(uint32_t)hmh[3] + bch - 1) / bch;
   if (with_sigma && nhub_out > 0)
     GB_TRY(w.part.ensure(sizeof(double) * 64 * (size_t)G * nhub_out * maxch_out));
   // candidates: vertices with 1..hub in-edges, the same list for every group
@@ -593,12 +602,12 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
     GB_HIP(hipGetLastError());
     if (nhub_out > 0) {
       hipLaunchKernelGGL(mbc_backward_hub_kernel, dim3(wgrid((uint64_t)nhub_out * maxch_out), G),
-                         dim3(MB), 0, st, V, hub, rp, col, w.hub_out.as<int32_t>(), nhub_out,
+                         dim3(MB), 0, st, V, bch, rp, col, w.hub_out.as<int32_t>(), nhub_out,
                          maxch_out, w.levels[dd]->fm.as<unsigned long long>(),
                          w.levels[dd + 1]->fm.as<unsigned long long>(), w.sigma.as<double>(),
                          w.delta.as<double>(), w.part.as<double>());
       hipLaunchKernelGGL(mbc_backward_combine_kernel, dim3((nhub_out + 3) / 4, G), dim3(MB), 0, st,
-                         V, hub, rp, w.hub_out.as<int32_t>(), nhub_out, maxch_out,
+                         V, bch, rp, w.hub_out.as<int32_t>(), nhub_out, maxch_out,
                          w.levels[dd]->fm.as<unsigned long long>(), w.part.as<double>(),
                          w.delta.as<double>());
       GB_HIP(hipGetLastError());
