@@ -575,25 +575,36 @@ __global__ void light_end_kernel(int32_t V, const int32_t *__restrict__ rp,
     le[v] = rp[v] + (lrank[rp[v + 1]] - lrank[rp[v]]);
 }
 
-// largest row r in [lo, hi] with rp[r] <= i (the row holding edge i when
-// rp[lo] <= i < rp[hi + 1])
-__device__ __forceinline__ int32_t row_of_edge(const int32_t *__restrict__ rp, int32_t lo,
-                                               int32_t hi, int64_t i) {
+constexpr int SPLIT_IPT = 8;  // edges per lane and wave tile
+
+// Last row r in [lo, hi] with rp[r] <= i, given rp[lo] <= i (rp ascending):
+// the whole wave searches together, 64 probes per step (lane j reads
+// rp[lo + (j + 1) * span / 64]), so a range of V rows takes about log64 V
+// dependent reads.  lo, hi and i are the same in every lane.
+__device__ __forceinline__ int32_t wave_row_of_edge(const int32_t *__restrict__ rp, int32_t lo,
+                                                    int32_t hi, int64_t i) {
+  const int lane = gb_lane();
   while (lo < hi) {
-    const int32_t mid = (int32_t)(((int64_t)lo + hi + 1) >> 1);
-    if (rp[mid] <= i)
-      lo = mid;
-    else
-      hi = mid - 1;
+    const int64_t span = (int64_t)hi - lo;
+    const int32_t p = lo + (int32_t)((span * (lane + 1)) / GB_WAVE);  // p_63 == hi
+    const bool ok = rp[p] <= i;  // true on a prefix of the lanes
+    const int c = __popcll(__ballot(ok));
+    const int32_t pl = __shfl(p, c > 0 ? c - 1 : 0, GB_WAVE);
+    const int32_t ph = __shfl(p, c < GB_WAVE ? c : GB_WAVE - 1, GB_WAVE);
+    const int32_t nlo = c > 0 ? pl : lo;
+    hi = c < GB_WAVE ? ph - 1 : hi;
+    lo = nlo;
   }
   return lo;
 }
 
-constexpr int SPLIT_IPT = 8;  // edges per lane and wave tile
-
 // A wave takes tiles of 64 * SPLIT_IPT consecutive edges (lane-consecutive
-// edges, so loads and most stores coalesce); the rows spanned by the tile are
-// found once per tile, each edge's row by a short search among them.
+// within each of SPLIT_IPT steps).  The tile's first and last rows come from
+// two wave-wide searches; each lane then finds the rows of its SPLIT_IPT edges
+// with fixed-step binary searches run side by side (branch-free, on clamped
+// edge indices, so each step's reads are one batch), reads the edges and the
+// rows' bounds in two batches, and writes every edge to its slot: light edges
+// first in each row, both parts in row order.
 template <class W, class Pred>
 __global__ void split_scatter_kernel(int32_t V, int64_t E, const int32_t *__restrict__ rp,
                                      const int32_t *__restrict__ col, const W *__restrict__ w,
@@ -605,19 +616,58 @@ __global__ void split_scatter_kernel(int32_t V, int64_t E, const int32_t *__rest
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t t0 = wave * TILE; t0 < E; t0 += nwaves * TILE) {
     const int64_t t1 = t0 + TILE < E ? t0 + TILE : E;
-    const int32_t r_lo = row_of_edge(rp, 0, V - 1, t0);
-    const int32_t r_hi = row_of_edge(rp, r_lo, V - 1, t1 - 1);
+    const int32_t r_lo = wave_row_of_edge(rp, 0, V - 1, t0);
+    const int32_t r_hi = wave_row_of_edge(rp, r_lo, V - 1, t1 - 1);
+    int64_t ie[SPLIT_IPT];
+    int32_t lo[SPLIT_IPT], hi[SPLIT_IPT];
+#pragma unroll
+    for (int k = 0; k < SPLIT_IPT; k++) {
+      const int64_t i = t0 + (int64_t)k * GB_WAVE + lane;
+      ie[k] = i < t1 ? i : t1 - 1;
+      lo[k] = r_lo;
+      hi[k] = r_hi;
+    }
+    // steps: enough for a range of r_hi - r_lo + 1 rows
+    const uint32_t span = (uint32_t)(r_hi - r_lo);
+    const int steps = span ? 33 - __clz(span) : 0;
+    for (int s = 0; s < steps; s++) {
+      int32_t x[SPLIT_IPT], mid[SPLIT_IPT];
+#pragma unroll
+      for (int k = 0; k < SPLIT_IPT; k++) {
+        mid[k] = (int32_t)(((int64_t)lo[k] + hi[k] + 1) >> 1);  // lo when lo == hi
+        x[k] = rp[mid[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < SPLIT_IPT; k++) {
+        const bool le = x[k] <= ie[k];
+        lo[k] = le ? mid[k] : lo[k];
+        hi[k] = le ? hi[k] : mid[k] - 1;
+      }
+    }
+    int32_t rs[SPLIT_IPT], re[SPLIT_IPT], lr[SPLIT_IPT], ci[SPLIT_IPT];
+    W wi[SPLIT_IPT];
+#pragma unroll
+    for (int k = 0; k < SPLIT_IPT; k++) {
+      rs[k] = rp[lo[k]];
+      re[k] = rp[lo[k] + 1];
+      lr[k] = lrank[ie[k]];
+      wi[k] = w[ie[k]];
+      ci[k] = col[ie[k]];
+    }
+    int32_t lrs[SPLIT_IPT], lre[SPLIT_IPT];
+#pragma unroll
+    for (int k = 0; k < SPLIT_IPT; k++) {
+      lrs[k] = lrank[rs[k]];
+      lre[k] = lrank[re[k]];
+    }
 #pragma unroll
     for (int k = 0; k < SPLIT_IPT; k++) {
       const int64_t i = t0 + (int64_t)k * GB_WAVE + lane;
       if (i < t1) {
-        const int32_t v = row_of_edge(rp, r_lo, r_hi, i);
-        const int32_t rs = rp[v];
-        const int32_t lrs = lrank[rs], nl = lrank[rp[v + 1]] - lrs, lr = lrank[i] - lrs;
-        const W wi = w[i];
-        const int64_t dst = light(wi) ? rs + lr : rs + nl + (i - rs) - lr;
-        col_out[dst] = col[i];
-        w_out[dst] = wi;
+        const int32_t nl = lre[k] - lrs[k], l = lr[k] - lrs[k];
+        const int64_t dst = light(wi[k]) ? rs[k] + l : rs[k] + nl + (i - rs[k]) - l;
+        col_out[dst] = ci[k];
+        w_out[dst] = wi[k];
       }
     }
   }
