@@ -164,9 +164,9 @@ extern "C" gb_status gb_graph_features_multi(int32_t n, const gb_graph *graphs, 
   GB_HIP(hipMemcpyAsync(w.in.p, hs.data(), sizeof(GStat) * n, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(fill_zero_u64_kernel, dim3((unsigned)std::min<int64_t>((3 * n + 255) / 256, 1024)),
                      dim3(256), 0, st, w.out.as<unsigned long long>(), (int64_t)3 * n);
-  // x blocks per graph: enough for the largest graph, the chip filled over all graphs
-  const int64_t bx = std::max<int64_t>(1, std::min<int64_t>((maxE + 255) / 256,
-                                                            std::max<int64_t>(1, gb_num_cus() * 8 / n)));
+  // x blocks per graph, from the largest graph: about 256 items per thread,
+  // at most 256 blocks (the blocks of smaller graphs finish at once)
+  const int64_t bx = std::max<int64_t>(1, std::min<int64_t>((maxE + 65535) / 65536, 256));
   hipLaunchKernelGGL(graph_stats_multi_kernel, dim3((unsigned)bx, (unsigned)n), dim3(256), 0, st,
                      w.in.as<GStat>(), w.out.as<unsigned long long>());
   GB_HIP(hipGetLastError());
