@@ -289,7 +289,7 @@ gb_status gb_pr_warm_start(gb_graph g, const float *d_params, const float *h_nor
 /* ---- synthetic graphs (host) -------------------------------------------------
  * Deterministic, counter-based generators used by tests and benchmarks.
  * Rows are sorted by (destination, weight).  Output arrays are host memory
- * allocated by the library (free with gb_host_free). */
+ * allocated by the library (free with gb_host_csr_free). */
 typedef struct gb_host_csr {
   int32_t V;
   int64_t E;
