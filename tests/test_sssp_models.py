@@ -26,8 +26,10 @@ def test_state_machine_model_matches_dijkstra(tmp_path, model, trials):
         pytest.skip("hipcc not available")
     exe = tmp_path / model
     src = os.path.join(HERE, "models", model + ".hip")
-    subprocess.run([HIPCC, "-O2", "-std=c++17", "-I", CSRC, "-o", str(exe), src], check=True,
-                   capture_output=True)
+    # the models run on the host; the device flags match the library's build
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950", "-Xclang",
+                    "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-o", str(exe),
+                    src], check=True, capture_output=True)
     r = subprocess.run([str(exe), str(trials)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("ok"), r.stdout
