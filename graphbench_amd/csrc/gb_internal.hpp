@@ -114,7 +114,9 @@ gb_status gb_graph_max_outdeg(gb_graph g, int32_t *out, hipStream_t st);
 // Max weight of the graph's uint32 weights (cached).
 gb_status gb_graph_max_w(gb_graph g, uint32_t *out, hipStream_t st);
 // Light-first ordering of the forward CSR for a threshold (cached per handle).
-gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st);
+// sync false: the split stays queued on st (later splits on st only; see
+// split_rows)
+gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st, bool sync = true);
 gb_status gb_graph_split64(gb_graph g, double light_max, hipStream_t st);
 // Reference-form PageRank with a hook called at the top of every iteration
 // with the device wb array (before that iteration's gather).

@@ -688,25 +688,32 @@ static int split_grid(int64_t n, int64_t per_thread) {
   return (int)std::max<int64_t>(1, std::min(blocks, cap));
 }
 
-// light/heavy split of a CSR into col_out / w_out / le (see above); returns
-// after the work has run (the scratch is reused by the next split)
+// light/heavy split of a CSR into col_out / w_out / le (see above).  The
+// scratch is shared by all splits: with sync it returns after the work has
+// run; without, the work stays queued on st, and a later split may queue
+// behind it on the same stream only (a scratch buffer that must grow waits
+// for the stream before it is freed).
 template <class W, class Pred>
 static gb_status split_rows(int32_t V, int64_t E, const int32_t *rp, const int32_t *col,
                             const W *w, Pred light, int32_t *col_out, W *w_out, int32_t *le,
-                            hipStream_t st) {
+                            hipStream_t st, bool sync) {
   if (V <= 0) return GB_OK;
   DevBuf &flags = g_splitw().flags, &lrank = g_splitw().lrank, &tmp = g_splitw().tmp;
-  GB_TRY(lrank.ensure(sizeof(int32_t) * ((size_t)E + 1)));
+  auto reserve = [st](DevBuf &b, size_t n) -> gb_status {
+    if (n > b.bytes) GB_HIP(hipStreamSynchronize(st));
+    return b.ensure(n);
+  };
+  GB_TRY(reserve(lrank, sizeof(int32_t) * ((size_t)E + 1)));
   GB_HIP(hipMemsetAsync(lrank.p, 0, sizeof(int32_t), st));
   if (E > 0) {
-    GB_TRY(flags.ensure(sizeof(int32_t) * (size_t)E));
+    GB_TRY(reserve(flags, sizeof(int32_t) * (size_t)E));
     hipLaunchKernelGGL((light_flags_kernel<W, Pred>), dim3(split_grid(E, 4)), dim3(256), 0, st, E,
                        w, light, flags.as<int32_t>());
     GB_HIP(hipGetLastError());
     size_t tb = 0;
     GB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, flags.as<int32_t>(),
                                             lrank.as<int32_t>() + 1, (int)E, st));
-    GB_TRY(tmp.ensure(tb + 16));
+    GB_TRY(reserve(tmp, tb + 16));
     GB_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, flags.as<int32_t>(),
                                             lrank.as<int32_t>() + 1, (int)E, st));
     hipLaunchKernelGGL((split_scatter_kernel<W, Pred>), dim3(split_grid(E, 4 * SPLIT_IPT)),
@@ -717,11 +724,11 @@ static gb_status split_rows(int32_t V, int64_t E, const int32_t *rp, const int32
   hipLaunchKernelGGL(light_end_kernel, dim3(split_grid(V, 4)), dim3(256), 0, st, V, rp,
                      lrank.as<int32_t>(), le);
   GB_HIP(hipGetLastError());
-  GB_HIP(hipStreamSynchronize(st));  // the scratch is reused by the next split
+  if (sync) GB_HIP(hipStreamSynchronize(st));
   return GB_OK;
 }
 
-gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st) {
+gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st, bool sync) {
   if (g->split_valid && g->split_light_max == light_max) return GB_OK;
   if (!g->w && g->E > 0) return gb_fail(GB_ERR_INVALID, "graph has no uint32 weights");
   GB_TRY(g->split_col.ensure(sizeof(int32_t) * (g->E + 1)));
@@ -729,7 +736,7 @@ gb_status gb_graph_split(gb_graph g, uint32_t light_max, hipStream_t st) {
   GB_TRY(g->split_le.ensure(sizeof(int32_t) * (g->V + 1)));
   GB_TRY(split_rows<uint32_t>(g->V, g->E, g->rp, g->col, g->w, LightU32{light_max},
                               g->split_col.as<int32_t>(), g->split_w.as<uint32_t>(),
-                              g->split_le.as<int32_t>(), st));
+                              g->split_le.as<int32_t>(), st, sync));
   g->split_valid = true;
   g->split_light_max = light_max;
   return GB_OK;
@@ -743,7 +750,7 @@ gb_status gb_graph_split64(gb_graph g, double light_max, hipStream_t st) {
   GB_TRY(g->split64_le.ensure(sizeof(int32_t) * (g->V + 1)));
   GB_TRY(split_rows<double>(g->V, g->E, g->rp, g->col, g->w64, LightF64{light_max},
                             g->split64_col.as<int32_t>(), g->split64_w.as<double>(),
-                            g->split64_le.as<int32_t>(), st));
+                            g->split64_le.as<int32_t>(), st, true));
   g->split64_valid = true;
   g->split64_delta = light_max;
   return GB_OK;
@@ -755,5 +762,5 @@ extern "C" gb_status gb_separate(gb_graph g, uint32_t light_max, int32_t *d_col_
     return gb_fail(GB_ERR_INVALID, "gb_separate: bad arguments");
   hipStream_t st = gb_stream(stream);
   return split_rows<uint32_t>(g->V, g->E, g->rp, g->col, g->w, LightU32{light_max}, d_col_out,
-                              d_w_out, d_light_end, st);
+                              d_w_out, d_light_end, st, true);
 }

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <unordered_map>
 #include <vector>
 
 #include "gb_device.hpp"
@@ -753,14 +754,15 @@ uint32_t plain_delta(uint32_t max_w, int32_t V, int64_t E) {
 // The graph's rows as light and heavy parts for threshold lm.  When no weight
 // exceeds lm every edge is light: the graph's own arrays serve, with the row
 // end as the light end, and nothing is copied.
-gb_status light_rows(gb_graph g, uint32_t lm, hipStream_t st, SGraph<uint32_t> &out) {
+gb_status light_rows(gb_graph g, uint32_t lm, hipStream_t st, SGraph<uint32_t> &out,
+                     bool sync = true) {
   uint32_t mw = 0;
   GB_TRY(gb_graph_max_w(g, &mw, st));
   if (lm >= mw) {
     out = SGraph<uint32_t>{g->rp, g->rp + 1, g->col, g->w, g->V, 0, g->E};
     return GB_OK;
   }
-  GB_TRY(gb_graph_split(g, lm, st));
+  GB_TRY(gb_graph_split(g, lm, st, sync));
   out = SGraph<uint32_t>{g->rp, g->split_le.as<int32_t>(), g->split_col.as<int32_t>(),
                          g->split_w.as<uint32_t>(), g->V, 0, g->E};
   return GB_OK;
@@ -853,20 +855,37 @@ extern "C" gb_status gb_sssp_multi(int32_t ngraphs, const gb_graph *gs, const in
   hipStream_t st = gb_stream(stream);
   std::vector<SGraph<uint32_t>> hg(ngraphs);
   std::vector<STask<TU32>> ht(ngraphs);
-  for (int32_t i = 0; i < ngraphs; i++) {
+  // a handle caches one split: a handle repeated with another threshold that
+  // needs a split would find its rows overwritten
+  std::unordered_map<gb_graph, uint32_t> split_lm;
+  auto prep = [&](int32_t i) -> gb_status {
     gb_graph g = gs[i];
     if (!g || (!g->w && g->E > 0) || sources[i] < 0 || sources[i] >= g->V || !d_dist[i])
       return gb_fail(GB_ERR_INVALID, "gb_sssp_multi: bad graph %d", i);
+    uint32_t mw = 0;
+    GB_TRY(gb_graph_max_w(g, &mw, st));
     uint32_t delta = deltas ? deltas[i] : 0;
-    if (delta == 0) {
-      uint32_t mw = 0;
-      GB_TRY(gb_graph_max_w(g, &mw, st));
-      delta = plain_delta(mw, g->V, g->E);
-    }
+    if (delta == 0) delta = plain_delta(mw, g->V, g->E);
     uint32_t lm = light_max && light_max[i] ? light_max[i] : delta;
+    if (lm < mw) {
+      auto it = split_lm.find(g);
+      if (it != split_lm.end() && it->second != lm)
+        return gb_fail(GB_ERR_INVALID,
+                       "gb_sssp_multi: graph %d repeats a handle with another light threshold", i);
+      split_lm.emplace(g, lm);
+    }
     GB_TRY(check_degree(g, st));
-    GB_TRY(light_rows(g, lm, st, hg[i]));
+    // the splits stay queued on st, ahead of the solve (no wait per graph)
+    GB_TRY(light_rows(g, lm, st, hg[i], false));
     ht[i] = STask<TU32>{d_dist[i], 0, (uint64_t)delta, 0, i, sources[i]};
+    return GB_OK;
+  };
+  for (int32_t i = 0; i < ngraphs; i++) {
+    const gb_status s = prep(i);
+    if (s != GB_OK) {
+      (void)hipStreamSynchronize(st);  // queued splits end before the scratch is free
+      return s;
+    }
   }
   return sssp_engine<TU32>(hg, ht, 0, stats, st);
 }

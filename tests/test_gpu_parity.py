@@ -188,6 +188,26 @@ def test_sssp_multi_graphs(gpu):
         assert np.array_equal(gb.u32(o), oracle.dijkstra(h.row_ptr, h.col, h.w, s))
 
 
+def test_sssp_multi_repeated_handles(gpu):
+    """One handle twice: the same light threshold shares the handle's cached
+    split; two thresholds that both need a split are refused (a handle holds
+    one split), and the handle still solves afterwards."""
+    gb = gpu
+    _say("sssp multi repeated handles")
+    h = gb.gen_rmat(11, 8, seed=321, weights="uniform", wlo=1, whi=1023)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    s = gb.pick_sources(h, 2, 9)
+    srcs = [int(s[0]), int(s[1])]
+    outs = gb.sssp_multi([g, g], srcs, deltas=[64, 64], light_max=[64, 64])
+    for x, o in zip(srcs, outs):
+        assert np.array_equal(gb.u32(o), oracle.dijkstra(h.row_ptr, h.col, h.w, x))
+    with pytest.raises(gb.GBError) as ei:
+        gb.sssp_multi([g, g], srcs, deltas=[64, 128], light_max=[64, 128])
+    assert ei.value.status == 1 and "light threshold" in str(ei.value)
+    outs = gb.sssp_multi([g], srcs[:1], deltas=[128], light_max=[128])
+    assert np.array_equal(gb.u32(outs[0]), oracle.dijkstra(h.row_ptr, h.col, h.w, srcs[0]))
+
+
 def test_sssp_overflow_reported(gpu):
     gb = gpu
     _say("sssp overflow")
