@@ -9,8 +9,9 @@ in HBM.  value = GTEPS = E * iterations / step time.
 The JSON line also carries the roofline of the dominant kernel (pr_gather:
 algorithmic bytes per launch / mean launch time from HIP events recorded on
 the launch stream inside the timed region), the CPU baseline (the oracle's
-FP64 OpenMP power iteration on the host cores, same graph) and secondary
-measurements for SSSP and Brandes in "extra".
+FP64 OpenMP power iteration on the host cores, same graph, median of 5) and,
+in "extra", the other four workloads (graphbench_amd/workloads.py), each with
+a CPU baseline of the oracle on a bounded sample of the same work.
 """
 from __future__ import annotations
 
@@ -19,6 +20,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -33,14 +35,87 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host cores this process may run on: logical CPUs in its affinity
+    mask, the physical cores among them, the CPU model, and the thread count
+    the CPU baselines use (one per physical core, within OMP_NUM_THREADS when
+    the environment sets it)."""
+    aff = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in aff:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "core_id") as f1, open(base + "physical_package_id") as f2:
+                cores.add((f2.read().strip(), f1.read().strip()))
+        except OSError:
+            cores.add(("cpu", c))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = len(cores)
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return dict(nproc=os.cpu_count(), affinity_cpus=len(aff), physical_cores=len(cores),
+                model=model, omp_num_threads=omp or None, threads=max(1, threads))
+
+
+def median_time(fn, runs=5, warmup=1):
+    """Median wall time of `runs` calls after `warmup` untimed ones; returns
+    (median seconds, all times, last result)."""
+    import numpy as np
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), ts, out
+
+
+def cpu_record(value, unit, cpu, threads, sample, runs, **kw):
+    rec = dict(value=value, unit=unit, cores=threads, kind="port", sample=sample,
+               timing=f"median of {runs} runs after 1 untimed run" if runs > 1 else "one timed run",
+               host={k: cpu[k] for k in ("nproc", "affinity_cpus", "physical_cores", "model",
+                                         "omp_num_threads")})
+    rec.update(kw)
+    return rec
+
+
+def cpu_sssp(oracle, np, graphs, threads, runs, warmup):
+    """The oracle's delta-stepping (run_sspath_arr restated), one (graph,
+    source, delta) job per host thread; returns (median s, traversed edges)."""
+    def one(job):
+        h, s, delta = job
+        return oracle.delta_stepping(h.row_ptr, h.col, h.w, int(s), int(delta))
+
+    def run():
+        with ThreadPoolExecutor(threads) as ex:
+            return list(ex.map(one, graphs))
+    med, ts, outs = median_time(run, runs, warmup)
+    trav = 0
+    for (h, _, _), d in zip(graphs, outs):
+        trav += int(h.outdeg()[d != oracle.UINF].astype(np.int64).sum())
+    return med, trav
+
+
 def pagerank_bytes(V, E):
     # row_ptr 4(V+1) + col 4E + gathered contrib 4E + x_old 4V + 1/outdeg 4V + x_new 4V
     return 8 * E + 16 * V + 4
 
 
 def run_pagerank(gb, torch, np, args):
+    from graphbench_amd import workloads
     t0 = time.time()
-    h = gb.gen_rmat(22, 16, seed=7, weights="none")
+    h = workloads.c2()
     r = h.transpose()
     log(f"generated R-MAT 22: V={h.V} E={h.E} in {time.time() - t0:.1f}s")
     g = gb.Graph.from_host(h, reverse=r)
@@ -92,29 +167,32 @@ def run_pagerank(gb, torch, np, args):
     return h, r, out, x_host
 
 
-def cpu_baseline(oracle, np, h, r, x_gpu, budget_s):
-    threads = min(16, os.cpu_count() or 1)
+def cpu_baseline(oracle, np, h, r, x_gpu, budget_s, cpu):
+    """The oracle's FP64 power iteration (OpenMP over rows, deterministic
+    chunked reductions) on the same graph: a full solve when 6 of them fit the
+    budget (median of 5 after 1 warm-up), else a bounded number of
+    iterations."""
+    threads = cpu["threads"]
     oracle.set_threads(threads)
     od = h.outdeg()
     t = time.perf_counter()
     oracle.pagerank_cfg(r.row_ptr, r.col, od, 0.85, 1e-7, 2)
     per_it = (time.perf_counter() - t) / 2
-    full = per_it * 100 <= budget_s
-    k = 100 if full else max(2, int(budget_s / per_it))
-    t = time.perf_counter()
-    xr, it, rr = oracle.pagerank_cfg(r.row_ptr, r.col, od, 0.85, 1e-7, k)
-    el = time.perf_counter() - t
-    res = dict(value=h.E * it / el / 1e9, unit="GTEPS", cores=threads, kind="port",
-               sample=(f"oracle FP64 power iteration (OpenMP, {threads} threads) on the same "
-                       f"R-MAT 22 graph, {it} iterations" + (" (full solve)" if full else " (bounded sample)")),
-               seconds=el)
+    full = per_it * 100 * 6 <= budget_s
+    k = 100 if full else max(2, int(budget_s / 6 / per_it))
+    med, ts, out = median_time(lambda: oracle.pagerank_cfg(r.row_ptr, r.col, od, 0.85, 1e-7, k))
+    xr, it, rr = out
+    res = cpu_record(h.E * it / med / 1e9, "GTEPS", cpu, threads,
+                     (f"oracle FP64 power iteration (OpenMP, {threads} threads) on the same "
+                      f"R-MAT 22 graph, {it} iterations" + (" (full solve)" if full else " (bounded sample)")),
+                     5, seconds=med, runs_s=ts)
     err = float(np.abs(x_gpu - xr).sum()) if full else None
     return res, err, it
 
 
-def run_sssp_c1(gb, oracle, torch, np):
-    h = gb.gen_rmat(16, 16, seed=42, weights="uniform", wlo=1, whi=255)
-    srcs = gb.pick_sources(h, 8, 42)
+def run_sssp_c1(gb, oracle, torch, np, cpu, with_cpu):
+    from graphbench_amd import learned, workloads
+    h, srcs = workloads.c1()
     g = gb.Graph.from_host(h, need_reverse=False)
     gb.sssp(g, srcs)  # warm
     torch.cuda.synchronize()
@@ -127,14 +205,22 @@ def run_sssp_c1(gb, oracle, torch, np):
     dd = gb.u32(d)
     bad = sum(int((dd[i] != oracle.dijkstra(h.row_ptr, h.col, h.w, int(x))).sum())
               for i, x in enumerate(srcs))
-    return dict(workload="sssp_rmat16_8src", V=h.V, E=h.E, ms=s * 1e3,
-                gteps=st["traversed"] / s / 1e9, buckets=st["buckets"], phases=st["phases"],
-                relaxations=st["relax_edges"], mismatches_vs_dijkstra=bad)
+    out = dict(workload="sssp_rmat16_8src", V=h.V, E=h.E, ms=s * 1e3,
+               gteps=st["traversed"] / s / 1e9, buckets=st["buckets"], phases=st["phases"],
+               relaxations=st["relax_edges"], mismatches_vs_dijkstra=bad)
+    if with_cpu:
+        plain = learned.plain_delta(h)
+        threads = min(cpu["threads"], len(srcs))
+        med, trav = cpu_sssp(oracle, np, [(h, x, plain) for x in srcs], threads, 5, 1)
+        out["cpu"] = cpu_record(trav / med / 1e9, "GTEPS", cpu, threads,
+                                f"oracle delta-stepping (delta {plain}), the same 8 sources, "
+                                f"one source per thread", 5, seconds=med)
+    return out
 
 
-def run_bc_c3(gb, oracle, torch, np):
-    h = gb.gen_rmat(20, 16, seed=11, weights="none", symmetric=True)
-    srcs = gb.pick_sources(h, 256, 11)
+def run_bc_c3(gb, oracle, torch, np, cpu, with_cpu):
+    from graphbench_amd import workloads
+    h, srcs = workloads.c3()
     g = gb.Graph.from_host(h, need_reverse=False)
     gb.bc(g, srcs[:8])
     torch.cuda.synchronize()
@@ -142,16 +228,30 @@ def run_bc_c3(gb, oracle, torch, np):
     bc, st = gb.bc(g, srcs, stats=True)
     torch.cuda.synchronize()
     s = time.perf_counter() - t
-    # error vs the FP64 oracle on 4 sources (the full 256 would take minutes on the host)
+    # error vs the FP64 oracle on 4 sources (the 256-source comparison is a GPU test)
     sub = srcs[:4]
     bsub = gb.bc(g, sub).cpu().numpy()
+    oracle.set_threads(cpu["threads"])
     ref = oracle.brandes_bfs(h.row_ptr, h.col, sub)
     rel = float(np.max(np.abs(bsub - ref) / np.maximum(np.abs(ref), 1e-300)))
     M = h.E
     B = 40 * M + 48 * h.V + 8
-    return dict(workload="bc_rmat20_sym_256src", V=h.V, arcs=M, ms=s * 1e3,
-                gteps=256 * M / s / 1e9, max_depth=st["max_depth"],
-                achieved_gbs=256 * B / s / 1e9, max_rel_err_vs_oracle_4src=rel)
+    out = dict(workload="bc_rmat20_sym_256src", V=h.V, arcs=M, ms=s * 1e3,
+               gteps=256 * M / s / 1e9, max_depth=st["max_depth"],
+               # 256 x the one-source byte count over the time: what 256
+               # separate single-source solves would have to move, not a
+               # bandwidth (64 sources share each col and mask read)
+               per_source_equivalent_gbs=256 * B / s / 1e9, max_rel_err_vs_oracle_4src=rel)
+    if with_cpu:
+        threads = cpu["threads"]
+        ns = min(len(srcs), 2 * threads)
+        sample = srcs[:ns]
+        oracle.set_threads(threads)
+        med, _, _ = median_time(lambda: oracle.brandes_bfs(h.row_ptr, h.col, sample), 5, 1)
+        out["cpu"] = cpu_record(ns * M / med / 1e9, "GTEPS", cpu, threads,
+                                f"oracle Brandes (FP64), the first {ns} of the 256 sources, "
+                                f"one source per thread", 5, seconds=med)
+    return out
 
 
 def _timed(torch, fn):
@@ -162,20 +262,27 @@ def _timed(torch, fn):
     return out, time.perf_counter() - t
 
 
-def run_learned_c5(gb, oracle, torch, np, cpu_budget):
+def sssp_batch_bytes(V, E, k):
+    """Algorithmic bytes of one batched solve of k sources sharing the CSR:
+    row_ptr 4(V+1), col + w 8E, and per source one dist access per edge (4E)
+    plus dist init and the source-side read (8V)."""
+    return 4 * (V + 1) + 8 * E + k * (4 * E + 8 * V)
+
+
+def run_learned_c5(gb, oracle, torch, np, cpu, with_cpu):
     """R-MAT 24 (seed 99, weights 50/50 uniform [1,255] / log-uniform
-    [1,65535]), 32 SSSP sources + PageRank to 1e-7: the plain GPU form (one
-    source per solve, delta = max_w / avg_degree; PageRank from 1/N) against
-    the learned + batched form (one 32-source solve with the predicted delta
-    and light threshold; PageRank from the predicted warm start).  Model
-    inference is inside the learned timings."""
-    from graphbench_amd import learned
+    [1,65535]), 32 SSSP sources + PageRank to 1e-7, three forms on the same
+    GPU: plain (one source per solve with the per-source engine, delta =
+    max_w / avg_degree; PageRank from 1/N), batched with the plain delta (one
+    32-source solve of the batched engine), and learned + batched (the same
+    solve with the predicted delta; PageRank from the predicted warm start).
+    Model inference is inside the learned timings."""
+    from graphbench_amd import learned, workloads
     models = learned.LearnedModels.load()
     t0 = time.time()
-    h = gb.gen_rmat(24, 16, seed=99, weights="mix", wlo=1, whi=255)
+    h, srcs = workloads.c5()
     r = h.transpose()
     log(f"c5: generated R-MAT 24: V={h.V} E={h.E} in {time.time() - t0:.1f}s")
-    srcs = gb.pick_sources(h, 32, 99)
     g = gb.Graph.from_host(h, reverse=r)
     V, E = h.V, h.E
     plain = learned.plain_delta(h)
@@ -189,6 +296,11 @@ def run_learned_c5(gb, oracle, torch, np, cpu_budget):
             tr += st["traversed"]
         return tr
     traversed, t_sp = _timed(torch, plain_sssp)
+    gb.sssp(g, srcs[:2], delta=plain, light_max=plain)  # warm-up of the batched engine
+    (d_bp, st_bp), t_sbp = _timed(torch, lambda: gb.sssp(g, srcs, delta=plain, light_max=plain,
+                                                          stats=True))
+    exact_bp = bool(torch.equal(d_plain, d_bp))
+    del d_bp
 
     def learned_sssp():
         dl, lm = models.predict_delta(g, len(srcs))
@@ -197,8 +309,9 @@ def run_learned_c5(gb, oracle, torch, np, cpu_budget):
     (d_learn, st_l, dl, lm), t_sl = _timed(torch, learned_sssp)
     exact = bool(torch.equal(d_plain, d_learn))
     del d_plain, d_learn
+    B32 = sssp_batch_bytes(V, E, len(srcs))
     pr_kw = dict(damping=0.85, tol=1e-7, max_iter=100)
-    gb.pagerank(g, max_iter=3)  # warm-up (autotunes the tile size)
+    gb.pagerank(g, max_iter=3)  # warm-up (autotunes the gather)
     (x_p, res_p), t_pp = _timed(torch, lambda: gb.pagerank(g, **pr_kw))
 
     def learned_pr():
@@ -208,24 +321,35 @@ def run_learned_c5(gb, oracle, torch, np, cpu_budget):
     xp = x_p.cpu().numpy().astype(np.float64)
     xw = x_w.cpu().numpy().astype(np.float64)
     l1 = {"plain_vs_warm": float(np.abs(xp - xw).sum())}
-    oracle.set_threads(min(16, os.cpu_count() or 1))
+    oracle.set_threads(cpu["threads"])
+    t_cpu_pr = None
     tq = time.perf_counter()
-    oracle.pagerank_cfg(r.row_ptr, r.col, h.outdeg(), 0.85, 1e-10, 1)
-    if (time.perf_counter() - tq) * 40 <= 2 * cpu_budget:
-        ref, _, _ = oracle.pagerank_cfg(r.row_ptr, r.col, h.outdeg(), 0.85, 1e-10, 150)
-        l1["plain_vs_fp64_oracle"] = float(np.abs(xp - ref).sum())
-        l1["warm_vs_fp64_oracle"] = float(np.abs(xw - ref).sum())
+    ref, it_ref, _ = oracle.pagerank_cfg(r.row_ptr, r.col, h.outdeg(), 0.85, 1e-7, 100)
+    t_cpu_pr = time.perf_counter() - tq
+    l1["plain_vs_fp64_oracle"] = float(np.abs(xp - ref).sum())
+    l1["warm_vs_fp64_oracle"] = float(np.abs(xw - ref).sum())
     work_pr = E * res_p["iterations"]  # the plain solve's edge work, credited to both forms
     plain_s = t_sp + t_pp
     learned_s = t_sl + t_pw
-    return dict(
+    out = dict(
         workload="learned_rmat24_mix_32src_plus_pagerank", V=V, E=E,
-        sssp={"plain_ms": t_sp * 1e3, "learned_batched_ms": t_sl * 1e3,
+        sssp={"plain_ms": t_sp * 1e3, "batched_plain_delta_ms": t_sbp * 1e3,
+              "learned_batched_ms": t_sl * 1e3,
               "plain_delta": plain, "learned_delta": dl, "learned_light_max": lm,
-              "plain_gteps": traversed / t_sp / 1e9, "learned_batched_gteps": traversed / t_sl / 1e9,
-              "speedup": t_sp / t_sl, "distances_bit_identical": exact,
+              "plain_gteps": traversed / t_sp / 1e9,
+              "batched_plain_delta_gteps": traversed / t_sbp / 1e9,
+              "learned_batched_gteps": traversed / t_sl / 1e9,
+              "speedup": t_sp / t_sl, "batching_speedup": t_sp / t_sbp,
+              "learned_over_batched_plain": t_sbp / t_sl,
+              "distances_bit_identical": exact and exact_bp,
               "buckets": st_l["buckets"], "phases": st_l["phases"],
-              "relaxations": st_l["relax_edges"], "traversed": traversed},
+              "relaxations": st_l["relax_edges"], "traversed": traversed,
+              "batched_plain_delta": {"buckets": st_bp["buckets"], "phases": st_bp["phases"],
+                                      "relaxations": st_bp["relax_edges"]},
+              "roofline": {"bytes": B32, "achieved_gbs": B32 / t_sl / 1e9,
+                           "frac": B32 / t_sl / 1e9 / HBM_PEAK_GBS,
+                           "note": "B(32) = 4(V+1) + 8E + 32(4E + 8V) over the learned + "
+                                   "batched solve time"}},
         pagerank={"plain_ms": t_pp * 1e3, "warm_ms": t_pw * 1e3,
                   "plain_iterations": res_p["iterations"], "warm_iterations": res_w["iterations"],
                   "plain_gteps": work_pr / t_pp / 1e9, "warm_effective_gteps": work_pr / t_pw / 1e9,
@@ -233,42 +357,35 @@ def run_learned_c5(gb, oracle, torch, np, cpu_budget):
         combined={"plain_gteps": (traversed + work_pr) / plain_s / 1e9,
                   "learned_batched_gteps": (traversed + work_pr) / learned_s / 1e9,
                   "speedup": plain_s / learned_s})
-
-
-def c4_specs(n=512, seed=2024):
-    """512 directed graphs, V = round(2^(10 + 10 U)) (log-uniform in [1024,
-    1048576]); even indices R-MAT with edge factor U{8..32}, odd indices
-    Erdos-Renyi with average degree U{8..32}; weights uniform [1, 1023]."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(n):
-        V = int(round(2.0 ** (10 + 10 * rng.random())))
-        deg = int(rng.integers(8, 33))
-        out.append(("rmat" if i % 2 == 0 else "er", V, deg, 5000 + i))
+    if with_cpu:
+        threads = cpu["threads"]
+        ns = min(len(srcs), threads)
+        med, trav = cpu_sssp(oracle, np, [(h, x, plain) for x in srcs[:ns]], threads, 1, 0)
+        out["cpu"] = {
+            "sssp": cpu_record(trav / med / 1e9, "GTEPS", cpu, threads,
+                               f"oracle delta-stepping (delta {plain}), the first {ns} of the 32 "
+                               f"sources, one source per thread", 1, seconds=med),
+            "pagerank": cpu_record(E * it_ref / t_cpu_pr / 1e9, "GTEPS", cpu, threads,
+                                   f"oracle FP64 power iteration, full solve ({it_ref} iterations)",
+                                   1, seconds=t_cpu_pr)}
     return out
 
 
-def run_batched_c4(gb, oracle, torch, np):
-    """512 uneven graphs, one source each: one gb_sssp_multi solve with a
-    learned delta per graph, against 512 plain single-graph solves (delta =
-    max_w / avg_degree).  Distances of the two forms must be identical."""
-    from graphbench_amd import learned
+def run_batched_c4(gb, oracle, torch, np, cpu, with_cpu):
+    """512 uneven graphs, one source each, three forms: 512 plain
+    single-graph solves (delta = max_w / avg_degree), one gb_sssp_multi solve
+    with the plain delta per graph, and one with a learned delta per graph.
+    Distances of the three forms must be identical."""
+    from graphbench_amd import learned, workloads
     models = learned.LearnedModels.load()
     t0 = time.time()
-    hs = []
-    for kind, V, deg, seed in c4_specs():
-        if kind == "rmat":
-            sc = max(1, (V - 1).bit_length())
-            hs.append(gb.gen_rmat(sc, deg, seed=seed, weights="uniform", wlo=1, whi=1023, nverts=V))
-        else:
-            hs.append(gb.gen_er(V, deg, seed=seed, weights="uniform", wlo=1, whi=1023))
+    hs, srcs = workloads.c4()
     sumV = sum(h.V for h in hs)
     sumE = sum(h.E for h in hs)
     log(f"c4: generated 512 graphs, sum V={sumV} sum E={sumE} in {time.time() - t0:.1f}s")
     gs = [gb.Graph.from_host(h, need_reverse=False) for h in hs]
-    srcs = [int(gb.pick_sources(h, 1, 7 + i)[0]) for i, h in enumerate(hs)]
     outs_p = [torch.empty(h.V, dtype=torch.int32, device="cuda") for h in hs]
+    outs_b = [torch.empty(h.V, dtype=torch.int32, device="cuda") for h in hs]
     outs_l = [torch.empty(h.V, dtype=torch.int32, device="cuda") for h in hs]
     gb.sssp(gs[0], [srcs[0]], out=outs_p[0].view(1, -1))  # warm-up
 
@@ -279,6 +396,7 @@ def run_batched_c4(gb, oracle, torch, np):
             tr += st["traversed"]
         return tr
     traversed, t_p = _timed(torch, plain)
+    st_b, t_b = _timed(torch, lambda: gb.sssp_multi(gs, srcs, outs=outs_b, stats=True)[1])
 
     def batched():
         dl = models.predict_deltas(gs, 1)
@@ -286,18 +404,33 @@ def run_batched_c4(gb, oracle, torch, np):
                               outs=outs_l, stats=True)
         return st
     st_l, t_l = _timed(torch, batched)
-    exact = all(bool(torch.equal(a, b)) for a, b in zip(outs_p, outs_l))
+    exact = all(bool(torch.equal(a, b)) and bool(torch.equal(a, c))
+                for a, b, c in zip(outs_p, outs_b, outs_l))
     small = sorted(range(len(hs)), key=lambda i: hs[i].E)[:4]
     bad = sum(int((gb.u32(outs_l[i]) != oracle.dijkstra(hs[i].row_ptr, hs[i].col, hs[i].w,
                                                           srcs[i])).sum()) for i in small)
     B = sum(12 * h.E + 12 * h.V + 4 for h in hs)
-    return dict(workload="sssp_512_uneven_graphs", graphs=len(hs), sum_V=sumV, sum_E=sumE,
-                plain_ms=t_p * 1e3, batched_learned_ms=t_l * 1e3,
-                plain_gteps=traversed / t_p / 1e9, batched_learned_gteps=traversed / t_l / 1e9,
-                speedup=t_p / t_l, distances_bit_identical=exact,
-                mismatches_vs_dijkstra_4_smallest=bad, buckets=st_l["buckets"],
-                phases=st_l["phases"], relaxations=st_l["relax_edges"],
-                achieved_gbs=B / t_l / 1e9)
+    out = dict(workload="sssp_512_uneven_graphs", graphs=len(hs), sum_V=sumV, sum_E=sumE,
+               plain_ms=t_p * 1e3, batched_plain_delta_ms=t_b * 1e3, batched_learned_ms=t_l * 1e3,
+               plain_gteps=traversed / t_p / 1e9, batched_plain_delta_gteps=traversed / t_b / 1e9,
+               batched_learned_gteps=traversed / t_l / 1e9,
+               speedup=t_p / t_l, batching_speedup=t_p / t_b, learned_over_batched_plain=t_b / t_l,
+               distances_bit_identical=exact,
+               mismatches_vs_dijkstra_4_smallest=bad, buckets=st_l["buckets"],
+               phases=st_l["phases"], relaxations=st_l["relax_edges"],
+               batched_plain_delta_stats={"buckets": st_b["buckets"], "phases": st_b["phases"],
+                                          "relaxations": st_b["relax_edges"]},
+               achieved_gbs=B / t_l / 1e9, frac=B / t_l / 1e9 / HBM_PEAK_GBS)
+    if with_cpu:
+        threads = cpu["threads"]
+        jobs = [(hs[i], srcs[i], learned.plain_delta(hs[i])) for i in range(0, len(hs), 8)]
+        med, trav = cpu_sssp(oracle, np, jobs, threads, 5, 1)
+        out["cpu"] = cpu_record(trav / med / 1e9, "GTEPS", cpu, threads,
+                                f"oracle delta-stepping (plain delta per graph) on every 8th "
+                                f"graph ({len(jobs)} graphs, sum E = "
+                                f"{sum(j[0].E for j in jobs)}), one graph per thread", 5,
+                                seconds=med)
+    return out
 
 
 def main():
@@ -312,7 +445,8 @@ def main():
                     help="PageRank gather form: 0 tuned, 1024/2048/4096 row blocks, -1 merge path")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of host time for the headline CPU baseline (6 solves)")
     ap.add_argument("--only", choices=["sssp", "bc", "c4", "c5"], default=None,
                     help="time one secondary workload alone (diagnostics; no JSON line)")
     args = ap.parse_args()
@@ -326,34 +460,35 @@ def main():
     import oracle
 
     gb.init(0)
+    cpu = host_cpu()
+    with_cpu = not args.no_cpu
     if args.only:
-        if args.only == "c5":
-            print(json.dumps(run_learned_c5(gb, oracle, torch, np, args.cpu_budget)), flush=True)
-            return
-        fn = {"sssp": run_sssp_c1, "bc": run_bc_c3, "c4": run_batched_c4}[args.only]
-        print(json.dumps(fn(gb, oracle, torch, np)), flush=True)
+        fn = {"sssp": run_sssp_c1, "bc": run_bc_c3, "c4": run_batched_c4,
+              "c5": run_learned_c5}[args.only]
+        print(json.dumps(fn(gb, oracle, torch, np, cpu, with_cpu)), flush=True)
         return
     log("pagerank R-MAT 22")
     h, r, pr, x = run_pagerank(gb, torch, np, args)
     log(f"pagerank: {pr['step_s']*1e3:.2f} ms/step, {pr['gteps']:.1f} GTEPS, gather "
         f"{pr['gather_ms_avg']:.3f} ms, {pr['achieved_gbs']:.0f} GB/s")
-    cpu, err, cpu_it = (None, None, None)
-    if not args.no_cpu:
-        log("cpu baseline (oracle)")
-        cpu, err, cpu_it = cpu_baseline(oracle, np, h, r, x, args.cpu_budget)
-        log(f"cpu: {cpu['value']:.3f} GTEPS, L1 vs FP64 oracle {err}")
+    cpu_pr, err, cpu_it = (None, None, None)
+    if with_cpu:
+        log(f"cpu baseline (oracle, {cpu['threads']} threads: {cpu['model']})")
+        cpu_pr, err, cpu_it = cpu_baseline(oracle, np, h, r, x, args.cpu_budget, cpu)
+        log(f"cpu: {cpu_pr['value']:.3f} GTEPS, L1 vs FP64 oracle {err}")
     extra = {"pagerank_graph_mode": {"ms_per_step": pr["graph_step_ms"], "gteps": pr["graph_gteps"]},
              "pagerank_l1_vs_fp64_oracle": err, "pagerank_iterations": pr["iterations"],
              "pagerank_residual": pr["residual"], "oracle_iterations": cpu_it}
     if not args.no_extra:
+        del h, r, x
         log("sssp config 1")
-        extra["sssp_c1"] = run_sssp_c1(gb, oracle, torch, np)
+        extra["sssp_c1"] = run_sssp_c1(gb, oracle, torch, np, cpu, with_cpu)
         log("bc config 3")
-        extra["bc_c3"] = run_bc_c3(gb, oracle, torch, np)
+        extra["bc_c3"] = run_bc_c3(gb, oracle, torch, np, cpu, with_cpu)
         log("512 uneven graphs")
-        extra["batched_c4"] = run_batched_c4(gb, oracle, torch, np)
+        extra["batched_c4"] = run_batched_c4(gb, oracle, torch, np, cpu, with_cpu)
         log("learned + batched vs plain, R-MAT 24")
-        extra["learned_c5"] = run_learned_c5(gb, oracle, torch, np, args.cpu_budget)
+        extra["learned_c5"] = run_learned_c5(gb, oracle, torch, np, cpu, with_cpu)
     traffic = {}
     if os.path.exists(TRAFFIC_JSON):
         with open(TRAFFIC_JSON) as f:
@@ -384,7 +519,7 @@ def main():
                      "shape": {"items_per_thread": pr["items_per_thread"], "hot": pr["hot"],
                                "row_block": pr["row_block"], "relabeled": pr["relabeled"]},
                      "first_call_ms": pr["first_call_ms"]},
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_pr,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

@@ -62,7 +62,8 @@ class _SsspParams(C.Structure):
 class _SsspStats(C.Structure):
     _fields_ = [("buckets", C.c_int64), ("phases", C.c_int64), ("relax_edges", C.c_int64),
                 ("reached", C.c_int64), ("traversed", C.c_int64), ("ms", C.c_double),
-                ("improved_near", C.c_int64), ("improved_far", C.c_int64)]
+                ("improved_near", C.c_int64), ("improved_far", C.c_int64),
+                ("expanded_edges", C.c_int64)]
 
 
 class _PrParams(C.Structure):

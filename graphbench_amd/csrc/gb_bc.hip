@@ -110,6 +110,25 @@ __global__ __launch_bounds__(BC_BLOCK) void bcw_indeg_kernel(
   }
 }
 
+// Reached (slot, vertex) pairs: finite distances, one atomic per block.
+__global__ void bcw_reached_kernel(int64_t n, const double *__restrict__ dist,
+                                   unsigned long long *__restrict__ count) {
+  unsigned long long c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c += dist[i] != __longlong_as_double(0x7FF0000000000000ll) ? 1ull : 0ull;
+  __shared__ unsigned long long s_c[256 / GB_WAVE];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, GB_WAVE);
+  if (gb_lane() == 0) s_c[threadIdx.x / GB_WAVE] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < (int)(blockDim.x / GB_WAVE); k++) t += s_c[k];
+    atomicAdd(count, t);
+  }
+}
+
 struct PredTight {
   const double *dist;
   const double *w;   // forward weights
@@ -266,7 +285,7 @@ gb_status bc_run(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t bat
   GB_TRY(w.delta.ensure(sizeof(double) * (n + 1)));
   GB_TRY(w.order.ensure(sizeof(uint64_t) * (n + 1)));
   GB_TRY(w.src.ensure(sizeof(int32_t) * batch));
-  GB_TRY(w.cnt.ensure(sizeof(unsigned long long)));
+  GB_TRY(w.cnt.ensure(2 * sizeof(unsigned long long)));
   GB_TRY(w.indeg.ensure(sizeof(int32_t) * (n + 1)));
   GB_TRY(w.dist.ensure(sizeof(double) * (n + 1)));
   hipEvent_t e0, e1;
@@ -298,6 +317,25 @@ gb_status bc_run(gb_graph g, const int32_t *h_sources, int32_t nsrc, int32_t bat
                        w.order.as<uint64_t>());
     GB_HIP(hipGetLastError());
     GB_TRY(forward_levels(g, ns, st, off, true, w.level.as<int32_t>()));
+    // Every reached pair must have been given a depth.  A cycle of tight edges
+    // (a zero weight, or a weight below half an ulp of the distance it is
+    // added to) never lets Kahn's count reach zero: its vertices would keep
+    // sigma 0 and divide by it below.
+    {
+      unsigned long long *rc = w.cnt.as<unsigned long long>() + 1;
+      GB_HIP(hipMemsetAsync(rc, 0, sizeof(unsigned long long), st));
+      hipLaunchKernelGGL(bcw_reached_kernel, dim3(fill_grid(nn)), dim3(256), 0, st, nn,
+                         (const double *)w.dist.as<double>(), rc);
+      GB_HIP(hipGetLastError());
+      unsigned long long reached = 0;
+      GB_HIP(hipMemcpyAsync(&reached, rc, sizeof(reached), hipMemcpyDeviceToHost, st));
+      GB_HIP(hipStreamSynchronize(st));
+      if (reached != off.back())
+        return gb_fail(GB_ERR_INVALID,
+                       "gb_bc_weighted: %llu reached vertices lie on a cycle of tight edges "
+                       "(weights must be positive and not vanish against the distances)",
+                       (unsigned long long)(reached - off.back()));
+    }
     const int32_t depth = (int32_t)off.size() - 2;  // number of non-empty levels
     max_depth = std::max(max_depth, depth);
     PredTight pt{w.dist.as<double>(), g->w64, g->rw64};

@@ -141,14 +141,15 @@ void gb_msssp_release();
 // relaxations and the overflow flag.
 gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int32_t nsrc,
                        uint32_t *d_dist, int max_phases, int64_t *phases, int64_t *buckets,
-                       unsigned long long *relax, int64_t *imp_near, int64_t *imp_far, bool *ovf,
-                       hipStream_t st);
+                       unsigned long long *relax, int64_t *imp_near, int64_t *imp_far,
+                       int64_t *expanded, bool *ovf, hipStream_t st);
 gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64_t *reached,
                          int64_t *traversed, hipStream_t st);
 // Adds the per-wave statistics slots (nwaves * GB_WSTAT, gb_device.hpp) to
-// the counters: cnt[idx[k]] += sum over waves of slot k, k < 3.
+// the counters: cnt[idx[k]] += sum over waves of slot k (k < 4; idx3 < 0
+// leaves slot 3 out).
 gb_status gb_wstat_fold(int64_t nwaves, const unsigned long long *slots, unsigned long long *cnt,
-                        int idx0, int idx1, int idx2, hipStream_t st);
+                        int idx0, int idx1, int idx2, hipStream_t st, int idx3 = -1);
 void gb_bc_release();
 void gb_msbc_release();
 void gb_mlp_release();

@@ -147,7 +147,7 @@ __device__ __forceinline__ MsIn ms_load(const MsCtx &c, const unsigned long long
 template <int W>
 __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *lds_near,
                                          uint64_t *lds_far, uint32_t &relax, uint32_t &imp_near,
-                                         uint32_t &imp_far, bool &ovf) {
+                                         uint32_t &imp_far, uint32_t &expanded, bool &ovf) {
   using S = MsShape<W>;
   constexpr int ES = S::ES, UNROLL = S::UNROLL, G = S::G;
   const int lane = gb_lane();
@@ -180,6 +180,7 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
     if (i2 < n) enn = in[i2];
     const bool act = t < c.ntask && ((cur.m >> t) & 1ull);
     const int cnt = ecur.end - ecur.beg;
+    expanded += lane == 0 ? (uint32_t)cnt : 0u;
     for (int g0 = 0; g0 < cnt; g0 += G) {
       int32_t x[UNROLL];
       uint32_t nd[UNROLL], old[UNROLL];
@@ -314,17 +315,19 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
   MsEnt *ln = s_near[threadIdx.x / GB_WAVE];
   uint64_t *lf = s_far[threadIdx.x / GB_WAVE];
   if (act == ACT_LIGHT) {
-    uint32_t relax = 0, imp_near = 0, imp_far = 0;
+    uint32_t relax = 0, imp_near = 0, imp_far = 0, expanded = 0;
     bool ovf = false;
-    ms_light<W>(c, a, ln, lf, relax, imp_near, imp_far, ovf);
-    uint32_t tot, tn, tf;
+    ms_light<W>(c, a, ln, lf, relax, imp_near, imp_far, expanded, ovf);
+    uint32_t tot, tn, tf, te;
     (void)gb_wave_excl_sum(relax, &tot);
     (void)gb_wave_excl_sum(imp_near, &tn);
     (void)gb_wave_excl_sum(imp_far, &tf);
+    (void)gb_wave_excl_sum(expanded, &te);
     if (gb_lane() == 0) {
       gb_wstat_add(c.wstat, 0, tot);
       gb_wstat_add(c.wstat, 1, tn);
       gb_wstat_add(c.wstat, 2, tf);
+      gb_wstat_add(c.wstat, 3, te);
     }
     if (__ballot(ovf) && gb_lane() == 0) atomicOr(&c.cnt[MS_OVF], 1ull);
   } else {
@@ -433,7 +436,8 @@ MsWork &g_msw_ref() {
 template <int W>
 gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st,
                    uint32_t *out_rows, int64_t *phases, int64_t *buckets,
-                   unsigned long long *relax, int64_t *imp_near, int64_t *imp_far, bool *ovf) {
+                   unsigned long long *relax, int64_t *imp_near, int64_t *imp_far,
+                   int64_t *expanded, bool *ovf) {
   MsWork &w = g_msw;
   const int64_t V = c.V;
   hipLaunchKernelGGL(ms_fill_kernel, dim3((unsigned)std::min<int64_t>((V * W + 255) / 256, 65536)),
@@ -512,7 +516,8 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   hipLaunchKernelGGL(ms_unpack_kernel, dim3((unsigned)((V + 63) / 64), (unsigned)((W + 63) / 64)),
                      dim3(256), 0, st, V, W, c.ntask, (const uint32_t *)c.dist, out_rows);
   GB_HIP(hipGetLastError());
-  GB_TRY(gb_wstat_fold(nwaves, c.wstat, c.cnt, MS_RELAX, MS_IMP_NEAR, MS_IMP_FAR, st));
+  GB_TRY(gb_wstat_fold(nwaves, c.wstat, c.cnt, MS_RELAX, MS_IMP_NEAR, MS_IMP_FAR, st,
+                       MS_EXPANDED));
   unsigned long long h[MS_N];
   GB_HIP(hipMemcpyAsync(h, c.cnt, sizeof(h), hipMemcpyDeviceToHost, st));
   GB_HIP(hipStreamSynchronize(st));
@@ -521,6 +526,7 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   *relax += h[MS_RELAX];
   *imp_near += (int64_t)h[MS_IMP_NEAR];
   *imp_far += (int64_t)h[MS_IMP_FAR];
+  *expanded += (int64_t)h[MS_EXPANDED];
   if (h[MS_OVF]) *ovf = true;
   return GB_OK;
 }
@@ -551,8 +557,8 @@ gb_status gb_msssp_count(gb_graph g, int32_t nsrc, const uint32_t *d_dist, int64
 
 gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int32_t nsrc,
                        uint32_t *d_dist, int max_phases, int64_t *phases, int64_t *buckets,
-                       unsigned long long *relax, int64_t *imp_near, int64_t *imp_far, bool *ovf,
-                       hipStream_t st) {
+                       unsigned long long *relax, int64_t *imp_near, int64_t *imp_far,
+                       int64_t *expanded, bool *ovf, hipStream_t st) {
   MsWork &w = g_msw;
   const int64_t V = g->V, E = g->E;
   // queue capacity: a vertex enters a near queue at most once per phase with
@@ -599,16 +605,16 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
     const int32_t *ds = w.src.as<int32_t>();
     switch (W) {
       case 8:
-        GB_TRY(ms_group<8>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
+        GB_TRY(ms_group<8>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
         break;
       case 16:
-        GB_TRY(ms_group<16>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
+        GB_TRY(ms_group<16>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
         break;
       case 32:
-        GB_TRY(ms_group<32>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
+        GB_TRY(ms_group<32>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
         break;
       default:
-        GB_TRY(ms_group<64>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, ovf));
+        GB_TRY(ms_group<64>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
     }
   }
   return GB_OK;

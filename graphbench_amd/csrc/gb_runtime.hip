@@ -356,28 +356,29 @@ gb_status gb_node_ids(int64_t E, const int32_t *d_col, const int64_t *d_distrib,
 __global__ __launch_bounds__(256) void wstat_fold_kernel(int64_t nwaves,
                                                          const unsigned long long *__restrict__ s,
                                                          unsigned long long *__restrict__ cnt,
-                                                         int i0, int i1, int i2) {
-  __shared__ unsigned long long s_sum[3][4];
-  unsigned long long a[3] = {0ull, 0ull, 0ull};
+                                                         int i0, int i1, int i2, int i3) {
+  __shared__ unsigned long long s_sum[GB_WSTAT][4];
+  unsigned long long a[GB_WSTAT] = {};
   for (int64_t w = threadIdx.x; w < nwaves; w += blockDim.x)
-    for (int k = 0; k < 3; k++) a[k] += s[w * GB_WSTAT + k];
-  for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < GB_WSTAT; k++) a[k] += s[w * GB_WSTAT + k];
+  for (int k = 0; k < GB_WSTAT; k++) {
     unsigned long long v = a[k];
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     if (gb_lane() == 0) s_sum[k][threadIdx.x >> 6] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < GB_WSTAT) {
     const int k = threadIdx.x;
+    const int idx = k == 0 ? i0 : k == 1 ? i1 : k == 2 ? i2 : i3;
     const unsigned long long t = s_sum[k][0] + s_sum[k][1] + s_sum[k][2] + s_sum[k][3];
-    cnt[k == 0 ? i0 : k == 1 ? i1 : i2] += t;
+    if (idx >= 0) cnt[idx] += t;
   }
 }
 
 gb_status gb_wstat_fold(int64_t nwaves, const unsigned long long *slots, unsigned long long *cnt,
-                        int idx0, int idx1, int idx2, hipStream_t st) {
+                        int idx0, int idx1, int idx2, hipStream_t st, int idx3) {
   hipLaunchKernelGGL(wstat_fold_kernel, dim3(1), dim3(256), 0, st, nwaves, slots, cnt, idx0, idx1,
-                     idx2);
+                     idx2, idx3);
   GB_HIP(hipGetLastError());
   return GB_OK;
 }

@@ -707,6 +707,7 @@ gb_status sssp_engine(const std::vector<SGraph<typename Tr::W>> &hg,
     stats->relax_edges = (int64_t)h[C_RELAX];
     stats->improved_near = (int64_t)h[C_IMP_NEAR];
     stats->improved_far = (int64_t)h[C_IMP_FAR];
+    stats->expanded_edges = (int64_t)h[C_RELAX];  // one source per expansion
     stats->ms = ms;
     GB_TRY(w.counts.ensure(2 * sizeof(unsigned long long)));
     GB_HIP(hipMemsetAsync(w.counts.p, 0, 2 * sizeof(unsigned long long), st));
@@ -799,7 +800,7 @@ extern "C" gb_status gb_sssp(gb_graph g, const gb_sssp_params *p, const int32_t 
     GB_HIP(hipEventRecord(e0, st));
     gb_status r = gb_msssp_run(g, delta, h_sources, nsrc, d_dist, p ? p->max_phases : 0,
                                &s.phases, &s.buckets, &relax, &s.improved_near,
-                               &s.improved_far, &ovf, st);
+                               &s.improved_far, &s.expanded_edges, &ovf, st);
     if (r == GB_OK) r = hipEventRecord(e1, st) == hipSuccess ? GB_OK : gb_fail(GB_ERR_HIP, "event");
     if (r == GB_OK && stats) {
       float ms = 0.0f;
@@ -852,6 +853,8 @@ extern "C" gb_status gb_sssp_multi(int32_t ngraphs, const gb_graph *gs, const in
                                    uint32_t *const *d_dist, gb_sssp_stats *stats, void *stream) {
   if (ngraphs < 0 || (ngraphs > 0 && (!gs || !sources || !d_dist)))
     return gb_fail(GB_ERR_INVALID, "gb_sssp_multi: null");
+  // the engine's task limit, checked before any split is queued
+  if (ngraphs > 65535) return gb_fail(GB_ERR_INVALID, "gb_sssp_multi: at most 65535 graphs per call");
   hipStream_t st = gb_stream(stream);
   std::vector<SGraph<uint32_t>> hg(ngraphs);
   std::vector<STask<TU32>> ht(ngraphs);
@@ -887,5 +890,9 @@ extern "C" gb_status gb_sssp_multi(int32_t ngraphs, const gb_graph *gs, const in
       return s;
     }
   }
-  return sssp_engine<TU32>(hg, ht, 0, stats, st);
+  const gb_status r = sssp_engine<TU32>(hg, ht, 0, stats, st);
+  // on an early error the splits may still be queued: let them end before
+  // the caller reuses the split buffers or the shared scratch
+  if (r != GB_OK) (void)hipStreamSynchronize(st);
+  return r;
 }

@@ -140,9 +140,11 @@ __host__ __device__ inline void sssp_ctl_advance(SsspCtl &s, unsigned long long 
 // ---- batched form: one vertex entry serves a mask of tasks -----------------
 // Near queues 0/1 (with per-vertex task masks 0/1), far queues 2/3 (one far
 // mask per vertex), counters: MS_NEAR0..1, MS_FAR0..1, MS_MINB, MS_RELAX,
-// MS_OVF.  No light/heavy split: an expansion relaxes all edges of a vertex.
+// MS_OVF, the improvement outcomes and MS_EXPANDED (edges of expanded queue
+// entries, each read once for the entry's whole source mask).  No light/heavy
+// split: an expansion relaxes all edges of a vertex.
 enum { MS_NEAR0 = 0, MS_NEAR1 = 1, MS_FAR0 = 2, MS_FAR1 = 3, MS_MINB = 4, MS_RELAX = 5,
-       MS_OVF = 6, MS_IMP_NEAR = 7, MS_IMP_FAR = 8, MS_N = 9 };
+       MS_OVF = 6, MS_IMP_NEAR = 7, MS_IMP_FAR = 8, MS_EXPANDED = 9, MS_N = 10 };
 enum { MS_ST_LIGHT = 0, MS_ST_AFTER_BUCKET, MS_ST_FAR, MS_ST_AFTER_FAR };
 
 struct MsCtl {

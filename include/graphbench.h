@@ -161,6 +161,11 @@ typedef struct gb_sssp_stats {
    * the current bucket (re-expanded there) or in a later bucket */
   int64_t improved_near;
   int64_t improved_far;
+  /* (vertex, edge) pairs read by expansions: the batched engine reads an
+   * edge once for the mask of sources it is expanded for (relax_edges /
+   * expanded_edges is the mean number of sources per edge read); the
+   * per-source engine reads it once per source (equal to relax_edges) */
+  int64_t expanded_edges;
 } gb_sssp_stats;
 
 /* Distances from nsrc sources, all processed together in lockstep buckets.

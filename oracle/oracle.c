@@ -370,62 +370,97 @@ void orc_bfs_levels(int32_t V, const int32_t *rp, const int32_t *col,
  * comp_num_spath (70-96): sigma[w] = sum of sigma over predecessors.
  * comp_delta (98-123): delta[v] += ((1+delta[w])*sigma[v])/sigma[w].
  * run_bc (57-61): bc[j] += delta[j] for j != source. */
+/* One source of orc_brandes_bfs: delta[] (V entries) for source s, using the
+ * caller's scratch arrays. */
+static void brandes_bfs_one(int32_t V, const int32_t *rp, const int32_t *col,
+                            const int32_t *rrp, const int32_t *rcol, int32_t s,
+                            int32_t *level, int32_t *order, double *sigma,
+                            double *delta) {
+  for (int32_t i = 0; i < V; i++) {
+    level[i] = -1;
+    sigma[i] = 0.0;
+    delta[i] = 0.0;
+  }
+  int64_t h = 0, t = 0;
+  order[t++] = s;
+  level[s] = 0;
+  while (h < t) {
+    int32_t u = order[h++];
+    for (int32_t e = rp[u]; e < rp[u + 1]; e++) {
+      int32_t v = col[e];
+      if (level[v] < 0) {
+        level[v] = level[u] + 1;
+        order[t++] = v;
+      }
+    }
+  }
+  sigma[s] = 1.0;
+  for (int64_t k = 1; k < t; k++) {
+    int32_t v = order[k];
+    double acc = 0.0;
+    for (int32_t e = rrp[v]; e < rrp[v + 1]; e++) {
+      int32_t u = rcol[e];
+      if (level[u] == level[v] - 1) acc += sigma[u];
+    }
+    sigma[v] = acc;
+  }
+  for (int64_t k = t - 1; k >= 0; k--) {
+    int32_t v = order[k];
+    double acc = 0.0;
+    for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
+      int32_t x = col[e];
+      if (level[x] == level[v] + 1)
+        acc += ((1.0 + delta[x]) * sigma[v]) / sigma[x];
+    }
+    delta[v] = acc;
+  }
+}
+
+/* Brandes, btwcentral/csr_bc.c:16-130 on BFS levels.
+ * comp_num_spath (70-96): sigma[w] = sum of sigma over predecessors.
+ * comp_delta (98-123): delta[v] += ((1+delta[w])*sigma[v])/sigma[w].
+ * run_bc (57-61): bc[j] += delta[j] for j != source.
+ * Sources are solved in parallel, a batch of one per thread, and each
+ * batch's delta vectors are added to bc in source order, so the result does
+ * not depend on the thread count. */
 void orc_brandes_bfs(int32_t V, const int32_t *rp, const int32_t *col,
                      const int32_t *rrp, const int32_t *rcol,
                      const int32_t *sources, int32_t nsrc, double *bc) {
-  int32_t *level = (int32_t *)malloc(sizeof(int32_t) * (size_t)(V ? V : 1));
-  int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)(V ? V : 1));
-  double *sigma = (double *)malloc(sizeof(double) * (size_t)(V ? V : 1));
-  double *delta = (double *)malloc(sizeof(double) * (size_t)(V ? V : 1));
+  const size_t n = (size_t)(V ? V : 1);
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  if (nt > nsrc) nt = nsrc > 0 ? nsrc : 1;
+  int32_t *level = (int32_t *)malloc(sizeof(int32_t) * n * nt);
+  int32_t *order = (int32_t *)malloc(sizeof(int32_t) * n * nt);
+  double *sigma = (double *)malloc(sizeof(double) * n * nt);
+  double *delta = (double *)malloc(sizeof(double) * n * nt);
+  int32_t *ok = (int32_t *)malloc(sizeof(int32_t) * (size_t)nt);
   for (int32_t i = 0; i < V; i++) bc[i] = 0.0;
-  for (int32_t si = 0; si < nsrc; si++) {
-    int32_t s = sources[si];
-    if (s < 0 || s >= V) continue;
-    for (int32_t i = 0; i < V; i++) {
-      level[i] = -1;
-      sigma[i] = 0.0;
-      delta[i] = 0.0;
+  for (int32_t b0 = 0; b0 < nsrc; b0 += nt) {
+    const int nb = nsrc - b0 < nt ? nsrc - b0 : nt;
+#pragma omp parallel for schedule(static, 1) num_threads(nb)
+    for (int k = 0; k < nb; k++) {
+      const int32_t s = sources[b0 + k];
+      ok[k] = s >= 0 && s < V;
+      if (ok[k])
+        brandes_bfs_one(V, rp, col, rrp, rcol, s, level + n * k, order + n * k,
+                        sigma + n * k, delta + n * k);
     }
-    int64_t h = 0, t = 0;
-    order[t++] = s;
-    level[s] = 0;
-    while (h < t) {
-      int32_t u = order[h++];
-      for (int32_t e = rp[u]; e < rp[u + 1]; e++) {
-        int32_t v = col[e];
-        if (level[v] < 0) {
-          level[v] = level[u] + 1;
-          order[t++] = v;
-        }
-      }
+#pragma omp parallel for schedule(static)
+    for (int32_t j = 0; j < V; j++) {
+      double acc = bc[j];
+      for (int k = 0; k < nb; k++)
+        if (ok[k] && j != sources[b0 + k]) acc += delta[n * k + j];
+      bc[j] = acc;
     }
-    sigma[s] = 1.0;
-    for (int64_t k = 1; k < t; k++) {
-      int32_t v = order[k];
-      double acc = 0.0;
-      for (int32_t e = rrp[v]; e < rrp[v + 1]; e++) {
-        int32_t u = rcol[e];
-        if (level[u] == level[v] - 1) acc += sigma[u];
-      }
-      sigma[v] = acc;
-    }
-    for (int64_t k = t - 1; k >= 0; k--) {
-      int32_t v = order[k];
-      double acc = 0.0;
-      for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
-        int32_t x = col[e];
-        if (level[x] == level[v] + 1)
-          acc += ((1.0 + delta[x]) * sigma[v]) / sigma[x];
-      }
-      delta[v] = acc;
-    }
-    for (int32_t j = 0; j < V; j++)
-      if (j != s) bc[j] += delta[j];
   }
   free(level);
   free(order);
   free(sigma);
   free(delta);
+  free(ok);
 }
 
 /* Weighted Brandes: the run_bc loop (csr_bc.c:40-63) over exact distances;
@@ -550,26 +585,41 @@ int orc_pagerank_ref(int32_t V, const int32_t *irp, const int32_t *icol,
   return iter;
 }
 
+/* Sum of f(i) over [0, n) in fixed chunks of ORC_CHUNK: chunk partials in
+ * parallel, then added in chunk order, so the result does not depend on the
+ * thread count. */
+#define ORC_CHUNK 65536
+
 int orc_pagerank_cfg(int32_t V, const int32_t *irp, const int32_t *icol,
                      const int32_t *outdeg, double d, double tol,
                      int32_t max_iter, double *x, double *resid) {
   size_t n = (size_t)(V ? V : 1);
   double *contrib = (double *)malloc(sizeof(double) * n);
   double *xn = (double *)malloc(sizeof(double) * n);
+  const int64_t nch = ((int64_t)V + ORC_CHUNK - 1) / ORC_CHUNK;
+  double *part = (double *)malloc(sizeof(double) * (size_t)(nch ? nch : 1));
   const double N = (double)V;
   for (int32_t i = 0; i < V; i++) x[i] = 1.0 / N;
   int32_t it = 0;
   double r = INFINITY;
   while (it < max_iter) {
-    double dangling = 0.0;
-    for (int32_t u = 0; u < V; u++) {
-      if (outdeg[u] == 0) {
-        dangling += x[u];
-        contrib[u] = 0.0;
-      } else {
-        contrib[u] = x[u] / (double)outdeg[u];
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < nch; c++) {
+      const int32_t lo = (int32_t)(c * ORC_CHUNK);
+      const int32_t hi = (int32_t)((c + 1) * ORC_CHUNK < V ? (c + 1) * ORC_CHUNK : V);
+      double dg = 0.0;
+      for (int32_t u = lo; u < hi; u++) {
+        if (outdeg[u] == 0) {
+          dg += x[u];
+          contrib[u] = 0.0;
+        } else {
+          contrib[u] = x[u] / (double)outdeg[u];
+        }
       }
+      part[c] = dg;
     }
+    double dangling = 0.0;
+    for (int64_t c = 0; c < nch; c++) dangling += part[c];
     const double base = (1.0 - d) / N + d * dangling / N;
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int32_t v = 0; v < V; v++) {
@@ -577,11 +627,19 @@ int orc_pagerank_cfg(int32_t V, const int32_t *irp, const int32_t *icol,
       for (int32_t e = irp[v]; e < irp[v + 1]; e++) s += contrib[icol[e]];
       xn[v] = base + d * s;
     }
-    double acc = 0.0;
-    for (int32_t v = 0; v < V; v++) {
-      acc += fabs(xn[v] - x[v]);
-      x[v] = xn[v];
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < nch; c++) {
+      const int32_t lo = (int32_t)(c * ORC_CHUNK);
+      const int32_t hi = (int32_t)((c + 1) * ORC_CHUNK < V ? (c + 1) * ORC_CHUNK : V);
+      double acc = 0.0;
+      for (int32_t v = lo; v < hi; v++) {
+        acc += fabs(xn[v] - x[v]);
+        x[v] = xn[v];
+      }
+      part[c] = acc;
     }
+    double acc = 0.0;
+    for (int64_t c = 0; c < nch; c++) acc += part[c];
     r = acc;
     it++;
     if (r < tol) break;
@@ -589,6 +647,7 @@ int orc_pagerank_cfg(int32_t V, const int32_t *irp, const int32_t *icol,
   if (resid) *resid = r;
   free(contrib);
   free(xn);
+  free(part);
   return it;
 }
 

@@ -96,6 +96,50 @@ def test_sssp_relaxation_outcome_counters(gpu, nsrc, monkeypatch):
         monkeypatch.delenv("GB_SSSP_PER_SOURCE", raising=False)
 
 
+@pytest.mark.parametrize("nsrc", [1, 6, 40])
+def test_sssp_counters_exact_on_trees(gpu, nsrc, monkeypatch):
+    """The relaxation outcome totals (REV_STATS, csr_sspath_array.c:738-759)
+    pinned exactly on out-trees, where every reached vertex has one in-edge:
+    each (source, edge) out of a reached vertex is relaxed exactly once, each
+    reached non-source vertex is improved exactly once, and the improvement
+    stays in the expanding vertex's bucket (near) iff (d[u] + w) // delta ==
+    d[u] // delta.  Both engines, several deltas and light thresholds."""
+    gb = gpu
+    rng = np.random.default_rng(900 + nsrc)
+    V = 3000
+    parent = np.array([rng.integers(0, i) for i in range(1, V)])
+    child = np.arange(1, V)
+    w = rng.integers(1, 301, V - 1).astype(np.uint32)
+    h = gb.HostCSR.from_edges(V, parent, child, w)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    srcs = np.concatenate([[0], rng.choice(np.arange(1, V), nsrc - 1, replace=False)]).astype(np.int32)
+    src_of_edge = np.repeat(np.arange(V), np.diff(h.row_ptr))
+    _say(f"sssp counters on a tree, nsrc={nsrc}")
+    for per_source in (False, True):
+        if per_source:
+            monkeypatch.setenv("GB_SSSP_PER_SOURCE", "1")
+        for delta, light in ((1, 1), (50, 50), (200, 60), (1 << 20, 1 << 20)):
+            d, st = gb.sssp(g, srcs, delta=delta, light_max=light, stats=True)
+            d = gb.u32(d).astype(np.int64)
+            relax = near = far = reached = 0
+            for i in range(nsrc):
+                du = d[i][src_of_edge]
+                ok = du != 0xFFFFFFFF
+                nd = du[ok] + h.w[ok]
+                relax += int(ok.sum())
+                n_near = int((nd // delta == du[ok] // delta).sum())
+                near += n_near
+                far += int(ok.sum()) - n_near
+                reached += int((d[i] != 0xFFFFFFFF).sum())
+            tag = (per_source, delta, light)
+            assert st["reached"] == reached, tag
+            assert st["traversed"] == relax, tag
+            assert st["relax_edges"] == relax, (tag, st)
+            assert st["improved_near"] + st["improved_far"] == reached - nsrc, (tag, st)
+            assert (st["improved_near"], st["improved_far"]) == (near, far), (tag, st)
+        monkeypatch.delenv("GB_SSSP_PER_SOURCE", raising=False)
+
+
 def test_sssp_batched_overflow_reported(gpu):
     gb = gpu
     h = gb.HostCSR.from_edges(4, [0, 1, 2], [1, 2, 3], [0xFFFFFFF0, 0xFFFFFFF0, 5])
@@ -325,6 +369,35 @@ def test_bc_weighted(gpu):
     bc = gb.bc(g, srcs, weighted=True).cpu().numpy()
     ref = oracle.brandes_weighted(h.row_ptr, h.col, w64, r.row_ptr, r.col, rw64, srcs)
     assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+
+
+def test_bc_weighted_zero_weights(gpu):
+    """A zero-weight edge off any cycle is an ordinary tight edge; a cycle of
+    zero-weight edges has no Brandes order and is refused, not divided by a
+    zero path count."""
+    gb = gpu
+    _say("bc weighted zero weights")
+
+    def graph(src, dst, w, V):
+        h = gb.HostCSR.from_edges(V, src, dst, np.ones(len(src), np.uint32))
+        order = np.lexsort((np.asarray(dst), np.asarray(src)))
+        w64 = np.asarray(w, np.float64)[order]
+        r = h.transpose()
+        rorder = np.lexsort((np.asarray(src)[order], np.asarray(dst)[order]))
+        rw64 = w64[rorder]
+        return h, r, w64, rw64
+
+    # 0 -> 1 (0.0), 1 -> 2 (1.0), 0 -> 3 (0.5), 3 -> 2 (0.5): a DAG
+    h, r, w64, rw64 = graph([0, 1, 0, 3], [1, 2, 3, 2], [0.0, 1.0, 0.5, 0.5], 4)
+    g = gb.Graph.from_host(h, reverse=r, w64=w64, rw64=rw64)
+    bc = gb.bc(g, [0, 1, 2, 3], weighted=True).cpu().numpy()
+    ref = oracle.brandes_weighted(h.row_ptr, h.col, w64, r.row_ptr, r.col, rw64, [0, 1, 2, 3])
+    assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
+    # 0 -> 1 (1.0), 1 -> 2 (0.0), 2 -> 1 (0.0): 1 and 2 lie on a tight cycle
+    h, r, w64, rw64 = graph([0, 1, 2], [1, 2, 1], [1.0, 0.0, 0.0], 3)
+    g = gb.Graph.from_host(h, reverse=r, w64=w64, rw64=rw64)
+    with pytest.raises(gb.GBError, match="cycle of tight edges"):
+        gb.bc(g, [0], weighted=True)
 
 
 # --------------------------------------------------------------------- PageRank
