@@ -124,6 +124,15 @@ typedef void (*gb_pr_ref_hook)(int32_t iter, const double *d_wb, void *ctx);
 gb_status gb_pagerank_ref_impl(gb_graph g, const double *d_recip, int32_t max_iter, double *d_x,
                                int32_t *iterations, hipStream_t st, gb_pr_ref_hook hook,
                                void *ctx);
+// The same loop on caller-provided scratch: d_diff and d_wb (V doubles each)
+// and d_flag (one int); allocates nothing.
+gb_status gb_pagerank_ref_core(int32_t V, const int32_t *d_irp, const int32_t *d_icol,
+                               const double *d_recip, int32_t max_iter, double *d_x,
+                               int32_t *iterations, hipStream_t st, double *d_diff, double *d_wb,
+                               int *d_flag, gb_pr_ref_hook hook, void *ctx);
+// Looks up, with the device current, the per-device address of the
+// no-allocation scratch array (gb_noalloc.hip); called by gb_init.
+gb_status gb_noalloc_setup(int32_t device);
 // comp_distrib_csr on the device: distrib (p, int64) and, when non-null, the
 // per-part degree sums of two degree prefix arrays (send and recv).
 gb_status gb_edge_balanced_parts_ex(int32_t V, const int32_t *d_cut_rp, const int32_t *d_send_rp,

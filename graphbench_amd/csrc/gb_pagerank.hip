@@ -416,6 +416,8 @@ __global__ void pr_ref_gather_kernel(int32_t V, const int32_t *__restrict__ irp,
   wb[v] = acc;
 }
 
+__global__ void pr_ref_flag_clear_kernel(int *__restrict__ not_done) { *not_done = 0; }
+
 __global__ void pr_ref_update_kernel(int32_t V, const double *__restrict__ recip,
                                      double *__restrict__ x, double *__restrict__ diff,
                                      const double *__restrict__ wb, int *__restrict__ not_done) {
@@ -1290,6 +1292,41 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   return GB_OK;
 }
 
+gb_status gb_pagerank_ref_core(int32_t V, const int32_t *d_irp, const int32_t *d_icol,
+                               const double *d_recip, int32_t max_iter, double *d_x,
+                               int32_t *iterations, hipStream_t st, double *d_diff, double *d_wb,
+                               int *d_flag, gb_pr_ref_hook hook, void *ctx) {
+  const int blocks = (V + 255) / 256;
+  hipLaunchKernelGGL(pr_ref_init_kernel, dim3(blocks), dim3(256), 0, st, V, d_recip, d_x, d_diff,
+                     d_wb);
+  const double pj = 1.0 - 0.3;  // purpose_jump, pagerank_mpi.c:689
+  int32_t it = 0;
+  for (;;) {
+    if (max_iter > 0 && it >= max_iter) {
+      it = -1;
+      break;
+    }
+    it++;
+    if (hook) {
+      GB_HIP(hipStreamSynchronize(st));
+      hook(it, d_wb, ctx);  // pagerank_mpi.c:823-830 prints wb here
+    }
+    // a kernel, not a memset, clears the flag
+    hipLaunchKernelGGL(pr_ref_flag_clear_kernel, dim3(1), dim3(1), 0, st, d_flag);
+    hipLaunchKernelGGL(pr_ref_gather_kernel, dim3(blocks), dim3(256), 0, st, V, d_irp, d_icol,
+                       (const double *)d_diff, d_wb, pj);
+    hipLaunchKernelGGL(pr_ref_update_kernel, dim3(blocks), dim3(256), 0, st, V, d_recip, d_x,
+                       d_diff, (const double *)d_wb, d_flag);
+    int h = 0;
+    GB_HIP(hipMemcpyAsync(&h, d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    GB_HIP(hipStreamSynchronize(st));
+    if (!h) break;
+  }
+  GB_HIP(hipGetLastError());
+  if (iterations) *iterations = it;
+  return it < 0 ? GB_ERR_NOT_CONVERGED : GB_OK;
+}
+
 gb_status gb_pagerank_ref_impl(gb_graph g, const double *d_recip, int32_t max_iter, double *d_x,
                                int32_t *iterations, hipStream_t st, gb_pr_ref_hook hook,
                                void *ctx) {
@@ -1302,34 +1339,8 @@ gb_status gb_pagerank_ref_impl(gb_graph g, const double *d_recip, int32_t max_it
   GB_TRY(diff.ensure(sizeof(double) * V));
   GB_TRY(wb.ensure(sizeof(double) * V));
   GB_TRY(flag.ensure(sizeof(int)));
-  const int blocks = (V + 255) / 256;
-  hipLaunchKernelGGL(pr_ref_init_kernel, dim3(blocks), dim3(256), 0, st, V, d_recip, d_x,
-                     diff.as<double>(), wb.as<double>());
-  const double pj = 1.0 - 0.3;  // purpose_jump, pagerank_mpi.c:689
-  int32_t it = 0;
-  for (;;) {
-    if (max_iter > 0 && it >= max_iter) {
-      it = -1;
-      break;
-    }
-    it++;
-    if (hook) {
-      GB_HIP(hipStreamSynchronize(st));
-      hook(it, wb.as<double>(), ctx);  // pagerank_mpi.c:823-830 prints wb here
-    }
-    GB_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), st));
-    hipLaunchKernelGGL(pr_ref_gather_kernel, dim3(blocks), dim3(256), 0, st, V, g->rrp, g->rcol,
-                       diff.as<double>(), wb.as<double>(), pj);
-    hipLaunchKernelGGL(pr_ref_update_kernel, dim3(blocks), dim3(256), 0, st, V, d_recip, d_x,
-                       diff.as<double>(), wb.as<double>(), flag.as<int>());
-    int h = 0;
-    GB_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    GB_HIP(hipStreamSynchronize(st));
-    if (!h) break;
-  }
-  GB_HIP(hipGetLastError());
-  if (iterations) *iterations = it;
-  return it < 0 ? GB_ERR_NOT_CONVERGED : GB_OK;
+  return gb_pagerank_ref_core(V, g->rrp, g->rcol, d_recip, max_iter, d_x, iterations, st,
+                              diff.as<double>(), wb.as<double>(), flag.as<int>(), hook, ctx);
 }
 
 extern "C" gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t max_iter,

@@ -120,6 +120,7 @@ gb_status gb_init(int32_t device) {
   hipDeviceProp_t prop;
   GB_HIP(hipGetDeviceProperties(&prop, device));
   g_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  GB_TRY(gb_noalloc_setup(device));
   g_device = device;
   return GB_OK;
 }

@@ -230,6 +230,20 @@ gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
  * iterations receives the count, or -1 when max_iter is hit. */
 gb_status gb_pagerank_ref(gb_graph g, const double *d_recip, int32_t max_iter,
                           double *d_x, int32_t *iterations, void *stream);
+/* gb_pagerank_ref without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  The reverse CSR is passed as
+ * plain device pointers (d_irp: V+1 int32 row offsets, d_icol: the in-edge
+ * sources, in-CSR order).  Scratch (diff, wb, flag) lives in a fixed
+ * per-device array of 2 * 8 * GB_NOALLOC_PR_MAX_V + 256 bytes held until the
+ * process ends, so V may not exceed GB_NOALLOC_PR_MAX_V (GB_ERR_UNSUPPORTED
+ * otherwise).  Results, iteration count and status are those of
+ * gb_pagerank_ref on the same graph.  All calls on one device share its
+ * array: calls that overlap on one device race. */
+#define GB_NOALLOC_PR_MAX_V (1 << 22)
+gb_status gb_pagerank_ref_noalloc(int32_t V, const int32_t *d_irp, const int32_t *d_icol,
+                                  const double *d_recip, int32_t max_iter, double *d_x,
+                                  int32_t *iterations, void *stream);
 
 /* ---- BFS / Brandes ---------------------------------------------------------*/
 /* BFS levels (-1 unreached) from each source; d_level nsrc*V int32. */
