@@ -11,7 +11,7 @@
 #include <hipcub/device/device_radix_sort.hpp>
 #include <hipcub/device/device_scan.hpp>
 
-#include "../../include/graphbench.h"
+#include "graphbench.h"
 #include "gb_internal.hpp"
 
 namespace {

@@ -5,31 +5,42 @@
 // form a group and share everything but their distances:
 //   * distances are vertex-major, dist[v][W] (W = group width, 8..64), so the
 //     W sources' values of one vertex are one or two cache lines;
-//   * a queue entry is a 32/64-edge chunk of a vertex (vertex and edge range,
-//     16 bytes) together with the mask of sources for which it must be
-//     expanded (mask[v], u64);
+//   * a queue entry is a 32/64-edge chunk of a vertex (vertex, edge range and
+//     the mask of sources for which it must be expanded, 32 bytes);
 //   * a wave takes one entry and loads its col/w with one coalesced load; its
 //     lanes are then (edge slot, source) pairs, 64 / W edges per step, the
 //     slot's target and weight taken from the loading lane by a cross-lane
 //     read, so the W relaxations of one target are one coalesced line;
 //   * a group of steps is issued together: all target reads, then the
-//     atomicMin of every improving pair (no return value: the decision uses
-//     the value read), then one atomicOr per (step, slot) issued by one lane
-//     each in a single instruction, so a group costs three memory round trips
-//     whatever its size; the next entry's loads are in flight meanwhile;
+//     atomicMin of every improving pair and one atomicOr per (step, slot) on
+//     the targets' next-phase masks, issued by one lane each in a single
+//     instruction; neither returns a value (the decision uses the value read),
+//     so a group costs one memory round trip whatever its size, and the next
+//     entry's loads are in flight meanwhile;
 //   * an improved (target, source) pair sets the source's bit in the target's
-//     next-phase mask (same bucket) or far mask (later bucket), and the lane
-//     that turns a mask from empty to non-empty appends the target to the next
-//     queue (dedup without stamps).
+//     next-phase mask (same bucket) or far mask (later bucket); a far mask
+//     turned from empty to non-empty appends the target to the far queue;
+//   * between two phases a compaction pass scans the next-phase masks of all
+//     vertices (coalesced), zeroes them and writes one queue entry per 32/64-
+//     edge chunk of every marked vertex with out-edges, carrying the vertex's
+//     source mask, so the step kernel never waits for an atomic's return or
+//     reads row bounds inside its inner loop (the scan form, SCAN = true).
+// The scan costs a pass over V per phase.  Small graphs with many short
+// phases use the append form instead (SCAN = false): the lane whose atomicOr
+// turns a target's next-phase mask from empty to non-empty (its return value)
+// reads the target's row bounds and appends its chunks at once; the masks
+// are double-buffered (the phase reads its input mask array, which is
+// cleared by queue entry after the phase) and entries carry no mask.
 // All sources step through bucket indices in lockstep: light phases while the
 // near queue is non-empty, then a far scan that moves the next non-empty
 // bucket's (vertex, source) pairs to the near queue.  There is no light/heavy
 // split: an expansion relaxes all edges of the vertex.  Delta-stepping is
 // exact for any bucket width, so distances equal the per-source engine's.
 // The bucket loop is the state machine ms_ctl_advance (gb_sssp_ctl.hpp),
-// replayed from a captured hipGraph as in the per-source engine; a phase is
-// two launches, the state advance fused with the previous phase's mask clear,
-// then the step.
+// replayed from a captured hipGraph as in the per-source engine.  A phase of
+// the scan form is three launches (step, compaction, state advance); of the
+// append form two (the state advance fused with the clear of the previous
+// phase's input masks, then the step).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,8 +68,10 @@ struct MsShape {
 constexpr int32_t MS_MIN_CHUNK = 32;  // for queue capacities
 constexpr int MS_NCAP = 128, MS_FCAP = 128;  // LDS append buffers per wave (entries)
 
-// near-queue entry: one chunk [beg, end) of vertex v's edges
+// near-queue entry: one chunk [beg, end) of vertex v's edges and (scan form)
+// the mask of sources for which it is expanded
 struct __align__(16) MsEnt {
+  unsigned long long m;
   uint32_t v;
   int32_t beg, end;
   uint32_t pad;
@@ -72,9 +85,10 @@ __device__ __forceinline__ uint32_t ms_nchunks(int32_t deg) {
 
 // chunk q of the vertex whose edges are [r0, r1)
 template <int W>
-__device__ __forceinline__ MsEnt ms_chunk(uint32_t v, int32_t r0, int32_t r1, uint32_t q) {
+__device__ __forceinline__ MsEnt ms_chunk(unsigned long long m, uint32_t v, int32_t r0, int32_t r1,
+                                          uint32_t q) {
   const int32_t b = r0 + (int32_t)(q << MsShape<W>::CHUNK_SHIFT);
-  return MsEnt{v, b, min(r1, b + MsShape<W>::CHUNK), 0u};
+  return MsEnt{m, v, b, min(r1, b + MsShape<W>::CHUNK), 0u};
 }
 
 struct MsCtx {
@@ -85,7 +99,9 @@ struct MsCtx {
   int64_t E;
   int32_t ntask;                   // sources in this group (<= W)
   uint32_t *dist;                  // [V][W]
-  unsigned long long *mask[2];     // near masks, one per near queue
+  // next-phase masks: scan form mask[0] only (zero between phases), append
+  // form one per near queue
+  unsigned long long *mask[2];
   unsigned long long *far;         // far masks
   MsEnt *qn[2];                    // near queues 0/1 (chunks)
   uint64_t *qf[2];                 // far queues 0/1 (vertices)
@@ -104,18 +120,30 @@ __global__ void ms_fill_kernel(int64_t n, uint32_t *__restrict__ dist) {
     dist[i] = 0xFFFFFFFFu;
 }
 
-template <int W>
+// One wave (blockDim 64): lane t seeds source t; the lowest lane holding a
+// vertex writes its chunks with the mask of every lane holding it.
+template <int W, bool SCAN>
 __global__ void ms_seed_kernel(MsCtx c, const int32_t *__restrict__ src) {
   const int t = threadIdx.x;
-  if (t >= c.ntask) return;
-  const int32_t s = src[t];
+  const bool on = t < c.ntask;
+  const int32_t s = on ? src[t] : -1;
+  unsigned long long m = 0;
+  bool lowest = on;
+  for (int j = 0; j < GB_WAVE; j++) {
+    const int32_t sj = __shfl(s, j, GB_WAVE);
+    if (on && sj == s) {
+      m |= 1ull << j;
+      if (j < t) lowest = false;
+    }
+  }
+  if (!on) return;
   c.dist[(int64_t)s * W + t] = 0;
-  const unsigned long long old = atomicOr(&c.mask[0][s], 1ull << t);
-  if (old == 0) {
-    const int32_t r0 = c.rp[s], r1 = c.rp[s + 1];
+  if (!SCAN) atomicOr(&c.mask[0][s], 1ull << t);
+  const int32_t r0 = c.rp[s], r1 = c.rp[s + 1];
+  if (lowest && r1 > r0) {
     const uint32_t k = ms_nchunks<W>(r1 - r0);
     const unsigned long long b = atomicAdd(&c.cnt[MS_NEAR0], (unsigned long long)k);
-    for (uint32_t i = 0; i < k; i++) c.qn[0][b + i] = ms_chunk<W>((uint32_t)s, r0, r1, i);
+    for (uint32_t i = 0; i < k; i++) c.qn[0][b + i] = ms_chunk<W>(m, (uint32_t)s, r0, r1, i);
   }
 }
 
@@ -129,11 +157,11 @@ struct MsIn {
   uint32_t w;
 };
 
-template <int W>
+template <int W, bool SCAN>
 __device__ __forceinline__ MsIn ms_load(const MsCtx &c, const unsigned long long *min_,
                                         const MsEnt &e, int lane, int t) {
   MsIn r;
-  r.m = min_[e.v];
+  r.m = SCAN ? e.m : min_[e.v];
   r.du = t < c.ntask ? c.dist[(int64_t)e.v * W + t] : 0xFFFFFFFFu;
   const bool in = lane < e.end - e.beg;
   r.x = in ? c.col[e.beg + lane] : 0;
@@ -144,7 +172,7 @@ __device__ __forceinline__ MsIn ms_load(const MsCtx &c, const unsigned long long
 // Light phase: expand the entries of near queue a_in.  Entries are software
 // pipelined: entry i is expanded while entry i + nwaves's loads and entry
 // i + 2 nwaves's descriptor are in flight.
-template <int W>
+template <int W, bool SCAN>
 __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *lds_near,
                                          uint64_t *lds_far, uint32_t &relax, uint32_t &imp_near,
                                          uint32_t &imp_far, uint32_t &expanded, bool &ovf) {
@@ -157,26 +185,29 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
   const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
   const MsEnt *in = c.qn[a.a_in - MS_NEAR0];
   const unsigned long long *min_ = c.mask[a.a_in - MS_NEAR0];
-  unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
+  unsigned long long *mout = SCAN ? c.mask[0] : c.mask[a.a_near - MS_NEAR0];
   gb_wave_queue<MsEnt, MS_NCAP> nq(lds_near, c.qn[a.a_near - MS_NEAR0], &c.cnt[a.a_near]);
   gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.qf[a.a_far - MS_FAR0], &c.cnt[a.a_far]);
   const uint64_t n = c.cnt[a.a_in];
   const uint64_t b = a.a_bucket;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // wave-uniform indices (readfirstlane): the entries are read with scalar
+  // loads and live in scalar registers
+  const uint64_t wave = __builtin_amdgcn_readfirstlane(
+      (uint32_t)(((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   uint64_t i = wave, i1 = wave + nwaves;
   MsEnt ecur{}, enxt{};
   MsIn cur{};
   if (i < n) {
     ecur = in[i];
-    cur = ms_load<W>(c, min_, ecur, lane, t);
+    cur = ms_load<W, SCAN>(c, min_, ecur, lane, t);
   }
   if (i1 < n) enxt = in[i1];
   while (i < n) {
     MsIn nxt{};
     MsEnt enn{};
     const uint64_t i2 = i1 + nwaves;
-    if (i1 < n) nxt = ms_load<W>(c, min_, enxt, lane, t);
+    if (i1 < n) nxt = ms_load<W, SCAN>(c, min_, enxt, lane, t);
     if (i2 < n) enn = in[i2];
     const bool act = t < c.ntask && ((cur.m >> t) & 1ull);
     const int cnt = ecur.end - ecur.beg;
@@ -225,20 +256,32 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
       nm = (nm >> (ej * W)) & wmask;
       fm = (fm >> (ej * W)) & wmask;
       const uint32_t xj = (uint32_t)__shfl(cur.x, (g0 + lane) & (GB_WAVE - 1), GB_WAVE);
-      unsigned long long on = 1ull, of = 1ull;
-      if (lane < G && nm) on = atomicOr(&mout[xj], (unsigned long long)nm);
-      if (lane < G && fm) of = atomicOr(&c.far[xj], (unsigned long long)fm);
-      // a target without out-edges is not queued: its mask bits stay set,
-      // which only suppresses later appends that would expand nothing
-      int32_t r0 = 0, r1 = 0;
-      const bool newn = on == 0ull, newf = of == 0ull;
-      if (newn || newf) {
-        r0 = c.rp[xj];
-        r1 = c.rp[xj + 1];
+      if (SCAN) {
+        // no return value: the compaction pass after the phase queues the
+        // marked targets
+        if (lane < G && nm) atomicOr(&mout[xj], (unsigned long long)nm);
+        // far marks are rarer: the lane that starts a far mask queues the target
+        if (__ballot(lane < G && fm) != 0ull) {
+          unsigned long long of = 1ull;
+          if (lane < G && fm) of = atomicOr(&c.far[xj], (unsigned long long)fm);
+          fq.push(of == 0ull ? 1u : 0u, [&](uint32_t) { return (uint64_t)xj; });
+        }
+      } else {
+        unsigned long long on = 1ull, of = 1ull;
+        if (lane < G && nm) on = atomicOr(&mout[xj], (unsigned long long)nm);
+        if (lane < G && fm) of = atomicOr(&c.far[xj], (unsigned long long)fm);
+        // a target without out-edges is not queued: its mask bits stay set,
+        // which only suppresses later appends that would expand nothing
+        int32_t r0 = 0, r1 = 0;
+        const bool newn = on == 0ull, newf = of == 0ull;
+        if (newn || newf) {
+          r0 = c.rp[xj];
+          r1 = c.rp[xj + 1];
+        }
+        const uint32_t pn = newn && r1 > r0 ? ms_nchunks<W>(r1 - r0) : 0u;
+        nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(0ull, xj, r0, r1, q); });
+        fq.push(newf && r1 > r0 ? 1u : 0u, [&](uint32_t) { return (uint64_t)xj; });
       }
-      const uint32_t pn = newn && r1 > r0 ? ms_nchunks<W>(r1 - r0) : 0u;
-      nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(xj, r0, r1, q); });
-      fq.push(newf && r1 > r0 ? 1u : 0u, [&](uint32_t) { return (uint64_t)xj; });
     }
     ecur = enxt;
     cur = nxt;
@@ -246,12 +289,16 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
     i = i1;
     i1 = i2;
   }
-  gb_block_flush(nq, fq);
+  if (SCAN)
+    gb_block_flush(fq);
+  else
+    gb_block_flush(nq, fq);
 }
 
-// Far scan: far queue a_in -> bucket a_bucket's pairs to near queue a_near,
-// later buckets stay (far queue a_far), settled ones are dropped.
-template <int W>
+// Far scan: far queue a_in -> bucket a_bucket's pairs marked in the
+// next-phase masks (queued by the compaction), later buckets stay (far queue
+// a_far), settled ones are dropped.
+template <int W, bool SCAN>
 __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *lds_near,
                                        uint64_t *lds_far) {
   constexpr int ES = GB_WAVE / W;
@@ -259,7 +306,7 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *ld
   const int es = lane / W, t = lane % W;
   const uint64_t wmask = W == 64 ? ~0ull : ((1ull << W) - 1);
   const uint64_t *in = c.qf[a.a_in - MS_FAR0];
-  unsigned long long *mout = c.mask[a.a_near - MS_NEAR0];
+  unsigned long long *mout = SCAN ? c.mask[0] : c.mask[a.a_near - MS_NEAR0];
   gb_wave_queue<MsEnt, MS_NCAP> nq(lds_near, c.qn[a.a_near - MS_NEAR0], &c.cnt[a.a_near]);
   gb_wave_queue<uint64_t, MS_FCAP> fq(lds_far, c.qf[a.a_far - MS_FAR0], &c.cnt[a.a_far]);
   const uint64_t n = c.cnt[a.a_in];
@@ -290,34 +337,43 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *ld
     int32_t r0 = 0, r1 = 0;
     if (t == 0 && i < n) {
       c.far[x] = km;  // the entry is this vertex's only one in the scan
-      if (nm && atomicOr(&mout[x], (unsigned long long)nm) == 0ull) {
+      if (SCAN) {
+        if (nm) atomicOr(&mout[x], (unsigned long long)nm);
+      } else if (nm && atomicOr(&mout[x], (unsigned long long)nm) == 0ull) {
         r0 = c.rp[x];
         r1 = c.rp[x + 1];
         pn = ms_nchunks<W>(r1 - r0);
       }
       if (km) pf = 1;
     }
-    nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(x, r0, r1, q); });
-    fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
+    if (SCAN) {
+      fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
+    } else {
+      nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(0ull, x, r0, r1, q); });
+      fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
+    }
   }
-  gb_block_flush(nq, fq);
+  if (SCAN)
+    gb_block_flush(fq);
+  else
+    gb_block_flush(nq, fq);
   gb_block_atomic_min(minb, ~0ull, &c.cnt[MS_MINB]);
 }
 
-template <int W>
+template <int W, bool SCAN>
 __global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
                                                            const MsCtl *__restrict__ ctl) {
   const int act = ctl->action;
   if (act == ACT_NONE) return;
   const MsCtl a = *ctl;
-  __shared__ MsEnt s_near[MS_BLOCK / GB_WAVE][MS_NCAP];
   __shared__ uint64_t s_far[MS_BLOCK / GB_WAVE][MS_FCAP];
-  MsEnt *ln = s_near[threadIdx.x / GB_WAVE];
+  __shared__ MsEnt s_near[MS_BLOCK / GB_WAVE][SCAN ? 1 : MS_NCAP];
   uint64_t *lf = s_far[threadIdx.x / GB_WAVE];
+  MsEnt *ln = s_near[threadIdx.x / GB_WAVE];
   if (act == ACT_LIGHT) {
     uint32_t relax = 0, imp_near = 0, imp_far = 0, expanded = 0;
     bool ovf = false;
-    ms_light<W>(c, a, ln, lf, relax, imp_near, imp_far, expanded, ovf);
+    ms_light<W, SCAN>(c, a, ln, lf, relax, imp_near, imp_far, expanded, ovf);
     uint32_t tot, tn, tf, te;
     (void)gb_wave_excl_sum(relax, &tot);
     (void)gb_wave_excl_sum(imp_near, &tn);
@@ -331,17 +387,23 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
     }
     if (__ballot(ovf) && gb_lane() == 0) atomicOr(&c.cnt[MS_OVF], 1ull);
   } else {
-    ms_far<W>(c, a, ln, lf);
+    ms_far<W, SCAN>(c, a, ln, lf);
   }
 }
 
-// Between two phases, one launch: thread 0 computes the next action from
-// the previous state `prev` into `next` (the state machine ms_ctl_advance),
-// and, when the previous action was a light phase, every thread clears that
-// phase's input masks (the buffer is the output of the phase after next).
-// The two states are distinct buffers, alternating per phase, and the clear
-// takes its entry count from `prev` rather than the counter, which the
-// advance may reset, so the two halves never touch the same data.
+// The state machine between two phases (one thread).
+__global__ void ms_ctrl_kernel(MsCtx c, MsCtl *__restrict__ ctl) {
+  MsCtl s = *ctl;
+  ms_ctl_advance(s, c.cnt);
+  *ctl = s;
+}
+
+// Append form, between two phases, one launch: thread 0 computes the next
+// action from the previous state `prev` into `next`, and, when the previous
+// action was a light phase, every thread clears that phase's input masks (the
+// buffer is the output of the phase after next).  The two states are
+// distinct buffers, alternating per phase, and the clear takes its entry
+// count from `prev` rather than the counter, which the advance may reset.
 __global__ void ms_clear_ctrl_kernel(MsCtx c, const MsCtl *__restrict__ prev,
                                      MsCtl *__restrict__ next) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -357,6 +419,58 @@ __global__ void ms_clear_ctrl_kernel(MsCtx c, const MsCtl *__restrict__ prev,
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     m[q[i].v] = 0;
+}
+
+// After a phase: every vertex whose next-phase mask is set gets one near-queue
+// entry per chunk of its edges, with the mask, into queue a_near; the mask is
+// zeroed for the phase after.  A wave scans 256 consecutive vertices at a time
+// (four per lane, their masks read together and coalesced) and skips the
+// batch when none is marked, which is most of them in short phases.
+// Vertices without out-edges are dropped.
+constexpr int MS_CPL = 4;  // vertices per lane and batch
+template <int W>
+__global__ __launch_bounds__(MS_BLOCK) void ms_compact_kernel(MsCtx c,
+                                                              const MsCtl *__restrict__ ctl) {
+  const int act = ctl->action;
+  if (act == ACT_NONE) return;
+  const int out = ctl->a_near;
+  __shared__ MsEnt s_near[MS_BLOCK / GB_WAVE][MS_NCAP];
+  gb_wave_queue<MsEnt, MS_NCAP> nq(s_near[threadIdx.x / GB_WAVE], c.qn[out - MS_NEAR0],
+                                   &c.cnt[out]);
+  const int lane = gb_lane();
+  const int64_t V = c.V;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  constexpr int64_t SPAN = (int64_t)GB_WAVE * MS_CPL;
+  for (int64_t b0 = wave * SPAN; b0 < V; b0 += nwaves * SPAN) {
+    unsigned long long m[MS_CPL];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < MS_CPL; j++) {
+      const int64_t v = b0 + (int64_t)j * GB_WAVE + lane;
+      m[j] = v < V ? c.mask[0][v] : 0ull;
+      any |= m[j] != 0ull;
+    }
+    if (__ballot(any) == 0ull) continue;
+    int32_t r0[MS_CPL], r1[MS_CPL];
+#pragma unroll
+    for (int j = 0; j < MS_CPL; j++) {
+      const int64_t v = b0 + (int64_t)j * GB_WAVE + lane;
+      r0[j] = r1[j] = 0;
+      if (m[j]) {
+        c.mask[0][v] = 0ull;
+        r0[j] = c.rp[v];
+        r1[j] = c.rp[v + 1];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MS_CPL; j++) {
+      const uint32_t v = (uint32_t)(b0 + (int64_t)j * GB_WAVE + lane);
+      const uint32_t k = r1[j] > r0[j] ? ms_nchunks<W>(r1[j] - r0[j]) : 0u;
+      nq.push(k, [&](uint32_t q) { return ms_chunk<W>(m[j], v, r0[j], r1[j], q); });
+    }
+  }
+  gb_block_flush(nq);
 }
 
 // group distances [V][W] -> caller rows out[t][V] for t < ntask
@@ -420,6 +534,24 @@ int ms_grid(int64_t E) {
   return gb_num_cus() * per_cu;
 }
 
+// The scan form pays a pass over V per phase, the append form a returning
+// atomic and a row-bound read per newly marked target inside the step.
+// Measured with 8 sources of R-MAT 16 (185 phases): append 3.3 ms, scan
+// 4.3 ms; with 32 sources of R-MAT 24 (12 phases): append 61 ms, scan 54 ms.
+bool ms_scan_form(int64_t V) {
+  const char *e = getenv("GB_MSSSP_SCAN");
+  if (e && e[0]) return e[0] != '0';
+  return V >= ((int64_t)1 << 20);
+}
+
+// Compaction blocks: one batch of MS_CPL vertices per lane where that takes
+// at most 16 blocks per CU, else several batches per wave.
+int ms_compact_grid(int64_t V) {
+  const int64_t per_block = (int64_t)MS_BLOCK * MS_CPL;
+  const int64_t want = (V + per_block - 1) / per_block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)gb_num_cus() * 16));
+}
+
 struct MsWork {
   DevBuf dist, m0, m1, far, q[4], cnt, ctl, src, wstat;
   void release() {
@@ -433,7 +565,7 @@ MsWork &g_msw_ref() {
 }
 #define g_msw (g_msw_ref())
 
-template <int W>
+template <int W, bool SCAN>
 gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st,
                    uint32_t *out_rows, int64_t *phases, int64_t *buckets,
                    unsigned long long *relax, int64_t *imp_near, int64_t *imp_far,
@@ -443,26 +575,33 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   hipLaunchKernelGGL(ms_fill_kernel, dim3((unsigned)std::min<int64_t>((V * W + 255) / 256, 65536)),
                      dim3(256), 0, st, V * W, c.dist);
   GB_HIP(hipMemsetAsync(c.mask[0], 0, sizeof(unsigned long long) * V, st));
-  GB_HIP(hipMemsetAsync(c.mask[1], 0, sizeof(unsigned long long) * V, st));
+  if (!SCAN) GB_HIP(hipMemsetAsync(c.mask[1], 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.far, 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.cnt, 0, sizeof(unsigned long long) * MS_N, st));
   const int grid = ms_grid(c.E);
   const int64_t nwaves = (int64_t)grid * (MS_BLOCK / GB_WAVE);
   GB_HIP(hipMemsetAsync(c.wstat, 0, sizeof(unsigned long long) * GB_WSTAT * nwaves, st));
-  hipLaunchKernelGGL(ms_seed_kernel<W>, dim3(1), dim3(64), 0, st, c, d_src);
+  hipLaunchKernelGGL((ms_seed_kernel<W, SCAN>), dim3(1), dim3(64), 0, st, c, d_src);
   GB_HIP(hipGetLastError());
-  // two loop states, alternating per phase: phase k acts on ctl[k % 2], which
-  // the launch before it computed from ctl[(k + 1) % 2]; the first launch
-  // starts from the initial state in ctl[1] (no action taken yet)
+  // the loop state.  Scan form: one state in ctl[0], advanced to the first
+  // action here, then after every phase's compaction.  Append form: two
+  // states alternating per phase: phase k acts on ctl[k % 2], which the launch
+  // before it computed from ctl[(k + 1) % 2]; the first launch starts from the
+  // initial state in ctl[1] (no action taken yet)
   MsCtl hc{};
   hc.max_phases = max_phases > 0 ? max_phases : ((int64_t)1 << 40);
   hc.state = MS_ST_LIGHT;
   hc.buckets = 1;
   hc.action = ACT_NONE;
   MsCtl *ctl = w.ctl.as<MsCtl>();
-  GB_HIP(hipMemcpyAsync(ctl + 1, &hc, sizeof(hc), hipMemcpyHostToDevice, st));
-  const int cgrid = gb_num_cus() * 2;
-  constexpr int PH = 16;  // even: the replay ends on ctl[1], where the next one starts
+  MsCtl *hstate = SCAN ? ctl : ctl + 1;  // where a replay leaves the state
+  GB_HIP(hipMemcpyAsync(hstate, &hc, sizeof(hc), hipMemcpyHostToDevice, st));
+  if (SCAN) {
+    hipLaunchKernelGGL(ms_ctrl_kernel, dim3(1), dim3(1), 0, st, c, ctl);
+    GB_HIP(hipGetLastError());
+  }
+  const int cgrid = SCAN ? ms_compact_grid(c.V) : gb_num_cus() * 2;
+  constexpr int PH = 16;  // phases per replay (even: the append form ends on ctl[1])
   hipStream_t cs = st, own = nullptr;
   if (!st) {  // the null stream cannot be captured
     GB_HIP(hipStreamCreate(&own));
@@ -479,11 +618,19 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     for (int k = 0; k < PH; k++) {
-      MsCtl *cur = ctl + (k % 2), *prev = ctl + ((k + 1) % 2);
-      hipLaunchKernelGGL(ms_clear_ctrl_kernel, dim3(cgrid), dim3(256), 0, cs, c,
-                         (const MsCtl *)prev, cur);
-      hipLaunchKernelGGL(ms_step_kernel<W>, dim3(grid), dim3(MS_BLOCK), 0, cs, c,
-                         (const MsCtl *)cur);
+      if (SCAN) {
+        hipLaunchKernelGGL((ms_step_kernel<W, true>), dim3(grid), dim3(MS_BLOCK), 0, cs, c,
+                           (const MsCtl *)ctl);
+        hipLaunchKernelGGL(ms_compact_kernel<W>, dim3(cgrid), dim3(MS_BLOCK), 0, cs, c,
+                           (const MsCtl *)ctl);
+        hipLaunchKernelGGL(ms_ctrl_kernel, dim3(1), dim3(1), 0, cs, c, ctl);
+      } else {
+        MsCtl *cur = ctl + (k % 2), *prev = ctl + ((k + 1) % 2);
+        hipLaunchKernelGGL(ms_clear_ctrl_kernel, dim3(cgrid), dim3(256), 0, cs, c,
+                           (const MsCtl *)prev, cur);
+        hipLaunchKernelGGL((ms_step_kernel<W, false>), dim3(grid), dim3(MS_BLOCK), 0, cs, c,
+                           (const MsCtl *)cur);
+      }
     }
     e = hipStreamEndCapture(cs, &graph);
   }
@@ -493,7 +640,7 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   int64_t last = -1;
   while (gst == GB_OK) {
     if (hipGraphLaunch(exec, cs) != hipSuccess ||
-        hipMemcpyAsync(&hs, ctl + 1, sizeof(hs), hipMemcpyDeviceToHost, cs) != hipSuccess ||
+        hipMemcpyAsync(&hs, hstate, sizeof(hs), hipMemcpyDeviceToHost, cs) != hipSuccess ||
         hipStreamSynchronize(cs) != hipSuccess) {
       gst = gb_fail(GB_ERR_HIP, "msssp: phase replay failed");
       break;
@@ -566,8 +713,10 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   const int64_t qn = V + E / MS_MIN_CHUNK + 1;
   const int Wmax = nsrc >= 64 ? 64 : (nsrc > 32 ? 64 : nsrc > 16 ? 32 : nsrc > 8 ? 16 : 8);
   GB_TRY(w.dist.ensure(sizeof(uint32_t) * (size_t)V * Wmax));
+  // the scan form for large graphs (GB_MSSSP_SCAN=0/1 forces a form)
+  const bool scan = ms_scan_form(V);
   GB_TRY(w.m0.ensure(sizeof(unsigned long long) * V));
-  GB_TRY(w.m1.ensure(sizeof(unsigned long long) * V));
+  if (!scan) GB_TRY(w.m1.ensure(sizeof(unsigned long long) * V));
   GB_TRY(w.far.ensure(sizeof(unsigned long long) * V));
   GB_TRY(w.q[0].ensure(sizeof(MsEnt) * qn));
   GB_TRY(w.q[1].ensure(sizeof(MsEnt) * qn));
@@ -586,7 +735,7 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   c.E = g->E;
   c.dist = w.dist.as<uint32_t>();
   c.mask[0] = w.m0.as<unsigned long long>();
-  c.mask[1] = w.m1.as<unsigned long long>();
+  c.mask[1] = scan ? nullptr : w.m1.as<unsigned long long>();
   c.far = w.far.as<unsigned long long>();
   c.qn[0] = w.q[0].as<MsEnt>();
   c.qn[1] = w.q[1].as<MsEnt>();
@@ -605,16 +754,28 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
     const int32_t *ds = w.src.as<int32_t>();
     switch (W) {
       case 8:
-        GB_TRY(ms_group<8>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
+        GB_TRY((scan ? ms_group<8, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                          imp_near, imp_far, expanded, ovf)
+                    : ms_group<8, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                           imp_near, imp_far, expanded, ovf)));
         break;
       case 16:
-        GB_TRY(ms_group<16>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
+        GB_TRY((scan ? ms_group<16, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                          imp_near, imp_far, expanded, ovf)
+                    : ms_group<16, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                           imp_near, imp_far, expanded, ovf)));
         break;
       case 32:
-        GB_TRY(ms_group<32>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
+        GB_TRY((scan ? ms_group<32, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                          imp_near, imp_far, expanded, ovf)
+                    : ms_group<32, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                           imp_near, imp_far, expanded, ovf)));
         break;
       default:
-        GB_TRY(ms_group<64>(c, ds, max_phases, st, rows, phases, buckets, relax, imp_near, imp_far, expanded, ovf));
+        GB_TRY((scan ? ms_group<64, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                          imp_near, imp_far, expanded, ovf)
+                    : ms_group<64, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
+                                           imp_near, imp_far, expanded, ovf)));
     }
   }
   return GB_OK;

@@ -47,12 +47,16 @@ def test_sssp_small_bit_exact(gpu, seed, delta, light):
         assert np.array_equal(d[i], ref), f"source {s}"
 
 
+@pytest.mark.parametrize("scan", ["0", "1"])
 @pytest.mark.parametrize("nsrc", [2, 9, 17, 33, 64, 100])
-def test_sssp_batched_engine_matches_per_source(gpu, nsrc, monkeypatch):
+def test_sssp_batched_engine_matches_per_source(gpu, nsrc, scan, monkeypatch):
     """Several sources run in the batched engine (groups of up to 64 sources,
-    widths 8..64); distances equal the per-source engine's and Dijkstra's,
-    with duplicate sources and sources without out-edges included."""
+    widths 8..64), in both forms (appends inside the step, or the compaction
+    scan between phases); distances equal the per-source engine's and
+    Dijkstra's, with duplicate sources and sources without out-edges
+    included."""
     gb = gpu
+    monkeypatch.setenv("GB_MSSSP_SCAN", scan)
     h = gb.gen_rmat(12, 8, seed=50 + nsrc, weights="uniform", wlo=1, whi=300)
     g = gb.Graph.from_host(h, need_reverse=False)
     rng = np.random.default_rng(nsrc)
@@ -96,15 +100,18 @@ def test_sssp_relaxation_outcome_counters(gpu, nsrc, monkeypatch):
         monkeypatch.delenv("GB_SSSP_PER_SOURCE", raising=False)
 
 
+@pytest.mark.parametrize("scan", ["0", "1"])
 @pytest.mark.parametrize("nsrc", [1, 6, 40])
-def test_sssp_counters_exact_on_trees(gpu, nsrc, monkeypatch):
+def test_sssp_counters_exact_on_trees(gpu, nsrc, scan, monkeypatch):
     """The relaxation outcome totals (REV_STATS, csr_sspath_array.c:738-759)
     pinned exactly on out-trees, where every reached vertex has one in-edge:
     each (source, edge) out of a reached vertex is relaxed exactly once, each
     reached non-source vertex is improved exactly once, and the improvement
     stays in the expanding vertex's bucket (near) iff (d[u] + w) // delta ==
-    d[u] // delta.  Both engines, several deltas and light thresholds."""
+    d[u] // delta.  Both engines (the batched one in both forms), several
+    deltas and light thresholds."""
     gb = gpu
+    monkeypatch.setenv("GB_MSSSP_SCAN", scan)
     rng = np.random.default_rng(900 + nsrc)
     V = 3000
     parent = np.array([rng.integers(0, i) for i in range(1, V)])
