@@ -1,28 +1,53 @@
 // gb_msbc.hip -- batched multi-source BFS and Brandes betweenness for gfx950.
 //
 // Replaces the per-source loop of run_bc (btwcentral/csr_bc.c:40-63) for
-// unweighted graphs: 64 sources share one wave, one source per lane, and the
-// per-source state is stored vertex-major so that a wave reads one vertex's
-// state for all 64 sources with one coalesced load:
+// unweighted graphs.  64 sources share one wave, one source per lane, and the
+// per-source state is vertex-major so that a wave reads one vertex's state for
+// all 64 sources with one coalesced load:
 //   sigma[v][64], delta[v][64]  FP64 path counts and dependencies
-//   fm[d][v]                    64-bit mask: sources for which v is at level d
 //   visited[v]                  64-bit mask: sources that have reached v
-// Forward step d -> d+1 (pull, comp_num_spath csr_bc.c:70-96): one wave per
-// candidate vertex x (some source has not reached it yet).  Lanes read 64 of
-// x's in-edges at a time and test fm[d][u]; for each edge whose mask meets the
-// unreached lanes, lane s adds sigma[u][s].  Each lane sums its in-edges in
-// CSR order, so sigma is identical to a sequential pull.  Lanes whose sum is
-// non-zero set level d+1: fm[d+1][x], visited[x] and the level list L_{d+1}.
-// Backward step d (comp_delta csr_bc.c:98-123, pull form): one wave per vertex
-// of L_d; lane s sums ((1+delta[w][s])*sigma[v][s])/sigma[w][s] over out-edges
-// (v,w) with bit s of fm[d+1][w] set, in CSR order.
+//   fm[2][v]                    64-bit masks of the current and next level
+// G groups of 64 sources run side by side in grid.y.
+//
+// Levels.  The vertices of level d (for some source of the group) form the
+// list L_d, kept in a pool of (vertex, source mask) entries; BFS keeps two
+// levels in the pool, Brandes every level for the backward pass.  The pool
+// grows on the host between replays when a level could overflow it, so the
+// memory per group is O(V + sum of level sizes) <= O(65 V) whatever the depth.
+// A level d -> d+1 of the forward pass (comp_num_spath, csr_bc.c:70-96):
+//   * control (one wave): the previous level's append count gives L_d's size
+//     and the start of L_{d+1}; each group chooses push or pull from the
+//     out-edges of L_d against the in-edges of vertices some source has not
+//     reached (push while the former is below 1/16 of the latter);
+//   * push: one wave per item of at most PUSH_CH out-edges of an L_d vertex;
+//     an edge to x carries the sources of the vertex's level-d mask that have
+//     not reached x: their sigma is added to x's by FP64 atomicAdd (integers:
+//     exact in any order), the bits are ORed into x's next-level mask, and the
+//     lane whose OR found the mask empty appends x to L_{d+1};
+//   * pull: a scan lists the vertices with 1..hub in-edges that some source
+//     has not reached; one wave per listed vertex sums, lane s, sigma[u][s]
+//     over its in-edges u at level d for s, in CSR order; vertices above hub
+//     in-edges are pulled in chunks of hub edges, one wave each, with FP64
+//     atomicAdd;
+//   * finalize: each L_{d+1} entry takes its final mask, sets visited, writes
+//     BFS levels, and queues its out-edges as push items; the L_d masks are
+//     cleared for reuse as level d+2.
+// Backward (comp_delta, csr_bc.c:98-123, pull form), levels depth-2 .. 0: one
+// wave per L_d vertex; lane s sums ((1+delta[w][s])*sigma[v][s])/sigma[w][s]
+// over out-edges (v,w) with w at level d+1 for s, in CSR order (the operand
+// order of csr_bc.c:116).  The level masks are looked up in the two dense
+// mask arrays of the forward pass (zero when it ends): level l's entries are
+// written from the pool into array l & 1 before level l - 1 is processed,
+// and the entries of level l + 2 are cleared from it first.  Vertices above
+// hub out-edges are summed in chunks whose partials are added in chunk order.
 // Accumulation (csr_bc.c:57-61): bc[v] += delta[v][s] for v != source(s), in
-// source order; so bc matches the sequential oracle bit for bit.
-// Batches of 64 sources are independent: G of them run side by side in
-// grid.y, so one launch per level serves up to 64*G sources.
+// source order.
+// Both level loops are state machines on the device, replayed from captured
+// hipGraphs LPR levels at a time; the host reads one small state per replay.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "gb_device.hpp"
@@ -30,21 +55,71 @@
 
 namespace {
 
-constexpr int MB = 256;   // threads per block (4 waves)
-constexpr int MBQ = 256;  // staged list appends per wave
+constexpr int MB = 256;            // threads per block (4 waves)
+constexpr int MBQ = 256;           // staged appends per wave
+constexpr int32_t PUSH_CH = 256;   // out-edges per push item at most
+constexpr int LPR = 16;            // levels per replay
 
-__global__ void mbc_cand_init_kernel(int32_t V, const int32_t *__restrict__ rrp, int32_t hub,
-                                     int32_t *__restrict__ cand, unsigned long long *count) {
-  // vertices with 1..hub in-edges (none: never reached; more: the hub kernels)
-  const int lane = gb_lane();
-  for (int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; b0 < V;
-       b0 += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = b0 + lane;
-    const int32_t deg = v < V ? rrp[v + 1] - rrp[v] : 0;
-    const bool keep = deg > 0 && deg <= hub;
-    const uint64_t pos = gb_wave_reserve(count, keep ? 1u : 0u);
-    if (keep) cand[pos] = (int32_t)v;
-  }
+enum { MODE_NONE = 0, MODE_PUSH = 1, MODE_PULL = 2 };
+// per-group counters (u64), CN per group
+enum {
+  C_NEXT = 0,   // entries appended to L_{d+1}
+  C_CUR = 1,    // entries of L_d
+  C_MF = 2,     // out-edges of the entries of the newest finalized level
+  C_MPULL = 3,  // in-edges of vertices some source has not reached
+  C_CAND = 4,   // pull candidates of this level
+  C_PQ0 = 5,    // push items for even levels (C_PQ0 + 1: odd levels)
+  CN = 8
+};
+
+struct __align__(16) PushItem {
+  int32_t v, beg, end, pad;
+};
+
+struct BcLoop {
+  int32_t d;      // forward: the level last processed (-1 after seeding)
+  int32_t done;   // forward finished
+  int32_t stall;  // the pool or the level table needs room: the host grows it
+  int32_t depth;  // forward: number of non-empty levels once done
+  int32_t bd;     // backward: the level last processed
+  int32_t bdone;  // backward finished
+  int32_t pad[2];
+};
+
+struct BcCtx {
+  int32_t V, G, hub, lvl_cap;
+  int64_t E;  // edges of the CSR (both directions have E)
+  const int32_t *rp, *col, *rrp, *rcol;
+  const unsigned long long *act;  // [G] sources of the group
+  unsigned long long *visited;    // [G][V]
+  unsigned long long *fm[2];      // [G][V]
+  int32_t *pool_v;                // [G][pool_cap]
+  unsigned long long *pool_m;     // [G][pool_cap]
+  int64_t pool_cap;
+  int64_t *loff;  // [G][lvl_cap] pool offset of each level (two slots: BFS)
+  int32_t keep;   // 1: every level stays in the pool (Brandes)
+  PushItem *pq[2];  // [G][pq_cap] push items of even / odd levels
+  int64_t pq_cap;
+  int32_t *cand;            // [G][V]
+  unsigned long long *cnt;  // [G][CN]
+  int32_t *mode;            // [G]
+  BcLoop *st;
+  double *sigma, *delta;  // [G][V][64] (Brandes)
+  int32_t *level_out;     // BFS levels, rows g * 64 + s (or null)
+  const int32_t *hub_in;
+  int32_t nhub_in, maxch_in;
+  const int32_t *hub_out;
+  int32_t nhub_out, maxch_out, bch;
+  double *part;  // backward hub chunk partials
+};
+
+__device__ __forceinline__ int lslot(const BcCtx &c, int d) { return c.keep ? d : (d & 1); }
+
+__device__ __forceinline__ uint64_t wave_id() {
+  return ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+}
+__device__ __forceinline__ uint64_t wave_count() {
+  return ((uint64_t)gridDim.x * blockDim.x) >> 6;
 }
 
 // vertices with more than `hub` edges in a CSR, and the largest such degree
@@ -63,6 +138,167 @@ __global__ void mbc_hub_list_kernel(int32_t V, const int32_t *__restrict__ rp, i
       atomicMax(maxdeg, (unsigned int)deg);
     }
   }
+}
+
+// Sources of group g (blockDim 64, lane s = source 64 g + s): level 0.  The
+// lowest lane holding a vertex appends it to L_0 with the mask of every lane
+// holding it.
+__global__ void mbc_seed_kernel(BcCtx c, const int32_t *__restrict__ src, int32_t nsrc) {
+  const int32_t g = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t gV = (int64_t)g * c.V;
+  const bool on = g * 64 + lane < nsrc;
+  const int32_t v = on ? src[g * 64 + lane] : -1;
+  unsigned long long m = 0;
+  bool lowest = on;
+  for (int j = 0; j < GB_WAVE; j++) {
+    const int32_t vj = __shfl(v, j, GB_WAVE);
+    if (on && vj == v) {
+      m |= 1ull << j;
+      if (j < lane) lowest = false;
+    }
+  }
+  if (!on) return;
+  if (c.sigma) c.sigma[(gV + v) * 64 + lane] = 1.0;
+  if (lowest) {
+    c.fm[0][gV + v] = m;
+    const unsigned long long p = atomicAdd(&c.cnt[(int64_t)g * CN + C_NEXT], 1ull);
+    c.pool_v[(int64_t)g * c.pool_cap + p] = v;
+  }
+}
+
+// Forward control, one wave: advances to level d = st->d + 1 (see the header).
+__global__ void mbc_fwd_ctl_kernel(BcCtx c) {
+  BcLoop *st = c.st;
+  if (st->done || st->stall) return;
+  const int lane = threadIdx.x;
+  const int d = st->d + 1;
+  bool ok = true;
+  unsigned long long total = 0;
+  for (int g = lane; g < c.G; g += 64) {
+    const unsigned long long n = c.cnt[(int64_t)g * CN + C_NEXT];
+    total += n;
+    if (c.keep) {
+      const int64_t start = c.loff[(int64_t)g * c.lvl_cap + d];
+      if (start + (int64_t)n + c.V > c.pool_cap || d + 1 >= c.lvl_cap) ok = false;
+    }
+  }
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) total += __shfl_xor(total, k, 64);
+  const bool all_ok = __ballot(!ok) == 0ull;
+  __builtin_amdgcn_wave_barrier();
+  if (!all_ok) {
+    if (lane == 0) st->stall = 1;
+    return;
+  }
+  if (total == 0) {
+    if (lane == 0) {
+      st->done = 1;
+      st->depth = d;
+    }
+    return;
+  }
+  for (int g = lane; g < c.G; g += 64) {
+    unsigned long long *cg = c.cnt + (int64_t)g * CN;
+    const unsigned long long n = cg[C_NEXT];
+    cg[C_CUR] = n;
+    int64_t *lo = c.loff + (int64_t)g * c.lvl_cap;
+    lo[lslot(c, d + 1)] = c.keep ? lo[d] + (int64_t)n : (int64_t)((d + 1) & 1) * c.V;
+    const unsigned long long mf = cg[C_MF], mp = cg[C_MPULL];
+    c.mode[g] = n == 0 ? MODE_NONE : (16 * mf < mp ? MODE_PUSH : MODE_PULL);
+    cg[C_NEXT] = 0;
+    cg[C_MF] = 0;
+    cg[C_CAND] = 0;
+    cg[C_PQ0 + ((d + 1) & 1)] = 0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) st->d = d;
+}
+
+// The next level's queue of group g
+struct NextList {
+  int32_t *base;
+  unsigned long long *count;
+};
+__device__ __forceinline__ NextList next_list(const BcCtx &c, int32_t g, int d) {
+  const int64_t start = c.loff[(int64_t)g * c.lvl_cap + lslot(c, d + 1)];
+  return NextList{c.pool_v + (int64_t)g * c.pool_cap + start, &c.cnt[(int64_t)g * CN + C_NEXT]};
+}
+
+// Pull candidates: vertices with 1..hub in-edges that some source of the
+// group has not reached.
+__global__ __launch_bounds__(MB) void mbc_scan_kernel(BcCtx c) {
+  const int32_t g = blockIdx.y;
+  if (c.st->done || c.st->stall || c.mode[g] != MODE_PULL) return;
+  const int64_t gV = (int64_t)g * c.V;
+  const uint64_t act = c.act[g];
+  const int lane = gb_lane();
+  __shared__ int32_t s_q[MB / GB_WAVE][MBQ];
+  gb_wave_queue<int32_t, MBQ> q(s_q[threadIdx.x / GB_WAVE], c.cand + gV,
+                                &c.cnt[(int64_t)g * CN + C_CAND]);
+  for (int64_t b0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; b0 < c.V;
+       b0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = b0 + lane;
+    bool keep = false;
+    if (v < c.V) {
+      const int32_t deg = c.rrp[v + 1] - c.rrp[v];
+      keep = deg > 0 && deg <= c.hub && (~c.visited[gV + v] & act) != 0ull;
+    }
+    q.push(keep ? 1u : 0u, [&](uint32_t) { return (int32_t)v; });
+  }
+  gb_block_flush(q);
+}
+
+// Push over the level's items (see the header).
+template <bool SIGMA>
+__global__ __launch_bounds__(MB) void mbc_push_kernel(BcCtx c) {
+  const int32_t g = blockIdx.y;
+  if (c.st->done || c.st->stall || c.mode[g] != MODE_PUSH) return;
+  const int d = c.st->d;
+  const int64_t gV = (int64_t)g * c.V;
+  const int lane = gb_lane();
+  const uint64_t n = c.cnt[(int64_t)g * CN + C_PQ0 + (d & 1)];
+  const PushItem *items = c.pq[d & 1] + (int64_t)g * c.pq_cap;
+  const unsigned long long *fmc = c.fm[d & 1] + gV;
+  unsigned long long *fmn = c.fm[(d + 1) & 1] + gV;
+  const unsigned long long *vis = c.visited + gV;
+  const uint64_t act = c.act[g];
+  const NextList nl = next_list(c, g, d);
+  __shared__ int32_t s_list[MB / GB_WAVE][MBQ];
+  gb_wave_queue<int32_t, MBQ> lq(s_list[threadIdx.x / GB_WAVE], nl.base, nl.count);
+  const uint64_t nwaves = wave_count();
+  for (uint64_t i = wave_id(); i < n; i += nwaves) {
+    const PushItem it = items[i];
+    const uint64_t mu = fmc[it.v];
+    const double su = SIGMA && ((mu >> lane) & 1ull) ? c.sigma[(gV + it.v) * 64 + lane] : 0.0;
+    for (int32_t e0 = it.beg; e0 < it.end; e0 += GB_WAVE) {
+      const int32_t e = e0 + lane;
+      int32_t x = 0;
+      uint64_t nb = 0;
+      if (e < it.end) {
+        x = c.col[e];
+        nb = mu & ~vis[x] & act;
+      }
+      unsigned long long old = 1ull;
+      if (nb) old = atomicOr(&fmn[x], (unsigned long long)nb);
+      lq.push(nb && old == 0ull ? 1u : 0u, [&](uint32_t) { return x; });
+      if (SIGMA) {
+        uint64_t hit = __ballot(nb != 0);
+        while (hit) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            const bool okk = hit != 0;
+            const int j = okk ? __ffsll((unsigned long long)hit) - 1 : 0;
+            hit &= hit - 1;
+            const int32_t xj = __shfl(x, j, GB_WAVE);
+            const uint64_t nj = okk ? gb_shfl_u64(nb, j) : 0ull;
+            if ((nj >> lane) & 1ull) atomicAdd(&c.sigma[(gV + xj) * 64 + lane], su);
+          }
+        }
+      }
+    }
+  }
+  gb_block_flush(lq);
 }
 
 // Pull over in-edges [beg, end) of one vertex: lane s adds sigma[u][s] for every
@@ -106,8 +342,185 @@ __device__ __forceinline__ void pull_in(int lane, int32_t beg, int32_t end,
   }
 }
 
+template <bool SIGMA>
+__global__ __launch_bounds__(MB) void mbc_pull_kernel(BcCtx c) {
+  const int32_t g = blockIdx.y;
+  if (c.st->done || c.st->stall || c.mode[g] != MODE_PULL) return;
+  const int d = c.st->d;
+  const int64_t gV = (int64_t)g * c.V;
+  const int lane = gb_lane();
+  const uint64_t n = c.cnt[(int64_t)g * CN + C_CAND];
+  const unsigned long long *fmc = c.fm[d & 1] + gV;
+  unsigned long long *fmn = c.fm[(d + 1) & 1] + gV;
+  const uint64_t act = c.act[g];
+  const NextList nl = next_list(c, g, d);
+  __shared__ int32_t s_list[MB / GB_WAVE][MBQ];
+  gb_wave_queue<int32_t, MBQ> lq(s_list[threadIdx.x / GB_WAVE], nl.base, nl.count);
+  const uint64_t nwaves = wave_count();
+  for (uint64_t i = wave_id(); i < n; i += nwaves) {
+    const int32_t x = c.cand[gV + i];
+    const uint64_t need = ~c.visited[gV + x] & act;
+    double acc = 0.0;
+    uint64_t any = 0;
+    pull_in<SIGMA>(lane, c.rrp[x], c.rrp[x + 1], c.rcol, fmc, need, c.sigma, gV, acc, any);
+    const uint64_t found = SIGMA ? (__ballot(acc > 0.0) & need) : (any & need);
+    if (found) {
+      if (SIGMA && ((found >> lane) & 1ull)) c.sigma[(gV + x) * 64 + lane] = acc;
+      if (lane == 0) fmn[x] = found;  // one wave per candidate: a plain store
+    }
+    lq.push(lane == 0 && found ? 1u : 0u, [&](uint32_t) { return x; });
+  }
+  gb_block_flush(lq);
+}
+
+// Pull for in-degree hubs: one wave per (hub, chunk of `hub` edges).  Chunk
+// sums of sigma are integers, so atomicAdd keeps them exact.
+template <bool SIGMA>
+__global__ __launch_bounds__(MB) void mbc_pull_hub_kernel(BcCtx c) {
+  const int32_t g = blockIdx.y;
+  if (c.st->done || c.st->stall || c.mode[g] != MODE_PULL || c.nhub_in == 0) return;
+  const int d = c.st->d;
+  const int64_t gV = (int64_t)g * c.V;
+  const int lane = gb_lane();
+  const uint64_t items = (uint64_t)c.nhub_in * c.maxch_in;
+  const unsigned long long *fmc = c.fm[d & 1] + gV;
+  unsigned long long *fmn = c.fm[(d + 1) & 1] + gV;
+  const uint64_t act = c.act[g];
+  const NextList nl = next_list(c, g, d);
+  __shared__ int32_t s_list[MB / GB_WAVE][MBQ];
+  gb_wave_queue<int32_t, MBQ> lq(s_list[threadIdx.x / GB_WAVE], nl.base, nl.count);
+  const uint64_t nwaves = wave_count();
+  for (uint64_t it = wave_id(); it < items; it += nwaves) {
+    const int32_t x = c.hub_in[it / c.maxch_in];
+    const int32_t k = (int32_t)(it % c.maxch_in);
+    const int32_t beg = c.rrp[x] + k * c.hub;
+    const int32_t end = min(c.rrp[x + 1], beg + c.hub);
+    const uint64_t need = beg < end ? ~c.visited[gV + x] & act : 0ull;
+    uint64_t found = 0;
+    if (need) {
+      double acc = 0.0;
+      uint64_t any = 0;
+      pull_in<SIGMA>(lane, beg, end, c.rcol, fmc, need, c.sigma, gV, acc, any);
+      found = SIGMA ? (__ballot(acc > 0.0) & need) : (any & need);
+      if (SIGMA && ((found >> lane) & 1ull)) atomicAdd(&c.sigma[(gV + x) * 64 + lane], acc);
+    }
+    unsigned long long old = 1ull;
+    if (found && lane == 0) old = atomicOr(&fmn[x], (unsigned long long)found);
+    lq.push(lane == 0 && old == 0ull ? 1u : 0u, [&](uint32_t) { return x; });
+  }
+  gb_block_flush(lq);
+}
+
+// Finalize L_{d+1} (final masks, visited, BFS levels, push items, the counts
+// that choose the next mode) and clear the L_d masks (see the header).
+__global__ __launch_bounds__(MB) void mbc_fin_kernel(BcCtx c) {
+  const int32_t g = blockIdx.y;
+  if (c.st->done || c.st->stall) return;
+  const int d = c.st->d;
+  const int64_t gV = (int64_t)g * c.V;
+  const int lane = gb_lane();
+  const unsigned long long *cg = c.cnt + (int64_t)g * CN;
+  const uint64_t n_next = cg[C_NEXT];
+  const uint64_t n_cur = d >= 0 ? cg[C_CUR] : 0;
+  const int64_t *lo = c.loff + (int64_t)g * c.lvl_cap;
+  const int64_t pbase = (int64_t)g * c.pool_cap;
+  const int64_t base_next = pbase + lo[lslot(c, d + 1)];
+  const int64_t base_cur = d >= 0 ? pbase + lo[lslot(c, d)] : 0;
+  unsigned long long *fmn = c.fm[(d + 1) & 1] + gV;
+  unsigned long long *fmc = c.fm[d & 1] + gV;
+  const uint64_t act = c.act[g];
+  __shared__ PushItem s_pq[MB / GB_WAVE][MBQ / 2];
+  gb_wave_queue<PushItem, MBQ / 2> pq(s_pq[threadIdx.x / GB_WAVE],
+                                      c.pq[(d + 1) & 1] + (int64_t)g * c.pq_cap,
+                                      &c.cnt[(int64_t)g * CN + C_PQ0 + ((d + 1) & 1)]);
+  unsigned long long mf = 0, mp = 0;
+  const uint64_t n = n_next > n_cur ? n_next : n_cur;
+  for (uint64_t b0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; b0 < n;
+       b0 += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = b0 + lane;
+    int32_t v = 0, r0 = 0, r1 = 0;
+    if (i < n_next) {
+      v = c.pool_v[base_next + i];
+      const unsigned long long m = fmn[v];
+      c.pool_m[base_next + i] = m;
+      const unsigned long long old = c.visited[gV + v];
+      const unsigned long long nv = old | m;
+      c.visited[gV + v] = nv;
+      if (c.level_out) {
+        for (unsigned long long b = m; b; b &= b - 1) {
+          const int s = __ffsll(b) - 1;
+          c.level_out[((int64_t)g * 64 + s) * c.V + v] = d + 1;
+        }
+      }
+      r0 = c.rp[v];
+      r1 = c.rp[v + 1];
+      mf += (unsigned long long)(r1 - r0);
+      if (old != act && nv == act) mp += (unsigned long long)(c.rrp[v + 1] - c.rrp[v]);
+    }
+    const uint32_t k = (uint32_t)((r1 - r0 + PUSH_CH - 1) / PUSH_CH);
+    pq.push(k, [&](uint32_t q) {
+      const int32_t b = r0 + (int32_t)q * PUSH_CH;
+      return PushItem{v, b, min(r1, b + PUSH_CH), 0};
+    });
+    if (i < n_cur) fmc[c.pool_v[base_cur + i]] = 0ull;
+  }
+  gb_block_flush(pq);
+  __shared__ unsigned long long s_mf[MB / GB_WAVE], s_mp[MB / GB_WAVE];
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) {
+    mf += __shfl_xor(mf, k, 64);
+    mp += __shfl_xor(mp, k, 64);
+  }
+  if (lane == 0) {
+    s_mf[threadIdx.x / GB_WAVE] = mf;
+    s_mp[threadIdx.x / GB_WAVE] = mp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tf = 0, tp = 0;
+    for (int w = 0; w < MB / GB_WAVE; w++) {
+      tf += s_mf[w];
+      tp += s_mp[w];
+    }
+    if (tf) atomicAdd(&c.cnt[(int64_t)g * CN + C_MF], tf);
+    if (tp) atomicAdd(&c.cnt[(int64_t)g * CN + C_MPULL], (unsigned long long)0 - tp);
+  }
+}
+
+// ---- backward -------------------------------------------------------------
+
+// Backward control (one thread): level st->bd - 1, done below 0.
+__global__ void mbc_bwd_ctl_kernel(BcCtx c) {
+  BcLoop *st = c.st;
+  if (st->bdone) return;
+  const int bd = st->bd - 1;
+  if (bd < 0)
+    st->bdone = 1;
+  else
+    st->bd = bd;
+}
+
+// Level l's entries into mask array l & 1 (write) or their masks cleared
+// (!write), for l = st->bd + off.
+template <bool WRITE>
+__global__ void mbc_bwd_masks_kernel(BcCtx c, int off) {
+  const int32_t g = blockIdx.y;
+  if (c.st->bdone) return;
+  const int l = c.st->bd + off;
+  if (l < 0 || l >= c.st->depth) return;
+  const int64_t gV = (int64_t)g * c.V;
+  const int64_t *lo = c.loff + (int64_t)g * c.lvl_cap;
+  const int64_t base = (int64_t)g * c.pool_cap + lo[l];
+  const int64_t n = lo[l + 1] - lo[l];
+  unsigned long long *m = c.fm[l & 1] + gV;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m[c.pool_v[base + i]] = WRITE ? c.pool_m[base + i] : 0ull;
+}
+
 // Pull over out-edges [beg, end): lane s adds ((1+delta[w][s])*sv)/sigma[w][s]
-// for every successor w one level deeper for source s, in CSR order.
+// for every successor w one level deeper for source s (its mask in fmd1), in
+// CSR order.
 __device__ __forceinline__ double pull_out(int lane, int32_t beg, int32_t end,
                                            const int32_t *__restrict__ col,
                                            const unsigned long long *__restrict__ fmd1,
@@ -147,222 +560,75 @@ __device__ __forceinline__ double pull_out(int lane, int32_t beg, int32_t end,
   return acc;
 }
 
-// copy the shared candidate list to every group
-__global__ void mbc_cand_copy_kernel(int64_t n, int32_t G, int64_t stride,
-                                     int32_t *__restrict__ cand) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    for (int32_t g = 1; g < G; g++) cand[g * stride + i] = cand[i];
-}
-
-// sources of group g: slots [64g, 64g+64)
-__global__ void mbc_seed_kernel(int32_t V, int32_t nsrc, const int32_t *__restrict__ src,
-                                double *__restrict__ sigma, unsigned long long *__restrict__ fm0,
-                                unsigned long long *__restrict__ visited,
-                                int32_t *__restrict__ list0, unsigned long long *list0_count,
-                                bool with_sigma) {
-  const int32_t g = blockIdx.x;
-  const int lane = threadIdx.x;  // blockDim = 64
-  const int32_t s = g * 64 + lane;
-  const int64_t gV = (int64_t)g * V;
-  if (s < nsrc) {
-    const int32_t v = src[s];
-    atomicOr(&fm0[gV + v], 1ull << lane);
-    atomicOr(&visited[gV + v], 1ull << lane);
-    if (with_sigma) sigma[(gV + v) * 64 + lane] = 1.0;
-  }
-  __syncthreads();
-  // level-0 list: each distinct source vertex once (the lowest lane holding it)
-  if (s < nsrc) {
-    const int32_t v = src[s];
-    const uint64_t m = atomicOr(&fm0[gV + v], 0ull);  // read back from L2
-    if ((int)(__ffsll((unsigned long long)m) - 1) == lane) {
-      unsigned long long p = atomicAdd(&list0_count[g], 1ull);
-      list0[gV + p] = v;
-    }
-  }
-}
-
-template <bool SIGMA>
-__global__ __launch_bounds__(MB) void mbc_forward_kernel(
-    int32_t V, int32_t d, const int32_t *__restrict__ rrp, const int32_t *__restrict__ rcol,
-    const int32_t *__restrict__ cand_in, const unsigned long long *__restrict__ cand_in_count,
-    int32_t *__restrict__ cand_out, unsigned long long *cand_out_count,
-    const unsigned long long *__restrict__ fm_d, unsigned long long *__restrict__ fm_d1,
-    unsigned long long *__restrict__ visited, const unsigned long long *__restrict__ active,
-    double *__restrict__ sigma, int32_t *__restrict__ list_d1,
-    unsigned long long *list_d1_count, int32_t *__restrict__ level_out) {
+__global__ __launch_bounds__(MB) void mbc_bwd_kernel(BcCtx c) {
   const int32_t g = blockIdx.y;
-  const int64_t gV = (int64_t)g * V;
+  if (c.st->bdone) return;
+  const int bd = c.st->bd;
+  const int64_t gV = (int64_t)g * c.V;
   const int lane = gb_lane();
-  const uint64_t n = cand_in_count[g];
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const uint64_t act = active[g];
-  const unsigned long long *fmd = fm_d + gV;
-  // the level list and the candidate list are appended to by every wave:
-  // stage the appends per wave in LDS
-  __shared__ int32_t s_list[MB / GB_WAVE][MBQ], s_cand[MB / GB_WAVE][MBQ];
-  gb_wave_queue<int32_t, MBQ> lq(s_list[threadIdx.x / GB_WAVE], list_d1 + gV, &list_d1_count[g]);
-  gb_wave_queue<int32_t, MBQ> cq(s_cand[threadIdx.x / GB_WAVE], cand_out + gV,
-                                 &cand_out_count[g]);
-  for (uint64_t i = wave; i < n; i += nwaves) {
-    const int32_t x = cand_in[gV + i];
-    const uint64_t need = ~visited[gV + x] & act;
-    if (need == 0) continue;  // dropped: every source has reached x
-    double acc = 0.0;
-    uint64_t any = 0;
-    pull_in<SIGMA>(lane, rrp[x], rrp[x + 1], rcol, fmd, need, sigma, gV, acc, any);
-    const uint64_t found = SIGMA ? (__ballot(acc > 0.0) & need) : (any & need);
-    if (found) {
-      if ((found >> lane) & 1ull) {
-        if (SIGMA) sigma[(gV + x) * 64 + lane] = acc;
-        if (level_out) level_out[((int64_t)g * 64 + lane) * V + x] = d + 1;
-      }
-      if (lane == 0) {
-        fm_d1[gV + x] = found;
-        visited[gV + x] |= found;
-      }
-    }
-    lq.push(lane == 0 && found ? 1u : 0u, [&](uint32_t) { return x; });
-    // still unreached by some source: a candidate again at the next level
-    cq.push(lane == 0 && (need & ~found) ? 1u : 0u, [&](uint32_t) { return x; });
-  }
-  lq.flush();
-  cq.flush();
-}
-
-__global__ __launch_bounds__(MB) void mbc_backward_kernel(
-    int32_t V, int32_t hub, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
-    const int32_t *__restrict__ list_d, const unsigned long long *__restrict__ list_d_count,
-    const unsigned long long *__restrict__ fm_d, const unsigned long long *__restrict__ fm_d1,
-    const double *__restrict__ sigma, double *__restrict__ delta) {
-  const int32_t g = blockIdx.y;
-  const int64_t gV = (int64_t)g * V;
-  const int lane = gb_lane();
-  const uint64_t n = list_d_count[g];
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = wave; i < n; i += nwaves) {
-    const int32_t v = list_d[gV + i];
-    const int32_t beg = rp[v], end = rp[v + 1];
-    if (end - beg > hub) continue;  // split over several waves by the hub kernels
-    const uint64_t mine = fm_d[gV + v];
+  const int64_t *lo = c.loff + (int64_t)g * c.lvl_cap;
+  const int64_t base = (int64_t)g * c.pool_cap + lo[bd];
+  const uint64_t n = (uint64_t)(lo[bd + 1] - lo[bd]);
+  const unsigned long long *fm1 = c.fm[(bd + 1) & 1] + gV;
+  const uint64_t nwaves = wave_count();
+  for (uint64_t i = wave_id(); i < n; i += nwaves) {
+    const int32_t v = c.pool_v[base + i];
+    const int32_t beg = c.rp[v], end = c.rp[v + 1];
+    if (end - beg > c.bch) continue;  // split over several waves by the hub kernels
+    const uint64_t mine = c.pool_m[base + i];
     const bool me = (mine >> lane) & 1ull;
-    const double sv = me ? sigma[(gV + v) * 64 + lane] : 0.0;
-    const double acc = pull_out(lane, beg, end, col, fm_d1 + gV, mine, sv, sigma, delta, gV);
-    if (me) delta[(gV + v) * 64 + lane] = acc;
+    const double sv = me ? c.sigma[(gV + v) * 64 + lane] : 0.0;
+    const double acc = pull_out(lane, beg, end, c.col, fm1, mine, sv, c.sigma, c.delta, gV);
+    if (me) c.delta[(gV + v) * 64 + lane] = acc;
   }
 }
 
-// Forward step for in-degree hubs: one wave per (hub, chunk of `hub` edges).
-// Chunk sums of sigma are integers, so atomicAdd keeps them exact.
-template <bool SIGMA>
-__global__ __launch_bounds__(MB) void mbc_forward_hub_kernel(
-    int32_t V, int32_t hub, const int32_t *__restrict__ rrp, const int32_t *__restrict__ rcol,
-    const int32_t *__restrict__ hubs, int32_t nhubs, int32_t maxch,
-    const unsigned long long *__restrict__ fm_d, unsigned long long *__restrict__ fm_d1,
-    const unsigned long long *__restrict__ visited, const unsigned long long *__restrict__ active,
-    double *__restrict__ sigma, int32_t *__restrict__ list_d1, unsigned long long *list_d1_count) {
+// Backward for out-degree hubs: per-chunk partials ...
+__global__ __launch_bounds__(MB) void mbc_bwd_hub_kernel(BcCtx c) {
   const int32_t g = blockIdx.y;
-  const int64_t gV = (int64_t)g * V;
+  if (c.st->bdone || c.nhub_out == 0) return;
+  const int bd = c.st->bd;
+  const int64_t gV = (int64_t)g * c.V;
   const int lane = gb_lane();
-  const uint64_t items = (uint64_t)nhubs * maxch;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const uint64_t act = active[g];
-  for (uint64_t it = wave; it < items; it += nwaves) {
-    const int32_t x = hubs[it / maxch];
-    const int32_t k = (int32_t)(it % maxch);
-    const int32_t beg = rrp[x] + k * hub;
-    const int32_t end = min(rrp[x + 1], beg + hub);
+  const uint64_t items = (uint64_t)c.nhub_out * c.maxch_out;
+  const unsigned long long *fm0 = c.fm[bd & 1] + gV;
+  const unsigned long long *fm1 = c.fm[(bd + 1) & 1] + gV;
+  const uint64_t nwaves = wave_count();
+  for (uint64_t it = wave_id(); it < items; it += nwaves) {
+    const int32_t x = c.hub_out[it / c.maxch_out];
+    const int32_t k = (int32_t)(it % c.maxch_out);
+    const int32_t beg = c.rp[x] + k * c.bch;
+    const int32_t end = min(c.rp[x + 1], beg + c.bch);
     if (beg >= end) continue;
-    const uint64_t need = ~visited[gV + x] & act;
-    if (need == 0) continue;
-    double acc = 0.0;
-    uint64_t any = 0;
-    pull_in<SIGMA>(lane, beg, end, rcol, fm_d + gV, need, sigma, gV, acc, any);
-    const uint64_t found = SIGMA ? (__ballot(acc > 0.0) & need) : (any & need);
-    if (found) {
-      if (SIGMA && ((found >> lane) & 1ull)) atomicAdd(&sigma[(gV + x) * 64 + lane], acc);
-      if (lane == 0) {
-        const unsigned long long old = atomicOr(&fm_d1[gV + x], found);
-        if (old == 0) {  // the first chunk to reach x enters it in level d+1
-          const unsigned long long p = atomicAdd(&list_d1_count[g], 1ull);
-          list_d1[gV + p] = x;
-        }
-      }
-    }
-  }
-}
-
-// After the hub chunks of a level: record the hubs' new sources as visited
-// and, for BFS, write their levels.
-__global__ void mbc_hub_finalize_kernel(int32_t V, int32_t d, const int32_t *__restrict__ hubs,
-                                        int32_t nhubs,
-                                        const unsigned long long *__restrict__ fm_d1,
-                                        unsigned long long *__restrict__ visited,
-                                        int32_t *__restrict__ level_out) {
-  const int32_t g = blockIdx.y;
-  const int64_t gV = (int64_t)g * V;
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nhubs) return;
-  const int32_t x = hubs[i];
-  const uint64_t f = fm_d1[gV + x];
-  if (!f) return;
-  visited[gV + x] |= f;
-  if (level_out)
-    for (int s = 0; s < 64; s++)
-      if ((f >> s) & 1ull) level_out[((int64_t)g * 64 + s) * V + x] = d + 1;
-}
-
-// Backward step for out-degree hubs: per-chunk partials ...
-__global__ __launch_bounds__(MB) void mbc_backward_hub_kernel(
-    int32_t V, int32_t hub, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
-    const int32_t *__restrict__ hubs, int32_t nhubs, int32_t maxch,
-    const unsigned long long *__restrict__ fm_d, const unsigned long long *__restrict__ fm_d1,
-    const double *__restrict__ sigma, const double *__restrict__ delta,
-    double *__restrict__ part) {
-  const int32_t g = blockIdx.y;
-  const int64_t gV = (int64_t)g * V;
-  const int lane = gb_lane();
-  const uint64_t items = (uint64_t)nhubs * maxch;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t it = wave; it < items; it += nwaves) {
-    const int32_t x = hubs[it / maxch];
-    const int32_t k = (int32_t)(it % maxch);
-    const int32_t beg = rp[x] + k * hub;
-    const int32_t end = min(rp[x + 1], beg + hub);
-    if (beg >= end) continue;
-    const uint64_t mine = fm_d[gV + x];
+    const uint64_t mine = fm0[x];
     if (mine == 0) continue;
     const bool me = (mine >> lane) & 1ull;
-    const double sv = me ? sigma[(gV + x) * 64 + lane] : 0.0;
-    const double acc = pull_out(lane, beg, end, col, fm_d1 + gV, mine, sv, sigma, delta, gV);
-    part[((int64_t)g * items + it) * 64 + lane] = acc;
+    const double sv = me ? c.sigma[(gV + x) * 64 + lane] : 0.0;
+    const double acc = pull_out(lane, beg, end, c.col, fm1, mine, sv, c.sigma, c.delta, gV);
+    c.part[((int64_t)g * items + it) * 64 + lane] = acc;
   }
 }
 
 // ... summed in chunk order (deterministic), one wave per hub.
-__global__ __launch_bounds__(MB) void mbc_backward_combine_kernel(
-    int32_t V, int32_t hub, const int32_t *__restrict__ rp, const int32_t *__restrict__ hubs,
-    int32_t nhubs, int32_t maxch, const unsigned long long *__restrict__ fm_d,
-    const double *__restrict__ part, double *__restrict__ delta) {
+__global__ __launch_bounds__(MB) void mbc_bwd_combine_kernel(BcCtx c) {
   const int32_t g = blockIdx.y;
-  const int64_t gV = (int64_t)g * V;
+  if (c.st->bdone || c.nhub_out == 0) return;
+  const int bd = c.st->bd;
+  const int64_t gV = (int64_t)g * c.V;
   const int lane = gb_lane();
-  const uint64_t items = (uint64_t)nhubs * maxch;
-  const int32_t i = (int32_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (i >= nhubs) return;
-  const int32_t x = hubs[i];
-  const uint64_t mine = fm_d[gV + x];
-  if (!((mine >> lane) & 1ull)) return;
-  const int32_t nch = (rp[x + 1] - rp[x] + hub - 1) / hub;
-  double acc = 0.0;
-  for (int32_t k = 0; k < nch; k++)
-    acc += part[((int64_t)g * items + (int64_t)i * maxch + k) * 64 + lane];
-  delta[(gV + x) * 64 + lane] = acc;
+  const uint64_t items = (uint64_t)c.nhub_out * c.maxch_out;
+  const unsigned long long *fm0 = c.fm[bd & 1] + gV;
+  const uint64_t nwaves = wave_count();
+  for (uint64_t i = wave_id(); i < (uint64_t)c.nhub_out; i += nwaves) {
+    const int32_t x = c.hub_out[i];
+    const uint64_t mine = fm0[x];
+    if (!((mine >> lane) & 1ull)) continue;
+    const int32_t nch = (c.rp[x + 1] - c.rp[x] + c.bch - 1) / c.bch;
+    double acc = 0.0;
+    for (int32_t k = 0; k < nch; k++)
+      acc += c.part[((int64_t)g * items + (int64_t)i * c.maxch_out + k) * 64 + lane];
+    c.delta[(gV + x) * 64 + lane] = acc;
+  }
 }
 
 // bc[v] += delta[v][s] (v != source s) over groups and lanes in source order
@@ -382,20 +648,14 @@ __global__ void mbc_accum_kernel(int32_t V, int32_t G, int32_t nsrc,
   bc[v] = acc;
 }
 
-struct LevelBuf {
-  DevBuf fm, list;
-};
-
 struct MsWork {
-  DevBuf sigma, delta, visited, cand[2], counts, src, active, allcand, hub_in, hub_out, hub_meta,
-      part;
-  std::vector<LevelBuf *> levels;
+  DevBuf sigma, delta, visited, fm[2], pool_v, pool_m, loff, pq[2], cand, counts, mode, loop, src,
+      act, hub_in, hub_out, hub_meta, part;
   void release() {
-    for (DevBuf *b : {&sigma, &delta, &visited, &cand[0], &cand[1], &counts, &src, &active,
-                      &allcand, &hub_in, &hub_out, &hub_meta, &part})
+    for (DevBuf *b : {&sigma, &delta, &visited, &fm[0], &fm[1], &pool_v, &pool_m, &loff, &pq[0],
+                      &pq[1], &cand, &counts, &mode, &loop, &src, &act, &hub_in, &hub_out,
+                      &hub_meta, &part})
       b->release();
-    for (auto *l : levels) delete l;
-    levels.clear();
   }
 };
 MsWork &msw() {
@@ -418,204 +678,274 @@ int32_t bc_bchunk(int32_t hub) {
   return v >= 64 ? v : hub;
 }
 
-int wgrid(uint64_t n) {
-  uint64_t b = (n + 3) / 4;
-  uint64_t cap = (uint64_t)gb_num_cus() * 8;
-  if (b > cap) b = cap;
-  return b < 1 ? 1 : (int)b;
+// Grows a [G][old] array to [G][want] entries of `elem` bytes, keeping each
+// group's first `old` entries.
+gb_status grow_rows(DevBuf &buf, size_t elem, int32_t G, int64_t old, int64_t want,
+                    hipStream_t st) {
+  DevBuf nb;
+  GB_TRY(nb.ensure(elem * (size_t)G * (size_t)want));
+  GB_HIP(hipMemcpy2DAsync(nb.p, elem * want, buf.p, elem * old, elem * old, G,
+                          hipMemcpyDeviceToDevice, st));
+  GB_HIP(hipStreamSynchronize(st));
+  std::swap(buf.p, nb.p);
+  std::swap(buf.bytes, nb.bytes);
+  return GB_OK;  // nb frees the old buffer
 }
 
-// counts layout (u64): [0,G) cand_in, [G,2G) cand_out, [2G,3G) list
-enum { CNT_STRIDE = 3 };
+// One replayable chunk of LPR levels: forward or backward.
+struct LevelGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~LevelGraph() { reset(); }
+};
+
+template <class F>
+gb_status capture(LevelGraph &lg, hipStream_t cs, F launches) {
+  lg.reset();
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    launches();
+    e = hipStreamEndCapture(cs, &lg.graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&lg.exec, lg.graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return gb_fail(GB_ERR_HIP, "bc: graph capture failed: %s", hipGetErrorString(e));
+  return GB_OK;
+}
 
 // Runs one batch of up to 64*G sources.  with_sigma: Brandes (sigma, delta,
 // accumulate into d_bc); otherwise BFS levels into d_level [nsrc][V].
-gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma,
-                 double *d_bc, int32_t *d_level, hipStream_t st, int32_t *depth_out) {
+gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma, double *d_bc,
+                 int32_t *d_level, hipStream_t st, int32_t *depth_out) {
   MsWork &w = msw();
   const int32_t V = g->V;
   const int32_t G = (nsrc + 63) / 64;
   const int64_t GV = (int64_t)G * V;
-  const int32_t *rp = g->rp ? g->rp : g->rrp;
-  const int32_t *col = g->rp ? g->col : g->rcol;
-  const int32_t *rrp = g->rrp ? g->rrp : g->rp;
-  const int32_t *rcol = g->rrp ? g->rcol : g->col;
+  BcCtx c{};
+  c.V = V;
+  c.G = G;
+  c.E = g->E;
+  c.rp = g->rp ? g->rp : g->rrp;
+  c.col = g->rp ? g->col : g->rcol;
+  c.rrp = g->rrp ? g->rrp : g->rp;
+  c.rcol = g->rrp ? g->rcol : g->col;
+  c.keep = with_sigma ? 1 : 0;
+  c.hub = hub_edges();
+  c.bch = bc_bchunk(c.hub);
   if (with_sigma) {
     GB_TRY(w.sigma.ensure(sizeof(double) * GV * 64));
     GB_TRY(w.delta.ensure(sizeof(double) * GV * 64));
     GB_HIP(hipMemsetAsync(w.sigma.p, 0, sizeof(double) * GV * 64, st));
     GB_HIP(hipMemsetAsync(w.delta.p, 0, sizeof(double) * GV * 64, st));
+    c.sigma = w.sigma.as<double>();
+    c.delta = w.delta.as<double>();
   }
+  c.level_out = d_level;
   GB_TRY(w.visited.ensure(sizeof(uint64_t) * GV));
-  GB_TRY(w.cand[0].ensure(sizeof(int32_t) * GV));
-  GB_TRY(w.cand[1].ensure(sizeof(int32_t) * GV));
-  GB_TRY(w.counts.ensure(sizeof(unsigned long long) * CNT_STRIDE * G));
-  GB_TRY(w.src.ensure(sizeof(int32_t) * nsrc));
-  GB_TRY(w.active.ensure(sizeof(uint64_t) * G));
   GB_HIP(hipMemsetAsync(w.visited.p, 0, sizeof(uint64_t) * GV, st));
-  GB_HIP(hipMemcpyAsync(w.src.p, h_src, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
-  std::vector<uint64_t> act(G);
-  for (int32_t k = 0; k < G; k++) {
-    const int32_t n = std::min(64, nsrc - 64 * k);
-    act[k] = n == 64 ? ~0ull : ((1ull << n) - 1);
+  c.visited = w.visited.as<unsigned long long>();
+  for (int k = 0; k < 2; k++) {
+    GB_TRY(w.fm[k].ensure(sizeof(uint64_t) * GV));
+    GB_HIP(hipMemsetAsync(w.fm[k].p, 0, sizeof(uint64_t) * GV, st));
+    c.fm[k] = w.fm[k].as<unsigned long long>();
   }
-  GB_HIP(hipMemcpyAsync(w.active.p, act.data(), sizeof(uint64_t) * G, hipMemcpyHostToDevice, st));
-  auto level = [&](int d) -> gb_status {
-    while ((int)w.levels.size() <= d) w.levels.push_back(new LevelBuf());
-    LevelBuf *L = w.levels[d];
-    GB_TRY(L->fm.ensure(sizeof(uint64_t) * GV));
-    GB_TRY(L->list.ensure(sizeof(int32_t) * GV));
-    GB_HIP(hipMemsetAsync(L->fm.p, 0, sizeof(uint64_t) * GV, st));
-    return GB_OK;
-  };
-  unsigned long long *cnt = w.counts.as<unsigned long long>();
-  GB_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * CNT_STRIDE * G, st));
-  GB_TRY(level(0));
-  // vertices above `hub` edges are split into chunks of `hub` edges, one wave
+  // the pool: two levels (BFS) or every level, starting at 6 V entries per group
+  int64_t pool_cap = with_sigma ? 6 * (int64_t)V + 64 : 2 * (int64_t)V;
+  int32_t lvl_cap = with_sigma ? 64 : 2;
+  GB_TRY(w.pool_v.ensure(sizeof(int32_t) * (size_t)G * pool_cap));
+  GB_TRY(w.pool_m.ensure(sizeof(uint64_t) * (size_t)G * pool_cap));
+  GB_TRY(w.loff.ensure(sizeof(int64_t) * (size_t)G * lvl_cap));
+  GB_HIP(hipMemsetAsync(w.loff.p, 0, sizeof(int64_t) * (size_t)G * lvl_cap, st));
+  c.pq_cap = (int64_t)V + g->E / PUSH_CH + 1;
+  for (int k = 0; k < 2; k++) {
+    GB_TRY(w.pq[k].ensure(sizeof(PushItem) * (size_t)G * c.pq_cap));
+    c.pq[k] = w.pq[k].as<PushItem>();
+  }
+  GB_TRY(w.cand.ensure(sizeof(int32_t) * GV));
+  c.cand = w.cand.as<int32_t>();
+  GB_TRY(w.counts.ensure(sizeof(unsigned long long) * CN * G));
+  c.cnt = w.counts.as<unsigned long long>();
+  GB_TRY(w.mode.ensure(sizeof(int32_t) * G));
+  c.mode = w.mode.as<int32_t>();
+  GB_TRY(w.loop.ensure(sizeof(BcLoop)));
+  c.st = w.loop.as<BcLoop>();
+  GB_TRY(w.src.ensure(sizeof(int32_t) * nsrc));
+  GB_TRY(w.act.ensure(sizeof(uint64_t) * G));
+  c.act = w.act.as<unsigned long long>();
+  GB_HIP(hipMemcpyAsync(w.src.p, h_src, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
+  {
+    std::vector<uint64_t> act(G);
+    std::vector<unsigned long long> cnt0((size_t)CN * G, 0ull);
+    for (int32_t k = 0; k < G; k++) {
+      const int32_t n = std::min(64, nsrc - 64 * k);
+      act[k] = n == 64 ? ~0ull : ((1ull << n) - 1);
+      cnt0[(size_t)k * CN + C_MPULL] = (unsigned long long)g->E;  // every in-edge
+    }
+    GB_HIP(hipMemcpyAsync(w.act.p, act.data(), sizeof(uint64_t) * G, hipMemcpyHostToDevice, st));
+    GB_HIP(hipMemcpyAsync(c.cnt, cnt0.data(), sizeof(unsigned long long) * CN * G,
+                          hipMemcpyHostToDevice, st));
+    BcLoop l{};
+    l.d = -1;
+    GB_HIP(hipMemcpyAsync(c.st, &l, sizeof(l), hipMemcpyHostToDevice, st));
+    // the host copies read stack memory: wait before leaving the scope
+    GB_HIP(hipStreamSynchronize(st));
+  }
+  // vertices above `hub` edges are pulled in chunks of `hub` edges, one wave
   // each (in-degree for the forward pulls, out-degree for the backward ones)
-  const int32_t hub = hub_edges();
   const int vgrid = std::max(1, std::min((V + 255) / 256, 2048));
   GB_TRY(w.hub_in.ensure(sizeof(int32_t) * (V + 1)));
   GB_TRY(w.hub_out.ensure(sizeof(int32_t) * (V + 1)));
   GB_TRY(w.hub_meta.ensure(sizeof(unsigned long long) * 4));
   GB_HIP(hipMemsetAsync(w.hub_meta.p, 0, sizeof(unsigned long long) * 4, st));
   unsigned long long *hm = w.hub_meta.as<unsigned long long>();
-  hipLaunchKernelGGL(mbc_hub_list_kernel, dim3(vgrid), dim3(256), 0, st, V, rrp, hub,
+  hipLaunchKernelGGL(mbc_hub_list_kernel, dim3(vgrid), dim3(256), 0, st, V, c.rrp, c.hub,
                      w.hub_in.as<int32_t>(), hm + 0, (unsigned int *)(hm + 1));
-  hipLaunchKernelGGL(mbc_hub_list_kernel, dim3(vgrid), dim3(256), 0, st, V, rp, hub,
+  hipLaunchKernelGGL(mbc_hub_list_kernel, dim3(vgrid), dim3(256), 0, st, V, c.rp, c.bch,
                      w.hub_out.as<int32_t>(), hm + 2, (unsigned int *)(hm + 3));
   unsigned long long hmh[4];
   GB_HIP(hipMemcpyAsync(hmh, hm, sizeof(hmh), hipMemcpyDeviceToHost, st));
   GB_HIP(hipStreamSynchronize(st));
-  const int32_t nhub_in = (int32_t)hmh[0], nhub_out = (int32_t)hmh[2];
-  const int32_t maxch_in = ((int32_t)(uint32_t)hmh[1] + hub - 1) / hub;
-  const int32_t bch = bc_bchunk(hub);
-  const int32_t maxch_out = ((int32_t)(uint32_t)hmh[3] + bch - 1) / bch;
-  if (with_sigma && nhub_out > 0)
-    GB_TRY(w.part.ensure(sizeof(double) * 64 * (size_t)G * nhub_out * maxch_out));
-  // candidates: vertices with 1..hub in-edges, the same list for every group
-  hipLaunchKernelGGL(mbc_cand_init_kernel, dim3(vgrid), dim3(256), 0, st, V, rrp, hub,
-                     w.cand[0].as<int32_t>(), cnt);
-  unsigned long long nc = 0;
-  GB_HIP(hipMemcpyAsync(&nc, cnt, sizeof(nc), hipMemcpyDeviceToHost, st));
-  GB_HIP(hipStreamSynchronize(st));
-  if (G > 1 && nc > 0) {
-    hipLaunchKernelGGL(mbc_cand_copy_kernel, dim3(std::max<int64_t>(1, std::min<int64_t>((nc + 255) / 256, 4096))),
-                       dim3(256), 0, st, (int64_t)nc, G, (int64_t)V, w.cand[0].as<int32_t>());
-    std::vector<unsigned long long> c0(G, nc);
-    GB_HIP(hipMemcpyAsync(cnt, c0.data(), sizeof(unsigned long long) * G, hipMemcpyHostToDevice,
-                          st));
+  c.hub_in = w.hub_in.as<int32_t>();
+  c.nhub_in = (int32_t)hmh[0];
+  c.maxch_in = std::max<int32_t>(1, ((int32_t)(uint32_t)hmh[1] + c.hub - 1) / c.hub);
+  c.hub_out = w.hub_out.as<int32_t>();
+  c.nhub_out = with_sigma ? (int32_t)hmh[2] : 0;
+  c.maxch_out = std::max<int32_t>(1, ((int32_t)(uint32_t)hmh[3] + c.bch - 1) / c.bch);
+  if (with_sigma && c.nhub_out > 0) {
+    GB_TRY(w.part.ensure(sizeof(double) * 64 * (size_t)G * c.nhub_out * c.maxch_out));
+    c.part = w.part.as<double>();
   }
-  hipLaunchKernelGGL(mbc_seed_kernel, dim3(G), dim3(64), 0, st, V, nsrc, w.src.as<int32_t>(),
-                     with_sigma ? w.sigma.as<double>() : nullptr,
-                     w.levels[0]->fm.as<unsigned long long>(), w.visited.as<unsigned long long>(),
-                     w.levels[0]->list.as<int32_t>(), cnt + 2 * G, with_sigma);
+  auto set_pool = [&]() {
+    c.pool_v = w.pool_v.as<int32_t>();
+    c.pool_m = w.pool_m.as<unsigned long long>();
+    c.pool_cap = pool_cap;
+    c.loff = w.loff.as<int64_t>();
+    c.lvl_cap = lvl_cap;
+  };
+  set_pool();
+  // level 0 and its finalize (as the end of level -1)
+  const int gx = std::max(16, std::min(gb_num_cus() * 8, 32768 / G));
+  const dim3 grid(gx, G), block(MB);
+  hipLaunchKernelGGL(mbc_seed_kernel, dim3(G), dim3(64), 0, st, c, w.src.as<int32_t>(), nsrc);
+  hipLaunchKernelGGL(mbc_fin_kernel, grid, block, 0, st, c);
   GB_HIP(hipGetLastError());
-  if (d_level) {
-    // level 0 of each source: handled by a host-side fill below (sources are known)
+  hipStream_t cs = st, own = nullptr;
+  if (!st) {  // the null stream cannot be captured
+    GB_HIP(hipStreamCreate(&own));
+    cs = own;
   }
-  std::vector<std::vector<unsigned long long>> list_n;  // [level][group]
-  std::vector<unsigned long long> h(CNT_STRIDE * G);
-  GB_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(unsigned long long) * CNT_STRIDE * G,
-                        hipMemcpyDeviceToHost, st));
-  GB_HIP(hipStreamSynchronize(st));
-  list_n.push_back(std::vector<unsigned long long>(h.begin() + 2 * G, h.begin() + 3 * G));
-  int cur = 0;
-  int d = 0;
-  for (;;) {
-    uint64_t maxc = 0;
-    for (int32_t k = 0; k < G; k++) maxc = std::max<uint64_t>(maxc, h[k]);
-    if (maxc == 0 && nhub_in == 0) break;
-    GB_TRY(level(d + 1));
-    // zero cand_out and list counts for the next level
-    GB_HIP(hipMemsetAsync(cnt + G, 0, sizeof(unsigned long long) * 2 * G, st));
-    dim3 grid(wgrid(std::max<uint64_t>(maxc, 1)), G);
-    if (with_sigma)
-      hipLaunchKernelGGL(mbc_forward_kernel<true>, grid, dim3(MB), 0, st, V, d, rrp, rcol,
-                         w.cand[cur].as<int32_t>(), cnt, w.cand[cur ^ 1].as<int32_t>(), cnt + G,
-                         w.levels[d]->fm.as<unsigned long long>(),
-                         w.levels[d + 1]->fm.as<unsigned long long>(),
-                         w.visited.as<unsigned long long>(), w.active.as<unsigned long long>(),
-                         w.sigma.as<double>(), w.levels[d + 1]->list.as<int32_t>(), cnt + 2 * G,
-                         (int32_t *)nullptr);
-    else
-      hipLaunchKernelGGL(mbc_forward_kernel<false>, grid, dim3(MB), 0, st, V, d, rrp, rcol,
-                         w.cand[cur].as<int32_t>(), cnt, w.cand[cur ^ 1].as<int32_t>(), cnt + G,
-                         w.levels[d]->fm.as<unsigned long long>(),
-                         w.levels[d + 1]->fm.as<unsigned long long>(),
-                         w.visited.as<unsigned long long>(), w.active.as<unsigned long long>(),
-                         (double *)nullptr, w.levels[d + 1]->list.as<int32_t>(), cnt + 2 * G,
-                         d_level);
-    GB_HIP(hipGetLastError());
-    if (nhub_in > 0) {
-      dim3 hgrid(wgrid((uint64_t)nhub_in * maxch_in), G);
-      if (with_sigma)
-        hipLaunchKernelGGL(mbc_forward_hub_kernel<true>, hgrid, dim3(MB), 0, st, V, hub, rrp, rcol,
-                           w.hub_in.as<int32_t>(), nhub_in, maxch_in,
-                           w.levels[d]->fm.as<unsigned long long>(),
-                           w.levels[d + 1]->fm.as<unsigned long long>(),
-                           w.visited.as<unsigned long long>(), w.active.as<unsigned long long>(),
-                           w.sigma.as<double>(), w.levels[d + 1]->list.as<int32_t>(), cnt + 2 * G);
-      else
-        hipLaunchKernelGGL(mbc_forward_hub_kernel<false>, hgrid, dim3(MB), 0, st, V, hub, rrp,
-                           rcol, w.hub_in.as<int32_t>(), nhub_in, maxch_in,
-                           w.levels[d]->fm.as<unsigned long long>(),
-                           w.levels[d + 1]->fm.as<unsigned long long>(),
-                           w.visited.as<unsigned long long>(), w.active.as<unsigned long long>(),
-                           (double *)nullptr, w.levels[d + 1]->list.as<int32_t>(), cnt + 2 * G);
-      hipLaunchKernelGGL(mbc_hub_finalize_kernel, dim3((nhub_in + 255) / 256, G), dim3(256), 0, st,
-                         V, d, w.hub_in.as<int32_t>(), nhub_in,
-                         w.levels[d + 1]->fm.as<unsigned long long>(),
-                         w.visited.as<unsigned long long>(), d_level);
-      GB_HIP(hipGetLastError());
+  struct OwnStream {
+    hipStream_t s;
+    ~OwnStream() {
+      if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
     }
-    GB_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(unsigned long long) * CNT_STRIDE * G,
-                          hipMemcpyDeviceToHost, st));
-    GB_HIP(hipStreamSynchronize(st));
-    std::vector<unsigned long long> ln(h.begin() + 2 * G, h.begin() + 3 * G);
-    uint64_t tot = 0;
-    for (auto x : ln) tot += x;
-    if (tot == 0) break;
-    list_n.push_back(ln);
-    // the next level reads the surviving candidates
-    GB_HIP(hipMemcpyAsync(cnt, cnt + G, sizeof(unsigned long long) * G, hipMemcpyDeviceToDevice,
-                          st));
-    for (int32_t k = 0; k < G; k++) h[k] = h[G + k];
-    cur ^= 1;
-    d++;
+  } own_guard{own};
+  if (own) {
+    hipEvent_t ready;
+    GB_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    GB_HIP(hipEventRecord(ready, st));
+    GB_HIP(hipStreamWaitEvent(cs, ready, 0));
+    (void)hipEventDestroy(ready);
   }
-  const int depth = (int)list_n.size();  // levels 0..depth-1 are non-empty
+  auto fwd_launches = [&]() {
+    for (int k = 0; k < LPR; k++) {
+      hipLaunchKernelGGL(mbc_fwd_ctl_kernel, dim3(1), dim3(64), 0, cs, c);
+      hipLaunchKernelGGL(mbc_scan_kernel, grid, block, 0, cs, c);
+      if (with_sigma) {
+        hipLaunchKernelGGL(mbc_push_kernel<true>, grid, block, 0, cs, c);
+        hipLaunchKernelGGL(mbc_pull_kernel<true>, grid, block, 0, cs, c);
+        hipLaunchKernelGGL(mbc_pull_hub_kernel<true>, grid, block, 0, cs, c);
+      } else {
+        hipLaunchKernelGGL(mbc_push_kernel<false>, grid, block, 0, cs, c);
+        hipLaunchKernelGGL(mbc_pull_kernel<false>, grid, block, 0, cs, c);
+        hipLaunchKernelGGL(mbc_pull_hub_kernel<false>, grid, block, 0, cs, c);
+      }
+      hipLaunchKernelGGL(mbc_fin_kernel, grid, block, 0, cs, c);
+    }
+  };
+  LevelGraph fg;
+  GB_TRY(capture(fg, cs, fwd_launches));
+  BcLoop hl{};
+  int32_t last_d = -2;
+  for (;;) {
+    GB_HIP(hipGraphLaunch(fg.exec, cs));
+    GB_HIP(hipMemcpyAsync(&hl, c.st, sizeof(hl), hipMemcpyDeviceToHost, cs));
+    GB_HIP(hipStreamSynchronize(cs));
+    if (hl.done) break;
+    if (hl.stall) {
+      // room for one more level of V entries per group, and for the level table
+      int64_t used = 0;
+      std::vector<int64_t> lo((size_t)G * lvl_cap);
+      std::vector<unsigned long long> cn((size_t)CN * G);
+      GB_HIP(hipMemcpy(lo.data(), c.loff, sizeof(int64_t) * lo.size(), hipMemcpyDeviceToHost));
+      GB_HIP(hipMemcpy(cn.data(), c.cnt, sizeof(unsigned long long) * cn.size(),
+                       hipMemcpyDeviceToHost));
+      const int dn = hl.d + 1;
+      for (int32_t k = 0; k < G; k++)
+        used = std::max<int64_t>(used, lo[(size_t)k * lvl_cap + dn] +
+                                           (int64_t)cn[(size_t)k * CN + C_NEXT]);
+      if (used + V > pool_cap) {
+        const int64_t ncap = std::max<int64_t>(2 * pool_cap, used + 2 * (int64_t)V);
+        GB_TRY(grow_rows(w.pool_v, sizeof(int32_t), G, pool_cap, ncap, cs));
+        GB_TRY(grow_rows(w.pool_m, sizeof(uint64_t), G, pool_cap, ncap, cs));
+        pool_cap = ncap;
+      }
+      if (dn + 1 >= lvl_cap) {
+        GB_TRY(grow_rows(w.loff, sizeof(int64_t), G, lvl_cap, 2 * lvl_cap, cs));
+        lvl_cap *= 2;
+      }
+      set_pool();
+      hl.stall = 0;
+      GB_HIP(hipMemcpyAsync(c.st, &hl, sizeof(hl), hipMemcpyHostToDevice, cs));
+      GB_HIP(hipStreamSynchronize(cs));
+      GB_TRY(capture(fg, cs, fwd_launches));
+      continue;
+    }
+    if (hl.d == last_d) return gb_fail(GB_ERR_HIP, "bc: level loop made no progress");
+    last_d = hl.d;
+  }
+  const int depth = hl.depth;  // levels 0..depth-1 are non-empty
   if (depth_out) *depth_out = depth;
   if (!with_sigma) return GB_OK;
-  // backward: levels depth-2 .. 0 (the deepest level has delta 0)
-  for (int dd = depth - 2; dd >= 0; dd--) {
-    uint64_t maxn = 0;
-    for (auto x : list_n[dd]) maxn = std::max<uint64_t>(maxn, x);
-    if (maxn == 0) continue;
-    GB_HIP(hipMemcpyAsync(cnt + 2 * G, list_n[dd].data(), sizeof(unsigned long long) * G,
-                          hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(mbc_backward_kernel, dim3(wgrid(maxn), G), dim3(MB), 0, st, V, hub, rp,
-                       col, w.levels[dd]->list.as<int32_t>(), cnt + 2 * G,
-                       w.levels[dd]->fm.as<unsigned long long>(),
-                       w.levels[dd + 1]->fm.as<unsigned long long>(), w.sigma.as<double>(),
-                       w.delta.as<double>());
+  if (depth >= 2) {
+    // the deepest level's masks (the forward pass left both arrays zero),
+    // then levels depth-2 .. 0: clear level bd + 2, write level bd, process
+    hl.bd = depth - 1;
+    hl.bdone = 0;
+    GB_HIP(hipMemcpyAsync(c.st, &hl, sizeof(hl), hipMemcpyHostToDevice, cs));
+    hipLaunchKernelGGL(mbc_bwd_masks_kernel<true>, grid, block, 0, cs, c, 0);
     GB_HIP(hipGetLastError());
-    if (nhub_out > 0) {
-      hipLaunchKernelGGL(mbc_backward_hub_kernel, dim3(wgrid((uint64_t)nhub_out * maxch_out), G),
-                         dim3(MB), 0, st, V, bch, rp, col, w.hub_out.as<int32_t>(), nhub_out,
-                         maxch_out, w.levels[dd]->fm.as<unsigned long long>(),
-                         w.levels[dd + 1]->fm.as<unsigned long long>(), w.sigma.as<double>(),
-                         w.delta.as<double>(), w.part.as<double>());
-      hipLaunchKernelGGL(mbc_backward_combine_kernel, dim3((nhub_out + 3) / 4, G), dim3(MB), 0, st,
-                         V, bch, rp, w.hub_out.as<int32_t>(), nhub_out, maxch_out,
-                         w.levels[dd]->fm.as<unsigned long long>(), w.part.as<double>(),
-                         w.delta.as<double>());
-      GB_HIP(hipGetLastError());
+    GB_HIP(hipStreamSynchronize(cs));
+    LevelGraph bg;
+    GB_TRY(capture(bg, cs, [&]() {
+      for (int k = 0; k < LPR; k++) {
+        hipLaunchKernelGGL(mbc_bwd_ctl_kernel, dim3(1), dim3(1), 0, cs, c);
+        hipLaunchKernelGGL(mbc_bwd_masks_kernel<false>, grid, block, 0, cs, c, 2);
+        hipLaunchKernelGGL(mbc_bwd_masks_kernel<true>, grid, block, 0, cs, c, 0);
+        hipLaunchKernelGGL(mbc_bwd_kernel, grid, block, 0, cs, c);
+        hipLaunchKernelGGL(mbc_bwd_hub_kernel, grid, block, 0, cs, c);
+        hipLaunchKernelGGL(mbc_bwd_combine_kernel, grid, block, 0, cs, c);
+      }
+    }));
+    for (;;) {
+      GB_HIP(hipGraphLaunch(bg.exec, cs));
+      GB_HIP(hipMemcpyAsync(&hl, c.st, sizeof(hl), hipMemcpyDeviceToHost, cs));
+      GB_HIP(hipStreamSynchronize(cs));
+      if (hl.bdone) break;
     }
   }
-  hipLaunchKernelGGL(mbc_accum_kernel, dim3((V + 255) / 256), dim3(256), 0, st, V, G, nsrc,
-                     w.src.as<int32_t>(), w.delta.as<double>(), d_bc);
+  hipLaunchKernelGGL(mbc_accum_kernel, dim3((V + 255) / 256), dim3(256), 0, cs, V, G, nsrc,
+                     w.src.as<int32_t>(), c.delta, d_bc);
   GB_HIP(hipGetLastError());
+  if (own) GB_HIP(hipStreamSynchronize(cs));
   return GB_OK;
 }
 
@@ -623,11 +953,6 @@ __global__ void fill_i32_kernel(int64_t n, int32_t *__restrict__ a, int32_t v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     a[i] = v;
-}
-__global__ void set_sources_level_kernel(int32_t V, int32_t nsrc, const int32_t *__restrict__ src,
-                                         int32_t *__restrict__ level) {
-  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nsrc) level[(int64_t)s * V + src[s]] = 0;
 }
 
 }  // namespace
@@ -652,9 +977,6 @@ extern "C" gb_status gb_bfs(gb_graph g, const int32_t *h_sources, int32_t nsrc, 
   for (int32_t b0 = 0; b0 < nsrc; b0 += per) {
     const int32_t ns = std::min(per, nsrc - b0);
     GB_TRY(ms_run(g, h_sources + b0, ns, false, nullptr, d_level + (int64_t)b0 * V, st, nullptr));
-    hipLaunchKernelGGL(set_sources_level_kernel, dim3((ns + 255) / 256), dim3(256), 0, st, V, ns,
-                       msw().src.as<int32_t>(), d_level + (int64_t)b0 * V);
-    GB_HIP(hipGetLastError());
   }
   GB_HIP(hipStreamSynchronize(st));
   return GB_OK;
@@ -679,7 +1001,7 @@ extern "C" gb_status gb_bc(gb_graph g, const int32_t *h_sources, int32_t nsrc, i
   const int64_t cap_entries = (int64_t)3 << 28;  // sigma+delta entries budget (~12 GB)
   if (V > 0 && per * (int64_t)V > cap_entries) per = std::max<int64_t>(64, (cap_entries / V) / 64 * 64);
   int32_t max_depth = 0, batches = 0;
-  for (int32_t b0 = 0; b0 < nsrc; b0 += (int32_t)per) {
+  for (int32_t b0 = 0; b0 < nsrc && V > 0; b0 += (int32_t)per) {
     const int32_t ns = (int32_t)std::min<int64_t>(per, nsrc - b0);
     int32_t depth = 0;
     GB_TRY(ms_run(g, h_sources + b0, ns, true, d_bc, nullptr, st, &depth));
