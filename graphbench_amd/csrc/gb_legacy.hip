@@ -2,10 +2,12 @@
 //
 // run_sspath_arr / run_sspath / run_pagerank_csr_mpi / run_bc and the
 // readers keep the reference's signatures, t_csr layout, file formats,
-// printed output and exit(123) error handling (include/gb_legacy.h).  Each
-// converts t_csr (int64 end offsets, AoS vertex/edge info) into the device
-// layout once, calls the native gb_* path and writes results back into the
-// caller's t_csr.
+// printed output and exit(123) error handling (include/gb_legacy.h).  The
+// t_csr entry points convert t_csr (int64 end offsets, AoS vertex/edge info)
+// into the device layout once, call the native gb_* path and write results
+// back into the caller's t_csr; the file entry points (run_pagerank_csr_mpi,
+// run_bc) parse the file into an edge stream and build their CSRs on the
+// device.
 #include <hip/hip_runtime.h>
 
 #include <cctype>
@@ -69,11 +71,58 @@ void *amalloc(size_t n) {  // _mm_malloc(n, 64) equivalent
   return p;
 }
 
-// Whitespace token reader matching the reference's fscanf("%s") loop.
+// Buffered reader with the reference's fscanf semantics (csr.c:26-35,
+// 156-176): next() is "%99s" (a whitespace-delimited token, split after 99
+// characters), num() is "%lld" (optional sign and decimal digits, stopping
+// at the first other character, which is left for the next read).  One
+// 1 MiB read at a time instead of a libc call per field.
 struct Tok {
   FILE *f;
-  bool next(char *buf, size_t n) { return fscanf(f, "%99s", buf) == 1; }
-  bool num(long long *v) { return fscanf(f, "%lld", v) == 1; }
+  std::vector<char> buf = std::vector<char>(1 << 20);
+  size_t pos = 0, len = 0;
+  int peek() {
+    if (pos == len) {
+      len = fread(buf.data(), 1, buf.size(), f);
+      pos = 0;
+      if (len == 0) return EOF;
+    }
+    return (unsigned char)buf[pos];
+  }
+  void skip_space() {
+    int c;
+    while ((c = peek()) != EOF && isspace(c)) pos++;
+  }
+  bool next(char *out, size_t n) {
+    skip_space();
+    size_t k = 0;
+    int c;
+    while (k < 99 && k + 1 < n && (c = peek()) != EOF && !isspace(c)) {
+      out[k++] = (char)c;
+      pos++;
+    }
+    out[k] = '\0';
+    return k > 0;
+  }
+  bool num(long long *v) {
+    skip_space();
+    int c = peek();
+    bool neg = false;
+    if (c == '-' || c == '+') {
+      neg = c == '-';
+      pos++;
+      c = peek();
+    }
+    if (c == EOF || !isdigit(c)) return false;
+    unsigned long long x = 0;
+    int digits = 0;
+    while ((c = peek()) != EOF && isdigit(c)) {
+      if (++digits > 18) fatal("integer field out of range");
+      x = x * 10 + (unsigned)(c - '0');
+      pos++;
+    }
+    *v = neg ? -(long long)x : (long long)x;
+    return true;
+  }
 };
 
 FILE *open_in(const char *dir, const char *file, const char *mode) {
@@ -219,15 +268,180 @@ void add_list_one(p_list list, lint val) {  // list.c:11-26 (prepend)
 }
 
 void pr_print_hook(int32_t iter, const double *d_wb, void *ctx) {
-  const t_csr *gr = (const t_csr *)ctx;
-  const int n = gr->v_size < 10 ? (int)gr->v_size : 10;
+  const std::vector<double> &recip = *(const std::vector<double> *)ctx;
+  const int n = recip.size() < 10 ? (int)recip.size() : 10;
   double wb[10];
   if (hipMemcpy(wb, d_wb, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
     fatal("download");
   printf("ITER %d ", iter);  // pagerank_mpi.c:823-830
-  for (int i = 0; i < n; i++) printf("%f|%f ", wb[i], gr->vet_info[i].recip);
+  for (int i = 0; i < n; i++) printf("%f|%f ", wb[i], recip[i]);
   printf("\n");
   fflush(stdout);
+}
+
+// ---- graph files straight to a device CSR ------------------------------------
+// run_pagerank_csr_mpi and run_bc read a graph file the way scan_csr_idx +
+// read_graph_csr (text) or read_csr_bin (binary) do, but keep only the edge
+// stream: the host parses the file once into per-edge (row, column) int32
+// pairs in file order (text) or uploads the file's int64 pairs as they are
+// (binary), and the CSR is built on the device by gb_csr_from_edges.  Its
+// stable sort keeps each row's edges in file order, which is the order the
+// reference's scatter (csr.c:45-56, 271-291) leaves them in.
+enum { EW_NONE, EW_TEXT, EW_BIN };
+
+struct FileGraph {
+  lint v_num = 0, e_num = 0, offset = 0, g_v_size = 0;
+  DevBuf row, col;  // int32 per edge in file order; -1 marks an endpoint out of range
+  int weights = EW_NONE;  // which per-slot weight rule read_graph_csr / read_csr_bin use
+};
+
+// binary pairs (from, to) -> (row, column) for `type` (csr.c:250-291)
+__global__ void bin_pairs_kernel(int64_t E, const int64_t *__restrict__ pairs, int64_t offset,
+                                 int recv, int64_t V, int32_t *__restrict__ row,
+                                 int32_t *__restrict__ col) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t from = pairs[2 * i], to = pairs[2 * i + 1];
+    // OFFSET = 1 (graph.h:28-32): the other endpoint is 1-based
+    const int64_t r = recv ? to - offset : from - offset;
+    const int64_t c = recv ? from - 1 : to - 1;
+    row[i] = (r >= 0 && r < V) ? (int32_t)r : -1;
+    col[i] = (c >= 0 && c < V) ? (int32_t)c : -1;
+  }
+}
+
+// Edge weights by CSR slot, as the readers assign them: slot i gets the i-th
+// edge of the file's weight, v_to / v_num (csr.c:42, text), or 0.1 / 0.2 by
+// the parity of i (csr.c:293-298, binary).  Text SEND files: v_to = col + 1.
+__global__ void slot_weights_kernel(int64_t E, const int32_t *__restrict__ file_col, int text,
+                                    double v_num, double *__restrict__ w) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = text ? (double)(file_col[i] + 1) / v_num : ((i & 1) ? 0.2 : 0.1);
+}
+
+int grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+void load_text_edges(FileGraph &fg, const char *file, const char *dir, sint type) {
+  FILE *f = open_in(dir, file, "r");
+  Tok t{f};
+  char str[128];
+  long long v_num = 0, e_num = 0, offset = 0, g_v_size = 0, u, v, w;
+  bool header = false;
+  std::vector<int32_t> row, col;
+  size_t read = 0;
+  while (t.next(str, sizeof(str))) {
+    if (str[0] == 'p' && str[1] == '\0') {
+      char kind[128];
+      if (!t.next(kind, sizeof(kind)) || !t.num(&v_num) || !t.num(&e_num) || !t.num(&offset) ||
+          !t.num(&g_v_size))
+        fatal("bad 'p' line");
+      if (v_num <= 0 || v_num >= (1ll << 31) || e_num < 0 || e_num >= (1ll << 31))
+        fatal("graph size outside the int32 device layout");
+      row.resize(e_num);
+      col.resize(e_num);
+      read = 0;
+      header = true;
+    } else if (str[0] == 'a' && str[1] == '\0') {
+      if (!header) fatal("'a' line before the 'p' line");
+      if (!t.num(&u) || !t.num(&v) || !t.num(&w)) fatal("bad 'a' line");
+      // csr.c:37-48 / 165-176: RECV keys on the target, SEND on the source
+      const long long r = type == GB_LEGACY_RECV ? v - offset : u - offset;
+      const long long c = type == GB_LEGACY_RECV ? u - 1 : v - 1;
+      if (r < 0 || r >= v_num) fatal("edge endpoint out of range");
+      if (c < 0 || c >= v_num) fatal("edge endpoint out of range");
+      row[read] = (int32_t)r;
+      col[read] = (int32_t)c;
+      if ((long long)++read == e_num) break;
+    }
+  }
+  fclose(f);
+  if (!header) fatal("no 'p' line");
+  if ((long long)read != e_num) fatal("the file holds fewer edges than its 'p' line counts");
+  fg.v_num = v_num;
+  fg.e_num = e_num;
+  fg.offset = offset;
+  fg.g_v_size = g_v_size;
+  fg.weights = EW_TEXT;
+  upload(fg.row, row);
+  upload(fg.col, col);
+}
+
+void load_bin_edges(FileGraph &fg, const char *file, const char *dir, sint type) {
+  FILE *f = open_in(dir, file, "rb");
+  lint hdr[4];
+  if (fread(hdr, sizeof(lint), 4, f) != 4) fatal("short binary header");
+  const lint v_num = hdr[0], e_num = hdr[1];
+  if (v_num <= 0 || v_num >= (1ll << 31) || e_num < 0 || e_num >= (1ll << 31))
+    fatal("graph size outside the int32 device layout");
+  std::vector<lint> pairs(2 * (size_t)e_num);
+  if ((lint)fread(pairs.data(), sizeof(lint), 2 * e_num, f) != 2 * e_num)
+    fatal("short edge list");
+  fclose(f);
+  fg.v_num = v_num;
+  fg.e_num = e_num;
+  fg.offset = hdr[2];
+  fg.g_v_size = hdr[3];
+  fg.weights = EW_BIN;
+  DevBuf raw;
+  upload(raw, pairs);
+  check(fg.row.ensure(sizeof(int32_t) * (e_num + 1)), "device allocation");
+  check(fg.col.ensure(sizeof(int32_t) * (e_num + 1)), "device allocation");
+  if (e_num > 0) {
+    hipLaunchKernelGGL(bin_pairs_kernel, dim3(grid_for(e_num)), dim3(256), 0, 0, (int64_t)e_num,
+                       raw.as<int64_t>(), (int64_t)fg.offset, type == GB_LEGACY_RECV ? 1 : 0,
+                       (int64_t)v_num, fg.row.as<int32_t>(), fg.col.as<int32_t>());
+    hipcheck(hipGetLastError(), "binary edges");
+  }
+  hipcheck(hipDeviceSynchronize(), "binary edges");
+}
+
+void load_edges(FileGraph &fg, const char *file, const char *dir, sint type) {
+  if (bin)
+    load_bin_edges(fg, file, dir, type);
+  else
+    load_text_edges(fg, file, dir, type);
+}
+
+// CSR rows keyed by fg.row (end-exclusive row_ptr of V + 1 entries).
+void build_rows(const FileGraph &fg, DevBuf &rp, DevBuf &col) {
+  const int32_t V = (int32_t)fg.v_num;
+  const int64_t E = fg.e_num;
+  check(rp.ensure(sizeof(int32_t) * (V + 1)), "device allocation");
+  check(col.ensure(sizeof(int32_t) * (E + 1)), "device allocation");
+  check(gb_csr_from_edges(V, E, fg.row.as<int32_t>(), fg.col.as<int32_t>(), nullptr, 0,
+                          rp.as<int32_t>(), col.as<int32_t>(), nullptr, nullptr),
+        "CSR build");
+}
+
+// Forward CSR with the readers' per-slot f64 weights and its transpose.
+void build_send_graph(const FileGraph &fg, DevGraph &dg) {
+  const int32_t V = (int32_t)fg.v_num;
+  const int64_t E = fg.e_num;
+  build_rows(fg, dg.rp, dg.col);
+  check(dg.w64.ensure(sizeof(double) * (E + 1)), "device allocation");
+  if (E > 0) {
+    hipLaunchKernelGGL(slot_weights_kernel, dim3(grid_for(E)), dim3(256), 0, 0, E,
+                       fg.col.as<int32_t>(), fg.weights == EW_TEXT ? 1 : 0, (double)fg.v_num,
+                       dg.w64.as<double>());
+    hipcheck(hipGetLastError(), "edge weights");
+  }
+  check(dg.rrp.ensure(sizeof(int32_t) * (V + 1)), "device allocation");
+  check(dg.rcol.ensure(sizeof(int32_t) * (E + 1)), "device allocation");
+  check(dg.rw64.ensure(sizeof(double) * (E + 1)), "device allocation");
+  check(gb_csr_transpose(V, E, dg.rp.as<int32_t>(), dg.col.as<int32_t>(), dg.w64.as<double>(), 8,
+                         dg.rrp.as<int32_t>(), dg.rcol.as<int32_t>(), dg.rw64.as<double>(),
+                         nullptr),
+        "reverse CSR");
+  check(gb_graph_wrap(&dg.g, V, E, dg.rp.as<int32_t>(), dg.col.as<int32_t>(), nullptr,
+                      dg.rrp.as<int32_t>(), dg.rcol.as<int32_t>(), nullptr),
+        "graph handle");
+  check(gb_graph_set_f64_weights(dg.g, dg.w64.as<double>(), dg.rw64.as<double>()), "f64 weights");
+  dg.V = V;
+  dg.E = E;
 }
 
 }  // namespace
@@ -433,108 +647,88 @@ void run_pagerank_csr_mpi(char *file, char *dir) {
     fatal("run_pagerank_csr_mpi: this build runs on one GPU (num_nodes must be 1)");
   const int myrank = 0;
   char f_buf[4096];
-  t_csr gr_s{};
-  t_csr *gr = &gr_s;
+  // the in-edge file (pagerank_mpi.c:721-728): read_graph_csr RECV or
+  // read_csr_bin RECV; the CSR is built on the device
   snprintf(f_buf, sizeof(f_buf), "%s%s_rc_%d_%d", data_extend, file, myrank, num_nodes);
-  if (!bin) {
-    scan_csr_idx_impl(gr, f_buf, dir, GB_LEGACY_RECV);
-    read_graph_csr_impl(gr, f_buf, dir, GB_LEGACY_RECV);
-  } else {
-    read_csr_bin_impl(gr, f_buf, dir, GB_LEGACY_RECV, myrank);
-  }
-  // read_recip (csr_partition.c:166-182 / read_recip_bin for `bin`)
-  std::vector<double> recip(gr->v_size);
+  FileGraph fg;
+  load_edges(fg, f_buf, dir, GB_LEGACY_RECV);
+  const lint V = fg.v_num;
+  // read_recip (csr_partition.c:166-185: V doubles, then distrib, text) or
+  // read_recip_bin (csr_partition_bin_mpi.c:312-332: num_nodes lints of
+  // distrib, then V doubles, binary)
+  std::vector<double> recip(V);
   {
-    if (!bin) {
+    if (!bin)
       snprintf(f_buf, sizeof(f_buf), "%s%s%s_recip_%d_%s", dir, par_extend, file, myrank,
                nodes_extension ? nodes_extension : "(null)");
-      FILE *r = fopen(f_buf, "r");
-      if (!r) {
-        printf("the file %s does not exists!\n", f_buf);
-        fatal("missing recip file");
-      }
-      for (lint i = 0; i < gr->v_size; i++)
-        if (fscanf(r, "%lf", &recip[i]) != 1) fatal("short recip file");
-      fclose(r);
-    } else {
-      snprintf(f_buf, sizeof(f_buf), "%s%s%s_recip_%d_%s", dir, par_extend, file, myrank,
-               nodes_extension ? nodes_extension : "(null)");
-      FILE *r = fopen(f_buf, "rb");
-      if (!r) {
-        printf("the file %s does not exists!\n", f_buf);
-        fatal("missing recip file");
-      }
-      if ((lint)fread(recip.data(), sizeof(double), gr->v_size, r) != gr->v_size)
-        fatal("short recip file");
-      fclose(r);
+    else
+      snprintf(f_buf, sizeof(f_buf), "%s%s%s_recip_%d_%d", dir, par_extend, file, myrank,
+               num_nodes);
+    FILE *r = fopen(f_buf, bin ? "rb" : "r");
+    if (!r) {
+      printf("the file %s does not exists!\n", f_buf);
+      fatal("missing recip file");
     }
-    for (lint i = 0; i < gr->v_size; i++) gr->vet_info[i].recip = recip[i];
-    if (!bin) printf("%1.10f\n", gr->vet_info[0].recip);
+    if (!bin) {
+      for (lint i = 0; i < V; i++)
+        if (fscanf(r, "%lf", &recip[i]) != 1) fatal("short recip file");
+    } else {
+      std::vector<lint> distrib(num_nodes);
+      if ((lint)fread(distrib.data(), sizeof(lint), num_nodes, r) != (lint)num_nodes ||
+          (lint)fread(recip.data(), sizeof(double), V, r) != V)
+        fatal("short recip file");
+    }
+    fclose(r);
+    if (!bin) printf("%1.10f\n", recip[0]);
   }
   // the RECV CSR is the in-CSR: wrap it as the reverse graph of the handle
   DevGraph dg;
-  {
-    const int32_t V = (int32_t)gr->v_size;
-    const int64_t E = gr->e_size;
-    std::vector<int32_t> rp(V + 1), col(E);
-    rp[0] = 0;
-    for (int32_t i = 0; i < V; i++) rp[i + 1] = (int32_t)gr->vet_idx[i];
-    for (int64_t e = 0; e < E; e++) {
-      if (gr->edge_idx[e] < 0 || gr->edge_idx[e] >= V) fatal("edge_idx out of range");
-      col[e] = (int32_t)gr->edge_idx[e];
-    }
-    upload(dg.rrp, rp);
-    upload(dg.rcol, col);
-    check(gb_graph_wrap(&dg.g, V, E, nullptr, nullptr, nullptr, dg.rrp.as<int32_t>(),
-                        dg.rcol.as<int32_t>(), nullptr),
-          "graph handle");
-    dg.V = V;
-    dg.E = E;
-  }
+  build_rows(fg, dg.rrp, dg.rcol);
+  fg.row.release();
+  fg.col.release();
+  check(gb_graph_wrap(&dg.g, (int32_t)V, fg.e_num, nullptr, nullptr, nullptr,
+                      dg.rrp.as<int32_t>(), dg.rcol.as<int32_t>(), nullptr),
+        "graph handle");
+  dg.V = (int32_t)V;
+  dg.E = fg.e_num;
   DevBuf drecip, dx;
   upload(drecip, recip);
-  check(dx.ensure(sizeof(double) * (gr->v_size + 1)), "device allocation");
+  check(dx.ensure(sizeof(double) * (V + 1)), "device allocation");
   int32_t iters = 0;
   hipEvent_t e0, e1;
   hipcheck(hipEventCreate(&e0), "event");
   hipcheck(hipEventCreate(&e1), "event");
   hipcheck(hipEventRecord(e0, 0), "event");
   gb_status s = gb_pagerank_ref_impl(dg.g, drecip.as<double>(), 0, dx.as<double>(), &iters, 0,
-                                     pr_print_hook, gr);
+                                     pr_print_hook, &recip);
   hipcheck(hipEventRecord(e1, 0), "event");
   check(s, "pagerank");
-  std::vector<double> x(gr->v_size);
-  hipcheck(hipMemcpy(x.data(), dx.p, sizeof(double) * gr->v_size, hipMemcpyDeviceToHost),
-           "download");
-  for (lint i = 0; i < gr->v_size; i++) {
-    gr->vet_info[i].weight = x[i];
-    gr->vet_info[i].wb = x[i];
-  }
+  const int n = V < 10 ? (int)V : 10;
+  double x[10];
+  hipcheck(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
   float ms = 0;
   hipcheck(hipEventSynchronize(e1), "event");
   hipcheck(hipEventElapsedTime(&ms, e0, e1), "event");
+  hipcheck(hipEventDestroy(e0), "event");
+  hipcheck(hipEventDestroy(e1), "event");
   printf("Final pagerank values: \n");  // pagerank_mpi.c:1024-1030
-  for (lint ii = 0; ii < 10 && ii < gr->v_size; ii++)
-    printf("%lld|%lf ", ii + gr->offset, gr->vet_info[ii].weight);
+  for (int ii = 0; ii < n; ii++) printf("%lld|%lf ", ii + fg.offset, x[ii]);
   printf("\n");
   printf("iterations taken: %d\n", iters);
-  printf("gpu time: %f ms (%lld in-edges, %d iterations)\n", ms, gr->e_size, iters);
+  printf("gpu time: %f ms (%lld in-edges, %d iterations)\n", ms, (long long)fg.e_num, iters);
   fflush(stdout);
 }
 
 // ---- Brandes (csr_bc.c:16-68) --------------------------------------------------
 
 void run_bc(char *file, char *dir) {
-  t_csr gs_s{};
-  t_csr *gs = &gs_s;
-  if (!bin) {
-    scan_csr_idx_impl(gs, file, dir, GB_LEGACY_SEND);
-    read_graph_csr_impl(gs, file, dir, GB_LEGACY_SEND);
-  } else {
-    read_csr_bin_impl(gs, file, dir, GB_LEGACY_SEND, 0);
-  }
+  // scan_csr_idx + read_graph_csr SEND, or read_csr_bin SEND (csr_bc.c:20-33)
+  FileGraph fg;
+  load_edges(fg, file, dir, GB_LEGACY_SEND);
   DevGraph dg;
-  to_device(gs, dg, true, true);
+  build_send_graph(fg, dg);
+  fg.row.release();
+  fg.col.release();
   const int32_t V = dg.V;
   std::vector<int32_t> sources(V);
   for (int32_t i = 0; i < V; i++) sources[i] = i;  // every vertex (csr_bc.c:40)

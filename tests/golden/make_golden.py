@@ -17,6 +17,16 @@ outputs, as JSON under tests/golden/.  Cases:
   chunk.json       the reference CLI -k (write_recip + chunk_files,
                    csr_partition.c:32-164) on text graphs: every recip, part
                    and send file it writes
+  pagerank_bin.json
+                   the reference CLI -l -i (bin = 1: read_csr_bin,
+                   csr.c:211-299, and read_recip_bin,
+                   csr_partition_bin_mpi.c:312-332, feeding
+                   run_pagerank_csr_mpi): binary in-edge files whose pairs are
+                   not in row order, exact 1/out-degree recips
+  bc_bin.json      the reference CLI -l -e (read_csr_bin SEND feeding run_bc,
+                   csr_bc.c:16-68): binary edge files in shuffled order, so
+                   the per-slot 0.1 / 0.2 weights land on other edges than in
+                   a sorted file
 Arguments: fixture names to (re)write; none = all.
 """
 from __future__ import annotations
@@ -215,6 +225,99 @@ def bc_cases():
     return out
 
 
+def bin_graph_bytes(V, pairs):
+    """read_csr_bin's format (csr.c:221-229): int64 v_num, e_num, offset,
+    g_v_size, then e_num (from, to) int64 pairs, both 1-based."""
+    hdr = np.array([V, len(pairs), 1, V], np.int64)
+    return hdr.tobytes() + np.asarray(pairs, np.int64).reshape(-1, 2).tobytes()
+
+
+def pagerank_bin_cases():
+    out = []
+    rng = np.random.default_rng(17)
+    graphs = [(6, [0, 0, 1, 2, 3, 3, 4, 5, 5], [1, 2, 2, 0, 2, 4, 5, 3, 0])]
+    for V, E in [(10, 30), (12, 20)]:
+        u = rng.integers(0, V, E)
+        v = rng.integers(0, V, E)
+        m = u != v
+        o = np.argsort(u[m], kind="stable")
+        graphs.append((V, u[m][o].tolist(), v[m][o].tolist()))
+    ur, vr = rmat_edges(8, 8, 9)
+    o = np.argsort(ur, kind="stable")
+    graphs.append((256, ur[o].tolist(), vr[o].tolist()))
+    for V, u, v in graphs:
+        u = np.asarray(u, np.int64)
+        v = np.asarray(v, np.int64)
+        E = len(u)
+        deg = np.bincount(u, minlength=V).astype(np.float64)
+        with np.errstate(divide="ignore"):
+            recip = 1.0 / deg
+        perm = rng.permutation(E)  # in-edge file order: not grouped by target
+        rc_pairs = np.stack([u[perm] + 1, v[perm] + 1], 1)
+        with tempfile.TemporaryDirectory() as td:
+            g = os.path.join(td, "g") + "/"
+            for sub in ("datex", "parex", "log"):
+                os.makedirs(g + sub)
+            with open(g + "g.txt", "w") as f:
+                f.write(text_graph(V, u.tolist(), v.tolist()))
+            base = ["cli", "-f", "g.txt", "-b", g, "-n", "1"]
+            # the text pipeline writes the files the binary run also opens
+            # (partition index, send bounds); then the graph, recip and send
+            # files are replaced by their binary forms
+            run([*base[:1], "-k", *base[1:]], cwd=td)
+            run([*base[:1], "-r", *base[1:]], cwd=td)
+            distrib = [int(x) for x in open(g + "parex/g.txt_recip_0_1").read().split()[V:V + 1]]
+            send = [int(x) for x in open(g + "parex/g.txt_send_1").read().split()[:1]]
+            with open(g + "datex/g.txt_sd_0_1", "wb") as f:
+                f.write(bin_graph_bytes(V, np.stack([u + 1, v + 1], 1)))
+            with open(g + "datex/g.txt_rc_0_1", "wb") as f:
+                f.write(bin_graph_bytes(V, rc_pairs))
+            with open(g + "parex/g.txt_recip_0_1", "wb") as f:
+                f.write(np.array(distrib, np.int64).tobytes() + recip.tobytes())
+            with open(g + "parex/g.txt_send_1", "wb") as f:
+                f.write(np.array(send, np.int64).tobytes())
+            r = run(["cli", "-l", "-i", "-f", "g.txt", "-b", g, "-n", "1"], cwd=td, check=False)
+        text = r.stdout
+        iters = [[float(t.split("|")[0]) for t in ln.split()[2:]]
+                 for ln in text.splitlines() if ln.startswith("ITER")]
+        fin = text.split("Final pagerank values:")[1].splitlines()[1].split()
+        final = [float(t.split("|")[1]) for t in fin]
+        it = int(re.search(r"iterations taken: (\d+)", text).group(1))
+        out.append(dict(V=V, rc_pairs=rc_pairs.ravel().tolist(), distrib=distrib,
+                        recip=[float(x) for x in recip], iter_wb=iters,
+                        final=final[:min(10, V)], iterations=it))
+    return out
+
+
+def bc_bin_cases():
+    out = []
+    rng = np.random.default_rng(19)
+    graphs = [(5, [0, 1, 2, 3], [1, 2, 3, 4]),                       # path
+              (6, [0, 0, 0, 0, 0], [1, 2, 3, 4, 5]),                 # star
+              (6, [0, 0, 1, 2, 3, 4], [1, 2, 3, 3, 4, 5]),           # diamond + tail
+              (7, [0, 0, 1, 2, 3, 3, 4, 5], [1, 2, 3, 3, 4, 5, 6, 6])]
+    for V, E in [(12, 30), (20, 60)]:
+        u = rng.integers(0, V, E)
+        v = rng.integers(0, V, E)
+        m = u != v
+        graphs.append((V, u[m].tolist(), v[m].tolist()))
+    for V, u, v in graphs:
+        u = np.asarray(u, np.int64)
+        v = np.asarray(v, np.int64)
+        perm = rng.permutation(len(u))
+        pairs = np.stack([u[perm] + 1, v[perm] + 1], 1)
+        with tempfile.TemporaryDirectory() as td:
+            g = os.path.join(td, "g") + "/"
+            os.makedirs(g)
+            with open(g + "g.bin", "wb") as f:
+                f.write(bin_graph_bytes(V, pairs))
+            r = run(["cli", "-l", "-e", "-f", "g.bin", "-b", g], cwd=td)
+        vals = r.stdout.split("betweeness centrality is:")[1].split()
+        bc = [float(t.split(":")[1]) for t in vals]
+        out.append(dict(V=V, pairs=pairs.ravel().tolist(), bc=bc))
+    return out
+
+
 def chunk_cases():
     """The reference CLI -k (main.c:116-136: scan_csr_idx SEND/RECV,
     comp_distrib_csr, write_recip, chunk_files) on text graphs, num_nodes p:
@@ -296,6 +399,8 @@ def main():
         "bc.json": bc_cases,
         "remove_duplicate.json": remove_duplicate_cases,
         "chunk.json": chunk_cases,
+        "pagerank_bin.json": pagerank_bin_cases,
+        "bc_bin.json": bc_bin_cases,
     }
     wanted = sys.argv[1:] or list(builders)
     fixtures = {name: builders[name]() for name in wanted}

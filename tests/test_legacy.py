@@ -32,10 +32,11 @@ def legacy():
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def run_printing_entry(func, file, dirname, nodes_extension=None):
+def run_printing_entry(func, file, dirname, nodes_extension=None, binary=False):
     """Runs run_pagerank_csr_mpi / run_bc in a child process: they print their
     results (like the reference) and exit(123) on fatal errors, which must not
-    end the test runner.  Returns (returncode, stdout)."""
+    end the test runner.  `binary` sets the reference's global `bin` (main.c
+    option -l).  Returns (returncode, stdout)."""
     import subprocess
     import sys
     code = (
@@ -46,6 +47,7 @@ def run_printing_entry(func, file, dirname, nodes_extension=None):
         + (f"ext = C.create_string_buffer({nodes_extension!r})\n"
            "C.c_void_p.in_dll(lib, 'nodes_extension').value = C.addressof(ext)\n"
            if nodes_extension else "")
+        + ("C.c_int.in_dll(lib, 'bin').value = 1\n" if binary else "")
         + f"lib.{func}({file!r}, {dirname!r})\n")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     print(f"[case] {func} rc={r.returncode}\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}", flush=True)
@@ -105,6 +107,29 @@ def test_binary_reader(tmp_path):
     assert col.tolist() == [1, 2, 2, 4, 0]
     assert w.tolist() == [0.1, 0.2, 0.1, 0.2, 0.1]  # csr.c:293-298
     assert all(g.vet_info[i].weight == float("inf") for i in range(V))
+
+
+def write_bin_graph(path, V, pairs):
+    """read_csr_bin's format: int64 v_num, e_num, offset 1, g_v_size, then
+    the (from, to) pairs, 1-based (graph/csr.c:221-229)."""
+    with open(path, "wb") as f:
+        f.write(np.array([V, len(pairs), 1, V], np.int64).tobytes())
+        f.write(np.asarray(pairs, np.int64).tobytes())
+
+
+def test_binary_reader_golden_in_edge_file(tmp_path):
+    """The reference's binary in-edge files (pagerank_bin.json) through
+    read_csr_bin RECV: rows by target, file order inside a row."""
+    L = legacy()
+    for case in load("pagerank_bin.json"):
+        V = case["V"]
+        pairs = np.array(case["rc_pairs"], np.int64).reshape(-1, 2)
+        write_bin_graph(tmp_path / "rc.bin", V, pairs)
+        g = L.TCsr()
+        L.lib().read_csr_bin(C.byref(g), b"rc.bin", (str(tmp_path) + "/").encode(), L.RECV, 0)
+        rp, col, _ = L.csr_arrays(g)
+        erp, ecol, _ = _csr_sorted(V, pairs[:, 1] - 1, pairs[:, 0] - 1, np.zeros(len(pairs)))
+        assert rp.tolist() == erp.tolist() and col.tolist() == ecol.tolist()
 
 
 def test_bin_search_host():
@@ -218,6 +243,97 @@ def test_run_bc_vs_reference_and_oracle(gpu, case, tmp_path):
     assert np.allclose(bc, ref, atol=1e-6)  # printed with %lf
     if V <= 6:
         assert np.allclose(bc, case["bc"], atol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", load("pagerank_bin.json"), ids=lambda c: f"V{c['V']}")
+def test_run_pagerank_csr_mpi_binary_vs_reference(gpu, case, tmp_path):
+    """bin = 1 (main.c -l): the binary in-edge file, whose pairs are not in row
+    order, and the binary recip file (num_nodes distrib entries, then V
+    doubles) give the reference's iteration count, every printed ITER line
+    and its final values."""
+    d = str(tmp_path) + "/"
+    os.makedirs(d + "datex")
+    os.makedirs(d + "parex")
+    V = case["V"]
+    write_bin_graph(d + "datex/g.txt_rc_0_1", V, np.array(case["rc_pairs"]).reshape(-1, 2))
+    with open(d + "parex/g.txt_recip_0_1", "wb") as f:
+        f.write(np.array(case["distrib"], np.int64).tobytes())
+        f.write(np.array(case["recip"], np.float64).tobytes())
+    rc, out = run_printing_entry("run_pagerank_csr_mpi", b"g.txt", d.encode(), b"1", binary=True)
+    assert rc == 0
+    assert out.startswith("ITER 1 ")  # the recip line is printed for text files only
+    it = int(out.split("iterations taken:")[1].split()[0])
+    assert it == case["iterations"]
+    fin = out.split("Final pagerank values:")[1].splitlines()[1].split()
+    vals = np.array([float(t.split("|")[1]) for t in fin])
+    ref = np.array(case["final"])
+    assert np.max(np.abs(vals[:len(ref)] - ref)) <= 1.01e-6
+    iters = [ln for ln in out.splitlines() if ln.startswith("ITER")]
+    assert len(iters) == it
+    n = min(10, V)
+    for k, ln in enumerate(iters):
+        wb = [float(t.split("|")[0]) for t in ln.split()[2:]]
+        assert np.max(np.abs(np.array(wb[:n]) - np.array(case["iter_wb"][k][:n]))) <= 1.01e-6
+
+
+def _bc_from_output(out):
+    vals = out.split("betweeness centrality is:")[1].split()
+    return np.array([float(t.split(":")[1]) for t in vals])
+
+
+def _oracle_bc_slots(V, src, dst, slot_w):
+    """Brandes on the CSR a reader builds: rows by source in file order, the
+    weight of slot i given by slot_w[i] whichever edge lands there."""
+    rp, col, _ = _csr_sorted(V, src, dst, np.zeros(len(src)))
+    w = np.asarray(slot_w, np.float64)
+    s = np.repeat(np.arange(V), np.diff(rp))
+    o = np.argsort(col, kind="stable")
+    rrp = np.zeros(V + 1, np.int64)
+    np.add.at(rrp, col + 1, 1)
+    rrp = np.cumsum(rrp)
+    return oracle.brandes_weighted(rp.astype(np.int32), col, w, rrp.astype(np.int32), s[o], w[o],
+                                   np.arange(V))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", load("bc_bin.json"), ids=lambda c: f"V{c['V']}E{len(c['pairs']) // 2}")
+def test_run_bc_binary_vs_reference_and_oracle(gpu, case, tmp_path):
+    """bin = 1: read_csr_bin SEND on a shuffled binary edge file; slot i weighs
+    0.1 or 0.2 by its parity (graph/csr.c:293-298)."""
+    V = case["V"]
+    pairs = np.array(case["pairs"], np.int64).reshape(-1, 2)
+    write_bin_graph(tmp_path / "g.bin", V, pairs)
+    rc, out = run_printing_entry("run_bc", b"g.bin", (str(tmp_path) + "/").encode(), binary=True)
+    assert rc == 0
+    bc = _bc_from_output(out)
+    E = len(pairs)
+    ref = _oracle_bc_slots(V, pairs[:, 0] - 1, pairs[:, 1] - 1,
+                           np.where(np.arange(E) % 2 == 0, 0.1, 0.2))
+    assert np.allclose(bc, ref, atol=1e-6)  # printed with %lf
+    if V <= 6:  # no tied shortest paths: the reference's own output
+        assert np.allclose(bc, case["bc"], atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_run_bc_text_weights_follow_file_order(gpu, tmp_path):
+    """read_graph_csr gives CSR slot i the weight v_to / v_num of the i-th
+    edge of the file (graph/csr.c:42), not of the edge stored in slot i: on a
+    file not sorted by source the two differ."""
+    rng = np.random.default_rng(23)
+    V, E = 40, 160
+    u = rng.integers(0, V, E)
+    v = rng.integers(0, V, E)
+    m = u != v
+    u, v = u[m], v[m]
+    write_text_graph(tmp_path / "g.txt", V, u, v)
+    rc, out = run_printing_entry("run_bc", b"g.txt", (str(tmp_path) + "/").encode())
+    assert rc == 0
+    bc = _bc_from_output(out)
+    ref = _oracle_bc_slots(V, u, v, (v + 1) / V)  # file order = slot order
+    assert np.allclose(bc, ref, atol=1e-6)
+    moved = _oracle_bc_slots(V, u, v, ((v + 1) / V)[np.argsort(u, kind="stable")])
+    assert not np.allclose(ref, moved, atol=1e-3)
 
 
 @pytest.mark.gpu

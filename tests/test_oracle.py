@@ -252,6 +252,58 @@ def test_weighted_brandes_vs_reference_run_bc(case):
         assert np.allclose(bc, ref, atol=5.01e-7)
 
 
+@pytest.mark.parametrize("case", load("pagerank_bin.json"), ids=lambda c: f"V{c['V']}")
+def test_pagerank_ref_vs_reference_binary_pipeline(case):
+    """bin = 1: read_csr_bin RECV keeps each target's in-edges in file order
+    (graph/csr.c:271-291), read_recip_bin skips num_nodes distrib entries
+    (partition/csr_partition_bin_mpi.c:312-332).  Exact recips, so the
+    iteration counts and the printed trajectories match."""
+    V = case["V"]
+    p = np.array(case["rc_pairs"]).reshape(-1, 2) - 1
+    irp, icol, _ = csr(V, p[:, 1], p[:, 0])
+    recip = np.array(case["recip"])
+    x, it = oracle.pagerank_ref(irp, icol, recip)
+    assert it == case["iterations"]
+    n = len(case["final"])
+    assert np.max(np.abs(x[:n] - np.array(case["final"]))) <= 5.01e-7
+    for k in (2, it // 2, it):
+        ours, _ = oracle.pagerank_ref(irp, icol, recip, max_iter=k - 1)
+        assert np.max(np.abs(ours[:n] - np.array(case["iter_wb"][k - 1][:n]))) <= 5.01e-7
+
+
+def bin_bc_graph(case):
+    """read_csr_bin SEND: rows by source in file order, and the weight of CSR
+    slot i is 0.1 or 0.2 by the parity of i (graph/csr.c:293-298), whichever
+    edge lands there."""
+    V = case["V"]
+    p = np.array(case["pairs"]).reshape(-1, 2) - 1
+    rp, col, _ = csr(V, p[:, 0], p[:, 1])
+    w = np.where(np.arange(len(col)) % 2 == 0, 0.1, 0.2)
+    return V, rp, col, w
+
+
+@pytest.mark.parametrize("case", load("bc_bin.json"), ids=lambda c: f"V{c['V']}E{len(c['pairs']) // 2}")
+def test_weighted_brandes_vs_reference_binary_run_bc(case):
+    """The oracle on the binary reader's graph: equal to the reference where
+    no two shortest paths tie (path, star, diamond with unequal sides), equal
+    to networkx on every simple graph.  On the 7-vertex double diamond both
+    sides of each diamond weigh 0.1 + 0.2, and the reference's duplicated
+    predecessors (sssp/csr_sspath.c:94-122) skew its sigma ratios."""
+    V, rp, col, w = bin_bc_graph(case)
+    rrp, rcol, rw = transpose(V, rp, col, w)
+    bc = oracle.brandes_weighted(rp, col, w, rrp, rcol, rw, np.arange(V))
+    src = np.repeat(np.arange(V), np.diff(rp))
+    if len(set(zip(src.tolist(), col.tolist()))) == len(col):
+        nxv = _nx_bc_directed_weighted(V, src.tolist(), col.tolist(), w.tolist())
+        assert np.allclose(bc, nxv, rtol=1e-12, atol=1e-12)
+    ref = np.array(case["bc"])
+    if V <= 6:
+        assert np.allclose(bc, ref, atol=5.01e-7)
+    elif V == 7:
+        assert not np.allclose(bc, ref, atol=1e-3)  # the tie case
+        assert np.allclose(bc, [0, 2, 2, 9, 2.5, 1.5, 0], atol=1e-12)
+
+
 # ------------------------------------------------------------------ mathematics
 
 def rand_graph(seed, V, E, wlo=1, whi=255, allow_dup=True):
