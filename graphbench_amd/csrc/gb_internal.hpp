@@ -12,6 +12,8 @@ gb_status gb_fail(gb_status s, const char *fmt, ...);
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #define GB_HIP(call)                                                        \
   do {                                                                      \
     hipError_t _e = (call);                                                 \
@@ -80,6 +82,15 @@ struct gb_graph_s {
   // or not, row blocks among the candidates or not); merge-path form when
   // row-block edges is 0; all 0 = not tuned yet
   int32_t pr_tuned[20][3] = {};
+  // batched SSSP shapes picked by the autotuner, per (group width, delta)
+  struct MsTuned {
+    int32_t W;
+    uint64_t delta;
+    int8_t scan, per_cu, cshift;
+  };
+  std::vector<MsTuned> ms_tuned;
+  // batched BFS/Brandes shape {grid blocks per CU, push/pull ratio}; 0 = not tuned
+  int32_t bc_tuned[2] = {};
   int32_t pr_tile_items = 0;
   bool pr_tile_relabeled = false;
   int64_t pr_ntiles = 0;

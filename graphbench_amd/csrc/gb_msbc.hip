@@ -47,6 +47,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -111,6 +112,7 @@ struct BcCtx {
   const int32_t *hub_out;
   int32_t nhub_out, maxch_out, bch;
   double *part;  // backward hub chunk partials
+  int32_t alpha;  // push while alpha * (frontier out-edges) < (unvisited in-edges)
 };
 
 __device__ __forceinline__ int lslot(const BcCtx &c, int d) { return c.keep ? d : (d & 1); }
@@ -205,7 +207,7 @@ __global__ void mbc_fwd_ctl_kernel(BcCtx c) {
     int64_t *lo = c.loff + (int64_t)g * c.lvl_cap;
     lo[lslot(c, d + 1)] = c.keep ? lo[d] + (int64_t)n : (int64_t)((d + 1) & 1) * c.V;
     const unsigned long long mf = cg[C_MF], mp = cg[C_MPULL];
-    c.mode[g] = n == 0 ? MODE_NONE : (16 * mf < mp ? MODE_PUSH : MODE_PULL);
+    c.mode[g] = n == 0 ? MODE_NONE : ((unsigned long long)c.alpha * mf < mp ? MODE_PUSH : MODE_PULL);
     cg[C_NEXT] = 0;
     cg[C_MF] = 0;
     cg[C_CAND] = 0;
@@ -718,10 +720,24 @@ gb_status capture(LevelGraph &lg, hipStream_t cs, F launches) {
   return GB_OK;
 }
 
+// A launch shape: grid blocks per group (x) as blocks per CU, and the
+// push/pull ratio alpha.
+struct BcShape {
+  int per_cu, alpha;
+};
+
+// The rules when the autotuner does not run: 8 blocks per CU per group, at
+// least 16 and at most 32768 blocks over all groups; push while 16 times the
+// frontier's out-edges stay below the unvisited vertices' in-edges.
+int bc_grid(const BcShape &sh, int32_t G) {
+  return std::max(16, std::min(gb_num_cus() * sh.per_cu, 32768 / G));
+}
+constexpr BcShape BC_RULES{8, 16};
+
 // Runs one batch of up to 64*G sources.  with_sigma: Brandes (sigma, delta,
 // accumulate into d_bc); otherwise BFS levels into d_level [nsrc][V].
 gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma, double *d_bc,
-                 int32_t *d_level, hipStream_t st, int32_t *depth_out) {
+                 int32_t *d_level, hipStream_t st, int32_t *depth_out, const BcShape &sh) {
   MsWork &w = msw();
   const int32_t V = g->V;
   const int32_t G = (nsrc + 63) / 64;
@@ -737,6 +753,7 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
   c.keep = with_sigma ? 1 : 0;
   c.hub = hub_edges();
   c.bch = bc_bchunk(c.hub);
+  c.alpha = sh.alpha;
   if (with_sigma) {
     GB_TRY(w.sigma.ensure(sizeof(double) * GV * 64));
     GB_TRY(w.delta.ensure(sizeof(double) * GV * 64));
@@ -829,7 +846,7 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
   };
   set_pool();
   // level 0 and its finalize (as the end of level -1)
-  const int gx = std::max(16, std::min(gb_num_cus() * 8, 32768 / G));
+  const int gx = bc_grid(sh, G);
   const dim3 grid(gx, G), block(MB);
   hipLaunchKernelGGL(mbc_seed_kernel, dim3(G), dim3(64), 0, st, c, w.src.as<int32_t>(), nsrc);
   hipLaunchKernelGGL(mbc_fin_kernel, grid, block, 0, st, c);
@@ -955,6 +972,76 @@ __global__ void fill_i32_kernel(int64_t n, int32_t *__restrict__ a, int32_t v) {
     a[i] = v;
 }
 
+// The shape for graph g: GB_BC_GRID (4, 8 or 16 blocks per CU) and
+// GB_BC_ALPHA (4 .. 64) fix parts; else the shape cached in the handle, else
+// the rules.
+BcShape bc_shape(gb_graph g, int *fixed_parts) {
+  BcShape sh = BC_RULES;
+  if (g->bc_tuned[0]) sh = BcShape{g->bc_tuned[0], g->bc_tuned[1]};
+  int fixed = 0;
+  const char *e = getenv("GB_BC_GRID");
+  if (e && (atoi(e) == 4 || atoi(e) == 8 || atoi(e) == 16)) {
+    sh.per_cu = atoi(e);
+    fixed |= 1;
+  }
+  e = getenv("GB_BC_ALPHA");
+  if (e && atoi(e) >= 4 && atoi(e) <= 64) {
+    sh.alpha = atoi(e);
+    fixed |= 2;
+  }
+  if (fixed_parts) *fixed_parts = fixed;
+  return sh;
+}
+
+// Autotuner for the Brandes engine, run on a graph's first batch when the
+// handle has no shape yet (GB_BC_TUNE=0: rules only): the batch is solved in
+// the rules' shape, then with one change at a time (the other blocks-per-CU
+// counts, then alpha halved and doubled), into a scratch score array; the
+// fastest shape is kept in the handle.  Every shape gives the same scores.
+gb_status bc_tune(gb_graph g, const int32_t *h_src, int32_t nsrc, hipStream_t st) {
+  int fixed = 0;
+  const BcShape base = bc_shape(g, &fixed);
+  const char *e = getenv("GB_BC_TUNE");
+  if ((e && e[0] == '0') || g->bc_tuned[0] || fixed == 3) return GB_OK;
+  DevBuf scratch;
+  GB_TRY(scratch.ensure(sizeof(double) * (g->V + 1)));
+  auto timed = [&](const BcShape &s, double &ms) -> gb_status {
+    GB_HIP(hipMemsetAsync(scratch.p, 0, sizeof(double) * g->V, st));
+    GB_HIP(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    int32_t depth = 0;
+    GB_TRY(ms_run(g, h_src, nsrc, true, scratch.as<double>(), nullptr, st, &depth, s));
+    GB_HIP(hipStreamSynchronize(st));
+    ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return GB_OK;
+  };
+  BcShape best = base;
+  double best_ms = 0;
+  GB_TRY(timed(best, best_ms));
+  auto consider = [&](BcShape s) -> gb_status {
+    double ms = 0;
+    GB_TRY(timed(s, ms));
+    if (ms < best_ms) {
+      best = s;
+      best_ms = ms;
+    }
+    return GB_OK;
+  };
+  if (!(fixed & 1)) {
+    const int pc0 = best.per_cu;
+    for (int pc : {4, 8, 16})
+      if (pc != pc0) GB_TRY(consider(BcShape{pc, best.alpha}));
+  }
+  if (!(fixed & 2)) {
+    const int a0 = best.alpha;
+    for (int a : {a0 / 2, a0 * 2})
+      if (a >= 4 && a <= 64) GB_TRY(consider(BcShape{best.per_cu, a}));
+  }
+  g->bc_tuned[0] = best.per_cu;
+  g->bc_tuned[1] = best.alpha;
+  return GB_OK;
+}
+
 }  // namespace
 
 void gb_msbc_release() { msw().release(); }
@@ -976,7 +1063,8 @@ extern "C" gb_status gb_bfs(gb_graph g, const int32_t *h_sources, int32_t nsrc, 
   const int32_t per = 64 * 32;
   for (int32_t b0 = 0; b0 < nsrc; b0 += per) {
     const int32_t ns = std::min(per, nsrc - b0);
-    GB_TRY(ms_run(g, h_sources + b0, ns, false, nullptr, d_level + (int64_t)b0 * V, st, nullptr));
+    GB_TRY(ms_run(g, h_sources + b0, ns, false, nullptr, d_level + (int64_t)b0 * V, st, nullptr,
+                  bc_shape(g, nullptr)));
   }
   GB_HIP(hipStreamSynchronize(st));
   return GB_OK;
@@ -1004,7 +1092,8 @@ extern "C" gb_status gb_bc(gb_graph g, const int32_t *h_sources, int32_t nsrc, i
   for (int32_t b0 = 0; b0 < nsrc && V > 0; b0 += (int32_t)per) {
     const int32_t ns = (int32_t)std::min<int64_t>(per, nsrc - b0);
     int32_t depth = 0;
-    GB_TRY(ms_run(g, h_sources + b0, ns, true, d_bc, nullptr, st, &depth));
+    if (b0 == 0) GB_TRY(bc_tune(g, h_sources, ns, st));
+    GB_TRY(ms_run(g, h_sources + b0, ns, true, d_bc, nullptr, st, &depth, bc_shape(g, nullptr)));
     max_depth = std::max(max_depth, depth - 1);
     batches++;
   }

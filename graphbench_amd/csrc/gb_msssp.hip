@@ -44,6 +44,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "gb_device.hpp"
@@ -53,9 +54,11 @@
 namespace {
 
 constexpr int MS_BLOCK = 256;
-// Edges per queue entry at most: 32 for groups of up to 16 sources (many
+// Default edges per queue entry: 32 for groups of up to 16 sources (many
 // edge slots per step: a short chunk keeps the per-entry critical path short),
-// 64 for wider groups.
+// 64 for wider groups.  The chunk of a solve (32 or 64, MsCtx::cshift) is a
+// launch parameter the autotuner may change; the unrolling below follows the
+// default.
 template <int W>
 struct MsShape {
   static constexpr int CHUNK_SHIFT = W <= 16 ? 5 : 6;
@@ -77,18 +80,16 @@ struct __align__(16) MsEnt {
   uint32_t pad;
 };
 
-template <int W>
-__device__ __forceinline__ uint32_t ms_nchunks(int32_t deg) {
-  const uint32_t k = (uint32_t)((deg + MsShape<W>::CHUNK - 1) >> MsShape<W>::CHUNK_SHIFT);
+__device__ __forceinline__ uint32_t ms_nchunks(int32_t deg, int cshift) {
+  const uint32_t k = (uint32_t)((deg + (1 << cshift) - 1) >> cshift);
   return k ? k : 1u;
 }
 
 // chunk q of the vertex whose edges are [r0, r1)
-template <int W>
 __device__ __forceinline__ MsEnt ms_chunk(unsigned long long m, uint32_t v, int32_t r0, int32_t r1,
-                                          uint32_t q) {
-  const int32_t b = r0 + (int32_t)(q << MsShape<W>::CHUNK_SHIFT);
-  return MsEnt{m, v, b, min(r1, b + MsShape<W>::CHUNK), 0u};
+                                          uint32_t q, int cshift) {
+  const int32_t b = r0 + (int32_t)(q << cshift);
+  return MsEnt{m, v, b, min(r1, b + (1 << cshift)), 0u};
 }
 
 struct MsCtx {
@@ -108,6 +109,7 @@ struct MsCtx {
   unsigned long long *cnt;         // MS_* counters
   unsigned long long *wstat;       // per-wave statistics slots (GB_WSTAT per wave)
   uint64_t delta, magic;           // bucket width and its division constant
+  int32_t cshift;                  // log2 of the edges per queue entry (5 or 6)
 };
 
 __device__ __forceinline__ uint64_t ms_bucket(uint32_t d, uint64_t magic) {
@@ -141,9 +143,9 @@ __global__ void ms_seed_kernel(MsCtx c, const int32_t *__restrict__ src) {
   if (!SCAN) atomicOr(&c.mask[0][s], 1ull << t);
   const int32_t r0 = c.rp[s], r1 = c.rp[s + 1];
   if (lowest && r1 > r0) {
-    const uint32_t k = ms_nchunks<W>(r1 - r0);
+    const uint32_t k = ms_nchunks(r1 - r0, c.cshift);
     const unsigned long long b = atomicAdd(&c.cnt[MS_NEAR0], (unsigned long long)k);
-    for (uint32_t i = 0; i < k; i++) c.qn[0][b + i] = ms_chunk<W>(m, (uint32_t)s, r0, r1, i);
+    for (uint32_t i = 0; i < k; i++) c.qn[0][b + i] = ms_chunk(m, (uint32_t)s, r0, r1, i, c.cshift);
   }
 }
 
@@ -218,7 +220,7 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
       bool ok[UNROLL];
 #pragma unroll
       for (int k = 0; k < UNROLL; k++) {
-        const int r = g0 + k * ES + es;  // < CHUNK <= 64
+        const int r = g0 + k * ES + es;  // < chunk <= 64 (the chunk is a multiple of G)
         x[k] = __shfl(cur.x, r, GB_WAVE);
         const uint32_t wk = __shfl(cur.w, r, GB_WAVE);
         // branch-free, so that the reads below stay one batch
@@ -278,8 +280,8 @@ __device__ __forceinline__ void ms_light(const MsCtx &c, const MsCtl &a, MsEnt *
           r0 = c.rp[xj];
           r1 = c.rp[xj + 1];
         }
-        const uint32_t pn = newn && r1 > r0 ? ms_nchunks<W>(r1 - r0) : 0u;
-        nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(0ull, xj, r0, r1, q); });
+        const uint32_t pn = newn && r1 > r0 ? ms_nchunks(r1 - r0, c.cshift) : 0u;
+        nq.push(pn, [&](uint32_t q) { return ms_chunk(0ull, xj, r0, r1, q, c.cshift); });
         fq.push(newf && r1 > r0 ? 1u : 0u, [&](uint32_t) { return (uint64_t)xj; });
       }
     }
@@ -342,14 +344,14 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *ld
       } else if (nm && atomicOr(&mout[x], (unsigned long long)nm) == 0ull) {
         r0 = c.rp[x];
         r1 = c.rp[x + 1];
-        pn = ms_nchunks<W>(r1 - r0);
+        pn = ms_nchunks(r1 - r0, c.cshift);
       }
       if (km) pf = 1;
     }
     if (SCAN) {
       fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
     } else {
-      nq.push(pn, [&](uint32_t q) { return ms_chunk<W>(0ull, x, r0, r1, q); });
+      nq.push(pn, [&](uint32_t q) { return ms_chunk(0ull, x, r0, r1, q, c.cshift); });
       fq.push(pf, [&](uint32_t) { return (uint64_t)x; });
     }
   }
@@ -466,8 +468,8 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_compact_kernel(MsCtx c,
 #pragma unroll
     for (int j = 0; j < MS_CPL; j++) {
       const uint32_t v = (uint32_t)(b0 + (int64_t)j * GB_WAVE + lane);
-      const uint32_t k = r1[j] > r0[j] ? ms_nchunks<W>(r1[j] - r0[j]) : 0u;
-      nq.push(k, [&](uint32_t q) { return ms_chunk<W>(m[j], v, r0[j], r1[j], q); });
+      const uint32_t k = r1[j] > r0[j] ? ms_nchunks(r1[j] - r0[j], c.cshift) : 0u;
+      nq.push(k, [&](uint32_t q) { return ms_chunk(m[j], v, r0[j], r1[j], q, c.cshift); });
     }
   }
   gb_block_flush(nq);
@@ -524,24 +526,48 @@ __global__ void ms_count_kernel(int64_t V, int32_t nsrc, const int32_t *__restri
   }
 }
 
-// Step-kernel blocks: per CU, 4 below 2^23 edges (small phases: fewer idle
-// blocks to launch and flush), 8 up to 2^25, 16 beyond (more waves in flight
-// in the large phases).  Measured at 4 / 8 / 16 blocks per CU: R-MAT 16 with
-// 8 sources 3.3 / 3.7-3.9 / 4.7 ms; R-MAT 22 with 32 sources 27.9 /
-// 26.3-27.2 / 26.3-26.5 ms.
-int ms_grid(int64_t E) {
-  const int per_cu = E < ((int64_t)1 << 23) ? 4 : E < ((int64_t)1 << 25) ? 8 : 16;
-  return gb_num_cus() * per_cu;
+// Step-kernel blocks per CU when the autotuner does not run: 4 below 2^23
+// edges (small phases: fewer idle blocks to launch and flush), 8 up to 2^25,
+// 16 beyond (more waves in flight in the large phases).  Measured at 4 / 8 /
+// 16 blocks per CU: R-MAT 16 with 8 sources 3.3 / 3.7-3.9 / 4.7 ms; R-MAT 22
+// with 32 sources 27.9 / 26.3-27.2 / 26.3-26.5 ms.
+int ms_rule_per_cu(int64_t E) {
+  return E < ((int64_t)1 << 23) ? 4 : E < ((int64_t)1 << 25) ? 8 : 16;
 }
+constexpr int MS_MAX_PER_CU = 16;
 
 // The scan form pays a pass over V per phase, the append form a returning
 // atomic and a row-bound read per newly marked target inside the step.
 // Measured with 8 sources of R-MAT 16 (185 phases): append 3.3 ms, scan
 // 4.3 ms; with 32 sources of R-MAT 24 (12 phases): append 61 ms, scan 54 ms.
-bool ms_scan_form(int64_t V) {
+// This is the rule when the autotuner does not run.
+bool ms_rule_scan(int64_t V) { return V >= ((int64_t)1 << 20); }
+
+// A solve's shape: form, step-kernel blocks per CU, log2 edges per entry.
+struct MsShapeRt {
+  bool scan;
+  int per_cu;
+  int cshift;
+};
+
+// The parts of the shape fixed by the environment (-1: free):
+// GB_MSSSP_SCAN (0/1), GB_MSSSP_GRID (blocks per CU: 4, 8 or 16),
+// GB_MSSSP_CHUNK (32 or 64); GB_MSSSP_TUNE=0 keeps the rules for the rest.
+struct MsFixed {
+  int scan = -1, per_cu = -1, cshift = -1;
+  bool tune = true;
+};
+MsFixed ms_fixed() {
+  MsFixed f;
   const char *e = getenv("GB_MSSSP_SCAN");
-  if (e && e[0]) return e[0] != '0';
-  return V >= ((int64_t)1 << 20);
+  if (e && e[0]) f.scan = e[0] != '0' ? 1 : 0;
+  e = getenv("GB_MSSSP_GRID");
+  if (e && (atoi(e) == 4 || atoi(e) == 8 || atoi(e) == 16)) f.per_cu = atoi(e);
+  e = getenv("GB_MSSSP_CHUNK");
+  if (e && (atoi(e) == 32 || atoi(e) == 64)) f.cshift = atoi(e) == 32 ? 5 : 6;
+  e = getenv("GB_MSSSP_TUNE");
+  if (e && e[0] == '0') f.tune = false;
+  return f;
 }
 
 // Compaction blocks: one batch of MS_CPL vertices per lane where that takes
@@ -565,11 +591,25 @@ MsWork &g_msw_ref() {
 }
 #define g_msw (g_msw_ref())
 
+// Counters of one group's solve.
+struct MsStats {
+  int64_t phases = 0, buckets = 0, imp_near = 0, imp_far = 0, expanded = 0;
+  unsigned long long relax = 0;
+  bool ovf = false;
+  void add(const MsStats &o) {
+    phases += o.phases;
+    buckets += o.buckets;
+    imp_near += o.imp_near;
+    imp_far += o.imp_far;
+    expanded += o.expanded;
+    relax += o.relax;
+    ovf = ovf || o.ovf;
+  }
+};
+
 template <int W, bool SCAN>
-gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st,
-                   uint32_t *out_rows, int64_t *phases, int64_t *buckets,
-                   unsigned long long *relax, int64_t *imp_near, int64_t *imp_far,
-                   int64_t *expanded, bool *ovf) {
+gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, int grid, hipStream_t st,
+                   uint32_t *out_rows, MsStats &sx) {
   MsWork &w = g_msw;
   const int64_t V = c.V;
   hipLaunchKernelGGL(ms_fill_kernel, dim3((unsigned)std::min<int64_t>((V * W + 255) / 256, 65536)),
@@ -578,7 +618,6 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   if (!SCAN) GB_HIP(hipMemsetAsync(c.mask[1], 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.far, 0, sizeof(unsigned long long) * V, st));
   GB_HIP(hipMemsetAsync(c.cnt, 0, sizeof(unsigned long long) * MS_N, st));
-  const int grid = ms_grid(c.E);
   const int64_t nwaves = (int64_t)grid * (MS_BLOCK / GB_WAVE);
   GB_HIP(hipMemsetAsync(c.wstat, 0, sizeof(unsigned long long) * GB_WSTAT * nwaves, st));
   hipLaunchKernelGGL((ms_seed_kernel<W, SCAN>), dim3(1), dim3(64), 0, st, c, d_src);
@@ -668,13 +707,107 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, hipStream_t st
   unsigned long long h[MS_N];
   GB_HIP(hipMemcpyAsync(h, c.cnt, sizeof(h), hipMemcpyDeviceToHost, st));
   GB_HIP(hipStreamSynchronize(st));
-  *phases += hs.phases;
-  *buckets += hs.buckets;
-  *relax += h[MS_RELAX];
-  *imp_near += (int64_t)h[MS_IMP_NEAR];
-  *imp_far += (int64_t)h[MS_IMP_FAR];
-  *expanded += (int64_t)h[MS_EXPANDED];
-  if (h[MS_OVF]) *ovf = true;
+  sx.phases = hs.phases;
+  sx.buckets = hs.buckets;
+  sx.relax = h[MS_RELAX];
+  sx.imp_near = (int64_t)h[MS_IMP_NEAR];
+  sx.imp_far = (int64_t)h[MS_IMP_FAR];
+  sx.expanded = (int64_t)h[MS_EXPANDED];
+  sx.ovf = h[MS_OVF] != 0;
+  return GB_OK;
+}
+
+// One group's solve in shape sh.
+template <int W>
+gb_status ms_group_shape(MsCtx c, const int32_t *d_src, int max_phases, const MsShapeRt &sh,
+                         hipStream_t st, uint32_t *out_rows, MsStats &sx) {
+  c.cshift = sh.cshift;
+  const int grid = gb_num_cus() * sh.per_cu;
+  return sh.scan ? ms_group<W, true>(c, d_src, max_phases, grid, st, out_rows, sx)
+                 : ms_group<W, false>(c, d_src, max_phases, grid, st, out_rows, sx);
+}
+
+// The shape for a group of width W on graph g: the environment's fixed parts,
+// then the shape tuned for (W, delta) cached in the graph handle, else the
+// rules.  When tuning is on and the handle has no shape for (W, delta) yet,
+// the group is solved in several shapes (every solve is exact, so each leaves
+// the same distances): the rules' shape, then one change at a time (the other
+// form, the other blocks-per-CU counts, the other chunk), keeping a change
+// when it is faster.  A shape's time is the wall time of its solve, graph
+// capture and host waits included; below 20 ms, the better of two solves.
+// The stats returned are those of the kept shape's solve.
+template <int W>
+gb_status ms_group_tuned(gb_graph g, MsCtx c, const int32_t *d_src, int max_phases,
+                         hipStream_t st, uint32_t *out_rows, MsStats &sx) {
+  const MsFixed f = ms_fixed();
+  MsShapeRt sh{f.scan >= 0 ? f.scan == 1 : ms_rule_scan(c.V),
+               f.per_cu > 0 ? f.per_cu : ms_rule_per_cu(c.E),
+               f.cshift > 0 ? f.cshift : MsShape<W>::CHUNK_SHIFT};
+  const bool all_fixed = f.scan >= 0 && f.per_cu > 0 && f.cshift > 0;
+  const gb_graph_s::MsTuned *cached = nullptr;
+  for (const auto &t : g->ms_tuned)
+    if (t.W == W && t.delta == c.delta) cached = &t;
+  if (cached && !all_fixed) {
+    if (f.scan < 0) sh.scan = cached->scan == 1;
+    if (f.per_cu < 0) sh.per_cu = cached->per_cu;
+    if (f.cshift < 0) sh.cshift = cached->cshift;
+  }
+  if (!f.tune || all_fixed || cached || max_phases > 0)
+    return ms_group_shape<W>(c, d_src, max_phases, sh, st, out_rows, sx);
+  int reps = 1;
+  auto timed = [&](const MsShapeRt &s, double &ms, MsStats &o) -> gb_status {
+    ms = 1e300;
+    for (int r = 0; r < reps; r++) {
+      GB_HIP(hipStreamSynchronize(st));
+      const auto t0 = std::chrono::steady_clock::now();
+      GB_TRY(ms_group_shape<W>(c, d_src, max_phases, s, st, out_rows, o));
+      GB_HIP(hipStreamSynchronize(st));
+      ms = std::min(ms, std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0).count());
+    }
+    return GB_OK;
+  };
+  MsShapeRt best = sh;
+  double best_ms = 0;
+  MsStats best_st;
+  GB_TRY(timed(best, best_ms, best_st));
+  if (best_ms < 20.0) {  // short solves: the better of two from here on
+    reps = 2;
+    GB_TRY(timed(best, best_ms, best_st));
+  }
+  auto consider = [&](MsShapeRt s) -> gb_status {
+    double ms = 0;
+    MsStats o;
+    GB_TRY(timed(s, ms, o));
+    if (ms < best_ms) {
+      best = s;
+      best_ms = ms;
+      best_st = o;
+    }
+    return GB_OK;
+  };
+  if (f.scan < 0) {
+    MsShapeRt s = best;
+    s.scan = !s.scan;
+    GB_TRY(consider(s));
+  }
+  if (f.per_cu < 0) {
+    const int base = best.per_cu;
+    for (int pc : {4, 8, 16})
+      if (pc != base) {
+        MsShapeRt s = best;
+        s.per_cu = pc;
+        GB_TRY(consider(s));
+      }
+  }
+  if (f.cshift < 0) {
+    MsShapeRt s = best;
+    s.cshift = s.cshift == 5 ? 6 : 5;
+    GB_TRY(consider(s));
+  }
+  g->ms_tuned.push_back(gb_graph_s::MsTuned{W, c.delta, (int8_t)(best.scan ? 1 : 0),
+                                            (int8_t)best.per_cu, (int8_t)best.cshift});
+  sx = best_st;  // out_rows: the last solve's distances, the same in every shape
   return GB_OK;
 }
 
@@ -713,17 +846,16 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   const int64_t qn = V + E / MS_MIN_CHUNK + 1;
   const int Wmax = nsrc >= 64 ? 64 : (nsrc > 32 ? 64 : nsrc > 16 ? 32 : nsrc > 8 ? 16 : 8);
   GB_TRY(w.dist.ensure(sizeof(uint32_t) * (size_t)V * Wmax));
-  // the scan form for large graphs (GB_MSSSP_SCAN=0/1 forces a form)
-  const bool scan = ms_scan_form(V);
+  // both forms' buffers: the autotuner may try either
   GB_TRY(w.m0.ensure(sizeof(unsigned long long) * V));
-  if (!scan) GB_TRY(w.m1.ensure(sizeof(unsigned long long) * V));
+  GB_TRY(w.m1.ensure(sizeof(unsigned long long) * V));
   GB_TRY(w.far.ensure(sizeof(unsigned long long) * V));
   GB_TRY(w.q[0].ensure(sizeof(MsEnt) * qn));
   GB_TRY(w.q[1].ensure(sizeof(MsEnt) * qn));
   GB_TRY(w.q[2].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.q[3].ensure(sizeof(uint64_t) * (V + 1)));
   GB_TRY(w.cnt.ensure(sizeof(unsigned long long) * MS_N));
-  GB_TRY(w.wstat.ensure(sizeof(unsigned long long) * GB_WSTAT * ms_grid(E) *
+  GB_TRY(w.wstat.ensure(sizeof(unsigned long long) * GB_WSTAT * gb_num_cus() * MS_MAX_PER_CU *
                         (MS_BLOCK / GB_WAVE)));
   GB_TRY(w.ctl.ensure(2 * sizeof(MsCtl)));
   GB_TRY(w.src.ensure(sizeof(int32_t) * 64));
@@ -735,7 +867,7 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   c.E = g->E;
   c.dist = w.dist.as<uint32_t>();
   c.mask[0] = w.m0.as<unsigned long long>();
-  c.mask[1] = scan ? nullptr : w.m1.as<unsigned long long>();
+  c.mask[1] = w.m1.as<unsigned long long>();
   c.far = w.far.as<unsigned long long>();
   c.qn[0] = w.q[0].as<MsEnt>();
   c.qn[1] = w.q[1].as<MsEnt>();
@@ -745,6 +877,7 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
   c.wstat = w.wstat.as<unsigned long long>();
   c.delta = delta;
   c.magic = delta > 1 ? (~0ull / delta) + 1 : 0;
+  MsStats tot;
   for (int32_t g0 = 0; g0 < nsrc; g0 += 64) {
     const int n = std::min<int32_t>(64, nsrc - g0);
     c.ntask = n;
@@ -752,31 +885,28 @@ gb_status gb_msssp_run(gb_graph g, uint32_t delta, const int32_t *h_sources, int
     uint32_t *rows = d_dist + (int64_t)g0 * V;
     const int W = n > 32 ? 64 : n > 16 ? 32 : n > 8 ? 16 : 8;
     const int32_t *ds = w.src.as<int32_t>();
+    MsStats sx;
     switch (W) {
       case 8:
-        GB_TRY((scan ? ms_group<8, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                          imp_near, imp_far, expanded, ovf)
-                    : ms_group<8, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                           imp_near, imp_far, expanded, ovf)));
+        GB_TRY(ms_group_tuned<8>(g, c, ds, max_phases, st, rows, sx));
         break;
       case 16:
-        GB_TRY((scan ? ms_group<16, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                          imp_near, imp_far, expanded, ovf)
-                    : ms_group<16, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                           imp_near, imp_far, expanded, ovf)));
+        GB_TRY(ms_group_tuned<16>(g, c, ds, max_phases, st, rows, sx));
         break;
       case 32:
-        GB_TRY((scan ? ms_group<32, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                          imp_near, imp_far, expanded, ovf)
-                    : ms_group<32, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                           imp_near, imp_far, expanded, ovf)));
+        GB_TRY(ms_group_tuned<32>(g, c, ds, max_phases, st, rows, sx));
         break;
       default:
-        GB_TRY((scan ? ms_group<64, true>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                          imp_near, imp_far, expanded, ovf)
-                    : ms_group<64, false>(c, ds, max_phases, st, rows, phases, buckets, relax,
-                                           imp_near, imp_far, expanded, ovf)));
+        GB_TRY(ms_group_tuned<64>(g, c, ds, max_phases, st, rows, sx));
     }
+    tot.add(sx);
   }
+  *phases += tot.phases;
+  *buckets += tot.buckets;
+  *relax += tot.relax;
+  *imp_near += tot.imp_near;
+  *imp_far += tot.imp_far;
+  *expanded += tot.expanded;
+  if (tot.ovf) *ovf = true;
   return GB_OK;
 }

@@ -5,9 +5,15 @@ GPU by the library's own kernels (gb_mlp_loss_grad, gb_mlp_adam,
 gb_mlp_forward), on data this module generates with the library itself:
 
 * delta model: graph statistics (gb_graph_features: log2 V, log2 average
-  degree, log2 max / mean weight, log2 degree skew, log2 sources) ->
-  (log2 delta, log2 light_max/delta), the fastest setting found by sweeping
-  batched delta-stepping over generated R-MAT and Erdos-Renyi graphs;
+  degree, log2 max / mean weight, log2 degree skew, log2 of the tasks solved
+  together) and the engine (1: the per-source engine, for one source or many
+  graphs in lockstep; 0: the batched engine, several sources of one graph) ->
+  (log2 delta/plain, log2 light_max/delta), the fastest setting found by
+  sweeping delta-stepping over generated R-MAT and Erdos-Renyi graphs, alone
+  (1, 8 and 32 sources) and in lockstep batches of 32 to 512 graphs.  plain
+  is the reference's rule max_weight / average degree: the model predicts a
+  scale over it, which carries over to graphs larger than those swept better
+  than an absolute width would;
 * warm-start model: per-vertex (log2(1+indeg), log2(1+outdeg), dangling) ->
   standardised log2(N * converged rank), from PageRank solved to 1e-9.
 
@@ -37,6 +43,21 @@ MODELS_PATH = os.environ.get(
     "GB_LEARNED_MODELS",
     os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_models.json"))
 HIDDEN = 64
+DELTA_DIN = 7  # the 6 graph features and the engine flag
+
+
+def batched_engine(nsrc: int) -> bool:
+    """Several sources of one graph go to the batched engine (gb_sssp), unless
+    GB_SSSP_PER_SOURCE selects the per-source engine."""
+    e = os.environ.get("GB_SSSP_PER_SOURCE", "")
+    return nsrc >= 2 and not (e and e != "0")
+
+
+def delta_inputs(F: np.ndarray, per_source_engine: bool) -> np.ndarray:
+    """Delta-model inputs: graph features (n, 6) and the engine flag."""
+    F = np.atleast_2d(np.asarray(F, np.float32))
+    flag = np.full((len(F), 1), 1.0 if per_source_engine else 0.0, np.float32)
+    return np.concatenate([F, flag], axis=1)
 
 
 def _torch():
@@ -148,13 +169,23 @@ def _time_sssp(g, srcs, delta, light, reps=1):
     return best
 
 
+def _target(rows):
+    """The mean (log2 delta, log2 light_max/delta) of the settings within 5%
+    of the fastest and within 1.5 of its log2 delta (the time-vs-delta curve
+    can have two separate minima; averaging across them would land between
+    them)."""
+    kb, _, best = min(rows, key=lambda r: r[2])
+    good = [(k, lr) for k, lr, t in rows if t <= 1.05 * best and abs(k - kb) <= 1.5]
+    return np.array([np.mean([k for k, _ in good]), np.mean([lr for _, lr in good])],
+                    np.float32), best
+
+
 def delta_sweep(h: HostCSR, g: Graph, nsrc: int, seed: int, kmax: int = 18, kmin: int = 2):
-    """Times batched delta-stepping over delta = 2^k (descending, stopping once
-    a run is 3x slower than the best), then light_max / delta in {1/2, 1/4}
-    around the best k, and the plain rule.  The target is the mean
-    (log2 delta, log2 light_max/delta) of the settings within 5% of the fastest
-    and within 1.5 of its log2 delta (the time-vs-delta curve can have two
-    separate minima; averaging across them would land between them)."""
+    """Times delta-stepping over delta = 2^k (descending, stopping once a run
+    is 3x slower than the best), then, on the per-source engine (one source),
+    light_max / delta in {1/2, 1/4} around the best k, and the plain rule.
+    The batched engine (several sources) has no light/heavy split, so its
+    light ratio stays 0 (light_max = delta)."""
     srcs = pick_sources(h, nsrc, seed)
     _time_sssp(g, srcs[:1], 1 << 10, 1 << 10)  # warm-up (workspaces, code objects)
     reps = 2 if h.E < (1 << 25) else 1
@@ -167,19 +198,93 @@ def delta_sweep(h: HostCSR, g: Graph, nsrc: int, seed: int, kmax: int = 18, kmin
         if t > 3 * best:
             break
     kb = min(rows, key=lambda r: r[2])[0]
-    for k in (kb - 1, kb, kb + 1):
-        for lr in (-1.0, -2.0):
-            if kmin <= k <= kmax:
-                lm = max(1, int((1 << int(k)) * 2.0 ** lr))
-                rows.append((k, lr, _time_sssp(g, srcs, 1 << int(k), lm, reps)))
+    per_source = not batched_engine(nsrc)
+    if per_source:
+        for k in (kb - 1, kb, kb + 1):
+            for lr in (-1.0, -2.0):
+                if kmin <= k <= kmax:
+                    lm = max(1, int((1 << int(k)) * 2.0 ** lr))
+                    rows.append((k, lr, _time_sssp(g, srcs, 1 << int(k), lm, reps)))
     plain = plain_delta(h)
     t_plain = _time_sssp(g, srcs, plain, plain, reps)
     rows.append((math.log2(plain), 0.0, t_plain))
-    kb, _, best = min(rows, key=lambda r: r[2])
-    good = [(k, lr) for k, lr, t in rows if t <= 1.05 * best and abs(k - kb) <= 1.5]
-    target = np.array([np.mean([k for k, _ in good]), np.mean([lr for _, lr in good])], np.float32)
-    feats = graph_features(g, nsrc)
+    target, best = _target(rows)
+    target[0] -= math.log2(plain)  # a scale over the plain rule
+    feats = delta_inputs(graph_features(g, nsrc), per_source)[0]
     return feats, target, dict(best_s=best, plain_s=t_plain, sweep=rows)
+
+
+def multi_specs(quick: bool = False):
+    """Lockstep batches the delta model is fitted on: (graphs, log2 V range,
+    seed); graphs as in the 512-graph workload (half R-MAT with edge factor
+    U{8..32}, half Erdos-Renyi with average degree U{8..32}), weights
+    uniform in [1, whi]."""
+    specs = [(512, 10, 16, 1023, 7100), (512, 10, 17, 255, 7200), (256, 11, 18, 1023, 7300),
+             (128, 12, 19, 65535, 7400), (64, 12, 20, 1023, 7500), (32, 14, 20, 255, 7600),
+             (512, 10, 20, 1023, 7700)]
+    return specs[:2] if quick else specs
+
+
+def multi_sweep(spec, kmin: int = -3, kmax: int = 9, rows_per_batch: int = 64):
+    """Times one gb_sssp_multi solve of a generated batch for delta_i =
+    plain_i * 2^k (a common scale over the plain rule of each graph; the
+    buckets advance in lockstep), then light_max / delta in {1/2, 1/4} around
+    the best k.  Every graph's target is (log2 plain_i + k*, ratio*), k* the
+    mean of the settings within 5% of the fastest; the inputs carry the
+    batch's graph count as the tasks solved together."""
+    from . import sssp_multi
+    n, lo, hi, whi, seed = spec
+    rng = np.random.default_rng(seed)
+    hs = []
+    for i in range(n):
+        V = int(round(2.0 ** (lo + (hi - lo) * rng.random())))
+        deg = int(rng.integers(8, 33))
+        if i % 2 == 0:
+            sc = max(1, (V - 1).bit_length())
+            hs.append(gen_rmat(sc, deg, seed=seed + i, weights="uniform", wlo=1, whi=whi, nverts=V))
+        else:
+            hs.append(gen_er(V, deg, seed=seed + i, weights="uniform", wlo=1, whi=whi))
+    gs = [Graph.from_host(h, need_reverse=False) for h in hs]
+    srcs = [int(pick_sources(h, 1, seed + 3 * i)[0]) for i, h in enumerate(hs)]
+    plains = np.array([plain_delta(h) for h in hs], np.float64)
+    torch = _torch()
+
+    def run(k, lr):
+        dl = [max(1, int(round(p * 2.0 ** k))) for p in plains]
+        lm = [max(1, int(round(d * 2.0 ** lr))) for d in dl]
+        best = float("inf")
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            sssp_multi(gs, srcs, deltas=dl, light_max=lm)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best
+    run(0, 0.0)  # warm-up
+    rows = []
+    best = float("inf")
+    for k in range(kmin, kmax + 1):
+        t = run(k, 0.0)
+        rows.append((float(k), 0.0, t))
+        best = min(best, t)
+        if t > 3 * best:
+            break
+    kb = min(rows, key=lambda r: r[2])[0]
+    for k in (kb - 1, kb, kb + 1):
+        for lr in (-1.0, -2.0):
+            if kmin <= k <= kmax:
+                rows.append((k, lr, run(k, lr)))
+    target, best = _target(rows)
+    t_plain = [t for k, lr, t in rows if k == 0.0 and lr == 0.0][0]
+    F = delta_inputs(graph_features_multi(gs, n), True)
+    pick = rng.choice(n, size=min(n, rows_per_batch), replace=False)
+    X = F[pick]
+    T = np.stack([np.full(len(pick), target[0]), np.full(len(pick), target[1])],
+                 axis=1).astype(np.float32)
+    for g in gs:
+        g.close()
+    return X, T, dict(best_s=best, plain_s=t_plain, sweep=rows, scale=float(target[0]),
+                      ratio=float(target[1]))
 
 
 def vertex_features_host(h: HostCSR, r: HostCSR) -> np.ndarray:
@@ -218,31 +323,49 @@ class LearnedModels:
         with open(path) as f:
             return LearnedModels(json.load(f))
 
-    def predict_delta(self, g: Graph, nsrc: int):
-        """(delta, light_max) for a batch of nsrc sources on g."""
-        torch = _torch()
-        d = self.data["delta"]
-        f = graph_features(g, nsrc)
-        xs = (f - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
-        y = self.delta_mlp.forward(torch.from_numpy(xs[None, :].astype(np.float32)).cuda())
-        y = y.cpu().numpy()[0] * np.array(d["y_std"]) + np.array(d["y_mean"])
-        delta = int(round(2.0 ** float(np.clip(y[0], 0, 24))))
-        ratio = 2.0 ** float(np.clip(y[1], -3, 3))
-        return max(1, delta), max(1, int(round(delta * ratio)))
+    @staticmethod
+    def _plain(x) -> float:
+        """max_weight / average degree (the plain rule) from features 2 and 1."""
+        return max(1.0, math.floor(2.0 ** float(x[2]) / 2.0 ** float(x[1]) + 1e-3))
 
-    def predict_deltas(self, graphs, nsrc: int = 1):
-        """(delta, light_max) per graph, with one batched forward pass."""
+    def _predict(self, X, light: bool):
         torch = _torch()
         d = self.data["delta"]
-        F = graph_features_multi(graphs, nsrc)
-        xs = (F - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
+        xs = (X - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
         Y = self.delta_mlp.forward(torch.from_numpy(xs.astype(np.float32)).cuda()).cpu().numpy()
         Y = Y * np.array(d["y_std"]) + np.array(d["y_mean"])
         out = []
-        for y in Y:
-            delta = max(1, int(round(2.0 ** float(np.clip(y[0], 0, 24)))))
-            ratio = 2.0 ** float(np.clip(y[1], -3, 3))
+        for x, y in zip(X, Y):
+            plain = self._plain(x)
+            delta = max(1, int(round(plain * 2.0 ** float(np.clip(y[0], -8, 12)))))
+            delta = min(delta, 1 << 30)
+            ratio = 2.0 ** float(np.clip(y[1], -3, 0)) if light else 1.0
             out.append((delta, max(1, int(round(delta * ratio)))))
+        return out
+
+    def predict_delta(self, g: Graph, nsrc: int):
+        """(delta, light_max) for a batch of nsrc sources on g.  The batched
+        engine (several sources) has no light/heavy split: light_max = delta."""
+        per_source = not batched_engine(nsrc)
+        return self._predict(delta_inputs(graph_features(g, nsrc), per_source), per_source)[0]
+
+    def predict_deltas(self, graphs, nsrc: int = 1):
+        """(delta, light_max) per graph for one lockstep solve of all of them
+        (gb_sssp_multi, the per-source engine), with one batched forward pass.
+        The buckets of a lockstep solve advance together, so the model was
+        fitted on one scale over every graph's plain width per batch: the
+        batch takes the median of the per-graph predictions (scale and light
+        ratio) and applies it to each graph's plain width."""
+        F = graph_features_multi(graphs, max(1, len(graphs) * nsrc))
+        X = delta_inputs(F, True)
+        per_graph = self._predict(X, True)
+        plains = [self._plain(x) for x in X]
+        scale = float(np.median([math.log2(d / p) for (d, _), p in zip(per_graph, plains)]))
+        ratio = float(np.median([math.log2(l / d) for d, l in per_graph]))
+        out = []
+        for p in plains:
+            d = max(1, min(1 << 30, int(round(p * 2.0 ** scale))))
+            out.append((d, max(1, int(round(d * 2.0 ** ratio)))))
         return out
 
     def warm_start(self, g: Graph):
@@ -257,7 +380,7 @@ class LearnedModels:
         return x0
 
 
-def _fit_with_holdout(din, dout, X, T, groups, max_steps, seed, log, lr=3e-3, every=50):
+def _fit_with_holdout(din, dout, X, T, groups, max_steps, seed, log, lr=3e-3, every=10):
     """Chooses the number of Adam steps on a 20% hold-out of whole graphs
     (``groups``), then trains a fresh model on all rows for that many steps."""
     torch = _torch()
@@ -359,14 +482,26 @@ def train_all(out_path: str, quick: bool = False, log=print, cache_path: Optiona
     groups = [r["group"] for r in recs]
     info = [dict(spec=r["spec"], nsrc=r["nsrc"], target=r["target"], best_ms=r["best_ms"],
                  plain_ms=r["plain_ms"]) for r in recs]
+    ngroups = max(groups) + 1 if groups else 0
+    for j, spec in enumerate(multi_specs(quick)):
+        X, T, meta = multi_sweep(spec)
+        feats.extend(X.tolist())
+        targets.extend(T.tolist())
+        groups.extend([ngroups + j] * len(X))
+        info.append(dict(multi=list(spec), scale=meta["scale"], ratio=meta["ratio"],
+                         best_ms=meta["best_s"] * 1e3, plain_ms=meta["plain_s"] * 1e3,
+                         sweep=[[k, lr, t * 1e3] for k, lr, t in meta["sweep"]]))
+        log(f"[multi {j+1}] {spec[0]} graphs 2^{spec[1]}..2^{spec[2]} w<={spec[3]}: scale 2^"
+            f"{meta['scale']:.2f} ratio {meta['ratio']:.2f}, best {meta['best_s']*1e3:.1f} ms, "
+            f"plain {meta['plain_s']*1e3:.1f} ms ({time.time()-t0:.0f}s)")
     F = np.array(feats, np.float32)
     Tt = np.array(targets, np.float32)
     xm, xs = _standardise(F)
     ym, ys = _standardise(Tt)
-    dm, losses, hold = _fit_with_holdout(6, 2, (F - xm) / xs, (Tt - ym) / ys, np.array(groups),
-                                         600 if quick else 2000, 1, log)
+    dm, losses, hold = _fit_with_holdout(DELTA_DIN, 2, (F - xm) / xs, (Tt - ym) / ys,
+                                         np.array(groups), 600 if quick else 2000, 1, log)
     log(f"delta model: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-    delta_json = dict(din=6, dout=2, params=dm.params.cpu().numpy().tolist(),
+    delta_json = dict(din=DELTA_DIN, dout=2, params=dm.params.cpu().numpy().tolist(),
                       x_mean=xm.tolist(), x_std=xs.tolist(), y_mean=ym.tolist(),
                       y_std=ys.tolist(), final_loss=losses[-1], holdout=hold, samples=info)
     # ---- warm-start model: converged ranks
@@ -394,7 +529,7 @@ def train_all(out_path: str, quick: bool = False, log=print, cache_path: Optiona
     pr_json = dict(din=3, dout=1, params=pm.params.cpu().numpy().tolist(), x_mean=xm2.tolist(),
                    x_std=xs2.tolist(), y_mean=ym2.tolist(), y_std=ys2.tolist(),
                    final_loss=losses2[-1], holdout=hold2, samples=int(len(X)))
-    data = dict(format="graphbench-mi355x learned models v1", delta=delta_json,
+    data = dict(format="graphbench-mi355x learned models v3", delta=delta_json,
                 warm_start=pr_json, train_seconds=time.time() - t0)
     with open(out_path, "w") as f:
         json.dump(data, f)

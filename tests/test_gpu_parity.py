@@ -79,6 +79,65 @@ def test_sssp_batched_engine_matches_per_source(gpu, nsrc, scan, monkeypatch):
         assert st["reached"] == int((d != 0xFFFFFFFF).sum())
 
 
+@pytest.mark.parametrize("chunk", ["32", "64"])
+@pytest.mark.parametrize("grid", ["4", "16"])
+def test_sssp_batched_shapes_exact(gpu, grid, chunk, monkeypatch):
+    """Every launch shape the autotuner may pick for the batched engine (form,
+    step blocks per CU, edges per queue entry) gives the same distances; the
+    autotuned call (first call on a new handle and delta, then its cached
+    shape) too."""
+    gb = gpu
+    h = gb.gen_rmat(12, 8, seed=61, weights="uniform", wlo=1, whi=500)
+    srcs = gb.pick_sources(h, 40, 62)
+    ref = None
+    monkeypatch.setenv("GB_MSSSP_TUNE", "0")
+    monkeypatch.setenv("GB_MSSSP_GRID", grid)
+    monkeypatch.setenv("GB_MSSSP_CHUNK", chunk)
+    for scan in ("0", "1"):
+        monkeypatch.setenv("GB_MSSSP_SCAN", scan)
+        _say(f"sssp shape grid={grid} chunk={chunk} scan={scan}")
+        g = gb.Graph.from_host(h, need_reverse=False)
+        d = gb.u32(gb.sssp(g, srcs, delta=200))
+        if ref is None:
+            ref = np.stack([oracle.dijkstra(h.row_ptr, h.col, h.w, int(x)) for x in srcs])
+        assert np.array_equal(d, ref)
+    for k in ("GB_MSSSP_TUNE", "GB_MSSSP_GRID", "GB_MSSSP_CHUNK", "GB_MSSSP_SCAN"):
+        monkeypatch.delenv(k)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    for _ in range(2):  # tuned, then cached
+        d, st = gb.sssp(g, srcs, delta=200, stats=True)
+        assert np.array_equal(gb.u32(d), ref)
+        assert st["reached"] == int((ref != 0xFFFFFFFF).sum())
+
+
+@pytest.mark.parametrize("alpha", ["4", "64"])
+@pytest.mark.parametrize("grid", ["4", "16"])
+def test_bc_shapes_exact(gpu, grid, alpha, monkeypatch):
+    """Brandes in every launch shape the autotuner may pick (grid blocks per CU,
+    push/pull ratio alpha) matches the oracle; so does the autotuned call."""
+    gb = gpu
+    h = gb.gen_rmat(12, 8, seed=63, weights="none", symmetric=True)
+    srcs = gb.pick_sources(h, 70, 64)
+    oracle.set_threads(min(16, oracle.get_threads()))
+    ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
+    monkeypatch.setenv("GB_BC_TUNE", "0")
+    monkeypatch.setenv("GB_BC_GRID", grid)
+    monkeypatch.setenv("GB_BC_ALPHA", alpha)
+    _say(f"bc shape grid={grid} alpha={alpha}")
+    g = gb.Graph.from_host(h, need_reverse=False)
+    bc = gb.bc(g, srcs).cpu().numpy()
+    assert float(np.max(np.abs(bc - ref) / np.maximum(np.abs(ref), 1e-300))) <= 1e-9
+    lv = gb.bfs(g, srcs[:3]).cpu().numpy()
+    for i in range(3):
+        assert np.array_equal(lv[i], oracle.bfs_levels(h.row_ptr, h.col, int(srcs[i])))
+    for k in ("GB_BC_TUNE", "GB_BC_GRID", "GB_BC_ALPHA"):
+        monkeypatch.delenv(k)
+    g = gb.Graph.from_host(h, need_reverse=False)
+    for _ in range(2):  # tuned on the first batch, then cached
+        bc = gb.bc(g, srcs).cpu().numpy()
+        assert float(np.max(np.abs(bc - ref) / np.maximum(np.abs(ref), 1e-300))) <= 1e-9
+
+
 @pytest.mark.parametrize("nsrc", [1, 6])
 def test_sssp_relaxation_outcome_counters(gpu, nsrc, monkeypatch):
     """Relaxation outcomes (the reference's REV_STATS totals): every reached

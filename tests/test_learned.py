@@ -44,7 +44,7 @@ def test_models_file_format():
     if not os.path.exists(learned.MODELS_PATH):
         pytest.skip("learned_models.json not trained yet")
     d = json.load(open(learned.MODELS_PATH))
-    for key, din, dout in (("delta", 6, 2), ("warm_start", 3, 1)):
+    for key, din, dout in (("delta", learned.DELTA_DIN, 2), ("warm_start", 3, 1)):
         m = d[key]
         assert m["din"] == din and m["dout"] == dout
         H = learned.HIDDEN
