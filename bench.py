@@ -29,6 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per pr_gather launch measured by rocprofv3 (FETCH_SIZE and
 # WRITE_SIZE, scaled by a copy of known size); written with the profile summary
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pr_gather_rmat22_traffic.json")
+# HBM-side bytes of one learned + batched 32-source solve at R-MAT 24, from
+# copy-calibrated FETCH_SIZE / WRITE_SIZE runs (rocprofv3 --pmc)
+SSSP_TRAFFIC_JSON = os.path.join(ROOT, "profiles", "sssp_rmat24_32src_traffic.json")
 
 
 def log(msg):
@@ -313,6 +316,10 @@ def run_learned_c5(gb, oracle, torch, np, cpu, with_cpu):
     exact = bool(torch.equal(d_plain, d_learn))
     del d_plain, d_learn
     B32 = sssp_batch_bytes(V, E, len(srcs))
+    sssp_traffic = {}
+    if os.path.exists(SSSP_TRAFFIC_JSON):
+        with open(SSSP_TRAFFIC_JSON) as f:
+            sssp_traffic = json.load(f)
     pr_kw = dict(damping=0.85, tol=1e-7, max_iter=100)
     gb.pagerank(g, max_iter=3)  # warm-up (autotunes the gather)
     (x_p, res_p), t_pp = _timed(torch, lambda: gb.pagerank(g, **pr_kw))
@@ -351,8 +358,15 @@ def run_learned_c5(gb, oracle, torch, np, cpu, with_cpu):
                                       "relaxations": st_bp["relax_edges"]},
               "roofline": {"bytes": B32, "achieved_gbs": B32 / t_sl / 1e9,
                            "frac": B32 / t_sl / 1e9 / HBM_PEAK_GBS,
+                           "traffic": sssp_traffic.get("traffic_bytes_per_solve"),
+                           "traffic_over_bytes": (sssp_traffic["traffic_bytes_per_solve"] / B32
+                                                  if sssp_traffic else None),
+                           "traffic_gbs": (sssp_traffic["traffic_bytes_per_solve"] / t_sl / 1e9
+                                           if sssp_traffic else None),
+                           "traffic_source": sssp_traffic.get("source"),
                            "note": "B(32) = 4(V+1) + 8E + 32(4E + 8V) over the learned + "
-                                   "batched solve time"}},
+                                   "batched solve time; traffic = HBM-side bytes of one such "
+                                   "solve from the profile named"}},
         pagerank={"plain_ms": t_pp * 1e3, "warm_ms": t_pw * 1e3,
                   "plain_iterations": res_p["iterations"], "warm_iterations": res_w["iterations"],
                   "plain_gteps": work_pr / t_pp / 1e9, "warm_effective_gteps": work_pr / t_pw / 1e9,

@@ -65,8 +65,10 @@ struct MsShape {
   static constexpr int32_t CHUNK = 1 << CHUNK_SHIFT;
   static constexpr int ES = GB_WAVE / W;                // edge slots per step
   static constexpr int STEPS = CHUNK / ES;              // steps per full chunk
-  static constexpr int UNROLL = STEPS < 8 ? STEPS : 8;  // steps per group
+  static constexpr int UNROLL = STEPS < 4 ? STEPS : 4;  // steps per group
   static constexpr int G = ES * UNROLL;                 // edges per group (<= 64)
+  // waves per SIMD the step kernel's registers must allow (1: no bound)
+  static constexpr int MIN_WAVES = W >= 16 ? 6 : 1;
 };
 constexpr int32_t MS_MIN_CHUNK = 32;  // for queue capacities
 constexpr int MS_NCAP = 128, MS_FCAP = 128;  // LDS append buffers per wave (entries)
@@ -363,7 +365,8 @@ __device__ __forceinline__ void ms_far(const MsCtx &c, const MsCtl &a, MsEnt *ld
 }
 
 template <int W, bool SCAN>
-__global__ __launch_bounds__(MS_BLOCK) void ms_step_kernel(MsCtx c,
+__global__ __launch_bounds__(MS_BLOCK)
+__attribute__((amdgpu_waves_per_eu(MsShape<W>::MIN_WAVES))) void ms_step_kernel(MsCtx c,
                                                            const MsCtl *__restrict__ ctl) {
   const int act = ctl->action;
   if (act == ACT_NONE) return;
