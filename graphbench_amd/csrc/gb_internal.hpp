@@ -89,8 +89,12 @@ struct gb_graph_s {
     int8_t scan, per_cu, cshift;
   };
   std::vector<MsTuned> ms_tuned;
-  // batched BFS/Brandes shape {grid blocks per CU, push/pull ratio}; 0 = not tuned
-  int32_t bc_tuned[2] = {};
+  // batched BFS/Brandes shapes picked by the autotuner, per number of
+  // 64-source groups in a launch
+  struct BcTuned {
+    int32_t groups, per_cu, alpha;
+  };
+  std::vector<BcTuned> bc_tuned;
   int32_t pr_tile_items = 0;
   bool pr_tile_relabeled = false;
   int64_t pr_ntiles = 0;

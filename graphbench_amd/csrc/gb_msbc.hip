@@ -633,21 +633,49 @@ __global__ __launch_bounds__(MB) void mbc_bwd_combine_kernel(BcCtx c) {
   }
 }
 
-// bc[v] += delta[v][s] (v != source s) over groups and lanes in source order
-__global__ void mbc_accum_kernel(int32_t V, int32_t G, int32_t nsrc,
-                                 const int32_t *__restrict__ src,
-                                 const double *__restrict__ delta, double *__restrict__ bc) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  double acc = bc[v];
+// bc[v] += delta[v][s] (v != source s) over groups and lanes in source order.
+// A wave takes ACC_V consecutive vertices per group: it reads their delta rows
+// with one coalesced 512-byte load each (lane j: source j) into an LDS tile
+// (row stride 65 doubles, so that the column reads below hit distinct
+// banks), then lane i sums vertex i's row in source order.  Reading the rows
+// straight from each vertex's thread touched 64 lines per load and re-fetched
+// every line up to 16 times.
+constexpr int ACC_V = 32;
+constexpr int ACC_BLOCK = 256;
+__global__ __launch_bounds__(ACC_BLOCK) void mbc_accum_kernel(int32_t V, int32_t G, int32_t nsrc,
+                                                              const int32_t *__restrict__ src,
+                                                              const double *__restrict__ delta,
+                                                              double *__restrict__ bc) {
+  __shared__ double s_tile[ACC_BLOCK / GB_WAVE][ACC_V][GB_WAVE + 1];
+  const int lane = gb_lane(), wave = threadIdx.x / GB_WAVE;
+  const int64_t v0 = ((int64_t)blockIdx.x * (ACC_BLOCK / GB_WAVE) + wave) * ACC_V;
+  if (v0 >= V) return;  // wave-uniform: no barrier below
+  double (*tile)[GB_WAVE + 1] = s_tile[wave];
+  const int64_t v = v0 + lane;  // lanes < ACC_V sum vertex v
+  const bool mine = lane < ACC_V && v < V;
+  double acc = mine ? bc[v] : 0.0;
   for (int32_t g = 0; g < G; g++) {
-    const double *row = delta + ((int64_t)g * V + v) * 64;
-    for (int k = 0; k < 64; k++) {
-      const int32_t s = g * 64 + k;
-      if (s < nsrc && src[s] != (int32_t)v) acc += row[k];
+    const int32_t sj = g * 64 + lane < nsrc ? src[g * 64 + lane] : -1;
+    double r[ACC_V];
+#pragma unroll
+    for (int i = 0; i < ACC_V; i++) {
+      const int64_t vi = v0 + i < V ? v0 + i : V - 1;
+      r[i] = delta[((int64_t)g * V + vi) * 64 + lane];
     }
+#pragma unroll
+    for (int i = 0; i < ACC_V; i++) tile[i][lane] = r[i];
+    // the tile is this wave's alone: order its LDS writes before the reads
+    // of other lanes' values, and those reads before the next group's writes
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 0; k < 64; k++) {
+      const int32_t sk = __shfl(sj, k, GB_WAVE);  // -1 past nsrc
+      if (mine && sk >= 0 && sk != (int32_t)v) acc += tile[lane][k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
-  bc[v] = acc;
+  if (mine) bc[v] = acc;
 }
 
 struct MsWork {
@@ -959,7 +987,9 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
       if (hl.bdone) break;
     }
   }
-  hipLaunchKernelGGL(mbc_accum_kernel, dim3((V + 255) / 256), dim3(256), 0, cs, V, G, nsrc,
+  hipLaunchKernelGGL(mbc_accum_kernel,
+                     dim3((V + ACC_V * (ACC_BLOCK / GB_WAVE) - 1) / (ACC_V * (ACC_BLOCK / GB_WAVE))),
+                     dim3(ACC_BLOCK), 0, cs, V, G, nsrc,
                      w.src.as<int32_t>(), c.delta, d_bc);
   GB_HIP(hipGetLastError());
   if (own) GB_HIP(hipStreamSynchronize(cs));
@@ -972,12 +1002,17 @@ __global__ void fill_i32_kernel(int64_t n, int32_t *__restrict__ a, int32_t v) {
     a[i] = v;
 }
 
-// The shape for graph g: GB_BC_GRID (4, 8 or 16 blocks per CU) and
-// GB_BC_ALPHA (4 .. 64) fix parts; else the shape cached in the handle, else
-// the rules.
-BcShape bc_shape(gb_graph g, int *fixed_parts) {
+// The shape for a launch of G groups on graph g: GB_BC_GRID (4, 8 or 16
+// blocks per CU) and GB_BC_ALPHA (4 .. 64) fix parts; else the shape cached
+// in the handle for G groups, else the rules.
+const gb_graph_s::BcTuned *bc_cached(gb_graph g, int32_t G) {
+  for (const auto &t : g->bc_tuned)
+    if (t.groups == G) return &t;
+  return nullptr;
+}
+BcShape bc_shape(gb_graph g, int32_t G, int *fixed_parts) {
   BcShape sh = BC_RULES;
-  if (g->bc_tuned[0]) sh = BcShape{g->bc_tuned[0], g->bc_tuned[1]};
+  if (const auto *t = bc_cached(g, G)) sh = BcShape{t->per_cu, t->alpha};
   int fixed = 0;
   const char *e = getenv("GB_BC_GRID");
   if (e && (atoi(e) == 4 || atoi(e) == 8 || atoi(e) == 16)) {
@@ -993,16 +1028,18 @@ BcShape bc_shape(gb_graph g, int *fixed_parts) {
   return sh;
 }
 
-// Autotuner for the Brandes engine, run on a graph's first batch when the
-// handle has no shape yet (GB_BC_TUNE=0: rules only): the batch is solved in
-// the rules' shape, then with one change at a time (the other blocks-per-CU
-// counts, then alpha halved and doubled), into a scratch score array; the
-// fastest shape is kept in the handle.  Every shape gives the same scores.
+// Autotuner for the Brandes engine, run on a graph's first batch of G groups
+// when the handle has no shape for G yet (GB_BC_TUNE=0: rules only): the
+// batch is solved in the rules' shape, then with one change at a time (the
+// other blocks-per-CU counts, then alpha halved and doubled), into a scratch
+// score array; the fastest shape is kept in the handle.  Every shape gives
+// the same scores.
 gb_status bc_tune(gb_graph g, const int32_t *h_src, int32_t nsrc, hipStream_t st) {
+  const int32_t G = (nsrc + 63) / 64;
   int fixed = 0;
-  const BcShape base = bc_shape(g, &fixed);
+  const BcShape base = bc_shape(g, G, &fixed);
   const char *e = getenv("GB_BC_TUNE");
-  if ((e && e[0] == '0') || g->bc_tuned[0] || fixed == 3) return GB_OK;
+  if ((e && e[0] == '0') || bc_cached(g, G) || fixed == 3) return GB_OK;
   DevBuf scratch;
   GB_TRY(scratch.ensure(sizeof(double) * (g->V + 1)));
   auto timed = [&](const BcShape &s, double &ms) -> gb_status {
@@ -1037,8 +1074,7 @@ gb_status bc_tune(gb_graph g, const int32_t *h_src, int32_t nsrc, hipStream_t st
     for (int a : {a0 / 2, a0 * 2})
       if (a >= 4 && a <= 64) GB_TRY(consider(BcShape{best.per_cu, a}));
   }
-  g->bc_tuned[0] = best.per_cu;
-  g->bc_tuned[1] = best.alpha;
+  g->bc_tuned.push_back(gb_graph_s::BcTuned{G, best.per_cu, best.alpha});
   return GB_OK;
 }
 
@@ -1064,7 +1100,7 @@ extern "C" gb_status gb_bfs(gb_graph g, const int32_t *h_sources, int32_t nsrc, 
   for (int32_t b0 = 0; b0 < nsrc; b0 += per) {
     const int32_t ns = std::min(per, nsrc - b0);
     GB_TRY(ms_run(g, h_sources + b0, ns, false, nullptr, d_level + (int64_t)b0 * V, st, nullptr,
-                  bc_shape(g, nullptr)));
+                  bc_shape(g, (ns + 63) / 64, nullptr)));
   }
   GB_HIP(hipStreamSynchronize(st));
   return GB_OK;
@@ -1092,8 +1128,9 @@ extern "C" gb_status gb_bc(gb_graph g, const int32_t *h_sources, int32_t nsrc, i
   for (int32_t b0 = 0; b0 < nsrc && V > 0; b0 += (int32_t)per) {
     const int32_t ns = (int32_t)std::min<int64_t>(per, nsrc - b0);
     int32_t depth = 0;
-    if (b0 == 0) GB_TRY(bc_tune(g, h_sources, ns, st));
-    GB_TRY(ms_run(g, h_sources + b0, ns, true, d_bc, nullptr, st, &depth, bc_shape(g, nullptr)));
+    GB_TRY(bc_tune(g, h_sources + b0, ns, st));
+    GB_TRY(ms_run(g, h_sources + b0, ns, true, d_bc, nullptr, st, &depth,
+                  bc_shape(g, (ns + 63) / 64, nullptr)));
     max_depth = std::max(max_depth, depth - 1);
     batches++;
   }
