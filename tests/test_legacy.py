@@ -91,6 +91,34 @@ def test_text_readers_send_and_recv(tmp_path):
     assert rcol.tolist() == [5, 0, 0, 1, 3, 5, 3, 4]
 
 
+def test_text_reader_fscanf_semantics(tmp_path):
+    """The text readers tokenise as the reference's fscanf loop does
+    (graph/csr.c:26-35, 156-176): tokens other than a lone `p` or `a` are
+    skipped, `%lld` stops at the first non-digit (a weight `3.75` reads as 3
+    and leaves `.75` as the next token), a sign is accepted, tokens longer
+    than 99 characters are split, and edges past the header's count are not
+    read."""
+    L = legacy()
+    lib = L.lib()
+    text = ("c comment 7 8 9\n" + "c " + "x" * 150 + "\n"
+            "p sp 4 4 1 4\n"
+            "a 1 2 3.75\n"
+            "a 2 3 +5\n"
+            "a 3 4 7 extra\n"
+            "a 4 1 2\n"
+            "a 1 3 9\n")
+    (tmp_path / "g.txt").write_text(text)
+    d = (str(tmp_path) + "/").encode()
+    g = L.TCsr()
+    lib.scan_csr_idx(C.byref(g), b"g.txt", d, L.SEND)
+    lib.read_graph_csr(C.byref(g), b"g.txt", d, L.SEND)
+    rp, col, w = L.csr_arrays(g)
+    assert (g.v_size, g.e_size, g.offset) == (4, 4, 1)
+    assert rp.tolist() == [0, 1, 2, 3, 4]
+    assert col.tolist() == [1, 2, 3, 0]
+    assert w.tolist() == [2 / 4, 3 / 4, 4 / 4, 1 / 4]
+
+
 def test_binary_reader(tmp_path):
     L = legacy()
     V, off = 5, 1
