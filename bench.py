@@ -32,6 +32,8 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pr_gather_rmat22_traffic.json")
 # HBM-side bytes of one learned + batched 32-source solve at R-MAT 24, from
 # copy-calibrated FETCH_SIZE / WRITE_SIZE runs (rocprofv3 --pmc)
 SSSP_TRAFFIC_JSON = os.path.join(ROOT, "profiles", "sssp_rmat24_32src_traffic.json")
+# HBM-side bytes of the timed 256-source Brandes solve at R-MAT 20, likewise
+BC_TRAFFIC_JSON = os.path.join(ROOT, "profiles", "bc_rmat20_256src_traffic.json")
 
 
 def log(msg):
@@ -247,6 +249,14 @@ def run_bc_c3(gb, oracle, torch, np, cpu, with_cpu):
                # separate single-source solves would have to move, not a
                # bandwidth (64 sources share each col and mask read)
                per_source_equivalent_gbs=256 * B / s / 1e9, max_rel_err_vs_oracle_4src=rel)
+    if os.path.exists(BC_TRAFFIC_JSON):
+        with open(BC_TRAFFIC_JSON) as f:
+            tr = json.load(f)
+        # measured HBM-side bytes of this solve over its time: the bandwidth
+        out["traffic"] = {"bytes_per_solve": tr["traffic_bytes_per_solve"],
+                          "gbs": tr["traffic_bytes_per_solve"] / s / 1e9,
+                          "frac": tr["traffic_bytes_per_solve"] / s / 1e9 / HBM_PEAK_GBS,
+                          "source": tr["source"]}
     if with_cpu:
         threads = cpu["threads"]
         ns = min(len(srcs), 2 * threads)
