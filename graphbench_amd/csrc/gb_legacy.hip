@@ -347,6 +347,7 @@ void load_text_edges(FileGraph &fg, const char *file, const char *dir, sint type
       header = true;
     } else if (str[0] == 'a' && str[1] == '\0') {
       if (!header) fatal("'a' line before the 'p' line");
+      if ((long long)read >= e_num) break;  // e_num == 0: no edge is read
       if (!t.num(&u) || !t.num(&v) || !t.num(&w)) fatal("bad 'a' line");
       // csr.c:37-48 / 165-176: RECV keys on the target, SEND on the source
       const long long r = type == GB_LEGACY_RECV ? v - offset : u - offset;
@@ -481,6 +482,7 @@ static void scan_csr_idx_impl(t_csr *graph, char *file, char *dir, sint type) {
       header = true;
     } else if (str[0] == 'a' && str[1] == '\0') {
       if (!header) fatal("'a' line before the 'p' line");
+      if (read >= e_num) break;  // e_num == 0: no edge is counted
       if (!t.num(&u) || !t.num(&v) || !t.num(&w)) fatal("bad 'a' line");
       if (type == GB_LEGACY_RECV) {
         u -= 1;
@@ -525,18 +527,16 @@ static void read_graph_csr_impl(t_csr *graph, char *file, char *dir, sint type) 
       for (long long i = 1; i < v_num; i++) pos[i] = graph->vet_idx[i - 1];
     } else if (str[0] == 'a' && str[1] == '\0') {
       if (pos.empty()) fatal("'a' line before the 'p' line");
+      if (read >= graph->e_size) break;  // the edges scan_csr_idx counted
       if (!t.num(&u) || !t.num(&v) || !t.num(&w)) fatal("bad 'a' line");
       // csr.c:42: the weight of the read-th edge is v_to / v_num (file order)
-      if (read < graph->e_size) graph->edge_info[read].edge_weight = (double)v / (double)v_num;
-      if (type == GB_LEGACY_RECV) {
-        u -= 1;
-        v -= offset;
-        graph->edge_idx[pos[v]++] = u;
-      } else {
-        u -= offset;
-        v -= 1;
-        graph->edge_idx[pos[u]++] = v;
-      }
+      graph->edge_info[read].edge_weight = (double)v / (double)v_num;
+      // csr.c:43-56: RECV keys on the target, SEND on the source
+      const long long key = type == GB_LEGACY_RECV ? v - offset : u - offset;
+      const long long other = type == GB_LEGACY_RECV ? u - 1 : v - 1;
+      if (key < 0 || key >= v_num || key >= graph->v_size || pos[key] >= graph->e_size)
+        fatal("edge does not match the scanned row counts");
+      graph->edge_idx[pos[key]++] = other;
       if (++read == e_num) break;
     }
   }

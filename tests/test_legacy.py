@@ -119,6 +119,32 @@ def test_text_reader_fscanf_semantics(tmp_path):
     assert w.tolist() == [2 / 4, 3 / 4, 4 / 4, 1 / 4]
 
 
+def test_text_reader_zero_edges_and_bad_endpoint(tmp_path):
+    """A header with 0 edges reads none of the 'a' lines after it; an edge
+    whose endpoint lies outside the header's vertex range ends the process
+    with ERROR_PRINT's exit(123) and a message (the reference would write
+    out of bounds)."""
+    import subprocess
+    import sys
+    L = legacy()
+    lib = L.lib()
+    (tmp_path / "z.txt").write_text("p sp 3 0 1 3\na 1 2 1\na 2 3 1\n")
+    d = (str(tmp_path) + "/").encode()
+    g = L.TCsr()
+    lib.scan_csr_idx(C.byref(g), b"z.txt", d, L.SEND)
+    lib.read_graph_csr(C.byref(g), b"z.txt", d, L.SEND)
+    rp, col, _ = L.csr_arrays(g)
+    assert rp.tolist() == [0, 0, 0, 0] and len(col) == 0
+    (tmp_path / "b.txt").write_text("p sp 3 2 1 3\na 1 2 1\na 9 3 1\n")
+    code = ("import ctypes as C, sys\n"
+            f"sys.path.insert(0, {ROOT!r})\n"
+            "from graphbench_amd import legacy as L\n"
+            "g = L.TCsr()\n"
+            f"L.lib().scan_csr_idx(C.byref(g), b'b.txt', {d!r}, L.SEND)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 123 and "out of range" in r.stdout
+
+
 def test_binary_reader(tmp_path):
     L = legacy()
     V, off = 5, 1
