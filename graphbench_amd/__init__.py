@@ -117,6 +117,7 @@ _SIGS = {
     "gb_mlp_param_count": (_i32, [_i32, _i32]),
     "gb_mlp_forward": (_i32, [_p, _i32, _i32, _p, _i64, _p, _p]),
     "gb_mlp_forward_bf16": (_i32, [_p, _i32, _i32, _p, _i64, _p, _p]),
+    "gb_mlp_forward_mfma": (_i32, [_p, _i32, _i32, _p, _i64, _p, _p]),
     "gb_mlp_loss_grad": (_i32, [_p, _i32, _i32, _p, _p, _i64, _p, C.POINTER(_f64), _p]),
     "gb_mlp_adam": (_i32, [_p, _p, _p, _p, _i32, C.c_float, C.c_float, C.c_float, C.c_float,
                            _i32, _p]),

@@ -208,9 +208,8 @@ extern "C" gb_status gb_pr_warm_start(gb_graph g, const float *d_params, const f
                      g->rrp, h_norm[0], h_norm[3], h_norm[1], h_norm[4], h_norm[2], h_norm[5],
                      X.as<float>());
   GB_HIP(hipGetLastError());
-  // one row per vertex: the MFMA inference (a warm start needs no more than
-  // bf16 hidden activations; the solve is judged by its own residual)
-  GB_TRY(gb_mlp_forward_bf16(d_params, 3, 1, X.as<float>(), V, Y.as<float>(), stream));
+  // one row per vertex: the FP32 inference with the hidden layer on MFMA
+  GB_TRY(gb_mlp_forward_mfma(d_params, 3, 1, X.as<float>(), V, Y.as<float>(), stream));
   hipLaunchKernelGGL(ranks_from_pred_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, Y.as<float>(),
                      h_norm[6], h_norm[7], d_x0, part.as<double>());
   hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(1024), 0, st, part.as<double>(), nb,

@@ -320,7 +320,105 @@ __global__ __launch_bounds__(MF_BLOCK) void mlp_infer_mfma_kernel(
   }
 }
 
+// ---- the same inference in FP32 on MFMA -----------------------------------
+// v_mfma_f32_32x32x2f32 multiplies FP32 operands exactly and sums in FP32, so
+// the 64 x 64 layer stays an FP32 layer: 2 output tiles x 32 k-steps of 2
+// inputs per 32 rows.  W2 lives in registers as B operands (64 per lane) and
+// the layer-1 activations of the lane's row as A operands (32 per lane).
+// Operand layouts of v_mfma_f32_32x32x2f32 (lane l):
+//   A (32 x 2):  row l % 32, k = l / 32
+//   B (2 x 32):  col l % 32, k = l / 32
+//   D (32 x 32): as for the bf16 form above
+__global__ __launch_bounds__(MF_BLOCK) void mlp_infer_mfma_f32_kernel(
+    const float *__restrict__ P, int din, int dout, const float *__restrict__ X, int64_t n,
+    float *__restrict__ Y) {
+  __shared__ float s_p[H * MAXIN + H + H + MAXOUT * H + MAXOUT];  // W1 b1 b2 W3 b3
+  __shared__ float s_h[MF_BLOCK / GB_WAVE][MF_ROWS * MF_LD];
+  const Lay L = layout(din, dout);
+  float *sW1 = s_p, *sb1 = sW1 + H * din, *sb2 = sb1 + H, *sW3 = sb2 + H, *sb3 = sW3 + dout * H;
+  for (int i = threadIdx.x; i < H * din; i += blockDim.x) sW1[i] = P[L.oW1 + i];
+  for (int i = threadIdx.x; i < H; i += blockDim.x) {
+    sb1[i] = P[L.ob1 + i];
+    sb2[i] = P[L.ob2 + i];
+  }
+  for (int i = threadIdx.x; i < dout * H; i += blockDim.x) sW3[i] = P[L.oW3 + i];
+  for (int i = threadIdx.x; i < dout; i += blockDim.x) sb3[i] = P[L.ob3 + i];
+  __syncthreads();
+  const int lane = gb_lane(), half = lane >> 5, c32 = lane & 31;
+  float *sh = s_h[threadIdx.x / GB_WAVE];
+  // W2 as B operands: tile nt (hidden outputs 32 nt ..), k-step ks (inputs 2 ks ..)
+  float bw[2][32];
+#pragma unroll
+  for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+    for (int ks = 0; ks < 32; ks++) bw[nt][ks] = P[L.oW2 + (32 * nt + c32) * H + 2 * ks + half];
+  const int64_t tiles = (n + MF_ROWS - 1) / MF_ROWS;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t tl = wave; tl < tiles; tl += nwaves) {
+    const int64_t r0 = tl * MF_ROWS;
+    int64_t row = r0 + c32;
+    if (row > n - 1) row = n - 1;  // rows past the end recompute the last row
+    float x[MAXIN];
+#pragma unroll
+    for (int d = 0; d < MAXIN; d++) x[d] = d < din ? X[row * din + d] : 0.0f;
+    // layer 1 into A operands: hidden unit k = 2 ks + half of this row
+    float a[32];
+#pragma unroll
+    for (int ks = 0; ks < 32; ks++) {
+      const int k = 2 * ks + half;
+      float z = sb1[k];
+      for (int d = 0; d < din; d++) z += sW1[k * din + d] * x[d];
+      a[ks] = z > 0.0f ? z : 0.0f;
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) {
+      mf_f32x16 acc = {};
+#pragma unroll
+      for (int ks = 0; ks < 32; ks++)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ks], bw[nt][ks], acc, 0, 0, 0);
+      const int col = 32 * nt + c32;
+      const float b = sb2[col];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int i = 8 * (r / 4) + 4 * half + (r % 4);
+        const float z = acc[r] + b;
+        sh[i * MF_LD + col] = z > 0.0f ? z : 0.0f;
+      }
+    }
+    // a wave's LDS accesses execute in order: its lanes' writes above are seen
+    // by the reads below
+    __builtin_amdgcn_wave_barrier();
+    float y[MAXOUT];
+#pragma unroll
+    for (int o = 0; o < MAXOUT; o++) {
+      float s = 0.0f;
+      if (o < dout)
+        for (int j = 0; j < 32; j++) s += sW3[o * H + 32 * half + j] * sh[c32 * MF_LD + 32 * half + j];
+      y[o] = s + __shfl_xor(s, 32, GB_WAVE);
+    }
+    __builtin_amdgcn_wave_barrier();  // the tile is rewritten next round
+    const int64_t out_row = r0 + c32;
+    if (half == 0 && out_row < n)
+      for (int o = 0; o < dout; o++) Y[out_row * dout + o] = y[o] + sb3[o];
+  }
+}
+
 }  // namespace
+
+extern "C" gb_status gb_mlp_forward_mfma(const float *d_params, int32_t din, int32_t dout,
+                                         const float *d_X, int64_t n, float *d_Y, void *stream) {
+  if (!dims_ok(din, dout) || !d_params || (n > 0 && (!d_X || !d_Y)))
+    return gb_fail(GB_ERR_INVALID, "gb_mlp_forward_mfma: bad arguments");
+  if (n == 0) return GB_OK;
+  const int64_t tiles = (n + MF_ROWS - 1) / MF_ROWS;
+  const int64_t blocks = std::max<int64_t>(
+      1, std::min<int64_t>((tiles + 3) / 4, (int64_t)gb_num_cus() * 8));
+  hipLaunchKernelGGL(mlp_infer_mfma_f32_kernel, dim3((unsigned)blocks), dim3(MF_BLOCK), 0,
+                     gb_stream(stream), d_params, din, dout, d_X, n, d_Y);
+  GB_HIP(hipGetLastError());
+  return GB_OK;
+}
 
 extern "C" gb_status gb_mlp_forward_bf16(const float *d_params, int32_t din, int32_t dout,
                                          const float *d_X, int64_t n, float *d_Y, void *stream) {

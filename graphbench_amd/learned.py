@@ -113,12 +113,14 @@ class MLP:
         _check(lib().gb_mlp_adam(_ptr(self.params), _ptr(self.grad), _ptr(self.m), _ptr(self.v),
                                  self.total, lr, b1, b2, eps, self.step, _stream()))
 
-    def forward(self, X, bf16: bool = False):
-        """FP32 forward, or with bf16=True the MFMA inference (hidden 64 x 64
-        layer in bf16 with FP32 sums)."""
+    def forward(self, X, bf16: bool = False, mfma: bool = False):
+        """FP32 forward (one thread per row); mfma=True: the same in FP32 with
+        the hidden 64 x 64 layer on MFMA; bf16=True: that layer on bf16 MFMA
+        with FP32 sums."""
         torch = _torch()
         Y = torch.empty((X.shape[0], self.dout), dtype=torch.float32, device="cuda")
-        fn = lib().gb_mlp_forward_bf16 if bf16 else lib().gb_mlp_forward
+        fn = (lib().gb_mlp_forward_bf16 if bf16 else
+              lib().gb_mlp_forward_mfma if mfma else lib().gb_mlp_forward)
         _check(fn(_ptr(self.params), self.din, self.dout, _ptr(X), X.shape[0], _ptr(Y), _stream()))
         return Y
 

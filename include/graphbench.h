@@ -275,6 +275,13 @@ int32_t gb_mlp_param_count(int32_t din, int32_t dout);
 /* Y[n][dout] = MLP(X[n][din]) */
 gb_status gb_mlp_forward(const float *d_params, int32_t din, int32_t dout,
                          const float *d_X, int64_t n, float *d_Y, void *stream);
+/* The same inference with the 64 x 64 hidden layer on FP32 MFMA
+ * (v_mfma_f32_32x32x2f32: exact FP32 products, FP32 sums; layers 1 and 3 on
+ * the vector units): for many rows, e.g. one per vertex.  Results agree with
+ * gb_mlp_forward to FP32 summation order.  The PageRank warm start uses it.
+ * No counterpart in the reference (the learned models are this library's). */
+gb_status gb_mlp_forward_mfma(const float *d_params, int32_t din, int32_t dout,
+                              const float *d_X, int64_t n, float *d_Y, void *stream);
 /* The same inference with the 64 x 64 hidden layer on bf16 MFMA (FP32 sums,
  * layers 1 and 3 in FP32): for many rows, e.g. one per vertex.  Results agree
  * with gb_mlp_forward to bf16 rounding of the hidden activations and W2.

@@ -136,6 +136,26 @@ def test_mlp_forward_bf16_mfma(gpu, din, dout, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("din,dout,n", [(3, 1, 10007), (6, 2, 4096), (16, 4, 33)])
+def test_mlp_forward_f32_mfma(gpu, din, dout, n):
+    """The FP32 MFMA inference (the warm start's) against an FP64 forward and
+    the per-row FP32 forward: any slip in the MFMA operand or result layouts
+    mixes hidden units and fails it; otherwise only the FP32 summation order
+    differs."""
+    torch = _torch()
+    m = learned.MLP(din, dout, seed=13)
+    X, _ = _data(n, din, dout, seed=14)
+    Y = m.forward(torch.from_numpy(X).cuda(), mfma=True).cpu().numpy()
+    Y32 = m.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    W1, b1, W2, b2, W3, b3 = [a.astype(np.float64) for a in _np_params(m)]
+    h1 = np.maximum(X.astype(np.float64) @ W1.T + b1, 0)
+    ref = np.maximum(h1 @ W2.T + b2, 0) @ W3.T + b3
+    scale = max(1.0, float(np.abs(ref).max()))
+    np.testing.assert_allclose(Y, ref, rtol=1e-5, atol=1e-5 * scale)
+    np.testing.assert_allclose(Y, Y32, rtol=1e-5, atol=1e-5 * scale)
+
+
+@pytest.mark.gpu
 def test_mlp_adam_vs_torch_optim(gpu):
     torch = _torch()
     m = learned.MLP(3, 1, seed=5)
