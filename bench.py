@@ -545,6 +545,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": pr["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": pr["achieved_gbs"] / HBM_PEAK_GBS,
+                     # what the gather's counters show it waits on (the profile
+                     # named in traffic_source): random 4-byte gathers served
+                     # by the L2, not HBM bytes
+                     "limiter": traffic.get("limiter"),
                      "traffic": traffic.get("traffic_bytes_per_launch"),
                      "traffic_source": traffic.get("source"),
                      "kernel": traffic.get("kernel", "pr_gather"),
