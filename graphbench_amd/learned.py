@@ -43,7 +43,7 @@ MODELS_PATH = os.environ.get(
     "GB_LEARNED_MODELS",
     os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_models.json"))
 HIDDEN = 64
-DELTA_DIN = 7  # the 6 graph features and the engine flag
+DELTA_DIN = 8  # the 6 graph features, the engine flag, the solve's largest graph
 
 
 def batched_engine(nsrc: int) -> bool:
@@ -53,11 +53,16 @@ def batched_engine(nsrc: int) -> bool:
     return nsrc >= 2 and not (e and e != "0")
 
 
-def delta_inputs(F: np.ndarray, per_source_engine: bool) -> np.ndarray:
-    """Delta-model inputs: graph features (n, 6) and the engine flag."""
+def delta_inputs(F: np.ndarray, per_source_engine: bool, lockstep: bool = False) -> np.ndarray:
+    """Delta-model inputs: graph features (n, 6), the engine flag, and, for a
+    lockstep batch of graphs (the rows of F), log2 of its largest edge count
+    (log2 V + log2 avg degree), 0 for a single graph.  A lockstep batch
+    advances its buckets together, so its best common scale depends on its
+    largest graphs as much as on each graph's own shape."""
     F = np.atleast_2d(np.asarray(F, np.float32))
     flag = np.full((len(F), 1), 1.0 if per_source_engine else 0.0, np.float32)
-    return np.concatenate([F, flag], axis=1)
+    big = np.full((len(F), 1), float(np.max(F[:, 0] + F[:, 1])) if lockstep else 0.0, np.float32)
+    return np.concatenate([F, flag, big], axis=1)
 
 
 def _torch():
@@ -172,14 +177,19 @@ def _time_sssp(g, srcs, delta, light, reps=1):
 
 
 def _target(rows):
-    """The mean (log2 delta, log2 light_max/delta) of the settings within 5%
-    of the fastest and within 1.5 of its log2 delta (the time-vs-delta curve
-    can have two separate minima; averaging across them would land between
-    them)."""
-    kb, _, best = min(rows, key=lambda r: r[2])
-    good = [(k, lr) for k, lr, t in rows if t <= 1.05 * best and abs(k - kb) <= 1.5]
-    return np.array([np.mean([k for k, _ in good]), np.mean([lr for _, lr in good])],
-                    np.float32), best
+    """(log2 delta, log2 light_max/delta) from a sweep's (k, ratio, time) rows:
+    one step above the smallest log2 delta within 3% of the fastest, at most
+    the largest one within 5%, and the mean light ratio of the settings within
+    5% at most 1 away from it.  Many time-vs-delta curves are flat beyond a
+    knee (one bucket holds every distance, or buckets stop mattering), and the
+    fastest point inside the flat part is noise: the knee is what carries over
+    between graphs."""
+    best = min(r[2] for r in rows)
+    near = [k for k, _, t in rows if t <= 1.03 * best]
+    good = [(k, lr) for k, lr, t in rows if t <= 1.05 * best]
+    kt = min(min(near) + 1.0, max(k for k, _ in good))
+    lrs = [lr for k, lr in good if abs(k - kt) <= 1.0] or [lr for _, lr in good]
+    return np.array([kt, np.mean(lrs)], np.float32), best
 
 
 def delta_sweep(h: HostCSR, g: Graph, nsrc: int, seed: int, kmax: int = 18, kmin: int = 2):
@@ -223,7 +233,8 @@ def multi_specs(quick: bool = False):
     uniform in [1, whi]."""
     specs = [(512, 10, 16, 1023, 7100), (512, 10, 17, 255, 7200), (256, 11, 18, 1023, 7300),
              (128, 12, 19, 65535, 7400), (64, 12, 20, 1023, 7500), (32, 14, 20, 255, 7600),
-             (512, 10, 20, 1023, 7700)]
+             (512, 10, 20, 1023, 7700), (512, 10, 18, 1023, 7800), (256, 10, 20, 255, 7900),
+             (1024, 10, 15, 1023, 8000), (384, 12, 20, 65535, 8100)]
     return specs[:2] if quick else specs
 
 
@@ -278,7 +289,7 @@ def multi_sweep(spec, kmin: int = -3, kmax: int = 9, rows_per_batch: int = 64):
                 rows.append((k, lr, run(k, lr)))
     target, best = _target(rows)
     t_plain = [t for k, lr, t in rows if k == 0.0 and lr == 0.0][0]
-    F = delta_inputs(graph_features_multi(gs, n), True)
+    F = delta_inputs(graph_features_multi(gs, n), True, lockstep=True)
     pick = rng.choice(n, size=min(n, rows_per_batch), replace=False)
     X = F[pick]
     T = np.stack([np.full(len(pick), target[0]), np.full(len(pick), target[1])],
@@ -333,7 +344,10 @@ class LearnedModels:
     def _predict(self, X, light: bool):
         torch = _torch()
         d = self.data["delta"]
-        xs = (X - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
+        # inputs outside the swept range are held at its edge (a graph larger
+        # than any swept one is predicted as the largest swept one)
+        Xc = np.clip(X, np.array(d["x_min"], np.float32), np.array(d["x_max"], np.float32))
+        xs = (Xc - np.array(d["x_mean"], np.float32)) / np.array(d["x_std"], np.float32)
         Y = self.delta_mlp.forward(torch.from_numpy(xs.astype(np.float32)).cuda()).cpu().numpy()
         Y = Y * np.array(d["y_std"]) + np.array(d["y_mean"])
         out = []
@@ -359,7 +373,7 @@ class LearnedModels:
         batch takes the median of the per-graph predictions (scale and light
         ratio) and applies it to each graph's plain width."""
         F = graph_features_multi(graphs, max(1, len(graphs) * nsrc))
-        X = delta_inputs(F, True)
+        X = delta_inputs(F, True, lockstep=True)
         per_graph = self._predict(X, True)
         plains = [self._plain(x) for x in X]
         scale = float(np.median([math.log2(d / p) for (d, _), p in zip(per_graph, plains)]))
@@ -475,9 +489,12 @@ def collect_delta_data(specs, cache_path: Optional[str], log=print):
 
 
 def train_all(out_path: str, quick: bool = False, log=print, cache_path: Optional[str] = None):
-    """Generates the training data on the GPU and fits both models."""
+    """Generates the training data on the GPU and fits both models.  The
+    batched engine's launch-shape autotuner is off during the sweeps (its
+    first solve per delta would be timed); the models only pick delta."""
     torch = _torch()
     t0 = time.time()
+    os.environ["GB_MSSSP_TUNE"] = "0"
     recs = collect_delta_data(delta_specs(quick), cache_path, log)
     feats = [r["features"] for r in recs]
     targets = [r["target"] for r in recs]
@@ -485,8 +502,10 @@ def train_all(out_path: str, quick: bool = False, log=print, cache_path: Optiona
     info = [dict(spec=r["spec"], nsrc=r["nsrc"], target=r["target"], best_ms=r["best_ms"],
                  plain_ms=r["plain_ms"]) for r in recs]
     ngroups = max(groups) + 1 if groups else 0
+    # 8 graphs per lockstep batch: a batch's target is one common scale, and
+    # more rows of it would outweigh the single-graph sweeps in the loss
     for j, spec in enumerate(multi_specs(quick)):
-        X, T, meta = multi_sweep(spec)
+        X, T, meta = multi_sweep(spec, rows_per_batch=8)
         feats.extend(X.tolist())
         targets.extend(T.tolist())
         groups.extend([ngroups + j] * len(X))
@@ -504,7 +523,8 @@ def train_all(out_path: str, quick: bool = False, log=print, cache_path: Optiona
                                          np.array(groups), 600 if quick else 2000, 1, log)
     log(f"delta model: loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     delta_json = dict(din=DELTA_DIN, dout=2, params=dm.params.cpu().numpy().tolist(),
-                      x_mean=xm.tolist(), x_std=xs.tolist(), y_mean=ym.tolist(),
+                      x_mean=xm.tolist(), x_std=xs.tolist(), x_min=F.min(axis=0).tolist(),
+                      x_max=F.max(axis=0).tolist(), y_mean=ym.tolist(),
                       y_std=ys.tolist(), final_loss=losses[-1], holdout=hold, samples=info)
     # ---- warm-start model: converged ranks
     Xs, Ys, Gs = [], [], []
@@ -531,7 +551,7 @@ def train_all(out_path: str, quick: bool = False, log=print, cache_path: Optiona
     pr_json = dict(din=3, dout=1, params=pm.params.cpu().numpy().tolist(), x_mean=xm2.tolist(),
                    x_std=xs2.tolist(), y_mean=ym2.tolist(), y_std=ys2.tolist(),
                    final_loss=losses2[-1], holdout=hold2, samples=int(len(X)))
-    data = dict(format="graphbench-mi355x learned models v3", delta=delta_json,
+    data = dict(format="graphbench-mi355x learned models v4", delta=delta_json,
                 warm_start=pr_json, train_seconds=time.time() - t0)
     with open(out_path, "w") as f:
         json.dump(data, f)
