@@ -126,7 +126,7 @@ def run_pagerank(gb, torch, np, args):
     g = gb.Graph.from_host(h, reverse=r)
     V, E = h.V, h.E
     kw = dict(damping=0.85, tol=1e-7, max_iter=100, items_per_thread=args.ipt,
-              relabel=args.relabel, row_block=args.row_block)
+              relabel=args.relabel, row_block=args.row_block, binned=args.binned)
     # first call: builds the per-graph caches (1/outdeg, out-degree relabelling,
     # merge-path tiles) and autotunes the gather shape; reported, not timed
     torch.cuda.synchronize()
@@ -165,7 +165,7 @@ def run_pagerank(gb, torch, np, args):
     out = dict(V=V, E=E, iterations=it, residual=res["residual"], step_s=step_s,
                first_call_ms=prep_ms, items_per_thread=res["items_per_thread"],
                hot=res["hot"], relabeled=res["relabeled"], row_block=res["row_block"],
-               gteps=gteps, gather_ms_avg=avg_gather_s * 1e3, gather_launches=gl,
+               binned=res["binned"], gteps=gteps, gather_ms_avg=avg_gather_s * 1e3, gather_launches=gl,
                tiles=res["tiles"], bytes_per_launch=B, achieved_gbs=achieved,
                graph_step_ms=graph_step_s * 1e3, graph_gteps=E * resg["iterations"] / graph_step_s / 1e9)
     x_host = x.cpu().numpy().astype(np.float64)
@@ -477,6 +477,8 @@ def main():
                     help="PageRank gather in out-degree order: 0 auto, 1 on, -1 off")
     ap.add_argument("--row-block", type=int, default=0,
                     help="PageRank gather form: 0 tuned, 1024/2048/4096 row blocks, -1 merge path")
+    ap.add_argument("--binned", type=int, default=0,
+                    help="PageRank binned form: 0 tuned, -1 never, 1/2/3 that chunk x bin shape")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0,
@@ -545,9 +547,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": pr["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": pr["achieved_gbs"] / HBM_PEAK_GBS,
-                     # what the gather's counters show it waits on (the profile
-                     # named in traffic_source): random 4-byte gathers served
-                     # by the L2, not HBM bytes
+                     # what the gather's counters show bounds it (the profile
+                     # named in traffic_source)
                      "limiter": traffic.get("limiter"),
                      "traffic": traffic.get("traffic_bytes_per_launch"),
                      "traffic_source": traffic.get("source"),
@@ -555,7 +556,8 @@ def main():
                      "bytes_per_launch": pr["bytes_per_launch"],
                      "mean_launch_ms": pr["gather_ms_avg"],
                      "shape": {"items_per_thread": pr["items_per_thread"], "hot": pr["hot"],
-                               "row_block": pr["row_block"], "relabeled": pr["relabeled"]},
+                               "row_block": pr["row_block"], "binned": pr["binned"],
+                               "relabeled": pr["relabeled"]},
                      "first_call_ms": pr["first_call_ms"]},
         "cpu_baseline": cpu_pr,
         "extra": extra,

@@ -70,7 +70,7 @@ class _PrParams(C.Structure):
     _fields_ = [("damping", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int32),
                 ("items_per_thread", C.c_int32), ("use_graph", C.c_int32),
                 ("time_kernels", C.c_int32), ("d_x0", C.c_void_p), ("relabel", C.c_int32),
-                ("row_block", C.c_int32)]
+                ("row_block", C.c_int32), ("binned", C.c_int32)]
 
 
 class _PrResult(C.Structure):
@@ -78,7 +78,7 @@ class _PrResult(C.Structure):
                 ("dangling", C.c_double), ("gather_ms", C.c_double),
                 ("gather_launches", C.c_int32), ("tiles", C.c_int32),
                 ("items_per_thread", C.c_int32), ("hot", C.c_int32), ("relabeled", C.c_int32),
-                ("row_block", C.c_int32)]
+                ("row_block", C.c_int32), ("binned", C.c_int32)]
 
 
 class _BcStats(C.Structure):
@@ -371,19 +371,21 @@ def sssp_multi(graphs: Sequence[Graph], sources: Sequence[int], deltas=None, lig
 
 def pagerank(g: Graph, damping: float = 0.85, tol: float = 1e-7, max_iter: int = 100,
              items_per_thread: int = 0, use_graph: bool = True, time_kernels: bool = False,
-             x0=None, out=None, relabel: int = 0, row_block: int = 0):
-    """Configured PageRank (FP32 ranks, compensated row sums, FP64 residual).
+             x0=None, out=None, relabel: int = 0, row_block: int = 0, binned: int = 0):
+    """Configured PageRank (FP32 ranks, compensated or fixed-point row sums,
+    FP64 residual).
 
     relabel: 0 auto (on for E >= 2^20), 1 on, -1 off -- gather in out-degree
     order (built once per graph; results are in the caller's vertex ids).
     row_block: 0 autotuned form, 1024/2048/4096 row blocks of that many
-    edges, -1 merge path only."""
+    edges, -1 merge path only.  binned: 0 left to the autotuner, -1 never,
+    1/2/3 the binned (propagation-blocking) form with that chunk x bin shape."""
     torch = _torch()
     if out is None:
         out = torch.empty(g.V, dtype=torch.float32, device="cuda")
     p = _PrParams(damping, tol, max_iter, items_per_thread, 1 if use_graph else 0,
                   1 if time_kernels else 0, x0.data_ptr() if x0 is not None else None,
-                  relabel, row_block)
+                  relabel, row_block, binned)
     r = _PrResult()
     _check(lib().gb_pagerank(g.handle, C.byref(p), _ptr(out), C.byref(r), _stream()))
     return out, {k: getattr(r, k) for k, _ in _PrResult._fields_}

@@ -77,11 +77,12 @@ struct gb_graph_s {
   bool max_w_valid = false;
   DevBuf inv_outdeg;  // float[V]
   bool inv_outdeg_valid = false;
-  // PageRank gather shape {items per thread, hot-set size, row-block edges}
-  // picked by the autotuner, per (fixed items per thread or none, relabelled
-  // or not, row blocks among the candidates or not); merge-path form when
-  // row-block edges is 0; all 0 = not tuned yet
-  int32_t pr_tuned[20][3] = {};
+  // PageRank gather shape {items per thread, hot-set size, row-block edges,
+  // binned shape} picked by the autotuner, per (fixed items per thread or
+  // none, relabelled or not, row blocks among the candidates or not, binned
+  // among them or not); merge-path form when the last two are 0; all 0 = not
+  // tuned yet
+  int32_t pr_tuned[40][4] = {};
   // batched SSSP shapes picked by the autotuner, per (group width, delta)
   struct MsTuned {
     int32_t W;
@@ -105,6 +106,15 @@ struct gb_graph_s {
   int64_t pr_rb_nblocks = 0;
   int32_t pr_rb_nlong = 0;
   DevBuf pr_rb, pr_rb_long;
+  // PageRank binned form (gb_pagerank.hip): per (layout, chunk and bin
+  // widths) the edge streams and work lists, built once per handle
+  struct PrBinned {
+    bool valid = false, relabeled = false;
+    int32_t csl = 0, bsl = 0, piece = 0;  // log2 chunk / bin widths, piece entries
+    int64_t padded = 0;                   // stream entries (segments padded to 4)
+    int64_t ntasks = 0, nunits = 0, npieces = 0, nmulti = 0;
+    DevBuf src16, dst16, vals, tasks, units, pieces, multi, sum64;
+  } pr_pb;
   // PageRank relabelling: vertices renumbered by out-degree (descending), the
   // in-CSR and 1/outdeg in the new numbering (gb_graph_pr_relabel)
   bool rl_valid = false;

@@ -23,6 +23,9 @@
 // an optional LDS copy of the hottest contributions) is autotuned per graph.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/device/device_radix_sort.hpp>
+
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -43,6 +46,7 @@ struct PrState {
   int32_t done;
   int32_t max_iter;
   int32_t V;
+  int32_t fx_shift;  // binned form: sums in fixed point, value * 2^fx_shift
 };
 
 __global__ void inv_outdeg_kernel(int32_t V, const int32_t *__restrict__ rp,
@@ -81,30 +85,51 @@ __device__ __forceinline__ double block_sum_f64(double v, double *s_red) {
 __global__ __launch_bounds__(PR_BLOCK) void pr_init_kernel(
     int32_t V, const float *__restrict__ x0, const int32_t *__restrict__ perm,
     const float *__restrict__ inv_od, float *__restrict__ x, float *__restrict__ contrib,
-    double *__restrict__ part_dang) {
+    double *__restrict__ part_dang, double *__restrict__ part_mass) {
   __shared__ double s_red[PR_BLOCK / 64];
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double dang = 0.0;
+  double dang = 0.0, mass = 0.0;
   if (v < V) {
     float xv = x0 ? x0[perm ? perm[v] : v] : (float)(1.0 / (double)V);
     float io = inv_od[v];
     x[v] = xv;
     contrib[v] = xv * io;
     if (io == 0.0f) dang = (double)xv;
+    mass = fabs((double)xv);
   }
   double t = block_sum_f64(dang, s_red);
-  if (threadIdx.x == 0) part_dang[blockIdx.x] = t;
+  double m = block_sum_f64(mass, s_red);
+  if (threadIdx.x == 0) {
+    part_dang[blockIdx.x] = t;
+    part_mass[blockIdx.x] = m;
+  }
 }
 
+// Sets the first base and the binned form's fixed-point shift.  The L1 mass
+// M = sum |x| bounds every contribution and every row sum, and after an
+// iteration M' <= (1 - d) + d * M <= max(1, M), so with 2^e >= M (e >= 0)
+// every contribution, row sum and partial sum stays at most 2^50 in fixed
+// point (value * 2^fx_shift) for the whole solve: fx_shift = 50 - e.  Each
+// contribution is rounded to 2^-51 of the mass bound, whatever the start's
+// scale.
 __global__ void pr_init_finalize_kernel(int32_t nparts, const double *__restrict__ part_dang,
+                                        const double *__restrict__ part_mass,
                                         PrState *__restrict__ st) {
   __shared__ double s_red[1024 / 64];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) a += part_dang[i];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    a += part_dang[i];
+    b += part_mass[i];
+  }
   double dang = block_sum_f64(a, s_red);
+  double mass = block_sum_f64(b, s_red);
   if (threadIdx.x == 0) {
     const double N = (double)st->V;
     const double d = (double)st->damping;
+    int e = 0;
+    const double mb = mass * (1.0 + 1.0 / 1024.0);  // FP32 rounding of later iterations
+    while (e < 60 && ldexp(1.0, e) < mb) ++e;
+    st->fx_shift = 50 - e;
     st->dangling = dang;
     st->base = (float)((1.0 - d) / N + d * dang / N);
     st->iter = 0;
@@ -765,6 +790,316 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_rb_fixup_kernel(
   }
 }
 
+// ---- binned form -----------------------------------------------------------
+//
+// The pull gather reads contrib[u] once per in-edge at a random address: one
+// L2 request per edge, and the L2's request rate bounds it.  The binned form
+// (propagation blocking) streams instead.  Sources are cut into chunks of
+// 2^CSL ids, destinations into bins of 2^BSL ids; the edges of each (chunk,
+// bin) pair form a segment, padded to a multiple of 4 entries.
+//   pb_scatter  one block per unit (a run of one chunk's segments): the
+//               chunk's contributions are copied into LDS, then each wave
+//               task (<= 256 entries of one segment) reads 4 u16 local source
+//               ids per lane and writes the 4 contributions to the value
+//               stream, where segments lie in (bin, chunk) order.
+//   pb_accum    one block per piece (a run of one bin's stream): 4 values and
+//               4 u16 local destination ids per lane, added into 2^BSL int64
+//               LDS accumulators.  A bin held by one piece finishes its rows
+//               in the block; a bin split over several pieces adds its sums
+//               to sum64 with integer atomics and
+//   pb_fixup    finishes those rows, sums the per-wave residual and dangling
+//               partials and sets the next base and stop flag (as the other
+//               forms' fix-ups do).
+// Sums are fixed point (value * 2^fx_shift, rounded to nearest, int64), so
+// the order of the LDS and global atomics changes nothing: results are
+// run-to-run identical, and each row sum is exact up to one rounding per
+// edge of 2^-51 of the mass bound (2^-51 for ranks summing to 1) and the
+// final rounding to FP32.
+constexpr int PB_TASK = 256;  // stream entries per wave task, 4 per lane
+constexpr int PB_SCATTER_BLOCK = 512;
+constexpr int PB_ACCUM_BLOCK = 1024;
+constexpr int PB_ACCUM_WAVES = PB_ACCUM_BLOCK / GB_WAVE;
+constexpr uint16_t PB_PAD = 0xFFFF;  // padding entry of the source stream
+
+struct PbTask {
+  int32_t in0, out0, n;  // n entries (multiple of 4) at src16[in0..], vals[out0..]
+};
+struct PbUnit {
+  int32_t src0, t0, t1;  // chunk's first source, tasks [t0, t1)
+};
+// Stream entries [e0, e1) of bin; multi: the bin is split over pieces.  The
+// bin's first segment (chunk 0: the first 2^CSL sources, the hubs once the
+// graph is relabelled) ends at hub_e1 and is read from src16 at e + hub_off:
+// its contributions come from an LDS copy, the scatter skips chunk 0.
+struct PbPiece {
+  int32_t bin, e0, e1, multi, hub_e1, hub_off;
+};
+typedef uint32_t pb_u32x2 __attribute__((ext_vector_type(2)));  // 4 u16 ids
+typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ pb_u32x2 pb_ids4(const uint16_t *p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const pb_u32x2 *>(p));
+}
+__device__ __forceinline__ pb_f32x4 pb_vals4(const float *p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const pb_f32x4 *>(p));
+}
+
+// round(v * scale) as int64 (nearest, ties to even), scale a power of two
+// with |v| * scale < 2^51: the product is exact in FP64, and adding 1.5 * 2^52
+// leaves the rounded integer in the low mantissa bits.
+__device__ __forceinline__ long long pb_fixed(float v, double scale) {
+  const double y = (double)v * scale + 6755399441055744.0;
+  return __double_as_longlong(y) - 0x4338000000000000LL;
+}
+
+__device__ __forceinline__ float pb_value(unsigned long long a, int S) {
+  return (float)ldexp((double)(long long)a, -S);
+}
+
+// contributions of 4 local source ids (padding ids, >= CS, give 0)
+template <int CS>
+__device__ __forceinline__ pb_f32x4 pb_lookup4(const float *s_c, pb_u32x2 s) {
+  const uint32_t i0 = s.x & 0xFFFFu, i1 = s.x >> 16, i2 = s.y & 0xFFFFu, i3 = s.y >> 16;
+  pb_f32x4 o;
+  o.x = i0 < (uint32_t)CS ? s_c[i0] : 0.0f;
+  o.y = i1 < (uint32_t)CS ? s_c[i1] : 0.0f;
+  o.z = i2 < (uint32_t)CS ? s_c[i2] : 0.0f;
+  o.w = i3 < (uint32_t)CS ? s_c[i3] : 0.0f;
+  return o;
+}
+
+template <int CSL>
+__global__ __launch_bounds__(PB_SCATTER_BLOCK) void pb_scatter_kernel(
+    const PbUnit *__restrict__ units, const PbTask *__restrict__ tasks,
+    const uint16_t *__restrict__ src16, const float *__restrict__ contrib,
+    float *__restrict__ vals, const PrState *__restrict__ st) {
+  constexpr int CS = 1 << CSL;
+  constexpr int PER = CS / PB_SCATTER_BLOCK;
+  constexpr int WAVES = PB_SCATTER_BLOCK / GB_WAVE;
+  __shared__ float s_c[CS];
+  if (st->done) return;
+  const PbUnit u = units[blockIdx.x];
+  // the chunk's contributions (the last chunk may be short: clamped reads,
+  // entries past it are never indexed)
+  const int32_t n = min(CS, st->V - u.src0);
+  float c[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = (int)threadIdx.x + j * PB_SCATTER_BLOCK;
+    c[j] = contrib[u.src0 + (k < n ? k : n - 1)];
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s_c[threadIdx.x + j * PB_SCATTER_BLOCK] = c[j];
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / GB_WAVE);
+  const int lane = gb_lane(), l4 = lane * 4;
+  // a wave takes 64 consecutive tasks (one descriptor per lane, one load) and
+  // works through them R at a time, their id loads in flight together
+  constexpr int R = 8;
+  for (int32_t base = u.t0 + wave * GB_WAVE; base < u.t1; base += WAVES * GB_WAVE) {
+    const int nt = min(GB_WAVE, u.t1 - base);
+    PbTask my = {0, 0, 0};
+    if (lane < nt) my = tasks[base + lane];
+    for (int j = 0; j < nt; j += R) {
+      int32_t in0[R], out0[R], n[R];
+      pb_u32x2 ids[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int q = (j + k) & (GB_WAVE - 1);
+        n[k] = j + k < nt ? __builtin_amdgcn_readlane(my.n, q) : 0;
+        in0[k] = __builtin_amdgcn_readlane(my.in0, q);
+        out0[k] = __builtin_amdgcn_readlane(my.out0, q);
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (l4 < n[k]) ids[k] = pb_ids4(src16 + in0[k] + l4);
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (l4 < n[k])
+          *reinterpret_cast<pb_f32x4 *>(vals + out0[k] + l4) = pb_lookup4<CS>(s_c, ids[k]);
+    }
+  }
+}
+
+template <int BSL>
+__device__ __forceinline__ void pb_add4(unsigned long long *s_acc, pb_f32x4 v, pb_u32x2 d,
+                                        double scale) {
+  constexpr uint32_t M = (1u << BSL) - 1;
+  const long long f0 = pb_fixed(v.x, scale), f1 = pb_fixed(v.y, scale);
+  const long long f2 = pb_fixed(v.z, scale), f3 = pb_fixed(v.w, scale);
+  if (f0) atomicAdd(&s_acc[d.x & M], (unsigned long long)f0);
+  if (f1) atomicAdd(&s_acc[(d.x >> 16) & M], (unsigned long long)f1);
+  if (f2) atomicAdd(&s_acc[d.y & M], (unsigned long long)f2);
+  if (f3) atomicAdd(&s_acc[(d.y >> 16) & M], (unsigned long long)f3);
+}
+
+template <int CSL, int BSL>
+__global__ __launch_bounds__(PB_ACCUM_BLOCK) void pb_accum_kernel(
+    const PbPiece *__restrict__ pieces, const float *__restrict__ vals,
+    const uint16_t *__restrict__ src16, const uint16_t *__restrict__ dst16,
+    const float *__restrict__ contrib, RowOut out, unsigned long long *__restrict__ sum64,
+    double *__restrict__ part_res, double *__restrict__ part_dang,
+    const PrState *__restrict__ st) {
+  constexpr int BS = 1 << BSL;
+  constexpr int CS = 1 << CSL;
+  constexpr int STEP = PB_ACCUM_BLOCK * 4;
+  constexpr int U = 4;  // batches of loads in flight per thread
+  __shared__ unsigned long long s_acc[BS];
+  __shared__ float s_hot[CS];
+  if (st->done) return;
+  out.base = st->base;
+  const int S = st->fx_shift;
+  const double scale = ldexp(1.0, S);
+  const PbPiece pc = pieces[blockIdx.x];
+  const int32_t he = min(pc.e1, pc.hub_e1);  // chunk-0 entries of the piece: [e0, he)
+  for (int k = threadIdx.x; k < BS; k += PB_ACCUM_BLOCK) s_acc[k] = 0ull;
+  if (pc.e0 < he) {
+    constexpr int PER = CS / PB_ACCUM_BLOCK;
+    const int32_t n = min(CS, st->V);
+    float c[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int k = (int)threadIdx.x + j * PB_ACCUM_BLOCK;
+      c[j] = contrib[k < n ? k : n - 1];
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) s_hot[threadIdx.x + j * PB_ACCUM_BLOCK] = c[j];
+  }
+  __syncthreads();
+  // e0, e1, hub_e1 and every e are multiples of 4
+  int32_t e = pc.e0 + (int32_t)threadIdx.x * 4;
+  for (; e + (U - 1) * STEP < he; e += U * STEP) {
+    pb_u32x2 q[U], d[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      q[k] = pb_ids4(src16 + pc.hub_off + e + k * STEP);
+      d[k] = pb_ids4(dst16 + e + k * STEP);
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) pb_add4<BSL>(s_acc, pb_lookup4<CS>(s_hot, q[k]), d[k], scale);
+  }
+  for (; e < he; e += STEP)
+    pb_add4<BSL>(s_acc, pb_lookup4<CS>(s_hot, pb_ids4(src16 + pc.hub_off + e)),
+                 pb_ids4(dst16 + e), scale);
+  e = max(pc.e0, pc.hub_e1) + (int32_t)threadIdx.x * 4;
+  for (; e + (U - 1) * STEP < pc.e1; e += U * STEP) {
+    pb_f32x4 v[U];
+    pb_u32x2 d[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      v[k] = pb_vals4(vals + e + k * STEP);
+      d[k] = pb_ids4(dst16 + e + k * STEP);
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) pb_add4<BSL>(s_acc, v[k], d[k], scale);
+  }
+  for (; e < pc.e1; e += STEP)
+    pb_add4<BSL>(s_acc, pb_vals4(vals + e), pb_ids4(dst16 + e), scale);
+  __syncthreads();
+  const int32_t v0 = pc.bin << BSL;
+  const int32_t nv = min(BS, st->V - v0);
+  double res = 0.0, dang = 0.0;
+  if (pc.multi) {
+    for (int k = threadIdx.x; k < nv; k += PB_ACCUM_BLOCK) {
+      const unsigned long long a = s_acc[k];
+      if (a) atomicAdd(&sum64[v0 + k], a);
+    }
+  } else {
+    for (int k = threadIdx.x; k < nv; k += PB_ACCUM_BLOCK)
+      out.put(v0 + k, pb_value(s_acc[k], S), out.x_old[v0 + k], out.inv_od[v0 + k], res, dang);
+  }
+  __shared__ double s_red[PB_ACCUM_WAVES];
+  res = block_sum_f64(res, s_red);
+  dang = block_sum_f64(dang, s_red);
+  if (threadIdx.x == 0) {
+    part_res[blockIdx.x] = res;
+    part_dang[blockIdx.x] = dang;
+  }
+}
+
+// Rows of split bins from sum64 (cleared for the next iteration), one
+// residual / dangling partial per block.  pr_rb_fixup_one_kernel then sums
+// these and pb_accum's partials in a fixed order and sets the next base and
+// the stop flag.
+__global__ __launch_bounds__(PR_BLOCK) void pb_rows_kernel(
+    int64_t nrows, int bsl, const int32_t *__restrict__ multi,
+    unsigned long long *__restrict__ sum64, RowOut out, double *__restrict__ part_res,
+    double *__restrict__ part_dang, const PrState *__restrict__ st) {
+  __shared__ double s_red[PR_BLOCK / 64];
+  if (st->done) return;
+  out.base = st->base;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double res = 0.0, dang = 0.0;
+  if (t < nrows) {
+    const int32_t v = (multi[t >> bsl] << bsl) + (int32_t)(t & ((1 << bsl) - 1));
+    if (v < st->V) {
+      const unsigned long long a = sum64[v];
+      sum64[v] = 0ull;
+      out.put(v, pb_value(a, st->fx_shift), out.x_old[v], out.inv_od[v], res, dang);
+    }
+  }
+  res = block_sum_f64(res, s_red);
+  dang = block_sum_f64(dang, s_red);
+  if (threadIdx.x == 0) {
+    part_res[blockIdx.x] = res;
+    part_dang[blockIdx.x] = dang;
+  }
+}
+
+// -- binned layout build (once per graph handle and shape) --
+
+// Per in-CSR edge u -> v (row v): key (bin of v, u), value v.  Rows are found
+// by binary search on the row offsets.
+__global__ void pb_keys_kernel(int32_t V, int64_t E, const int32_t *__restrict__ irp,
+                               const int32_t *__restrict__ icol, int bsl, int vbits,
+                               uint64_t *__restrict__ keys, int32_t *__restrict__ vs) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int32_t lo = 0, hi = V;  // irp[lo] <= e < irp[hi]
+  while (hi - lo > 1) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if (irp[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  keys[e] = ((uint64_t)(lo >> bsl) << vbits) | (uint32_t)icol[e];
+  vs[e] = lo;
+}
+
+// First sorted index of every non-empty (bin, chunk) segment.
+__global__ void pb_first_kernel(int64_t E, const uint64_t *__restrict__ keys, int vbits, int csl,
+                                int64_t nchunks, int32_t *__restrict__ first) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  const uint64_t key = keys[k];
+  const int64_t seg = (int64_t)(key >> vbits) * nchunks +
+                      (int64_t)((key & ((1ull << vbits) - 1)) >> csl);
+  bool head = k == 0;
+  if (!head) {
+    const uint64_t p = keys[k - 1];
+    head = (int64_t)(p >> vbits) * nchunks + (int64_t)((p & ((1ull << vbits) - 1)) >> csl) != seg;
+  }
+  if (head) first[seg] = (int32_t)k;
+}
+
+// Sorted edge k of segment (b, c) goes to src16[k + d1[seg]] (chunk-major
+// order) and dst16[k + d2[seg]] (bin-major order) as local ids.
+__global__ void pb_place_kernel(int64_t E, const uint64_t *__restrict__ keys,
+                                const int32_t *__restrict__ vs, int vbits, int csl, int bsl,
+                                int64_t nchunks, const int32_t *__restrict__ d1,
+                                const int32_t *__restrict__ d2, uint16_t *__restrict__ src16,
+                                uint16_t *__restrict__ dst16) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  const uint64_t key = keys[k];
+  const int32_t b = (int32_t)(key >> vbits);
+  const int32_t u = (int32_t)(key & ((1ull << vbits) - 1));
+  const int32_t c = u >> csl;
+  const int64_t seg = (int64_t)b * nchunks + c;
+  src16[k + d1[seg]] = (uint16_t)(u - (c << csl));
+  dst16[k + d2[seg]] = (uint16_t)(vs[k] - (b << bsl));
+}
+
 // Row blocks of an in-CSR (host side; irp copied from the device).
 void build_row_blocks(const std::vector<int32_t> &rp, int nb, std::vector<RowBlock> &blocks,
                       std::vector<LongRow> &longs) {
@@ -953,16 +1288,209 @@ static gb_status ensure_inv_outdeg(gb_graph g, hipStream_t st) {
   return GB_OK;
 }
 
+// Binned layout of the gather's in-CSR (irp, icol) for chunks of 2^csl
+// sources and bins of 2^bsl destinations, cached on the handle.  Edges are
+// sorted by (bin, source) with a stable radix sort (in-CSR order, so by
+// destination, among equal keys); each (bin, chunk) segment is padded to a
+// multiple of 4 entries.  Host side: segment offsets in both orders, wave
+// tasks (<= 256 entries of one segment), units (runs of one chunk's tasks of
+// about `unit` entries: one scatter block each) and pieces (runs of a bin's
+// stream of at most `piece` entries: one accumulation block each; largest
+// first, so that the block scheduler hands out the long ones first).
+static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const int32_t *icol,
+                          int csl, int bsl, int piece, hipStream_t st) {
+  gb_graph_s::PrBinned &L = g->pr_pb;
+  if (L.valid && L.relabeled == relabel && L.csl == csl && L.bsl == bsl && L.piece == piece)
+    return GB_OK;
+  L.valid = false;
+  const int32_t V = g->V;
+  const int64_t E = g->E;
+  const int64_t nchunks = ((int64_t)V + (1 << csl) - 1) >> csl;
+  const int64_t nbins = ((int64_t)V + (1 << bsl) - 1) >> bsl;
+  const int64_t nseg = nchunks * nbins;
+  if (nseg > ((int64_t)1 << 26) || E + 3 * std::min<int64_t>(nseg, E) > INT32_MAX - 4)
+    return gb_fail(GB_ERR_UNSUPPORTED, "gb_pagerank: graph too large for the binned form");
+  int vbits = 1;
+  while (((int64_t)1 << vbits) < V) vbits++;
+  int bbits = 1;
+  while (((int64_t)1 << bbits) < nbins) bbits++;
+  // sorted (bin, source) keys and destinations
+  std::vector<int32_t> first((size_t)nseg, -1);
+  DevBuf keys, keys2, vs, vs2, dfirst, tmp;
+  if (E > 0) {
+    GB_TRY(keys.ensure(sizeof(uint64_t) * E));
+    GB_TRY(keys2.ensure(sizeof(uint64_t) * E));
+    GB_TRY(vs.ensure(sizeof(int32_t) * E));
+    GB_TRY(vs2.ensure(sizeof(int32_t) * E));
+    const unsigned eb = (unsigned)((E + 255) / 256);
+    hipLaunchKernelGGL(pb_keys_kernel, dim3(eb), dim3(256), 0, st, V, E, irp, icol, bsl, vbits,
+                       keys.as<uint64_t>(), vs.as<int32_t>());
+    size_t tb = 0;
+    GB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.as<uint64_t>(),
+                                              keys2.as<uint64_t>(), vs.as<int32_t>(),
+                                              vs2.as<int32_t>(), (int)E, 0, vbits + bbits, st));
+    GB_TRY(tmp.ensure(tb));
+    GB_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, keys.as<uint64_t>(),
+                                              keys2.as<uint64_t>(), vs.as<int32_t>(),
+                                              vs2.as<int32_t>(), (int)E, 0, vbits + bbits, st));
+    GB_HIP(hipStreamSynchronize(st));
+    tmp.release();
+    keys.release();
+    vs.release();
+    GB_TRY(dfirst.ensure(sizeof(int32_t) * nseg));
+    GB_HIP(hipMemsetAsync(dfirst.p, 0xFF, sizeof(int32_t) * nseg, st));
+    hipLaunchKernelGGL(pb_first_kernel, dim3(eb), dim3(256), 0, st, E, keys2.as<uint64_t>(),
+                       vbits, csl, nchunks, dfirst.as<int32_t>());
+    GB_HIP(hipMemcpyAsync(first.data(), dfirst.p, sizeof(int32_t) * nseg, hipMemcpyDeviceToHost,
+                          st));
+    GB_HIP(hipStreamSynchronize(st));
+  }
+  // segment sizes (bin-major = sorted order), padded sizes and offsets
+  std::vector<int32_t> cnt((size_t)nseg, 0);
+  {
+    int64_t next = E;
+    for (int64_t i = nseg - 1; i >= 0; --i)
+      if (first[i] >= 0) {
+        cnt[i] = (int32_t)(next - first[i]);
+        next = first[i];
+      }
+  }
+  auto pad4 = [](int64_t n) { return (n + 3) & ~(int64_t)3; };
+  std::vector<int32_t> d1((size_t)nseg), d2((size_t)nseg);
+  std::vector<int32_t> out_start((size_t)nseg);
+  std::vector<int32_t> bin_start((size_t)nbins + 1);
+  int64_t P = 0;
+  for (int64_t b = 0; b < nbins; ++b) {
+    bin_start[b] = (int32_t)P;
+    for (int64_t c = 0; c < nchunks; ++c) {
+      const int64_t i = b * nchunks + c;
+      out_start[i] = (int32_t)P;
+      d2[i] = (int32_t)(P - (first[i] >= 0 ? first[i] : 0));
+      P += pad4(cnt[i]);
+    }
+  }
+  bin_start[nbins] = (int32_t)P;
+  std::vector<PbTask> tasks;
+  std::vector<PbUnit> units;
+  const int64_t unit = 65536;
+  std::vector<int32_t> hub_in((size_t)nbins);  // chunk-major offset of segment (0, b)
+  {
+    int64_t in = 0;
+    for (int64_t c = 0; c < nchunks; ++c) {
+      int32_t ut0 = (int32_t)tasks.size();
+      int64_t ue = 0;
+      for (int64_t b = 0; b < nbins; ++b) {
+        const int64_t i = b * nchunks + c;
+        d1[i] = (int32_t)(in - (first[i] >= 0 ? first[i] : 0));
+        const int64_t n = pad4(cnt[i]);
+        if (c == 0) {  // read by pb_accum from src16, no scatter
+          hub_in[b] = (int32_t)in;
+          in += n;
+          continue;
+        }
+        for (int64_t j = 0; j < n; j += PB_TASK) {
+          tasks.push_back(PbTask{(int32_t)(in + j), (int32_t)(out_start[i] + j),
+                                 (int32_t)std::min<int64_t>(PB_TASK, n - j)});
+          ue += std::min<int64_t>(PB_TASK, n - j);
+          if (ue >= unit) {
+            units.push_back(PbUnit{(int32_t)(c << csl), ut0, (int32_t)tasks.size()});
+            ut0 = (int32_t)tasks.size();
+            ue = 0;
+          }
+        }
+        in += n;
+      }
+      if ((int32_t)tasks.size() > ut0)
+        units.push_back(PbUnit{(int32_t)(c << csl), ut0, (int32_t)tasks.size()});
+    }
+  }
+  std::vector<PbPiece> pieces;
+  std::vector<int32_t> multi;
+  for (int64_t b = 0; b < nbins; ++b) {
+    const int64_t s = bin_start[b], n = bin_start[b + 1] - s;
+    const int64_t k = std::max<int64_t>(1, (n + piece - 1) / piece);
+    if (k > 1) multi.push_back((int32_t)b);
+    const int32_t hub_e1 = (int32_t)(s + pad4(cnt[b * nchunks]));
+    for (int64_t j = 0; j < k; ++j)
+      pieces.push_back(PbPiece{(int32_t)b, (int32_t)(s + pad4(n * j / k)),
+                               (int32_t)(j + 1 == k ? s + n : s + pad4(n * (j + 1) / k)),
+                               k > 1 ? 1 : 0, hub_e1, (int32_t)(hub_in[b] - s)});
+  }
+  std::stable_sort(pieces.begin(), pieces.end(), [](const PbPiece &x, const PbPiece &y) {
+    return x.e1 - x.e0 > y.e1 - y.e0;
+  });
+  // device copies
+  GB_TRY(L.src16.ensure(sizeof(uint16_t) * (P + 4)));
+  GB_TRY(L.dst16.ensure(sizeof(uint16_t) * (P + 4)));
+  GB_TRY(L.vals.ensure(sizeof(float) * (P + 4)));
+  GB_TRY(L.tasks.ensure(sizeof(PbTask) * (tasks.size() + 1)));
+  GB_TRY(L.units.ensure(sizeof(PbUnit) * (units.size() + 1)));
+  GB_TRY(L.pieces.ensure(sizeof(PbPiece) * (pieces.size() + 1)));
+  GB_TRY(L.multi.ensure(sizeof(int32_t) * (multi.size() + 1)));
+  GB_TRY(L.sum64.ensure(sizeof(unsigned long long) * V));
+  GB_HIP(hipMemsetAsync(L.src16.p, 0xFF, sizeof(uint16_t) * (P + 4), st));
+  GB_HIP(hipMemsetAsync(L.dst16.p, 0, sizeof(uint16_t) * (P + 4), st));
+  GB_HIP(hipMemsetAsync(L.sum64.p, 0, sizeof(unsigned long long) * V, st));
+  if (E > 0) {
+    DevBuf dd1, dd2;
+    GB_TRY(dd1.ensure(sizeof(int32_t) * nseg));
+    GB_TRY(dd2.ensure(sizeof(int32_t) * nseg));
+    GB_HIP(hipMemcpyAsync(dd1.p, d1.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, st));
+    GB_HIP(hipMemcpyAsync(dd2.p, d2.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(pb_place_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, E,
+                       keys2.as<uint64_t>(), vs2.as<int32_t>(), vbits, csl, bsl, nchunks,
+                       dd1.as<int32_t>(), dd2.as<int32_t>(), L.src16.as<uint16_t>(),
+                       L.dst16.as<uint16_t>());
+    GB_HIP(hipStreamSynchronize(st));  // dd1 / dd2 go out of scope
+  }
+  if (!tasks.empty())
+    GB_HIP(hipMemcpyAsync(L.tasks.p, tasks.data(), sizeof(PbTask) * tasks.size(),
+                          hipMemcpyHostToDevice, st));
+  if (!units.empty())
+    GB_HIP(hipMemcpyAsync(L.units.p, units.data(), sizeof(PbUnit) * units.size(),
+                          hipMemcpyHostToDevice, st));
+  GB_HIP(hipMemcpyAsync(L.pieces.p, pieces.data(), sizeof(PbPiece) * pieces.size(),
+                        hipMemcpyHostToDevice, st));
+  if (!multi.empty())
+    GB_HIP(hipMemcpyAsync(L.multi.p, multi.data(), sizeof(int32_t) * multi.size(),
+                          hipMemcpyHostToDevice, st));
+  GB_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
+  GB_HIP(hipGetLastError());
+  L.valid = true;
+  L.relabeled = relabel;
+  L.csl = csl;
+  L.bsl = bsl;
+  L.piece = piece;
+  L.padded = P;
+  L.ntasks = (int64_t)tasks.size();
+  L.nunits = (int64_t)units.size();
+  L.npieces = (int64_t)pieces.size();
+  L.nmulti = (int64_t)multi.size();
+  return GB_OK;
+}
+
 // The gather's form and shape: merge path with ipt items per thread (tile =
-// 256 x ipt items) and an LDS hot set of `hot` contributions, or row blocks of
-// rb edges (rb > 0; ipt and hot unused).
+// 256 x ipt items) and an LDS hot set of `hot` contributions, row blocks of
+// rb edges (rb > 0; ipt and hot unused), or the binned form (pb > 0: chunk
+// and bin widths PB_SHAPES[pb]; ipt, hot and rb unused).
 struct PrShape {
-  int ipt, hot, rb;
+  int ipt, hot, rb, pb;
 };
+struct PbShape {
+  int csl, bsl;
+};
+constexpr PbShape PB_SHAPES[] = {{0, 0}, {13, 13}, {14, 13}, {13, 12}};
+constexpr int PB_NSHAPES = 4;
+// stream entries per accumulation block at most (GB_PR_PB_PIECE, multiple of 4)
+static int pb_piece() {
+  const char *e = getenv("GB_PR_PB_PIECE");
+  const int v = e ? atoi(e) : 0;
+  return v >= 4096 && v <= (1 << 26) ? v & ~3 : 262144;
+}
 static bool valid_rb(int rb) { return rb == 1024 || rb == 2048 || rb == 4096; }
 
 static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_pr_result *res,
-                              void *stream, PrShape sh);
+                              void *stream, PrShape sh, double *iter_ms = nullptr);
 
 static bool use_relabel(gb_graph g, const gb_pr_params *p) {
   return p->relabel > 0 || (p->relabel == 0 && g->E >= ((int64_t)1 << 20));
@@ -976,19 +1504,30 @@ static bool use_relabel(gb_graph g, const gb_pr_params *p) {
 // item, large ones occupancy and rows that span threads; a hot set (relabelled
 // graphs only) takes the most-read sources off the L2 but its LDS lowers
 // occupancy.  Row-block candidates (unless row_block is -1) skip the merge-path
-// search and scan but leave lanes idle in blocks of uneven rows.
+// search and scan but leave lanes idle in blocks of uneven rows.  The binned
+// candidate (with the row-block ones, unless binned is -1) streams every edge
+// twice but issues no random reads.  Candidates are compared by the mean time
+// of a whole iteration (gather and fix-up).
 static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void *stream,
                             PrShape *out) {
   const bool rl = use_relabel(g, p);
   const char *ei = getenv("GB_PR_IPT");
   const char *eh = getenv("GB_PR_HOT");
   const char *er = getenv("GB_PR_RB");
+  const char *ep = getenv("GB_PR_PB");
+  const bool free_form = p->row_block == 0 && !p->items_per_thread && p->binned == 0;
+  int fixed_pb = p->binned > 0 ? p->binned : 0;
+  if (!fixed_pb && free_form && ep && atoi(ep) > 0 && atoi(ep) < PB_NSHAPES) fixed_pb = atoi(ep);
+  if (fixed_pb) {
+    *out = PrShape{0, 0, 0, fixed_pb};
+    return GB_OK;
+  }
   int fixed_rb = p->row_block > 0 ? p->row_block : 0;
   if (!fixed_rb && p->row_block == 0 && !p->items_per_thread && er && valid_rb(atoi(er)))
     fixed_rb = atoi(er);
   if (fixed_rb) {
     const int h = eh ? atoi(eh) : 0;
-    *out = PrShape{0, rl && (h == 4096 || h == 8192) ? h : 0, fixed_rb};
+    *out = PrShape{0, rl && (h == 4096 || h == 8192) ? h : 0, fixed_rb, 0};
     return GB_OK;
   }
   // fixed parts of the shape (0 / -1 = left to the tuner)
@@ -998,24 +1537,28 @@ static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void 
   if (eh && (atoi(eh) == 0 || atoi(eh) == 4096 || atoi(eh) == 8192)) fixed_hot = atoi(eh);
   if (!rl) fixed_hot = 0;
   if (fixed_ipt && fixed_hot >= 0) {
-    *out = PrShape{fixed_ipt, fixed_hot, 0};
+    *out = PrShape{fixed_ipt, fixed_hot, 0, 0};
     return GB_OK;
   }
   const bool try_rb = !fixed_ipt && fixed_hot <= 0 && p->row_block == 0;
-  // cached per (fixed ipt or none, row blocks allowed, layout)
-  const int key = (fixed_ipt ? fixed_ipt / 4 : 0) * 2 + (rl ? 1 : 0) + (try_rb ? 0 : 10);
-  if (g->pr_tuned[key][0] || g->pr_tuned[key][2]) {
-    *out = PrShape{g->pr_tuned[key][0], g->pr_tuned[key][1], g->pr_tuned[key][2]};
+  const bool try_pb = try_rb && p->binned == 0;
+  // cached per (fixed ipt or none, row blocks allowed, layout, binned allowed)
+  const int key = (fixed_ipt ? fixed_ipt / 4 : 0) * 2 + (rl ? 1 : 0) + (try_rb ? 0 : 10) +
+                  (try_pb ? 0 : 20);
+  if (g->pr_tuned[key][0] || g->pr_tuned[key][2] || g->pr_tuned[key][3]) {
+    *out = PrShape{g->pr_tuned[key][0], g->pr_tuned[key][1], g->pr_tuned[key][2],
+                   g->pr_tuned[key][3]};
     return GB_OK;
   }
   std::vector<PrShape> cands;
   for (int ci : fixed_ipt ? std::vector<int>{fixed_ipt} : std::vector<int>{4, 8, 12, 16})
     for (int ch : fixed_hot >= 0 ? std::vector<int>{fixed_hot} : std::vector<int>{0, 4096, 8192})
-      cands.push_back(PrShape{ci, ch, 0});
+      cands.push_back(PrShape{ci, ch, 0, 0});
   if (try_rb)
     for (int rb : {1024, 2048, 4096})
       for (int ch : rl ? std::vector<int>{0, 4096, 8192} : std::vector<int>{0})
-        cands.push_back(PrShape{0, ch, rb});
+        cands.push_back(PrShape{0, ch, rb, 0});
+  if (try_pb) cands.push_back(PrShape{0, 0, 0, 1});
   PrShape best = cands[0];
   double best_ms = 1e300;
   for (const PrShape &c : cands) {
@@ -1026,8 +1569,11 @@ static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void 
     q.time_kernels = 1;
     q.d_x0 = nullptr;
     gb_pr_result r{};
-    GB_TRY(pagerank_run(g, &q, d_x, &r, stream, c));
-    const double ms = r.gather_launches ? r.gather_ms / r.gather_launches : 1e300;
+    double it_ms = 0.0;
+    const gb_status s = pagerank_run(g, &q, d_x, &r, stream, c, &it_ms);
+    if (s == GB_ERR_UNSUPPORTED && c.pb) continue;  // graph too large for the binned layout
+    GB_TRY(s);
+    const double ms = r.gather_launches ? it_ms / r.gather_launches : 1e300;
     if (ms < best_ms) {
       best_ms = ms;
       best = c;
@@ -1036,6 +1582,7 @@ static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void 
   g->pr_tuned[key][0] = best.ipt;
   g->pr_tuned[key][1] = best.hot;
   g->pr_tuned[key][2] = best.rb;
+  g->pr_tuned[key][3] = best.pb;
   *out = best;
   return GB_OK;
 }
@@ -1054,15 +1601,19 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
     return gb_fail(GB_ERR_INVALID, "gb_pagerank: row_block must be -1, 0, 1024, 2048 or 4096");
   if (p->row_block > 0 && p->items_per_thread != 0)
     return gb_fail(GB_ERR_INVALID, "gb_pagerank: row_block and items_per_thread both set");
-  PrShape sh{8, 0, 0};
+  if (p->binned < -1 || p->binned >= PB_NSHAPES)
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: binned must be -1, 0, 1, 2 or 3");
+  if (p->binned > 0 && (p->row_block != 0 || p->items_per_thread != 0))
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: binned set with row_block or items_per_thread");
+  PrShape sh{8, 0, 0, 0};
   GB_TRY(ensure_inv_outdeg(g, gb_stream(stream)));
   GB_TRY(pick_shape(g, p, d_x, stream, &sh));
   return pagerank_run(g, p, d_x, res, stream, sh);
 }
 
 static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_pr_result *res,
-                              void *stream, PrShape sh) {
-  const int ipt = sh.rb ? 8 : sh.ipt, rb = sh.rb;
+                              void *stream, PrShape sh, double *iter_ms) {
+  const int ipt = sh.rb || sh.pb ? 8 : sh.ipt, rb = sh.rb, pb = sh.pb;
   int hot = sh.hot;
   hipStream_t st = gb_stream(stream);
   hipStream_t own = nullptr;
@@ -1091,7 +1642,9 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   const int32_t *irp = relabel ? g->rl_rrp.as<int32_t>() : g->rrp;
   const int32_t *icol = relabel ? g->rl_rcol.as<int32_t>() : g->rcol;
   const float *inv = relabel ? g->rl_inv.as<float>() : g->inv_outdeg.as<float>();
-  if (!rb && (g->pr_tile_items != tile || g->pr_tile_relabeled != relabel)) {
+  if (pb)
+    GB_TRY(pb_build(g, relabel, irp, icol, PB_SHAPES[pb].csl, PB_SHAPES[pb].bsl, pb_piece(), st));
+  if (!rb && !pb && (g->pr_tile_items != tile || g->pr_tile_relabeled != relabel)) {
     const int64_t ntiles = ((int64_t)V + g->E + tile - 1) / tile;
     GB_TRY(g->pr_tile_row.ensure(sizeof(int32_t) * (ntiles + 1)));
     GB_TRY(g->pr_tile_edge.ensure(sizeof(int32_t) * (ntiles + 1)));
@@ -1121,10 +1674,17 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     g->pr_rb_nblocks = (int64_t)blocks.size();
     g->pr_rb_nlong = (int32_t)longs.size();
   }
-  // merge path: tiles and their partials; row blocks: blocks, 4 partials each
-  const int64_t ntiles = rb ? g->pr_rb_nblocks : g->pr_ntiles;
-  const int64_t nparts = rb ? ntiles * (PR_BLOCK / GB_WAVE) : ntiles;
-  const int64_t nfix = (std::max<int64_t>(nparts, rb ? g->pr_rb_nlong : 0) + PR_BLOCK - 1) / PR_BLOCK;
+  // merge path: tiles and their partials; row blocks: blocks, 4 partials each;
+  // binned: pieces, 8 partials each, and the rows of split bins
+  const gb_graph_s::PrBinned &PB = g->pr_pb;
+  const int64_t ntiles = pb ? PB.npieces : rb ? g->pr_rb_nblocks : g->pr_ntiles;
+  const int64_t pb_rows = pb ? PB.nmulti << PB.bsl : 0;
+  const int64_t pb_row_blocks = (pb_rows + PR_BLOCK - 1) / PR_BLOCK;
+  const int64_t nparts = pb   ? ntiles + pb_row_blocks
+                         : rb ? ntiles * (PR_BLOCK / GB_WAVE)
+                              : ntiles;
+  const int64_t nfix =
+      (std::max<int64_t>(nparts, rb ? g->pr_rb_nlong : 0) + PR_BLOCK - 1) / PR_BLOCK;
   const int64_t vblocks = ((int64_t)V + PR_BLOCK - 1) / PR_BLOCK;
   PrWork &w = g_prw;
   GB_TRY(w.xa.ensure(sizeof(float) * V));
@@ -1152,18 +1712,52 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   float *ca = w.ca.as<float>(), *cb = w.cb.as<float>();
   hipLaunchKernelGGL(pr_init_kernel, dim3((unsigned)vblocks), dim3(PR_BLOCK), 0, st, V, p->d_x0,
                      relabel ? (const int32_t *)g->rl_perm.as<int32_t>() : nullptr, inv, xa, ca,
-                     w.pdang.as<double>());
+                     w.pdang.as<double>(), w.pres.as<double>());
   hipLaunchKernelGGL(pr_init_finalize_kernel, dim3(1), dim3(1024), 0, st, (int32_t)vblocks,
-                     w.pdang.as<double>(), state);
+                     w.pdang.as<double>(), (const double *)w.pres.as<double>(), state);
   GB_HIP(hipGetLastError());
 
   const int32_t *trow = g->pr_tile_row.as<int32_t>();
   const int32_t *tedge = g->pr_tile_edge.as<int32_t>();
-  // One iteration; ev (optional) brackets the gather launch.
+  // One iteration; ev0..ev1 (optional) bracket the gather launches.
   auto one_iter = [&](const float *x_old, const float *c_old, float *x_new, float *c_new,
                       hipEvent_t ev0, hipEvent_t ev1) {
     RowOut out{0.0f, (float)p->damping, x_old, inv, x_new, c_new};
     if (ev0) (void)hipEventRecord(ev0, st);
+    if (pb) {
+      if (PB.nunits > 0) {
+        if (PB.csl == 14)
+          hipLaunchKernelGGL(pb_scatter_kernel<14>, dim3((unsigned)PB.nunits),
+                             dim3(PB_SCATTER_BLOCK), 0, st, PB.units.as<PbUnit>(),
+                             PB.tasks.as<PbTask>(), PB.src16.as<uint16_t>(), c_old,
+                             PB.vals.as<float>(), state);
+        else
+          hipLaunchKernelGGL(pb_scatter_kernel<13>, dim3((unsigned)PB.nunits),
+                             dim3(PB_SCATTER_BLOCK), 0, st, PB.units.as<PbUnit>(),
+                             PB.tasks.as<PbTask>(), PB.src16.as<uint16_t>(), c_old,
+                             PB.vals.as<float>(), state);
+      }
+      auto accum = (PB.csl == 14)   ? pb_accum_kernel<14, 13>
+                   : (PB.bsl == 12) ? pb_accum_kernel<13, 12>
+                                    : pb_accum_kernel<13, 13>;
+      hipLaunchKernelGGL(accum, dim3((unsigned)PB.npieces), dim3(PB_ACCUM_BLOCK), 0, st,
+                         PB.pieces.as<PbPiece>(), (const float *)PB.vals.as<float>(),
+                         (const uint16_t *)PB.src16.as<uint16_t>(), PB.dst16.as<uint16_t>(),
+                         c_old, out, PB.sum64.as<unsigned long long>(), w.pres.as<double>(),
+                         w.pdang.as<double>(), state);
+      if (ev1) (void)hipEventRecord(ev1, st);
+      if (pb_rows > 0)
+        hipLaunchKernelGGL(pb_rows_kernel, dim3((unsigned)pb_row_blocks), dim3(PR_BLOCK), 0, st,
+                           pb_rows, PB.bsl, (const int32_t *)PB.multi.as<int32_t>(),
+                           PB.sum64.as<unsigned long long>(), out, w.pres.as<double>() + ntiles,
+                           w.pdang.as<double>() + ntiles, state);
+      // partials: one per piece, then one per row block, summed in that order
+      hipLaunchKernelGGL(pr_rb_fixup_one_kernel, dim3(1), dim3(RB_FIX1_THREADS), 0, st, nparts, 0,
+                         (const LongRow *)nullptr, (const F2 *)nullptr, out,
+                         (const double *)w.pres.as<double>(), (const double *)w.pdang.as<double>(),
+                         state);
+      return;
+    }
     if (rb) {
       const RbArgs a{irp, icol, g->pr_rb.as<RowBlock>(), c_old, out, w.tail.as<F2>(),
                      w.pres.as<double>(), w.pdang.as<double>(), state, ntiles};
@@ -1237,16 +1831,18 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     (void)hipGraphDestroy(graph);
   } else {
     // eager launches; with time_kernels every gather is bracketed by its own
-    // event pair and the stop flag is read every CH iterations only
-    if (p->time_kernels) GB_HIP(w.ensure_events(2 * (size_t)p->max_iter));
+    // event pair, a third event ends the iteration, and the stop flag is read
+    // every CH iterations only
+    if (p->time_kernels) GB_HIP(w.ensure_events(3 * (size_t)p->max_iter));
     const std::vector<hipEvent_t> &evs = w.events;
     for (int k = 0; k < p->max_iter; k++) {
       const float *xo = (k % 2 == 0) ? xa : xb;
       const float *co = (k % 2 == 0) ? ca : cb;
       float *xn = (k % 2 == 0) ? xb : xa;
       float *cn = (k % 2 == 0) ? cb : ca;
-      one_iter(xo, co, xn, cn, p->time_kernels ? evs[2 * k] : nullptr,
-               p->time_kernels ? evs[2 * k + 1] : nullptr);
+      one_iter(xo, co, xn, cn, p->time_kernels ? evs[3 * k] : nullptr,
+               p->time_kernels ? evs[3 * k + 1] : nullptr);
+      if (p->time_kernels) (void)hipEventRecord(evs[3 * k + 2], st);
       if ((k + 1) % CH == 0 || k + 1 == p->max_iter) {
         GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
         GB_HIP(hipStreamSynchronize(st));
@@ -1257,9 +1853,11 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
       GB_HIP(hipStreamSynchronize(st));
       // launches after the stop flag exit at once: count only working ones
       for (int k = 0; k < host_state.iter && k < p->max_iter; k++) {
-        float ms = 0.0f;
-        GB_HIP(hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
+        float ms = 0.0f, ims = 0.0f;
+        GB_HIP(hipEventElapsedTime(&ms, evs[3 * k], evs[3 * k + 1]));
+        GB_HIP(hipEventElapsedTime(&ims, evs[3 * k], evs[3 * k + 2]));
         gather_ms += ms;
+        if (iter_ms) *iter_ms += ims;
         gather_launches++;
       }
     }
@@ -1284,10 +1882,11 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     res->gather_ms = gather_ms;
     res->gather_launches = gather_launches;
     res->tiles = (int32_t)ntiles;
-    res->items_per_thread = rb ? 0 : ipt;
+    res->items_per_thread = rb || pb ? 0 : ipt;
     res->hot = hot;
     res->relabeled = relabel ? 1 : 0;
     res->row_block = rb;
+    res->binned = pb;
   }
   return GB_OK;
 }

@@ -200,9 +200,13 @@ typedef struct gb_pr_params {
   const float *d_x0; /* optional warm start (device, V; NULL = 1/N) */
   int32_t relabel;   /* gather in out-degree order: 0 auto (E >= 2^20), 1 on, -1 off;
                         built once per graph handle, results in the caller's ids */
-  int32_t row_block; /* gather form: 0 autotuned (merge path or row blocks),
-                        1024 / 2048 / 4096 row blocks of that many edges,
-                        -1 merge path only */
+  int32_t row_block; /* gather form: 0 autotuned (merge path, row blocks or
+                        binned), 1024 / 2048 / 4096 row blocks of that many
+                        edges, -1 merge path only */
+  int32_t binned;    /* binned (propagation-blocking) form: 0 left to the
+                        autotuner, -1 never, 1 / 2 / 3 that form with chunks
+                        x bins of 2^13 x 2^13, 2^14 x 2^13, 2^13 x 2^12 ids;
+                        its row sums are exact fixed-point sums */
 } gb_pr_params;
 
 typedef struct gb_pr_result {
@@ -217,6 +221,7 @@ typedef struct gb_pr_result {
   int32_t hot;              /* sources served from LDS (0 = none) */
   int32_t relabeled;        /* 1 if the gather ran in out-degree order */
   int32_t row_block;        /* row-block form used: edges per block (0 = merge path) */
+  int32_t binned;           /* binned form used (its gb_pr_params.binned value), 0 = not */
 } gb_pr_result;
 
 /* Configured PageRank: in-edge pull gather over the reverse CSR, FP32 ranks
