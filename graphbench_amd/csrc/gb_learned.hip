@@ -17,26 +17,53 @@
 
 namespace {
 
-__global__ void vertex_features_kernel(int32_t V, const int32_t *__restrict__ rp,
-                                       const int32_t *__restrict__ rrp, float m0, float s0,
-                                       float m1, float s1, float m2, float s2,
-                                       float *__restrict__ X) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  const float indeg = (float)(rrp[v + 1] - rrp[v]);
-  const float outdeg = (float)(rp[v + 1] - rp[v]);
-  X[v * 3 + 0] = (log2f(1.0f + indeg) - m0) / s0;
-  X[v * 3 + 1] = (log2f(1.0f + outdeg) - m1) / s1;
-  X[v * 3 + 2] = ((outdeg == 0.0f ? 1.0f : 0.0f) - m2) / s2;
+// Standardised features of a vertex with these degrees (one MLP input row).
+struct WsNorm {
+  float m0, s0, m1, s1, m2, s2;
+};
+__device__ __forceinline__ void ws_features(int32_t indeg, int32_t outdeg, const WsNorm &n,
+                                            float *__restrict__ x) {
+  x[0] = (log2f(1.0f + (float)indeg) - n.m0) / n.s0;
+  x[1] = (log2f(1.0f + (float)outdeg) - n.m1) / n.s1;
+  x[2] = ((outdeg == 0 ? 1.0f : 0.0f) - n.m2) / n.s2;
 }
 
-__global__ void ranks_from_pred_kernel(int32_t V, const float *__restrict__ Y, float ym, float ys,
+// The features depend on (indeg, outdeg) only, and the MLP maps each row on
+// its own, so vertices with both degrees below WS_T share the prediction of
+// one row of a WS_T x WS_T table (row indeg * WS_T + outdeg); the others get
+// rows of their own after the table.  The predictions are those of one row
+// per vertex, bit for bit, for a fraction of the MLP work.
+constexpr int32_t WS_T = 512;
+
+__global__ void ws_table_kernel(WsNorm n, float *__restrict__ X) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (int64_t)WS_T * WS_T) ws_features((int32_t)(i / WS_T), (int32_t)(i % WS_T), n, X + i * 3);
+}
+
+__global__ void ws_rows_kernel(int32_t V, const int32_t *__restrict__ rp,
+                               const int32_t *__restrict__ rrp, WsNorm n, float *__restrict__ X,
+                               int32_t *__restrict__ row, unsigned int *__restrict__ n_own) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const int32_t indeg = rrp[v + 1] - rrp[v];
+  const int32_t outdeg = rp[v + 1] - rp[v];
+  if (indeg < WS_T && outdeg < WS_T) {
+    row[v] = indeg * WS_T + outdeg;
+  } else {
+    const int64_t r = (int64_t)WS_T * WS_T + atomicAdd(n_own, 1u);
+    ws_features(indeg, outdeg, n, X + r * 3);
+    row[v] = (int32_t)r;
+  }
+}
+
+__global__ void ranks_from_pred_kernel(int32_t V, const float *__restrict__ Y,
+                                       const int32_t *__restrict__ row, float ym, float ys,
                                        float *__restrict__ x, double *__restrict__ part) {
   __shared__ double s_red[256 / 64];
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double s = 0.0;
   if (v < V) {
-    const float r = exp2f(Y[v] * ys + ym);  // N * rank
+    const float r = exp2f(Y[row[v]] * ys + ym);  // N * rank
     x[v] = r;
     s = (double)r;
   }
@@ -116,6 +143,7 @@ __global__ void fill_zero_u64_kernel(unsigned long long *a, int64_t n) {
 
 struct FeatWork {
   DevBuf in, out;
+  DevBuf wx, wy, wpart, wrow;  // gb_pr_warm_start: rows, predictions, sums, row of each vertex
 };
 FeatWork &g_featw() {  // heap singleton, never destroyed (as the other workspaces)
   static FeatWork *w = new FeatWork();
@@ -139,6 +167,10 @@ void features_from(const gb_graph g, int32_t nsrc, double sum_w, double max_w, d
 void gb_learned_release() {
   g_featw().in.release();
   g_featw().out.release();
+  g_featw().wx.release();
+  g_featw().wy.release();
+  g_featw().wpart.release();
+  g_featw().wrow.release();
 }
 
 extern "C" gb_status gb_graph_features_multi(int32_t n, const gb_graph *graphs, int32_t nsrc,
@@ -198,20 +230,35 @@ extern "C" gb_status gb_pr_warm_start(gb_graph g, const float *d_params, const f
     return gb_fail(GB_ERR_INVALID, "gb_pr_warm_start: bad arguments (needs both CSRs)");
   hipStream_t st = gb_stream(stream);
   const int32_t V = g->V;
-  DevBuf X, Y, part;
+  // reused across calls (allocating and freeing them cost more than the
+  // kernels at 16 M vertices); the call ends synchronised, so they are free
+  // again when it returns
+  DevBuf &X = g_featw().wx, &Y = g_featw().wy, &part = g_featw().wpart;
+  DevBuf &row = g_featw().wrow;
   const int64_t nb = (V + 255) / 256;
-  GB_TRY(X.ensure(sizeof(float) * 3 * (size_t)V));
-  GB_TRY(Y.ensure(sizeof(float) * (size_t)V));
-  GB_TRY(part.ensure(sizeof(double) * (nb + 1)));
+  const int64_t tab = (int64_t)WS_T * WS_T;
+  GB_TRY(X.ensure(sizeof(float) * 3 * (size_t)(tab + V)));
+  GB_TRY(Y.ensure(sizeof(float) * (size_t)(tab + V)));
+  GB_TRY(row.ensure(sizeof(int32_t) * (size_t)V));
+  GB_TRY(part.ensure(sizeof(double) * (nb + 2)));
   // h_norm: x mean[3], x std[3], y mean, y std
-  hipLaunchKernelGGL(vertex_features_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, g->rp,
-                     g->rrp, h_norm[0], h_norm[3], h_norm[1], h_norm[4], h_norm[2], h_norm[5],
+  const WsNorm n{h_norm[0], h_norm[3], h_norm[1], h_norm[4], h_norm[2], h_norm[5]};
+  unsigned int *n_own = reinterpret_cast<unsigned int *>(part.as<double>() + nb + 1);
+  GB_HIP(hipMemsetAsync(n_own, 0, sizeof(unsigned int), st));
+  hipLaunchKernelGGL(ws_table_kernel, dim3((unsigned)((tab + 255) / 256)), dim3(256), 0, st, n,
                      X.as<float>());
+  hipLaunchKernelGGL(ws_rows_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, g->rp, g->rrp, n,
+                     X.as<float>(), row.as<int32_t>(), n_own);
   GB_HIP(hipGetLastError());
-  // one row per vertex: the FP32 inference with the hidden layer on MFMA
-  GB_TRY(gb_mlp_forward_mfma(d_params, 3, 1, X.as<float>(), V, Y.as<float>(), stream));
+  unsigned int h_own = 0;
+  GB_HIP(hipMemcpyAsync(&h_own, n_own, sizeof(h_own), hipMemcpyDeviceToHost, st));
+  GB_HIP(hipStreamSynchronize(st));
+  // the table and the vertices' own rows: the FP32 inference, hidden layer on MFMA
+  GB_TRY(gb_mlp_forward_mfma(d_params, 3, 1, X.as<float>(), tab + (int64_t)h_own, Y.as<float>(),
+                             stream));
   hipLaunchKernelGGL(ranks_from_pred_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, Y.as<float>(),
-                     h_norm[6], h_norm[7], d_x0, part.as<double>());
+                     (const int32_t *)row.as<int32_t>(), h_norm[6], h_norm[7], d_x0,
+                     part.as<double>());
   hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(1024), 0, st, part.as<double>(), nb,
                      part.as<double>() + nb);
   hipLaunchKernelGGL(scale_kernel, dim3((unsigned)nb), dim3(256), 0, st, V, d_x0,

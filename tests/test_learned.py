@@ -229,6 +229,30 @@ def test_learned_delta_beats_plain_rule(gpu):
 
 
 @pytest.mark.gpu
+def test_warm_start_table_matches_per_vertex(gpu):
+    """gb_pr_warm_start runs the MLP once per (indeg, outdeg) pair with both
+    below 512 and once per other vertex; its x0 matches one MLP row per vertex
+    (features from the degrees, the same FP32 MFMA forward), normalised in
+    FP64.  R-MAT 16 has vertices past 512 on either degree and dangling ones."""
+    torch = _torch()
+    models = _models()
+    h = gb.gen_rmat(16, 16, seed=777, weights="none")
+    r = h.transpose()
+    g = gb.Graph.from_host(h, reverse=r)
+    indeg, outdeg = np.diff(r.row_ptr), np.diff(h.row_ptr)
+    assert (indeg >= 512).any() and (outdeg >= 512).any() and (outdeg == 0).any()
+    p = models.data["warm_start"]
+    xm, xs = np.array(p["x_mean"], np.float32), np.array(p["x_std"], np.float32)
+    F = np.stack([np.log2(1 + indeg.astype(np.float32)), np.log2(1 + outdeg.astype(np.float32)),
+                  (outdeg == 0).astype(np.float32)], 1)
+    X = np.ascontiguousarray((F - xm) / xs, np.float32)
+    Y = models.pr_mlp.forward(torch.from_numpy(X).cuda(), mfma=True).cpu().numpy()[:, 0]
+    rr = np.exp2(Y.astype(np.float64) * p["y_std"][0] + p["y_mean"][0])
+    x0 = models.warm_start(g).cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(x0, rr / rr.sum(), rtol=1e-5, atol=0)
+
+
+@pytest.mark.gpu
 def test_learned_warm_start_saves_iterations(gpu):
     """Unseen R-MAT (scale 18, seed 5151): warm start converges in fewer
     iterations to the same fixed point (L1 to the FP64 oracle <= 1e-6)."""
