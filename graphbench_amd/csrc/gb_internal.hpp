@@ -58,7 +58,12 @@ struct DevBuf {
   ~DevBuf() { release(); }
 };
 
+// A fresh id per graph handle, never reused (keys of per-handle caches that
+// outlive the handle, such as an instantiated PageRank hipGraph).
+uint64_t gb_next_graph_uid();
+
 struct gb_graph_s {
+  uint64_t uid = gb_next_graph_uid();
   int32_t V = 0;
   int64_t E = 0;
   const int32_t *rp = nullptr;

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "gb_device.hpp"
@@ -1137,6 +1138,11 @@ void build_row_blocks(const std::vector<int32_t> &rp, int nb, std::vector<RowBlo
 
 struct PrWork {
   DevBuf xa, xb, ca, cb, tail, head, pres, pdang, fres, fdang, state, counter;
+  // the instantiated hipGraph of CH iterations and what its kernels were
+  // captured with (graph id, form, sizes, every device pointer): replayed by
+  // later calls with the same key instead of captured again
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> exec_key;
   // gather timing events (time_kernels), created once and reused: creating
   // two per iteration on every call held up the first launches
   std::vector<hipEvent_t> events;
@@ -1155,6 +1161,9 @@ struct PrWork {
       b->release();
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    exec_key.clear();
   }
 };
 // Heap singleton, never destroyed: device memory must not be freed from a
@@ -1827,25 +1836,45 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   double gather_ms = 0.0;
   int gather_launches = 0;
   if (p->use_graph && !p->time_kernels) {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    GB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < CH; k++) {
-      if (k % 2 == 0)
-        one_iter(xa, ca, xb, cb, nullptr, nullptr);
-      else
-        one_iter(xb, cb, xa, ca, nullptr, nullptr);
+    // everything the captured kernels take as arguments
+    const gb_graph_s::PrBinned &L = g->pr_pb;
+    auto u = [](const void *q) { return (uint64_t)(uintptr_t)q; };
+    float dmp = (float)p->damping;
+    uint32_t dbits;
+    std::memcpy(&dbits, &dmp, sizeof(dbits));
+    const std::vector<uint64_t> key = {
+        g->uid, (uint64_t)ipt, (uint64_t)hot, (uint64_t)rb, (uint64_t)pb, relabel ? 1u : 0u,
+        dbits, (uint64_t)ntiles, (uint64_t)nparts, (uint64_t)nfix, (uint64_t)g->pr_rb_nlong,
+        u(irp), u(icol), u(inv), u(trow), u(tedge), u(g->pr_rb.p), u(g->pr_rb_long.p), u(xa),
+        u(xb), u(ca), u(cb), u(w.tail.p), u(w.head.p), u(w.pres.p), u(w.pdang.p), u(w.fres.p),
+        u(w.fdang.p), u(w.counter.p), u(state), u(L.src16.p), u(L.dst16.p), u(L.vals.p),
+        u(L.tasks.p), u(L.units.p), u(L.pieces.p), u(L.multi.p), u(L.sum64.p),
+        (uint64_t)L.nunits, (uint64_t)L.npieces, (uint64_t)L.nmulti, (uint64_t)L.hsl,
+        (uint64_t)L.csl, (uint64_t)L.bsl};
+    if (!w.exec || w.exec_key != key) {
+      if (w.exec) (void)hipGraphExecDestroy(w.exec);
+      w.exec = nullptr;
+      w.exec_key.clear();
+      hipGraph_t graph = nullptr;
+      GB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < CH; k++) {
+        if (k % 2 == 0)
+          one_iter(xa, ca, xb, cb, nullptr, nullptr);
+        else
+          one_iter(xb, cb, xa, ca, nullptr, nullptr);
+      }
+      GB_HIP(hipStreamEndCapture(st, &graph));
+      const hipError_t ie = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      GB_HIP(ie);
+      w.exec_key = key;
     }
-    GB_HIP(hipStreamEndCapture(st, &graph));
-    GB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     for (int done_it = 0; done_it < p->max_iter; done_it += CH) {
-      GB_HIP(hipGraphLaunch(exec, st));
+      GB_HIP(hipGraphLaunch(w.exec, st));
       GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
       GB_HIP(hipStreamSynchronize(st));
       if (host_state.done) break;
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
   } else {
     // eager launches; with time_kernels every gather is bracketed by its own
     // event pair, a third event ends the iteration, and the stop flag is read

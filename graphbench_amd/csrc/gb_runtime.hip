@@ -5,6 +5,7 @@
 #include <hipcub/device/device_scan.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -152,6 +153,11 @@ extern "C" gb_status gb_graph_wrap(gb_graph *out, int32_t V, int64_t E,
   g->rw = d_rw;
   *out = g;
   return GB_OK;
+}
+
+uint64_t gb_next_graph_uid() {
+  static std::atomic<uint64_t> next{1};
+  return next.fetch_add(1);
 }
 
 extern "C" gb_status gb_graph_upload(gb_graph *out, int32_t V, int64_t E,

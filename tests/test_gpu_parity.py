@@ -666,6 +666,37 @@ def test_pagerank_binned_args_rejected(gpu):
             gb.pagerank(g, **kw)
 
 
+def test_pagerank_graph_cache_across_handles_and_forms(gpu):
+    """The instantiated hipGraph is kept between calls and replayed only for
+    the same handle, form and buffers: alternating handles and forms, a handle
+    destroyed and another created, and a start vector each give the eager
+    result bit for bit."""
+    gb = gpu
+    hs = [gb.gen_rmat(12, 8, seed=31, weights="none"), gb.gen_er(5000, 6, seed=32, weights="none")]
+    gs = [gb.Graph.from_host(h, reverse=h.transpose()) for h in hs]
+    _say("pagerank graph cache")
+    forms = [dict(binned=1), dict(row_block=2048), dict(items_per_thread=8), dict(binned=4)]
+    eager = {}
+    for i, g in enumerate(gs):
+        for j, f in enumerate(forms):
+            eager[i, j] = gb.pagerank(g, use_graph=False, **f)[0].cpu().numpy()
+    for _ in range(2):
+        for j, f in enumerate(forms):
+            for i, g in enumerate(gs):
+                x = gb.pagerank(g, use_graph=True, **f)[0].cpu().numpy()
+                assert np.array_equal(x, eager[i, j]), (i, f)
+    del gs[0]
+    import gc
+    gc.collect()
+    g3 = gb.Graph.from_host(hs[0], reverse=hs[0].transpose())
+    for j, f in enumerate(forms):
+        assert np.array_equal(gb.pagerank(g3, use_graph=True, **f)[0].cpu().numpy(), eager[0, j])
+    x0 = gb.to_device(np.asarray(eager[0, 0], np.float32))
+    xa = gb.pagerank(g3, use_graph=True, x0=x0, binned=1)[0].cpu().numpy()
+    xb = gb.pagerank(g3, use_graph=False, x0=x0, binned=1)[0].cpu().numpy()
+    assert np.array_equal(xa, xb)
+
+
 def test_pagerank_graph_vs_eager(gpu):
     gb = gpu
     h = gb.gen_rmat(12, 8, seed=9, weights="none")
