@@ -823,6 +823,7 @@ constexpr int PB_SCATTER_BLOCK = 512;
 constexpr int PB_ACCUM_BLOCK = 1024;
 constexpr int PB_ACCUM_WAVES = PB_ACCUM_BLOCK / GB_WAVE;
 constexpr uint16_t PB_PAD = 0xFFFF;  // padding entry of the source stream
+constexpr int PB_ALIGN = 4;          // segments padded to a multiple of this many entries
 
 struct PbTask {
   int32_t in0, out0, n;  // n entries (multiple of 4) at src16[in0..], vals[out0..]
@@ -1331,7 +1332,8 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
       1 + (V > (1 << hsl) ? (((int64_t)V - (1 << hsl) + (1 << csl) - 1) >> csl) : 0);
   const int64_t nbins = ((int64_t)V + (1 << bsl) - 1) >> bsl;
   const int64_t nseg = nchunks * nbins;
-  if (nseg > ((int64_t)1 << 26) || E + 3 * std::min<int64_t>(nseg, E) > INT32_MAX - 4)
+  if (nseg > ((int64_t)1 << 26) ||
+      E + (PB_ALIGN - 1) * std::min<int64_t>(nseg, E) > INT32_MAX - PB_ALIGN)
     return gb_fail(GB_ERR_UNSUPPORTED, "gb_pagerank: graph too large for the binned form");
   int vbits = 1;
   while (((int64_t)1 << vbits) < V) vbits++;
@@ -1379,6 +1381,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
       }
   }
   auto pad4 = [](int64_t n) { return (n + 3) & ~(int64_t)3; };
+  auto padseg = [](int64_t n) { return (n + PB_ALIGN - 1) & ~(int64_t)(PB_ALIGN - 1); };
   std::vector<int32_t> d1((size_t)nseg), d2((size_t)nseg);
   std::vector<int32_t> out_start((size_t)nseg);
   std::vector<int32_t> bin_start((size_t)nbins + 1);
@@ -1389,7 +1392,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
       const int64_t i = b * nchunks + c;
       out_start[i] = (int32_t)P;
       d2[i] = (int32_t)(P - (first[i] >= 0 ? first[i] : 0));
-      P += pad4(cnt[i]);
+      P += padseg(cnt[i]);
     }
   }
   bin_start[nbins] = (int32_t)P;
@@ -1405,7 +1408,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
       for (int64_t b = 0; b < nbins; ++b) {
         const int64_t i = b * nchunks + c;
         d1[i] = (int32_t)(in - (first[i] >= 0 ? first[i] : 0));
-        const int64_t n = pad4(cnt[i]);
+        const int64_t n = padseg(cnt[i]);
         if (c == 0) {  // read by pb_accum from src16, no scatter
           hub_in[b] = (int32_t)in;
           in += n;
@@ -1433,7 +1436,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
     const int64_t s = bin_start[b], n = bin_start[b + 1] - s;
     const int64_t k = std::max<int64_t>(1, (n + piece - 1) / piece);
     if (k > 1) multi.push_back((int32_t)b);
-    const int32_t hub_e1 = (int32_t)(s + pad4(cnt[b * nchunks]));
+    const int32_t hub_e1 = (int32_t)(s + padseg(cnt[b * nchunks]));
     for (int64_t j = 0; j < k; ++j)
       pieces.push_back(PbPiece{(int32_t)b, (int32_t)(s + pad4(n * j / k)),
                                (int32_t)(j + 1 == k ? s + n : s + pad4(n * (j + 1) / k)),
