@@ -520,15 +520,21 @@ __global__ void mbc_bwd_masks_kernel(BcCtx c, int off) {
     m[c.pool_v[base + i]] = WRITE ? c.pool_m[base + i] : 0ull;
 }
 
-// Pull over out-edges [beg, end): lane s adds ((1+delta[w][s])*sv)/sigma[w][s]
-// for every successor w one level deeper for source s (its mask in fmd1), in
-// CSR order.
+// The backward pass sums, per vertex v and source s, psi[w][s] over the
+// successors w one level deeper, psi = (1 + delta) / sigma: delta[v][s] =
+// sigma[v][s] * sum psi (csr_bc.c:98-123 adds (1 + delta[w]) * sigma[v] /
+// sigma[w] per successor; the same sum, factored, so a successor costs one
+// 512-byte row read instead of two).  Once delta[v][s] is known, psi[v][s]
+// replaces sigma[v][s], which no later level reads.
+//
+// Pull over out-edges [beg, end): lane s adds psi[w][s] (held in the sigma
+// array) for every successor w one level deeper for source s (its mask in
+// fmd1), in CSR order.
 __device__ __forceinline__ double pull_out(int lane, int32_t beg, int32_t end,
                                            const int32_t *__restrict__ col,
                                            const unsigned long long *__restrict__ fmd1,
-                                           uint64_t mine, double sv,
-                                           const double *__restrict__ sigma,
-                                           const double *__restrict__ delta, int64_t gV) {
+                                           uint64_t mine, const double *__restrict__ psi,
+                                           int64_t gV) {
   double acc = 0.0;
   for (int32_t e0 = beg; e0 < end; e0 += GB_WAVE) {
     const int32_t e = e0 + lane;
@@ -540,7 +546,7 @@ __device__ __forceinline__ double pull_out(int lane, int32_t beg, int32_t end,
     }
     uint64_t hit = __ballot(m != 0);
     while (hit) {
-      double dw[8], sw[8];
+      double pw[8];
       bool on[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
@@ -550,13 +556,11 @@ __device__ __forceinline__ double pull_out(int lane, int32_t beg, int32_t end,
         const int32_t wj = __shfl(w, j, GB_WAVE);
         const uint64_t mj = ok ? gb_shfl_u64(m, j) : 0ull;
         on[k] = (mj >> lane) & 1ull;
-        const int64_t r = ((int64_t)gV + wj) * 64 + lane;
-        dw[k] = on[k] ? delta[r] : 0.0;
-        sw[k] = on[k] ? sigma[r] : 1.0;
+        pw[k] = on[k] ? psi[((int64_t)gV + wj) * 64 + lane] : 0.0;
       }
 #pragma unroll
       for (int k = 0; k < 8; k++)
-        if (on[k]) acc += ((1.0 + dw[k]) * sv) / sw[k];
+        if (on[k]) acc += pw[k];
     }
   }
   return acc;
@@ -572,6 +576,7 @@ __global__ __launch_bounds__(MB) void mbc_bwd_kernel(BcCtx c) {
   const int64_t base = (int64_t)g * c.pool_cap + lo[bd];
   const uint64_t n = (uint64_t)(lo[bd + 1] - lo[bd]);
   const unsigned long long *fm1 = c.fm[(bd + 1) & 1] + gV;
+  const bool deepest = bd == c.st->depth - 1;
   const uint64_t nwaves = wave_count();
   for (uint64_t i = wave_id(); i < n; i += nwaves) {
     const int32_t v = c.pool_v[base + i];
@@ -579,9 +584,15 @@ __global__ __launch_bounds__(MB) void mbc_bwd_kernel(BcCtx c) {
     if (end - beg > c.bch) continue;  // split over several waves by the hub kernels
     const uint64_t mine = c.pool_m[base + i];
     const bool me = (mine >> lane) & 1ull;
-    const double sv = me ? c.sigma[(gV + v) * 64 + lane] : 0.0;
-    const double acc = pull_out(lane, beg, end, c.col, fm1, mine, sv, c.sigma, c.delta, gV);
-    if (me) c.delta[(gV + v) * 64 + lane] = acc;
+    // the deepest level has no successors
+    const double acc = deepest ? 0.0 : pull_out(lane, beg, end, c.col, fm1, mine, c.sigma, gV);
+    if (me) {
+      const int64_t r = (gV + v) * 64 + lane;
+      const double sv = c.sigma[r];
+      const double d = sv * acc;
+      c.delta[r] = d;
+      c.sigma[r] = (1.0 + d) / sv;  // psi
+    }
   }
 }
 
@@ -590,6 +601,7 @@ __global__ __launch_bounds__(MB) void mbc_bwd_hub_kernel(BcCtx c) {
   const int32_t g = blockIdx.y;
   if (c.st->bdone || c.nhub_out == 0) return;
   const int bd = c.st->bd;
+  if (bd == c.st->depth - 1) return;  // no successors: the combine writes 0
   const int64_t gV = (int64_t)g * c.V;
   const int lane = gb_lane();
   const uint64_t items = (uint64_t)c.nhub_out * c.maxch_out;
@@ -604,9 +616,7 @@ __global__ __launch_bounds__(MB) void mbc_bwd_hub_kernel(BcCtx c) {
     if (beg >= end) continue;
     const uint64_t mine = fm0[x];
     if (mine == 0) continue;
-    const bool me = (mine >> lane) & 1ull;
-    const double sv = me ? c.sigma[(gV + x) * 64 + lane] : 0.0;
-    const double acc = pull_out(lane, beg, end, c.col, fm1, mine, sv, c.sigma, c.delta, gV);
+    const double acc = pull_out(lane, beg, end, c.col, fm1, mine, c.sigma, gV);
     c.part[((int64_t)g * items + it) * 64 + lane] = acc;
   }
 }
@@ -625,11 +635,15 @@ __global__ __launch_bounds__(MB) void mbc_bwd_combine_kernel(BcCtx c) {
     const int32_t x = c.hub_out[i];
     const uint64_t mine = fm0[x];
     if (!((mine >> lane) & 1ull)) continue;
-    const int32_t nch = (c.rp[x + 1] - c.rp[x] + c.bch - 1) / c.bch;
+    const int32_t nch = bd == c.st->depth - 1 ? 0 : (c.rp[x + 1] - c.rp[x] + c.bch - 1) / c.bch;
     double acc = 0.0;
     for (int32_t k = 0; k < nch; k++)
       acc += c.part[((int64_t)g * items + (int64_t)i * c.maxch_out + k) * 64 + lane];
-    c.delta[(gV + x) * 64 + lane] = acc;
+    const int64_t r = (gV + x) * 64 + lane;
+    const double sv = c.sigma[r];
+    const double d = sv * acc;
+    c.delta[r] = d;
+    c.sigma[r] = (1.0 + d) / sv;  // psi
   }
 }
 
@@ -961,13 +975,12 @@ gb_status ms_run(gb_graph g, const int32_t *h_src, int32_t nsrc, bool with_sigma
   if (depth_out) *depth_out = depth;
   if (!with_sigma) return GB_OK;
   if (depth >= 2) {
-    // the deepest level's masks (the forward pass left both arrays zero),
-    // then levels depth-2 .. 0: clear level bd + 2, write level bd, process
-    hl.bd = depth - 1;
+    // levels depth-1 .. 0 (the forward pass left both mask arrays zero): clear
+    // level bd + 2, write level bd, process; the deepest level only turns
+    // sigma into psi
+    hl.bd = depth;
     hl.bdone = 0;
     GB_HIP(hipMemcpyAsync(c.st, &hl, sizeof(hl), hipMemcpyHostToDevice, cs));
-    hipLaunchKernelGGL(mbc_bwd_masks_kernel<true>, grid, block, 0, cs, c, 0);
-    GB_HIP(hipGetLastError());
     GB_HIP(hipStreamSynchronize(cs));
     LevelGraph bg;
     GB_TRY(capture(bg, cs, [&]() {

@@ -352,10 +352,10 @@ def test_bc_rmat_undirected(gpu, batch):
     bc = gb.bc(g, srcs, batch=batch).cpu().numpy()
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
     assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
-    # lanes sum in CSR order and sources accumulate in order: identical bits
-    # (vertices above the hub threshold sum their delta in 1024-edge chunks)
-    if h.outdeg().max() <= 1024:
-        assert np.array_equal(bc, ref)
+    # the backward pass sums (1 + delta[w]) / sigma[w] over successors and
+    # multiplies by sigma[v] once, where the oracle multiplies and divides per
+    # successor as csr_bc.c does: a few ulps apart
+    np.testing.assert_allclose(bc, ref, rtol=1e-12, atol=1e-12)
 
 
 def test_bc_and_bfs_hub_split(gpu, monkeypatch):
@@ -391,12 +391,16 @@ def test_bc_many_sources_and_duplicates(gpu):
     _say(f"bc many sources n={len(srcs)}")
     g = gb.Graph.from_host(h, need_reverse=False)
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs)
-    exact = h.outdeg().max() <= 1024
+    first = None
     for batch in (0, 64, 100):
         bc = gb.bc(g, srcs, batch=batch).cpu().numpy()
         assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12), batch
-        if exact:
-            assert np.array_equal(bc, ref), batch
+        np.testing.assert_allclose(bc, ref, rtol=1e-12, atol=1e-12)
+        # lanes sum in CSR order and sources accumulate in order: the batch
+        # split changes no bit
+        if first is None:
+            first = bc
+        assert np.array_equal(bc, first), batch
 
 
 def test_bfs_many_sources(gpu):
@@ -420,7 +424,7 @@ def test_bc_directed(gpu):
     bc = gb.bc(g, srcs).cpu().numpy()
     ref = oracle.brandes_bfs(h.row_ptr, h.col, srcs, r.row_ptr, r.col)
     assert np.allclose(bc, ref, rtol=BC_RTOL, atol=1e-12)
-    assert np.array_equal(bc, ref)  # no vertex above the hub threshold here
+    np.testing.assert_allclose(bc, ref, rtol=1e-12, atol=1e-12)
 
 
 def test_bc_weighted(gpu):
