@@ -896,31 +896,35 @@ __global__ __launch_bounds__(PB_SCATTER_BLOCK) void pb_scatter_kernel(
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / GB_WAVE);
   const int lane = gb_lane(), l4 = lane * 4;
-  // a wave takes 64 consecutive tasks (one descriptor per lane, one load) and
-  // works through them R at a time, their id loads in flight together
+  // waves take R consecutive tasks in turn (wave w: tasks t0 + w * R + k *
+  // WAVES * R), so a unit's tasks spread over all its waves; a round's R
+  // descriptors come one per lane, loaded during the round before, and its R
+  // id loads are in flight together
   constexpr int R = 8;
-  for (int32_t base = u.t0 + wave * GB_WAVE; base < u.t1; base += WAVES * GB_WAVE) {
-    const int nt = min(GB_WAVE, u.t1 - base);
-    PbTask my = {0, 0, 0};
-    if (lane < nt) my = tasks[base + lane];
-    for (int j = 0; j < nt; j += R) {
-      int32_t in0[R], out0[R], n[R];
-      pb_u32x2 ids[R];
+  constexpr int32_t STRIDE = WAVES * R;
+  int32_t base = u.t0 + wave * R;
+  PbTask my = {0, 0, 0};
+  if (lane < R && base + lane < u.t1) my = tasks[base + lane];
+  for (; base < u.t1; base += STRIDE) {
+    const int nt = min(R, u.t1 - base);
+    PbTask nx = {0, 0, 0};
+    if (lane < R && base + STRIDE + lane < u.t1) nx = tasks[base + STRIDE + lane];
+    int32_t in0[R], out0[R], n[R];
+    pb_u32x2 ids[R];
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int q = (j + k) & (GB_WAVE - 1);
-        n[k] = j + k < nt ? __builtin_amdgcn_readlane(my.n, q) : 0;
-        in0[k] = __builtin_amdgcn_readlane(my.in0, q);
-        out0[k] = __builtin_amdgcn_readlane(my.out0, q);
-      }
-#pragma unroll
-      for (int k = 0; k < R; ++k)
-        if (l4 < n[k]) ids[k] = pb_ids4(src16 + in0[k] + l4);
-#pragma unroll
-      for (int k = 0; k < R; ++k)
-        if (l4 < n[k])
-          *reinterpret_cast<pb_f32x4 *>(vals + out0[k] + l4) = pb_lookup4<CS>(s_c, ids[k]);
+    for (int k = 0; k < R; ++k) {
+      n[k] = k < nt ? __builtin_amdgcn_readlane(my.n, k) : 0;
+      in0[k] = __builtin_amdgcn_readlane(my.in0, k);
+      out0[k] = __builtin_amdgcn_readlane(my.out0, k);
     }
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (l4 < n[k]) ids[k] = pb_ids4(src16 + in0[k] + l4);
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (l4 < n[k])
+        *reinterpret_cast<pb_f32x4 *>(vals + out0[k] + l4) = pb_lookup4<CS>(s_c, ids[k]);
+    my = nx;
   }
 }
 
