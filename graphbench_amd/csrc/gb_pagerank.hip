@@ -900,7 +900,7 @@ __global__ __launch_bounds__(PB_SCATTER_BLOCK) void pb_scatter_kernel(
   // WAVES * R), so a unit's tasks spread over all its waves; a round's R
   // descriptors come one per lane, loaded during the round before, and its R
   // id loads are in flight together
-  constexpr int R = 8;
+  constexpr int R = 16;
   constexpr int32_t STRIDE = WAVES * R;
   int32_t base = u.t0 + wave * R;
   PbTask my = {0, 0, 0};
@@ -959,16 +959,21 @@ __global__ __launch_bounds__(PB_ACCUM_BLOCK) void pb_accum_kernel(
   const double scale = ldexp(1.0, S);
   const PbPiece pc = pieces[blockIdx.x];
   const int32_t he = min(pc.e1, pc.hub_e1);  // chunk-0 entries of the piece: [e0, he)
-  for (int k = threadIdx.x; k < BS; k += PB_ACCUM_BLOCK) s_acc[k] = 0ull;
-  if (pc.e0 < he) {
-    constexpr int PER = CS / PB_ACCUM_BLOCK;
+  // the hub copy's loads go out first, the accumulators are cleared while
+  // they are in flight
+  constexpr int PER = CS / PB_ACCUM_BLOCK;
+  const bool hub = pc.e0 < he;
+  float c[PER];
+  if (hub) {
     const int32_t n = min(CS, st->V);
-    float c[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int k = (int)threadIdx.x + j * PB_ACCUM_BLOCK;
       c[j] = contrib[k < n ? k : n - 1];
     }
+  }
+  for (int k = threadIdx.x; k < BS; k += PB_ACCUM_BLOCK) s_acc[k] = 0ull;
+  if (hub) {
 #pragma unroll
     for (int j = 0; j < PER; ++j) s_hot[threadIdx.x + j * PB_ACCUM_BLOCK] = c[j];
   }
@@ -1012,8 +1017,22 @@ __global__ __launch_bounds__(PB_ACCUM_BLOCK) void pb_accum_kernel(
       if (a) atomicAdd(&sum64[v0 + k], a);
     }
   } else {
-    for (int k = threadIdx.x; k < nv; k += PB_ACCUM_BLOCK)
-      out.put(v0 + k, pb_value(s_acc[k], S), out.x_old[v0 + k], out.inv_od[v0 + k], res, dang);
+    // every row's old rank and 1/outdeg loads in flight together (rows
+    // clamped into the bin; nv >= 1)
+    constexpr int RPT = BS / PB_ACCUM_BLOCK;
+    float xo[RPT], io[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int k = (int)threadIdx.x + j * PB_ACCUM_BLOCK;
+      const int kc = k < nv ? k : nv - 1;
+      xo[j] = out.x_old[v0 + kc];
+      io[j] = out.inv_od[v0 + kc];
+    }
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int k = (int)threadIdx.x + j * PB_ACCUM_BLOCK;
+      if (k < nv) out.put(v0 + k, pb_value(s_acc[k], S), xo[j], io[j], res, dang);
+    }
   }
   __shared__ double s_red[PB_ACCUM_WAVES];
   res = block_sum_f64(res, s_red);
