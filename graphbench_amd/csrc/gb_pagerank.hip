@@ -1608,8 +1608,7 @@ static gb_status pick_shape(gb_graph g, const gb_pr_params *p, float *d_x, void 
     for (int rb : {1024, 2048, 4096})
       for (int ch : rl ? std::vector<int>{0, 4096, 8192} : std::vector<int>{0})
         cands.push_back(PrShape{0, ch, rb, 0});
-  if (try_pb)
-    for (int sh : {1, 4}) cands.push_back(PrShape{0, 0, 0, sh});
+  if (try_pb) cands.push_back(PrShape{0, 0, 0, 4});
   PrShape best = cands[0];
   double best_ms = 1e300;
   for (const PrShape &c : cands) {
