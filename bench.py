@@ -478,7 +478,7 @@ def main():
     ap.add_argument("--row-block", type=int, default=0,
                     help="PageRank gather form: 0 tuned, 1024/2048/4096 row blocks, -1 merge path")
     ap.add_argument("--binned", type=int, default=0,
-                    help="PageRank binned form: 0 tuned, -1 never, 1..4 that (hub, chunk, bin) shape")
+                    help="PageRank binned form: 0 tuned, -1 never, 1..6 that (hub, chunk, bin) shape")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0,

@@ -379,7 +379,7 @@ def pagerank(g: Graph, damping: float = 0.85, tol: float = 1e-7, max_iter: int =
     order (built once per graph; results are in the caller's vertex ids).
     row_block: 0 autotuned form, 1024/2048/4096 row blocks of that many
     edges, -1 merge path only.  binned: 0 left to the autotuner, -1 never,
-    1..4 the binned (propagation-blocking) form with that (hub, chunk, bin) shape."""
+    1..6 the binned (propagation-blocking) form with that (hub, chunk, bin) shape."""
     torch = _torch()
     if out is None:
         out = torch.empty(g.V, dtype=torch.float32, device="cuda")

@@ -115,7 +115,7 @@ struct gb_graph_s {
   // widths) the edge streams and work lists, built once per handle
   struct PrBinned {
     bool valid = false, relabeled = false;
-    int32_t hsl = 0, csl = 0, bsl = 0;  // log2 hub chunk / chunk / bin widths
+    int32_t hub = 0, csl = 0, bsl = 0;  // hub chunk width, log2 chunk / bin widths
     int32_t piece = 0;                  // entries per accumulation piece at most
     int64_t padded = 0;                   // stream entries (segments padded to 4)
     int64_t ntasks = 0, nunits = 0, npieces = 0, nmulti = 0;

@@ -796,7 +796,7 @@ __global__ __launch_bounds__(PR_BLOCK) void pr_rb_fixup_kernel(
 // The pull gather reads contrib[u] once per in-edge at a random address: one
 // L2 request per edge, and the L2's request rate bounds it.  The binned form
 // (propagation blocking) streams instead.  Sources are cut into a hub chunk
-// (chunk 0: the first 2^HSL ids, the highest out-degrees once relabelled)
+// (chunk 0: the first HUB ids, the highest out-degrees once relabelled)
 // and chunks of 2^CSL ids after it, destinations into bins of 2^BSL ids; the
 // edges of each (chunk, bin) pair form a segment, padded to a multiple of 4
 // entries.
@@ -940,7 +940,7 @@ __device__ __forceinline__ void pb_add4(unsigned long long *s_acc, pb_f32x4 v, p
   if (f3) atomicAdd(&s_acc[(d.y >> 16) & M], (unsigned long long)f3);
 }
 
-template <int HSL, int BSL>
+template <int HUB, int BSL>
 __global__ __launch_bounds__(PB_ACCUM_BLOCK) void pb_accum_kernel(
     const PbPiece *__restrict__ pieces, const float *__restrict__ vals,
     const uint16_t *__restrict__ src16, const uint16_t *__restrict__ dst16,
@@ -948,7 +948,8 @@ __global__ __launch_bounds__(PB_ACCUM_BLOCK) void pb_accum_kernel(
     double *__restrict__ part_res, double *__restrict__ part_dang,
     const PrState *__restrict__ st) {
   constexpr int BS = 1 << BSL;
-  constexpr int CS = 1 << HSL;  // hub chunk width
+  constexpr int CS = HUB;  // hub chunk width (a multiple of the block size)
+  static_assert(HUB % PB_ACCUM_BLOCK == 0, "hub chunk width");
   constexpr int STEP = PB_ACCUM_BLOCK * 4;
   constexpr int U = 4;  // batches of loads in flight per thread
   __shared__ unsigned long long s_acc[BS];
@@ -1093,26 +1094,26 @@ __global__ void pb_keys_kernel(int32_t V, int64_t E, const int32_t *__restrict__
   vs[e] = lo;
 }
 
-// Chunk of source u: 0 for the hub chunk [0, 2^hsl), then chunks of 2^csl.
-__host__ __device__ __forceinline__ int32_t pb_chunk(int32_t u, int hsl, int csl) {
-  return u < (1 << hsl) ? 0 : 1 + ((u - (1 << hsl)) >> csl);
+// Chunk of source u: 0 for the hub chunk [0, hub), then chunks of 2^csl.
+__host__ __device__ __forceinline__ int32_t pb_chunk(int32_t u, int hub, int csl) {
+  return u < hub ? 0 : 1 + ((u - hub) >> csl);
 }
-__host__ __device__ __forceinline__ int32_t pb_chunk_base(int32_t c, int hsl, int csl) {
-  return c == 0 ? 0 : (1 << hsl) + ((c - 1) << csl);
+__host__ __device__ __forceinline__ int32_t pb_chunk_base(int32_t c, int hub, int csl) {
+  return c == 0 ? 0 : hub + ((c - 1) << csl);
 }
 
 // First sorted index of every non-empty (bin, chunk) segment.
-__global__ void pb_first_kernel(int64_t E, const uint64_t *__restrict__ keys, int vbits, int hsl,
+__global__ void pb_first_kernel(int64_t E, const uint64_t *__restrict__ keys, int vbits, int hub,
                                 int csl, int64_t nchunks, int32_t *__restrict__ first) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= E) return;
   const uint64_t mask = (1ull << vbits) - 1;
   const uint64_t key = keys[k];
-  const int64_t seg = (int64_t)(key >> vbits) * nchunks + pb_chunk((int32_t)(key & mask), hsl, csl);
+  const int64_t seg = (int64_t)(key >> vbits) * nchunks + pb_chunk((int32_t)(key & mask), hub, csl);
   bool head = k == 0;
   if (!head) {
     const uint64_t p = keys[k - 1];
-    head = (int64_t)(p >> vbits) * nchunks + pb_chunk((int32_t)(p & mask), hsl, csl) != seg;
+    head = (int64_t)(p >> vbits) * nchunks + pb_chunk((int32_t)(p & mask), hub, csl) != seg;
   }
   if (head) first[seg] = (int32_t)k;
 }
@@ -1120,7 +1121,7 @@ __global__ void pb_first_kernel(int64_t E, const uint64_t *__restrict__ keys, in
 // Sorted edge k of segment (b, c) goes to src16[k + d1[seg]] (chunk-major
 // order) and dst16[k + d2[seg]] (bin-major order) as local ids.
 __global__ void pb_place_kernel(int64_t E, const uint64_t *__restrict__ keys,
-                                const int32_t *__restrict__ vs, int vbits, int hsl, int csl,
+                                const int32_t *__restrict__ vs, int vbits, int hub, int csl,
                                 int bsl,
                                 int64_t nchunks, const int32_t *__restrict__ d1,
                                 const int32_t *__restrict__ d2, uint16_t *__restrict__ src16,
@@ -1130,9 +1131,9 @@ __global__ void pb_place_kernel(int64_t E, const uint64_t *__restrict__ keys,
   const uint64_t key = keys[k];
   const int32_t b = (int32_t)(key >> vbits);
   const int32_t u = (int32_t)(key & ((1ull << vbits) - 1));
-  const int32_t c = pb_chunk(u, hsl, csl);
+  const int32_t c = pb_chunk(u, hub, csl);
   const int64_t seg = (int64_t)b * nchunks + c;
-  src16[k + d1[seg]] = (uint16_t)(u - pb_chunk_base(c, hsl, csl));
+  src16[k + d1[seg]] = (uint16_t)(u - pb_chunk_base(c, hub, csl));
   dst16[k + d2[seg]] = (uint16_t)(vs[k] - (b << bsl));
 }
 
@@ -1332,7 +1333,7 @@ static gb_status ensure_inv_outdeg(gb_graph g, hipStream_t st) {
   return GB_OK;
 }
 
-// Binned layout of the gather's in-CSR (irp, icol) for a hub chunk of 2^hsl
+// Binned layout of the gather's in-CSR (irp, icol) for a hub chunk of hub
 // sources, chunks of 2^csl sources after it and bins of 2^bsl destinations,
 // cached on the handle.  Edges are
 // sorted by (bin, source) with a stable radix sort (in-CSR order, so by
@@ -1343,16 +1344,16 @@ static gb_status ensure_inv_outdeg(gb_graph g, hipStream_t st) {
 // stream of at most `piece` entries: one accumulation block each; largest
 // first, so that the block scheduler hands out the long ones first).
 static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const int32_t *icol,
-                          int hsl, int csl, int bsl, int piece, hipStream_t st) {
+                          int hub, int csl, int bsl, int piece, hipStream_t st) {
   gb_graph_s::PrBinned &L = g->pr_pb;
-  if (L.valid && L.relabeled == relabel && L.hsl == hsl && L.csl == csl && L.bsl == bsl &&
+  if (L.valid && L.relabeled == relabel && L.hub == hub && L.csl == csl && L.bsl == bsl &&
       L.piece == piece)
     return GB_OK;
   L.valid = false;
   const int32_t V = g->V;
   const int64_t E = g->E;
   const int64_t nchunks =
-      1 + (V > (1 << hsl) ? (((int64_t)V - (1 << hsl) + (1 << csl) - 1) >> csl) : 0);
+      1 + (V > hub ? (((int64_t)V - hub + (1 << csl) - 1) >> csl) : 0);
   const int64_t nbins = ((int64_t)V + (1 << bsl) - 1) >> bsl;
   const int64_t nseg = nchunks * nbins;
   if (nseg > ((int64_t)1 << 26) ||
@@ -1388,7 +1389,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
     GB_TRY(dfirst.ensure(sizeof(int32_t) * nseg));
     GB_HIP(hipMemsetAsync(dfirst.p, 0xFF, sizeof(int32_t) * nseg, st));
     hipLaunchKernelGGL(pb_first_kernel, dim3(eb), dim3(256), 0, st, E, keys2.as<uint64_t>(),
-                       vbits, hsl, csl, nchunks, dfirst.as<int32_t>());
+                       vbits, hub, csl, nchunks, dfirst.as<int32_t>());
     GB_HIP(hipMemcpyAsync(first.data(), dfirst.p, sizeof(int32_t) * nseg, hipMemcpyDeviceToHost,
                           st));
     GB_HIP(hipStreamSynchronize(st));
@@ -1442,7 +1443,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
                                  (int32_t)std::min<int64_t>(PB_TASK, n - j)});
           ue += std::min<int64_t>(PB_TASK, n - j);
           if (ue >= unit) {
-            units.push_back(PbUnit{pb_chunk_base((int32_t)c, hsl, csl), ut0, (int32_t)tasks.size()});
+            units.push_back(PbUnit{pb_chunk_base((int32_t)c, hub, csl), ut0, (int32_t)tasks.size()});
             ut0 = (int32_t)tasks.size();
             ue = 0;
           }
@@ -1450,7 +1451,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
         in += n;
       }
       if ((int32_t)tasks.size() > ut0)
-        units.push_back(PbUnit{pb_chunk_base((int32_t)c, hsl, csl), ut0, (int32_t)tasks.size()});
+        units.push_back(PbUnit{pb_chunk_base((int32_t)c, hub, csl), ut0, (int32_t)tasks.size()});
     }
   }
   std::vector<PbPiece> pieces;
@@ -1487,7 +1488,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
     GB_HIP(hipMemcpyAsync(dd1.p, d1.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, st));
     GB_HIP(hipMemcpyAsync(dd2.p, d2.data(), sizeof(int32_t) * nseg, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(pb_place_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, E,
-                       keys2.as<uint64_t>(), vs2.as<int32_t>(), vbits, hsl, csl, bsl, nchunks,
+                       keys2.as<uint64_t>(), vs2.as<int32_t>(), vbits, hub, csl, bsl, nchunks,
                        dd1.as<int32_t>(), dd2.as<int32_t>(), L.src16.as<uint16_t>(),
                        L.dst16.as<uint16_t>());
     GB_HIP(hipStreamSynchronize(st));  // dd1 / dd2 go out of scope
@@ -1507,7 +1508,7 @@ static gb_status pb_build(gb_graph g, bool relabel, const int32_t *irp, const in
   GB_HIP(hipGetLastError());
   L.valid = true;
   L.relabeled = relabel;
-  L.hsl = hsl;
+  L.hub = hub;
   L.csl = csl;
   L.bsl = bsl;
   L.piece = piece;
@@ -1527,10 +1528,12 @@ struct PrShape {
   int ipt, hot, rb, pb;
 };
 struct PbShape {
-  int hsl, csl, bsl;  // log2 of the hub chunk, chunk and bin widths
+  int hub, csl, bsl;  // hub chunk width, log2 of the chunk and bin widths
 };
-constexpr PbShape PB_SHAPES[] = {{0, 0, 0}, {13, 13, 13}, {14, 14, 13}, {13, 13, 12}, {14, 13, 13}};
-constexpr int PB_NSHAPES = 5;
+constexpr PbShape PB_SHAPES[] = {{0, 0, 0},         {8192, 13, 13},  {16384, 14, 13},
+                                 {8192, 13, 12},    {16384, 13, 13}, {20480, 13, 13},
+                                 {16384, 12, 13}};
+constexpr int PB_NSHAPES = 7;
 // stream entries per accumulation block at most (GB_PR_PB_PIECE, multiple of 4)
 static int pb_piece() {
   const char *e = getenv("GB_PR_PB_PIECE");
@@ -1652,7 +1655,7 @@ extern "C" gb_status gb_pagerank(gb_graph g, const gb_pr_params *p, float *d_x,
   if (p->row_block > 0 && p->items_per_thread != 0)
     return gb_fail(GB_ERR_INVALID, "gb_pagerank: row_block and items_per_thread both set");
   if (p->binned < -1 || p->binned >= PB_NSHAPES)
-    return gb_fail(GB_ERR_INVALID, "gb_pagerank: binned must be -1 or 0 to 4");
+    return gb_fail(GB_ERR_INVALID, "gb_pagerank: binned must be -1 or 0 to 6");
   if (p->binned > 0 && (p->row_block != 0 || p->items_per_thread != 0))
     return gb_fail(GB_ERR_INVALID, "gb_pagerank: binned set with row_block or items_per_thread");
   PrShape sh{8, 0, 0, 0};
@@ -1693,7 +1696,7 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
   const int32_t *icol = relabel ? g->rl_rcol.as<int32_t>() : g->rcol;
   const float *inv = relabel ? g->rl_inv.as<float>() : g->inv_outdeg.as<float>();
   if (pb)
-    GB_TRY(pb_build(g, relabel, irp, icol, PB_SHAPES[pb].hsl, PB_SHAPES[pb].csl,
+    GB_TRY(pb_build(g, relabel, irp, icol, PB_SHAPES[pb].hub, PB_SHAPES[pb].csl,
                     PB_SHAPES[pb].bsl, pb_piece(), st));
   if (!rb && !pb && (g->pr_tile_items != tile || g->pr_tile_relabeled != relabel)) {
     const int64_t ntiles = ((int64_t)V + g->E + tile - 1) / tile;
@@ -1777,20 +1780,17 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     if (ev0) (void)hipEventRecord(ev0, st);
     if (pb) {
       if (PB.nunits > 0) {
-        if (PB.csl == 14)
-          hipLaunchKernelGGL(pb_scatter_kernel<14>, dim3((unsigned)PB.nunits),
-                             dim3(PB_SCATTER_BLOCK), 0, st, PB.units.as<PbUnit>(),
-                             PB.tasks.as<PbTask>(), PB.src16.as<uint16_t>(), c_old,
-                             PB.vals.as<float>(), state);
-        else
-          hipLaunchKernelGGL(pb_scatter_kernel<13>, dim3((unsigned)PB.nunits),
-                             dim3(PB_SCATTER_BLOCK), 0, st, PB.units.as<PbUnit>(),
-                             PB.tasks.as<PbTask>(), PB.src16.as<uint16_t>(), c_old,
-                             PB.vals.as<float>(), state);
+        auto scatter = PB.csl == 14   ? pb_scatter_kernel<14>
+                       : PB.csl == 12 ? pb_scatter_kernel<12>
+                                      : pb_scatter_kernel<13>;
+        hipLaunchKernelGGL(scatter, dim3((unsigned)PB.nunits), dim3(PB_SCATTER_BLOCK), 0, st,
+                           PB.units.as<PbUnit>(), PB.tasks.as<PbTask>(),
+                           PB.src16.as<uint16_t>(), c_old, PB.vals.as<float>(), state);
       }
-      auto accum = (PB.hsl == 14)   ? pb_accum_kernel<14, 13>
-                   : (PB.bsl == 12) ? pb_accum_kernel<13, 12>
-                                    : pb_accum_kernel<13, 13>;
+      auto accum = PB.hub == 20480  ? pb_accum_kernel<20480, 13>
+                   : PB.hub == 16384 ? pb_accum_kernel<16384, 13>
+                   : PB.bsl == 12    ? pb_accum_kernel<8192, 12>
+                                     : pb_accum_kernel<8192, 13>;
       hipLaunchKernelGGL(accum, dim3((unsigned)PB.npieces), dim3(PB_ACCUM_BLOCK), 0, st,
                          PB.pieces.as<PbPiece>(), (const float *)PB.vals.as<float>(),
                          (const uint16_t *)PB.src16.as<uint16_t>(), PB.dst16.as<uint16_t>(),
@@ -1874,7 +1874,7 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
         u(xb), u(ca), u(cb), u(w.tail.p), u(w.head.p), u(w.pres.p), u(w.pdang.p), u(w.fres.p),
         u(w.fdang.p), u(w.counter.p), u(state), u(L.src16.p), u(L.dst16.p), u(L.vals.p),
         u(L.tasks.p), u(L.units.p), u(L.pieces.p), u(L.multi.p), u(L.sum64.p),
-        (uint64_t)L.nunits, (uint64_t)L.npieces, (uint64_t)L.nmulti, (uint64_t)L.hsl,
+        (uint64_t)L.nunits, (uint64_t)L.npieces, (uint64_t)L.nmulti, (uint64_t)L.hub,
         (uint64_t)L.csl, (uint64_t)L.bsl};
     if (!w.exec || w.exec_key != key) {
       if (w.exec) (void)hipGraphExecDestroy(w.exec);

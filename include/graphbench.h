@@ -204,10 +204,12 @@ typedef struct gb_pr_params {
                         binned), 1024 / 2048 / 4096 row blocks of that many
                         edges, -1 merge path only */
   int32_t binned;    /* binned (propagation-blocking) form: 0 left to the
-                        autotuner, -1 never, 1..4 that form with (hub chunk,
-                        chunks, bins) of 2^(13,13,13), 2^(14,14,13),
-                        2^(13,13,12), 2^(14,13,13) ids; its row sums are
-                        fixed-point sums, independent of the order of adds */
+                        autotuner, -1 never, 1..6 that form with (hub chunk,
+                        chunks, bins) of (8192, 8192, 8192), (16384, 16384,
+                        8192), (8192, 8192, 4096), (16384, 8192, 8192),
+                        (20480, 8192, 8192), (16384, 4096, 8192) ids; its
+                        row sums are fixed-point sums, independent of the
+                        order of adds */
 } gb_pr_params;
 
 typedef struct gb_pr_result {

@@ -599,7 +599,7 @@ def test_pagerank_row_block_args_rejected(gpu):
             gb.pagerank(g, **kw)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6])
 def test_pagerank_binned_matches_oracle(gpu, shape, monkeypatch):
     """The binned form: contributions streamed into (bin, chunk) segments and
     summed per bin in fixed point.  Cases: skewed R-MAT (relabelled and not),
@@ -664,7 +664,7 @@ def test_pagerank_binned_args_rejected(gpu):
     gb = gpu
     h = gb.gen_rmat(8, 8, seed=3, weights="none")
     g = gb.Graph.from_host(h, reverse=h.transpose())
-    for kw in (dict(binned=5), dict(binned=-2), dict(binned=1, row_block=2048),
+    for kw in (dict(binned=7), dict(binned=-2), dict(binned=1, row_block=2048),
                dict(binned=1, items_per_thread=8)):
         with pytest.raises(gb.GBError):
             gb.pagerank(g, **kw)
