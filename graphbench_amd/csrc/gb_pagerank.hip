@@ -1902,8 +1902,9 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
     }
   } else {
     // eager launches; with time_kernels every gather is bracketed by its own
-    // event pair, a third event ends the iteration, and the stop flag is read
-    // every CH iterations only
+    // event pair, a third event ends the iteration when the autotuner asks
+    // for iteration times (every event record holds up the next launch by a
+    // few microseconds), and the stop flag is read every CH iterations only
     if (p->time_kernels) GB_HIP(w.ensure_events(3 * (size_t)p->max_iter));
     const std::vector<hipEvent_t> &evs = w.events;
     for (int k = 0; k < p->max_iter; k++) {
@@ -1913,7 +1914,7 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
       float *cn = (k % 2 == 0) ? cb : ca;
       one_iter(xo, co, xn, cn, p->time_kernels ? evs[3 * k] : nullptr,
                p->time_kernels ? evs[3 * k + 1] : nullptr);
-      if (p->time_kernels) (void)hipEventRecord(evs[3 * k + 2], st);
+      if (p->time_kernels && iter_ms) (void)hipEventRecord(evs[3 * k + 2], st);
       if ((k + 1) % CH == 0 || k + 1 == p->max_iter) {
         GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
         GB_HIP(hipStreamSynchronize(st));
@@ -1926,9 +1927,11 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
       for (int k = 0; k < host_state.iter && k < p->max_iter; k++) {
         float ms = 0.0f, ims = 0.0f;
         GB_HIP(hipEventElapsedTime(&ms, evs[3 * k], evs[3 * k + 1]));
-        GB_HIP(hipEventElapsedTime(&ims, evs[3 * k], evs[3 * k + 2]));
         gather_ms += ms;
-        if (iter_ms) *iter_ms += ims;
+        if (iter_ms) {
+          GB_HIP(hipEventElapsedTime(&ims, evs[3 * k], evs[3 * k + 2]));
+          *iter_ms += ims;
+        }
         gather_launches++;
       }
     }
