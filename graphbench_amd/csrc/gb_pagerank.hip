@@ -882,35 +882,30 @@ __global__ __launch_bounds__(PB_SCATTER_BLOCK) void pb_scatter_kernel(
   __shared__ float s_c[CS];
   if (st->done) return;
   const PbUnit u = units[blockIdx.x];
-  // the chunk's contributions (the last chunk may be short: clamped reads,
-  // entries past it are never indexed)
-  const int32_t n = min(CS, st->V - u.src0);
-  float c[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const int k = (int)threadIdx.x + j * PB_SCATTER_BLOCK;
-    c[j] = contrib[u.src0 + (k < n ? k : n - 1)];
-  }
-#pragma unroll
-  for (int j = 0; j < PER; ++j) s_c[threadIdx.x + j * PB_SCATTER_BLOCK] = c[j];
-  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / GB_WAVE);
   const int lane = gb_lane(), l4 = lane * 4;
   // waves take R consecutive tasks in turn (wave w: tasks t0 + w * R + k *
   // WAVES * R), so a unit's tasks spread over all its waves; a round's R
   // descriptors come one per lane, loaded during the round before, and its R
-  // id loads are in flight together
+  // id loads are in flight together.  The first round's descriptors and ids
+  // load together with the chunk's contributions.
   constexpr int R = 16;
   constexpr int32_t STRIDE = WAVES * R;
   int32_t base = u.t0 + wave * R;
   PbTask my = {0, 0, 0};
   if (lane < R && base + lane < u.t1) my = tasks[base + lane];
-  for (; base < u.t1; base += STRIDE) {
-    const int nt = min(R, u.t1 - base);
-    PbTask nx = {0, 0, 0};
-    if (lane < R && base + STRIDE + lane < u.t1) nx = tasks[base + STRIDE + lane];
-    int32_t in0[R], out0[R], n[R];
-    pb_u32x2 ids[R];
+  // the chunk's contributions (the last chunk may be short: clamped reads,
+  // entries past it are never indexed)
+  const int32_t nsrc = min(CS, st->V - u.src0);
+  float c[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = (int)threadIdx.x + j * PB_SCATTER_BLOCK;
+    c[j] = contrib[u.src0 + (k < nsrc ? k : nsrc - 1)];
+  }
+  int32_t in0[R], out0[R], n[R];
+  pb_u32x2 ids[R];
+  auto decode_and_load = [&](int nt) {
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       n[k] = k < nt ? __builtin_amdgcn_readlane(my.n, k) : 0;
@@ -920,6 +915,15 @@ __global__ __launch_bounds__(PB_SCATTER_BLOCK) void pb_scatter_kernel(
 #pragma unroll
     for (int k = 0; k < R; ++k)
       if (l4 < n[k]) ids[k] = pb_ids4(src16 + in0[k] + l4);
+  };
+  if (base < u.t1) decode_and_load(min(R, u.t1 - base));
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s_c[threadIdx.x + j * PB_SCATTER_BLOCK] = c[j];
+  __syncthreads();
+  for (bool first = true; base < u.t1; base += STRIDE, first = false) {
+    PbTask nx = {0, 0, 0};
+    if (lane < R && base + STRIDE + lane < u.t1) nx = tasks[base + STRIDE + lane];
+    if (!first) decode_and_load(min(R, u.t1 - base));
 #pragma unroll
     for (int k = 0; k < R; ++k)
       if (l4 < n[k])
