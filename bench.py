@@ -112,6 +112,23 @@ def cpu_sssp(oracle, np, graphs, threads, runs, warmup):
     return med, trav
 
 
+# (hub chunk, log2 chunk width, log2 bin width) of the binned PageRank shapes
+PB_SHAPES = {1: (8192, 13, 13), 2: (16384, 14, 13), 3: (8192, 13, 12), 4: (16384, 13, 13),
+             5: (20480, 13, 13), 6: (16384, 12, 13)}
+
+
+def gather_kernel_name(pr):
+    """The gather kernel(s) of the form the solve ran, as rocprofv3 names them."""
+    if pr["binned"]:
+        hub, csl, bsl = PB_SHAPES[pr["binned"]]
+        return f"pb_scatter_kernel<{csl}> + pb_accum_kernel<{hub}, {bsl}>"
+    if pr["row_block"]:
+        if pr["hot"]:
+            return f"pr_rowblock_hot_kernel<{pr['row_block']}, {pr['hot']}>"
+        return f"pr_rowblock_kernel<{pr['row_block']}>"
+    return f"pr_gather_kernel<{pr['items_per_thread']}, {pr['hot']}>"
+
+
 def pagerank_bytes(V, E):
     # row_ptr 4(V+1) + col 4E + gathered contrib 4E + x_old 4V + 1/outdeg 4V + x_new 4V
     return 8 * E + 16 * V + 4
@@ -529,6 +546,10 @@ def main():
     if os.path.exists(TRAFFIC_JSON):
         with open(TRAFFIC_JSON) as f:
             traffic = json.load(f)
+    ran = gather_kernel_name(pr)
+    # the counters describe one kernel: a profile of another one is not used
+    if traffic.get("kernel") != ran:
+        traffic = {"stale": f"{TRAFFIC_JSON} profiles {traffic.get('kernel')}; this run used {ran}"}
     line = {
         "metric": "GTEPS + % HBM peak (SSSP/PR/BC); learned-delta+batched vs plain, same GPU",
         "value": pr["gteps"],
@@ -551,8 +572,8 @@ def main():
                      # named in traffic_source)
                      "limiter": traffic.get("limiter"),
                      "traffic": traffic.get("traffic_bytes_per_launch"),
-                     "traffic_source": traffic.get("source"),
-                     "kernel": traffic.get("kernel", "pr_gather"),
+                     "traffic_source": traffic.get("source", traffic.get("stale")),
+                     "kernel": ran,
                      "bytes_per_launch": pr["bytes_per_launch"],
                      "mean_launch_ms": pr["gather_ms_avg"],
                      "shape": {"items_per_thread": pr["items_per_thread"], "hot": pr["hot"],

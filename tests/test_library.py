@@ -53,6 +53,30 @@ def test_library_has_gfx950_code_object():
     assert b"gfx950" in data
 
 
+def test_bench_gather_names_match_the_library():
+    """bench.py names the PageRank gather kernels of the form a solve ran, to
+    match them against the committed profile; every binned shape it knows
+    must be a kernel pair the library contains, and the committed traffic
+    profile must name one of the forms."""
+    import importlib.util
+    import json
+    import graphbench_amd as gb
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    data = open(gb.LIB_PATH, "rb").read()
+    names = []
+    for shape, (hub, csl, bsl) in bench.PB_SHAPES.items():
+        pr = dict(binned=shape, row_block=0, hot=0, items_per_thread=0)
+        names.append(bench.gather_kernel_name(pr))
+        assert f"pb_scatter_kernelILi{csl}E".encode() in data, shape
+        assert f"pb_accum_kernelILi{hub}ELi{bsl}E".encode() in data, shape
+    assert bench.gather_kernel_name(dict(binned=0, row_block=4096, hot=8192,
+                                         items_per_thread=0)) == "pr_rowblock_hot_kernel<4096, 8192>"
+    prof = json.load(open(os.path.join(ROOT, "profiles", "pr_gather_rmat22_traffic.json")))
+    assert prof["kernel"] in names
+
+
 def test_splitmix64_known_values():
     import graphbench_amd as gb
     # SplitMix64 finaliser of 0 and 1 (Steele, Lea, Flood 2014 reference outputs
