@@ -478,21 +478,29 @@ __global__ __launch_bounds__(MS_BLOCK) void ms_compact_kernel(MsCtx c,
   gb_block_flush(nq);
 }
 
-// group distances [V][W] -> caller rows out[t][V] for t < ntask
-__global__ __launch_bounds__(256) void ms_unpack_kernel(int64_t V, int W, int ntask,
+// group distances [V][W] -> caller rows out[t][V] for t < ntask.  A block
+// takes VT = 4096 / W vertices: it reads their W * VT distances as one
+// contiguous run (every lane busy) into an LDS tile whose row stride VT + 1
+// keeps the transposed writes on distinct banks, then writes each source's VT
+// distances as one run.
+template <int W>
+__global__ __launch_bounds__(256) void ms_unpack_kernel(int64_t V, int ntask,
                                                          const uint32_t *__restrict__ dist,
                                                          uint32_t *__restrict__ out) {
-  __shared__ uint32_t tile[64][65];
-  const int64_t v0 = (int64_t)blockIdx.x * 64;
-  const int t0 = blockIdx.y * 64;
-  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
-    const int vv = i / 64, tt = i % 64;
-    if (v0 + vv < V && t0 + tt < W) tile[tt][vv] = dist[(v0 + vv) * W + t0 + tt];
+  constexpr int VT = 4096 / W;
+  __shared__ uint32_t tile[W][VT + 1];
+  const int64_t v0 = (int64_t)blockIdx.x * VT;
+  const int64_t rest = V - v0;
+  const int nv = rest < VT ? (int)rest : VT;
+  const uint32_t *src = dist + v0 * W;
+  for (int i = threadIdx.x; i < W * VT; i += 256) {
+    const int vv = i / W, tt = i % W;
+    if (vv < nv) tile[tt][vv] = src[i];
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
-    const int tt = i / 64, vv = i % 64;
-    if (v0 + vv < V && t0 + tt < ntask) out[(int64_t)(t0 + tt) * V + v0 + vv] = tile[tt][vv];
+  for (int i = threadIdx.x; i < W * VT; i += 256) {
+    const int tt = i / VT, vv = i % VT;
+    if (vv < nv && tt < ntask) out[(int64_t)tt * V + v0 + vv] = tile[tt][vv];
   }
 }
 
@@ -702,8 +710,9 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, int grid, hipS
   }
   GB_TRY(gst);
   if (hs.capped) return gb_fail(GB_ERR_NOT_CONVERGED, "sssp: phase cap reached");
-  hipLaunchKernelGGL(ms_unpack_kernel, dim3((unsigned)((V + 63) / 64), (unsigned)((W + 63) / 64)),
-                     dim3(256), 0, st, V, W, c.ntask, (const uint32_t *)c.dist, out_rows);
+  constexpr int VT = 4096 / W;
+  hipLaunchKernelGGL(ms_unpack_kernel<W>, dim3((unsigned)((V + VT - 1) / VT)), dim3(256), 0, st,
+                     V, c.ntask, (const uint32_t *)c.dist, out_rows);
   GB_HIP(hipGetLastError());
   GB_TRY(gb_wstat_fold(nwaves, c.wstat, c.cnt, MS_RELAX, MS_IMP_NEAR, MS_IMP_FAR, st,
                        MS_EXPANDED));
