@@ -871,17 +871,71 @@ __device__ __forceinline__ pb_f32x4 pb_lookup4(const float *s_c, pb_u32x2 s) {
   return o;
 }
 
+// The end of a binned iteration: its nparts residual / dangling partials (one
+// per piece, then one per split-bin row block) summed in a fixed order by one
+// block of PB_SCATTER_BLOCK threads, then the next base and the stop flag.
+// Thread t sums partials t, t + NT, t + 2 NT, ... in that order whatever U
+// (the loads in flight per batch), so every U gives the same totals.
+template <int U>
+__device__ __forceinline__ void pb_fix_block(int64_t nparts, const double *__restrict__ p_res,
+                                             const double *__restrict__ p_dang, PrState *st,
+                                             double *s_red) {
+  constexpr int NT = PB_SCATTER_BLOCK;
+  double res = 0.0, dang = 0.0;
+  for (int64_t b = threadIdx.x; b < nparts; b += (int64_t)NT * U) {
+    double a[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + (int64_t)u * NT;
+      const int64_t j = i < nparts ? i : nparts - 1;
+      a[u] = p_res[j];
+      c[u] = p_dang[j];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = b + (int64_t)u * NT < nparts;
+      res += in ? a[u] : 0.0;
+      dang += in ? c[u] : 0.0;
+    }
+  }
+  const double rtot = block_sum_f64(res, s_red);
+  const double dtot = block_sum_f64(dang, s_red);
+  if (threadIdx.x == 0) pr_finish_iteration(st, rtot, dtot);
+}
+
+// pb_fix_block as a launch of its own (the last iteration of a chunk, or
+// when there is nothing to scatter).
+__global__ __launch_bounds__(PB_SCATTER_BLOCK) void pb_fix_kernel(
+    int64_t nparts, const double *__restrict__ p_res, const double *__restrict__ p_dang,
+    PrState *st) {
+  __shared__ double s_red[PB_SCATTER_BLOCK / GB_WAVE];
+  if (st->done) return;
+  pb_fix_block<8>(nparts, p_res, p_dang, st, s_red);
+}
+
+// fold = 1: block 0 ends the previous iteration (pb_fix_block on its
+// partials) while the other blocks scatter.  They read the stop flag as it
+// was before that iteration or as block 0 sets it: either way, this
+// iteration's accumulation runs only if the flag stays clear, so a scatter
+// done after the last iteration is only wasted work, and at most one is.
 template <int CSL>
 __global__ __launch_bounds__(PB_SCATTER_BLOCK) void pb_scatter_kernel(
     const PbUnit *__restrict__ units, const PbTask *__restrict__ tasks,
     const uint16_t *__restrict__ src16, const float *__restrict__ contrib,
-    float *__restrict__ vals, const PrState *__restrict__ st) {
+    float *__restrict__ vals, int fold, int64_t nparts, const double *__restrict__ p_res,
+    const double *__restrict__ p_dang, PrState *st) {
   constexpr int CS = 1 << CSL;
   constexpr int PER = CS / PB_SCATTER_BLOCK;
   constexpr int WAVES = PB_SCATTER_BLOCK / GB_WAVE;
   __shared__ float s_c[CS];
-  if (st->done) return;
-  const PbUnit u = units[blockIdx.x];
+  if (fold && blockIdx.x == 0) {
+    __shared__ double s_red[WAVES];
+    if (st->done) return;
+    pb_fix_block<2>(nparts, p_res, p_dang, st, s_red);  // few registers: the scatter's occupancy
+    return;
+  }
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const PbUnit u = units[blockIdx.x - fold];
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / GB_WAVE);
   const int lane = gb_lane(), l4 = lane * 4;
   // waves take R consecutive tasks in turn (wave w: tasks t0 + w * R + k *
@@ -1049,9 +1103,10 @@ __global__ __launch_bounds__(PB_ACCUM_BLOCK) void pb_accum_kernel(
 }
 
 // Rows of split bins from sum64 (cleared for the next iteration), one
-// residual / dangling partial per block.  pr_rb_fixup_one_kernel then sums
-// these and pb_accum's partials in a fixed order and sets the next base and
-// the stop flag.
+// residual / dangling partial per block.  pb_fix_block (in the next
+// iteration's scatter launch, or in pb_fix_kernel) then sums these and
+// pb_accum's partials in a fixed order and sets the next base and the stop
+// flag.
 __global__ __launch_bounds__(PR_BLOCK) void pb_rows_kernel(
     int64_t nrows, int bsl, const int32_t *__restrict__ multi,
     unsigned long long *__restrict__ sum64, RowOut out, double *__restrict__ part_res,
@@ -1777,9 +1832,12 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
 
   const int32_t *trow = g->pr_tile_row.as<int32_t>();
   const int32_t *tedge = g->pr_tile_edge.as<int32_t>();
-  // One iteration; ev0..ev1 (optional) bracket the gather launches.
+  // One iteration; ev0..ev1 (optional) bracket the gather launches.  Binned
+  // form: fold, the previous iteration's end runs in this one's scatter
+  // launch; defer, this iteration's end is left to the next one's.
+  const bool can_fold = pb && PB.nunits > 0;
   auto one_iter = [&](const float *x_old, const float *c_old, float *x_new, float *c_new,
-                      hipEvent_t ev0, hipEvent_t ev1) {
+                      hipEvent_t ev0, hipEvent_t ev1, bool fold, bool defer) {
     RowOut out{0.0f, (float)p->damping, x_old, inv, x_new, c_new};
     if (ev0) (void)hipEventRecord(ev0, st);
     if (pb) {
@@ -1787,9 +1845,12 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
         auto scatter = PB.csl == 14   ? pb_scatter_kernel<14>
                        : PB.csl == 12 ? pb_scatter_kernel<12>
                                       : pb_scatter_kernel<13>;
-        hipLaunchKernelGGL(scatter, dim3((unsigned)PB.nunits), dim3(PB_SCATTER_BLOCK), 0, st,
-                           PB.units.as<PbUnit>(), PB.tasks.as<PbTask>(),
-                           PB.src16.as<uint16_t>(), c_old, PB.vals.as<float>(), state);
+        const int f = fold ? 1 : 0;
+        hipLaunchKernelGGL(scatter, dim3((unsigned)PB.nunits + f), dim3(PB_SCATTER_BLOCK), 0,
+                           st, PB.units.as<PbUnit>(), PB.tasks.as<PbTask>(),
+                           PB.src16.as<uint16_t>(), c_old, PB.vals.as<float>(), f, nparts,
+                           (const double *)w.pres.as<double>(),
+                           (const double *)w.pdang.as<double>(), state);
       }
       auto accum = PB.hub == 20480  ? pb_accum_kernel<20480, 13>
                    : PB.hub == 16384 ? pb_accum_kernel<16384, 13>
@@ -1807,10 +1868,10 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
                            PB.sum64.as<unsigned long long>(), out, w.pres.as<double>() + ntiles,
                            w.pdang.as<double>() + ntiles, state);
       // partials: one per piece, then one per row block, summed in that order
-      hipLaunchKernelGGL(pr_rb_fixup_one_kernel, dim3(1), dim3(RB_FIX1_THREADS), 0, st, nparts, 0,
-                         (const LongRow *)nullptr, (const F2 *)nullptr, out,
-                         (const double *)w.pres.as<double>(), (const double *)w.pdang.as<double>(),
-                         state);
+      if (!defer)
+        hipLaunchKernelGGL(pb_fix_kernel, dim3(1), dim3(PB_SCATTER_BLOCK), 0, st, nparts,
+                           (const double *)w.pres.as<double>(),
+                           (const double *)w.pdang.as<double>(), state);
       return;
     }
     if (rb) {
@@ -1887,10 +1948,11 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
       hipGraph_t graph = nullptr;
       GB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < CH; k++) {
+        const bool fold = can_fold && k > 0, defer = can_fold && k + 1 < CH;
         if (k % 2 == 0)
-          one_iter(xa, ca, xb, cb, nullptr, nullptr);
+          one_iter(xa, ca, xb, cb, nullptr, nullptr, fold, defer);
         else
-          one_iter(xb, cb, xa, ca, nullptr, nullptr);
+          one_iter(xb, cb, xa, ca, nullptr, nullptr, fold, defer);
       }
       GB_HIP(hipStreamEndCapture(st, &graph));
       const hipError_t ie = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
@@ -1916,10 +1978,14 @@ static gb_status pagerank_run(gb_graph g, const gb_pr_params *p, float *d_x, gb_
       const float *co = (k % 2 == 0) ? ca : cb;
       float *xn = (k % 2 == 0) ? xb : xa;
       float *cn = (k % 2 == 0) ? cb : ca;
+      // the stop flag is read after iterations that end a chunk: their end
+      // runs on its own, every other iteration's in the next scatter launch
+      const bool read = (k + 1) % CH == 0 || k + 1 == p->max_iter;
       one_iter(xo, co, xn, cn, p->time_kernels ? evs[3 * k] : nullptr,
-               p->time_kernels ? evs[3 * k + 1] : nullptr);
+               p->time_kernels ? evs[3 * k + 1] : nullptr, can_fold && k % CH != 0,
+               can_fold && !read);
       if (p->time_kernels && iter_ms) (void)hipEventRecord(evs[3 * k + 2], st);
-      if ((k + 1) % CH == 0 || k + 1 == p->max_iter) {
+      if (read) {
         GB_HIP(hipMemcpyAsync(&host_state, state, sizeof(PrState), hipMemcpyDeviceToHost, st));
         GB_HIP(hipStreamSynchronize(st));
         if (host_state.done) break;
