@@ -428,54 +428,114 @@ __global__ void ms_clear_ctrl_kernel(MsCtx c, const MsCtl *__restrict__ prev,
 
 // After a phase: every vertex whose next-phase mask is set gets one near-queue
 // entry per chunk of its edges, with the mask, into queue a_near; the mask is
-// zeroed for the phase after.  A wave scans 256 consecutive vertices at a time
-// (four per lane, their masks read together and coalesced) and skips the
-// batch when none is marked, which is most of them in short phases.
-// Vertices without out-edges are dropped.
+// zeroed for the phase after.  Vertices without out-edges are dropped.  A block
+// takes `range` consecutive vertices (a multiple of MS_CSPAN) in batches of
+// MS_CSPAN, a wave 256 consecutive vertices of a batch, four per lane, their
+// masks read together and coalesced.  Two passes over the range: the first
+// counts its entries and reserves them with one atomic on the queue counter
+// (per-wave reservations, one per 128 entries, serialise on the counter's
+// cache line when most vertices are marked); the second, run only by blocks
+// with entries, writes them at their offsets and clears the masks, re-reading
+// the masks and row offsets the first pass has just brought into L2.  A
+// wave's entries are laid out vertex slot by slot (j = 0..3), so lanes with
+// one entry each store one contiguous run.
 constexpr int MS_CPL = 4;  // vertices per lane and batch
+constexpr int64_t MS_CSPAN = (int64_t)MS_BLOCK * MS_CPL;
 template <int W>
 __global__ __launch_bounds__(MS_BLOCK) void ms_compact_kernel(MsCtx c,
-                                                              const MsCtl *__restrict__ ctl) {
+                                                              const MsCtl *__restrict__ ctl,
+                                                              int64_t range) {
   const int act = ctl->action;
   if (act == ACT_NONE) return;
   const int out = ctl->a_near;
-  __shared__ MsEnt s_near[MS_BLOCK / GB_WAVE][MS_NCAP];
-  gb_wave_queue<MsEnt, MS_NCAP> nq(s_near[threadIdx.x / GB_WAVE], c.qn[out - MS_NEAR0],
-                                   &c.cnt[out]);
-  const int lane = gb_lane();
-  const int64_t V = c.V;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  constexpr int NW = MS_BLOCK / GB_WAVE;
   constexpr int64_t SPAN = (int64_t)GB_WAVE * MS_CPL;
-  for (int64_t b0 = wave * SPAN; b0 < V; b0 += nwaves * SPAN) {
+  __shared__ uint32_t s_wt[NW];
+  __shared__ unsigned long long s_base;
+  __shared__ uint32_t s_tot;
+  const int lane = gb_lane(), w = (int)(threadIdx.x >> 6);
+  const int64_t lo = (int64_t)blockIdx.x * range;
+  const int64_t hi = min((int64_t)c.V, lo + range);
+  unsigned long long *mask = c.mask[0];
+  // pass 1: count
+  uint32_t cnt = 0;
+  for (int64_t b0 = lo + w * SPAN; b0 < hi; b0 += MS_CSPAN) {
     unsigned long long m[MS_CPL];
     bool any = false;
 #pragma unroll
     for (int j = 0; j < MS_CPL; j++) {
       const int64_t v = b0 + (int64_t)j * GB_WAVE + lane;
-      m[j] = v < V ? c.mask[0][v] : 0ull;
+      m[j] = v < hi ? mask[v] : 0ull;
       any |= m[j] != 0ull;
     }
     if (__ballot(any) == 0ull) continue;
+#pragma unroll
+    for (int j = 0; j < MS_CPL; j++) {
+      const int64_t v = b0 + (int64_t)j * GB_WAVE + lane;
+      if (m[j]) {
+        const int32_t r0 = c.rp[v], r1 = c.rp[v + 1];
+        cnt += r1 > r0 ? ms_nchunks(r1 - r0, c.cshift) : 0u;
+      }
+    }
+  }
+  uint32_t wtot;
+  (void)gb_wave_excl_sum(cnt, &wtot);
+  if (lane == 0) s_wt[w] = wtot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int q = 0; q < NW; q++) t += s_wt[q];
+    s_tot = t;
+    s_base = t ? atomicAdd(&c.cnt[out], (unsigned long long)t) : 0ull;
+  }
+  __syncthreads();
+  if (s_tot == 0) return;  // block-uniform
+  // pass 2: write, in batch order, each batch's waves in wave order
+  MsEnt *q = c.qn[out - MS_NEAR0];
+  uint64_t off = s_base;
+  for (int64_t bb = lo; bb < hi; bb += MS_CSPAN) {  // block-uniform
+    const int64_t b0 = bb + w * SPAN;
+    unsigned long long m[MS_CPL];
     int32_t r0[MS_CPL], r1[MS_CPL];
+    uint32_t k[MS_CPL], ex[MS_CPL], tj[MS_CPL];
+#pragma unroll
+    for (int j = 0; j < MS_CPL; j++) {
+      const int64_t v = b0 + (int64_t)j * GB_WAVE + lane;
+      m[j] = v < hi ? mask[v] : 0ull;
+    }
+    uint32_t wsum = 0;
 #pragma unroll
     for (int j = 0; j < MS_CPL; j++) {
       const int64_t v = b0 + (int64_t)j * GB_WAVE + lane;
       r0[j] = r1[j] = 0;
       if (m[j]) {
-        c.mask[0][v] = 0ull;
+        mask[v] = 0ull;
         r0[j] = c.rp[v];
         r1[j] = c.rp[v + 1];
       }
+      k[j] = r1[j] > r0[j] ? ms_nchunks(r1[j] - r0[j], c.cshift) : 0u;
+      ex[j] = gb_wave_excl_sum(k[j], &tj[j]);
+      wsum += tj[j];
     }
+    if (lane == 0) s_wt[w] = wsum;
+    __syncthreads();
+    uint64_t p = off;
+    uint32_t btot = 0;
+    for (int qq = 0; qq < NW; qq++) {
+      const uint32_t t = s_wt[qq];
+      if (qq < w) p += t;
+      btot += t;
+    }
+    __syncthreads();  // s_wt is rewritten by the next batch
 #pragma unroll
     for (int j = 0; j < MS_CPL; j++) {
       const uint32_t v = (uint32_t)(b0 + (int64_t)j * GB_WAVE + lane);
-      const uint32_t k = r1[j] > r0[j] ? ms_nchunks(r1[j] - r0[j], c.cshift) : 0u;
-      nq.push(k, [&](uint32_t q) { return ms_chunk(m[j], v, r0[j], r1[j], q, c.cshift); });
+      for (uint32_t i = 0; i < k[j]; i++)
+        q[p + ex[j] + i] = ms_chunk(m[j], v, r0[j], r1[j], i, c.cshift);
+      p += tj[j];
     }
+    off += btot;
   }
-  gb_block_flush(nq);
 }
 
 // group distances [V][W] -> caller rows out[t][V] for t < ntask.  A block
@@ -651,6 +711,8 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, int grid, hipS
     GB_HIP(hipGetLastError());
   }
   const int cgrid = SCAN ? ms_compact_grid(c.V) : gb_num_cus() * 2;
+  // vertices per compaction block: a multiple of MS_CSPAN, cgrid blocks cover V
+  const int64_t crange = ((c.V + cgrid - 1) / cgrid + MS_CSPAN - 1) / MS_CSPAN * MS_CSPAN;
   constexpr int PH = 16;  // phases per replay (even: the append form ends on ctl[1])
   hipStream_t cs = st, own = nullptr;
   if (!st) {  // the null stream cannot be captured
@@ -672,7 +734,7 @@ gb_status ms_group(MsCtx c, const int32_t *d_src, int max_phases, int grid, hipS
         hipLaunchKernelGGL((ms_step_kernel<W, true>), dim3(grid), dim3(MS_BLOCK), 0, cs, c,
                            (const MsCtl *)ctl);
         hipLaunchKernelGGL(ms_compact_kernel<W>, dim3(cgrid), dim3(MS_BLOCK), 0, cs, c,
-                           (const MsCtl *)ctl);
+                           (const MsCtl *)ctl, crange);
         hipLaunchKernelGGL(ms_ctrl_kernel, dim3(1), dim3(1), 0, cs, c, ctl);
       } else {
         MsCtl *cur = ctl + (k % 2), *prev = ctl + ((k + 1) % 2);
