@@ -1,6 +1,6 @@
 """CPU oracle for the CSR analytics path -- TEST INFRASTRUCTURE ONLY.
 
-This package is the checker: only ``tests/``, ``__graft_entry__.smoke()`` and
+This package is the test oracle: only ``tests/``, ``__graft_entry__.smoke()`` and
 the ``cpu_baseline`` leg of ``bench.py`` may import it.  The shipped library
 (``graphbench_amd``) never imports, links or calls anything here.
 

@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement of the reference's CSR analytics algorithms.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline
+ * TEST INFRASTRUCTURE ONLY: what the HIP path is tested against, and the CPU baseline
  * timed by bench.py.  Nothing in graphbench_amd/ links or calls this file.
  *
  * Reference citations are file:line in stplaydog/GRAPHBENCH.

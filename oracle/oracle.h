@@ -1,7 +1,7 @@
 /*
  * oracle.h -- CPU oracle for the CSR analytics path (TEST INFRASTRUCTURE ONLY).
  *
- * This library is the checker, never the product: only tests/, the smoke()
+ * This library is the test oracle, never the product: only tests/, the smoke()
  * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load it.
  * The shipped library (graphbench_amd/libgraphbench.so) never links or calls it.
  *
